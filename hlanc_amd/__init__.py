@@ -1,0 +1,331 @@
+"""Python binding of libhlanc_hip.so, the MI355X implementation of HLanc's IRLM hot path.
+
+The names mirror the reference's C++ interface (HLanc.hpp): ``dcsr_matrix_t``
+(HLanc.hpp:74-100), ``dcsrmv_operator_gpu`` (HLanc.hpp:476-572),
+``dcsrmv_operator_mgpu`` (HLanc.hpp:750-893, one GPU) and ``ds_solver_gpu``
+(HLanc.hpp:1868-1872).  Everything below calls the C ABI declared in
+include/hlanc_capi.h through ctypes; there is no CPU fallback: if the library is
+not built, importing works but every call raises ``HLancError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhlanc_hip.so")
+
+HL_OK = 0
+HL_NOT_CONVERGED = 1
+HL_PREC_F64 = 0
+HL_PREC_F32 = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+HOST_OP = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, _dp)
+
+
+class HLancError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        super().__init__(f"hlanc error {code}: {msg}")
+        self.code = code
+
+
+class _Matrix(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int), ("ncols", ctypes.c_int), ("nnz", ctypes.c_int), ("sym", ctypes.c_int),
+                ("row_ptr", _ip), ("col", _ip), ("val", _dp)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("v0", _dp), ("seed", ctypes.c_uint64),
+                ("basis_prec", ctypes.c_int), ("use_mfma", ctypes.c_int), ("flags", ctypes.c_int),
+                ("reserved", ctypes.c_int * 5)]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("ret", ctypes.c_int), ("restarts", ctypes.c_int), ("nconv", ctypes.c_int), ("pad", ctypes.c_int),
+                ("spmv", ctypes.c_longlong), ("cgs_passes", ctypes.c_longlong), ("rnorm", ctypes.c_double),
+                ("seconds", ctypes.c_double), ("max_rel_resid", ctypes.c_double),
+                ("refine_shift", ctypes.c_double)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every entry point of include/hlanc_capi.h
+_SIGS = [
+    ("hl_last_error", ctypes.c_char_p, []),
+    ("hl_version", ctypes.c_char_p, []),
+    ("hl_device_count", ctypes.c_int, [_ip]),
+    ("hl_matrix_read", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_Matrix)]),
+    ("hl_matrix_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_Matrix)]),
+    ("hl_matrix_free", None, [ctypes.POINTER(_Matrix)]),
+    ("hl_csr_create", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp,
+                                     ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("hl_csr_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("hl_csr_info", ctypes.c_int, [ctypes.c_void_p, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_longlong)]),
+    ("hl_csr_apply", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("hl_csr_spmv", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("hl_csr_spmv_f32val", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("hl_csr_spmv_bytes", ctypes.c_double, [ctypes.c_void_p, ctypes.c_int]),
+    ("hl_cgs_pass", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
+                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p]),
+    ("hl_vq", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                             ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
+                             ctypes.c_int, ctypes.c_void_p]),
+    ("hl_options_default", None, [ctypes.POINTER(Options)]),
+    ("hl_ds_solve_csr", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
+                                       ctypes.POINTER(Options), ctypes.POINTER(Info)]),
+    ("hl_ds_solve_host_op", ctypes.c_int, [ctypes.c_int, ctypes.c_int, HOST_OP, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.POINTER(Options),
+                                           ctypes.POINTER(Info)]),
+]
+
+
+def build(verbose: bool = False) -> None:
+    """Compile libhlanc_hip.so for gfx950 with hipcc (no GPU needed)."""
+    subprocess.run(["make", "-s", "-j8", "-C", os.path.join(_HERE, "csrc")], check=True,
+                   stdout=None if verbose else subprocess.DEVNULL)
+
+
+def lib():
+    """The loaded C library; raises HLancError when it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HLancError(-2, f"{LIB_PATH} is not built; run hlanc_amd.build() or __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return [s[0] for s in _SIGS]
+
+
+def _check(rc: int, allow=(HL_OK,)):
+    if rc not in allow:
+        raise HLancError(rc, lib().hl_last_error().decode(errors="replace"))
+    return rc
+
+
+def device_count() -> int:
+    c = ctypes.c_int()
+    lib().hl_device_count(ctypes.byref(c))
+    return c.value
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+# ----------------------------------------------------------------- matrices
+
+class dcsr_matrix_t:
+    """Host CSR matrix (HLanc.hpp:74-100): reads "pp"/"mm"/"bin" files, or wraps arrays."""
+
+    def __init__(self, format=None, filename=None, is_symmetric=False, *, row_ptr=None, col=None, val=None,
+                 ncols=None):
+        if format is not None:
+            m = _Matrix()
+            _check(lib().hl_matrix_read(format.encode(), os.fsencode(filename), int(is_symmetric), ctypes.byref(m)))
+            try:
+                self.row_ptr = np.ctypeslib.as_array(m.row_ptr, shape=(m.nrows + 1,)).copy()
+                self.col = np.ctypeslib.as_array(m.col, shape=(max(m.nnz, 1),))[: m.nnz].copy()
+                self.val = np.ctypeslib.as_array(m.val, shape=(max(m.nnz, 1),))[: m.nnz].copy()
+                self.num_columns = m.ncols
+                self.is_symmetric = bool(m.sym)
+            finally:
+                lib().hl_matrix_free(ctypes.byref(m))
+        else:
+            self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+            self.col = np.ascontiguousarray(col, dtype=np.int32)
+            self.val = np.ascontiguousarray(val, dtype=np.float64)
+            n = len(self.row_ptr) - 1
+            self.num_columns = int(ncols if ncols is not None else n)
+            self.is_symmetric = bool(is_symmetric)
+
+    def get_num_rows(self):
+        return len(self.row_ptr) - 1
+
+    def get_nnz(self):
+        return len(self.val)
+
+    def get_num_columns(self):
+        return self.num_columns
+
+    def get_row_ptr(self):
+        return self.row_ptr
+
+    def get_column_index(self):
+        return self.col
+
+    def get_value(self):
+        return self.val
+
+    def symmetric(self):
+        return self.is_symmetric
+
+    def _as_struct(self):
+        self._keep = (np.ascontiguousarray(self.row_ptr, np.int32), np.ascontiguousarray(self.col, np.int32),
+                      np.ascontiguousarray(self.val, np.float64))
+        rp, c, v = self._keep
+        return _Matrix(self.get_num_rows(), self.num_columns, len(v), int(self.is_symmetric), _i(rp), _i(c), _d(v))
+
+    def save(self, format, filename):
+        m = self._as_struct()
+        _check(lib().hl_matrix_write(format.encode(), os.fsencode(filename), ctypes.byref(m)))
+
+    def save_pp(self, filename):
+        self.save("pp", filename)
+
+    def save_mm(self, filename):
+        self.save("mm", filename)
+
+    def save_bin(self, filename):
+        self.save("bin", filename)
+
+
+# ----------------------------------------------------------------- operator
+
+class dcsrmv_operator_gpu:
+    """y = A x on one MI355X (HLanc.hpp:476-572).  Callable with numpy arrays
+    (host staging, synchronous) or, through ``spmv``, with device pointers."""
+
+    def __init__(self, m: dcsr_matrix_t, dev: int = 0, fp32_values: bool = False):
+        self.m = m
+        self.dev = dev
+        h = ctypes.c_void_p()
+        rp = np.ascontiguousarray(m.row_ptr, np.int32)
+        c = np.ascontiguousarray(m.col, np.int32)
+        v = np.ascontiguousarray(m.val, np.float64)
+        _check(lib().hl_csr_create(dev, len(rp) - 1, m.num_columns, len(v), _i(rp), _i(c), _d(v),
+                                   int(m.is_symmetric), HL_PREC_F32 if fp32_values else HL_PREC_F64,
+                                   ctypes.byref(h)))
+        self.h = h
+        self.nrows = len(rp) - 1
+        self.ncols = m.num_columns
+        nnz = ctypes.c_longlong()
+        lib().hl_csr_info(h, None, None, None, ctypes.byref(nnz))
+        self.nnz_general = nnz.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hl_csr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __call__(self, x, y=None):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if y is None:
+            y = np.empty(self.nrows)
+        _check(lib().hl_csr_apply(self.h, x.ctypes.data, y.ctypes.data))
+        return y
+
+    def spmv(self, x_ptr: int, y_ptr: int, stream: int = 0, fp32_values: bool = False):
+        f = lib().hl_csr_spmv_f32val if fp32_values else lib().hl_csr_spmv
+        _check(f(self.h, x_ptr, y_ptr, stream))
+
+    def spmv_bytes(self, fp32_values: bool = False) -> float:
+        return lib().hl_csr_spmv_bytes(self.h, HL_PREC_F32 if fp32_values else HL_PREC_F64)
+
+
+class dcsrmv_operator_mgpu(dcsrmv_operator_gpu):
+    """dcsrmv_operator_mgpu (HLanc.hpp:750-893) on a one-GPU build: the first
+    device in ``mask`` holds the whole matrix; symmetric storage is refused as
+    the reference refuses it (HLanc.hpp:798-800)."""
+
+    def __init__(self, m: dcsr_matrix_t, mask: int = 0xFFFFFFFF):
+        if m.symmetric():
+            raise HLancError(-1, "dcsrmv_operator_mgpu can only be constructed with a general matrix.")
+        ndev = device_count()
+        devs = [d for d in range(ndev) if mask & (1 << d)]
+        if not devs:
+            raise HLancError(-2, "no device selected by mask")
+        super().__init__(m, devs[0])
+
+
+# ------------------------------------------------------------------- solver
+
+@dataclass
+class SolveResult:
+    ret: int
+    values: np.ndarray
+    vectors: np.ndarray
+    info: Info
+
+
+def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mfma: bool = True,
+            basis_prec: int = HL_PREC_F64, flags: int = 0) -> Options:
+    o = Options()
+    lib().hl_options_default(ctypes.byref(o))
+    o.tol = tol
+    o.maxiter = maxiter
+    o.seed = seed
+    o.use_mfma = int(use_mfma)
+    o.basis_prec = basis_prec
+    o.flags = flags
+    if v0 is not None:
+        o._v0 = np.ascontiguousarray(v0, dtype=np.float64)
+        o.v0 = _d(o._v0)
+    return o
+
+
+def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, v0=None,
+                  seed: int = 0, use_mfma: bool = True, ldv: Optional[int] = None) -> SolveResult:
+    """ds_solver_gpu (HLanc.hpp:1868-1872).  ``op`` is a dcsrmv_operator_gpu
+    (device-resident path) or any callable ``op(x, y)`` on numpy arrays (host
+    path, as the reference calls arbitrary operators with host pointers).
+    ``ret`` is 0 on success and 1 on the reference's numerical failures; values
+    and vectors are written only on success."""
+    ldv = n if ldv is None else ldv
+    values = np.zeros(ncv)
+    vectors = np.zeros((ncv, ldv)).T  # column-major ldv x ncv
+    o = options(tol, maxiter, v0, seed, use_mfma)
+    info = Info()
+    if isinstance(op, dcsrmv_operator_gpu):
+        if op.nrows != n:
+            raise HLancError(-1, f"operator has {op.nrows} rows, n={n}")
+        rc = lib().hl_ds_solve_csr(op.h, nev, ncv, _d(values), vectors.ctypes.data_as(_dp), ldv,
+                                   ctypes.byref(o), ctypes.byref(info))
+    else:
+        def tramp(ctx, x, y, _op=op, _n=n):
+            xa = np.ctypeslib.as_array(x, shape=(_n,))
+            ya = np.ctypeslib.as_array(y, shape=(_n,))
+            _op(xa, ya)
+
+        cb = HOST_OP(tramp)
+        rc = lib().hl_ds_solve_host_op(dev, n, cb, None, nev, ncv, _d(values), vectors.ctypes.data_as(_dp), ldv,
+                                       ctypes.byref(o), ctypes.byref(info))
+    _check(rc, allow=(HL_OK, HL_NOT_CONVERGED))
+    return SolveResult(rc, values[:nev].copy(), np.asarray(vectors[:n]), info)
+
+
+# ------------------------------------------------------- basis operations
+
+def cgs_pass(dev, n, m, V_ptr, ldv, w_in_ptr, w_out_ptr, h_in_ptr, h_out_ptr, nrm2_ptr, stream=0):
+    """One fused CGS pass on device pointers (hl_cgs_pass)."""
+    _check(lib().hl_cgs_pass(dev, n, m, V_ptr, ldv, w_in_ptr, w_out_ptr, h_in_ptr, h_out_ptr, nrm2_ptr, stream))
+
+
+def vq(dev, n, K, N, V_ptr, ldv, Q_ptr, ldq, Out_ptr, ldo, use_mfma=True, stream=0):
+    """Out = V Q on device pointers (hl_vq)."""
+    _check(lib().hl_vq(dev, n, K, N, V_ptr, ldv, Q_ptr, ldq, Out_ptr, ldo, int(use_mfma), stream))

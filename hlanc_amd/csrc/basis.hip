@@ -1,0 +1,598 @@
+// basis.hip -- kernels over the Krylov basis V for CDNA4 (gfx950).
+//
+// Replaces mat_v_gpu_t's cuBLAS calls (HLanc.hpp:973-1119) and the host BLAS-1
+// of dsaitr (HLanc.hpp:1411-1465):
+//   * cgs_kernel: one fused classical Gram-Schmidt pass
+//       w_out = w_in - V h_in ;  h_out = V^T w_out ;  ||w_out||^2
+//     A wave covers a 16-row x m tile as 4 lane groups of 16 rows x ceil(m/4)
+//     columns, so each load instruction touches four 128-byte column segments.
+//     Partial sums meet in a two-stage reduction whose last block also runs the
+//     Lanczos/DGKS bookkeeping (Ctl), so a whole dsaitr sweep needs no host
+//     round trip.  Grid shape depends only on n: results are bitwise
+//     repeatable, the same for a solve alone or beside others.
+//   * vq kernels: Out = V Q (dsapps' basis update and the Ritz vectors), as a
+//     v_mfma_f64_16x16x4f64 kernel computing Out^T = Q^T V^T (so stores are
+//     column-coalesced) or as a plain FMA kernel; optional fused residual
+//     update r = qlast r + beta Out(:, k) with ||r||.
+#include <cmath>
+
+#include "hl_internal.hpp"
+
+namespace hl {
+namespace {
+
+constexpr int kGroups = 4;                 // column groups per wave
+constexpr int kTileRows = kWave / kGroups; // 16 rows per wave tile
+
+__device__ inline void run_epilogue(int mode, int j, double nrm2, const double *h_in, Ctl *ctl,
+                                    double *h_sub, double *h_dia)
+{
+    switch (mode) {
+    case EPI_NONE:
+        ctl->nrm2 = nrm2;
+        break;
+    case EPI_WNORM:
+        ctl->wnorm = sqrt(nrm2);
+        break;
+    case EPI_PASS1: // HLanc.hpp:1431-1440
+        h_dia[j] = h_in[j];
+        h_sub[j] = (j == 0) ? 0.0 : ctl->rnorm;
+        ctl->rnorm = sqrt(nrm2);
+        ctl->need = ctl->rnorm < kDgks * ctl->wnorm;
+        ctl->iter = 0;
+        ctl->cgs += 1;
+        break;
+    case EPI_PASS2: { // HLanc.hpp:1444-1458, first correction
+        h_dia[j] += h_in[j];
+        const double r1 = sqrt(nrm2);
+        if (r1 > kDgks * ctl->rnorm) {
+            ctl->rnorm = r1;
+            ctl->need = 0;
+        } else {
+            ctl->rnorm = r1;
+            ctl->iter = 1;
+            ctl->need = r1 < kDgks * ctl->wnorm;
+        }
+        ctl->cgs += 1;
+        break;
+    }
+    case EPI_PASS3: { // second correction; failure zeroes resid (HLanc.hpp:1459-1463)
+        h_dia[j] += h_in[j];
+        const double r1 = sqrt(nrm2);
+        if (r1 > kDgks * ctl->rnorm) {
+            ctl->rnorm = r1;
+        } else {
+            ctl->rnorm = 0.0;
+            ctl->rscale = 0.0;
+        }
+        ctl->need = 0;
+        ctl->cgs += 1;
+        break;
+    }
+    case EPI_RNORM:
+        ctl->rnorm = sqrt(nrm2);
+        ctl->rscale = 1.0;
+        break;
+    default:
+        break;
+    }
+    if (mode >= EPI_PASS1) // STEP 2's scale for v_{j+1} (HLanc.hpp:1412-1419)
+        ctl->xscale = (ctl->rnorm > kSafmin ? 1.0 / ctl->rnorm : 1.0) * ctl->rscale;
+}
+
+// Block partials of M1 values (already in LDS vals[0..M1)) -> global partials;
+// the last block to arrive reduces all of them in block order into fin[0..M1)
+// (LDS) and returns true.
+__device__ inline bool reduce_blocks(const double *vals, int M1, double *partials, unsigned *counter,
+                                     double *scratch /* LDS kBlock */, double *fin /* LDS >= M1 */)
+{
+    __shared__ int is_last;
+    const int tid = threadIdx.x;
+    for (int c = tid; c < M1; c += kBlock)
+        __hip_atomic_store(&partials[(long long)blockIdx.x * M1 + c], vals[c], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const unsigned prev =
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (prev + 1 == gridDim.x);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    const int nb = gridDim.x;
+    const int nsplit = kBlock / M1 > 0 ? kBlock / M1 : 1;
+    for (int c0 = 0; c0 < M1; c0 += kBlock) {
+        const int span = (M1 - c0) < kBlock ? (M1 - c0) : kBlock;
+        const int ns = span == M1 ? nsplit : 1;
+        const int c = c0 + tid % span;
+        const int part = tid / span;
+        double s = 0.0;
+        if (part < ns) {
+            for (int b = part; b < nb; b += ns)
+                s += __hip_atomic_load(&partials[(long long)b * M1 + c], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
+        scratch[tid] = s;
+        __syncthreads();
+        if (tid < span) {
+            double t = 0.0;
+            for (int q = 0; q < ns; ++q) t += scratch[q * span + tid];
+            fin[c0 + tid] = t;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+template <int MAXC, typename VT>
+__global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__restrict__ V, long long ldv,
+                                                     const double *w_in, double *w_out,
+                                                     const double *__restrict__ h_in, double *h_out,
+                                                     double *partials, unsigned *counter, Ctl *ctl,
+                                                     double *h_sub, double *h_dia, int mode, int j)
+{
+    if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return; // DGKS decided on the device
+    __shared__ double red[kBlock / kWave][kGroups * MAXC + 1];
+    __shared__ double vals[kGroups * MAXC + 1];
+    __shared__ double scratch[kBlock];
+    __shared__ double fin[kGroups * MAXC + 1];
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int g = lane / kTileRows;
+    const int rl = lane % kTileRows;
+    const int cpg = (m + kGroups - 1) / kGroups;
+    const int c0 = g * cpg;
+
+    double hin[MAXC], acc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+        acc[c] = 0.0;
+        hin[c] = (h_in && c < cpg && c0 + c < m) ? h_in[c0 + c] : 0.0;
+    }
+    double nrm = 0.0;
+    const long long ntiles = (n + kTileRows - 1) / kTileRows;
+    const long long W = (long long)gridDim.x * (kBlock / kWave);
+    for (long long t = (long long)blockIdx.x * (kBlock / kWave) + wv; t < ntiles; t += W) {
+        const long long r = t * kTileRows + rl;
+        const bool ok = r < n;
+        const double win = ok ? w_in[r] : 0.0;
+        double v[MAXC];
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            const int colj = c0 + c;
+            v[c] = (ok && c < cpg && colj < m) ? static_cast<double>(V[(long long)colj * ldv + r]) : 0.0;
+        }
+        double a = win;
+        if (h_in) {
+            double p = 0.0;
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) p += v[c] * hin[c];
+            p += __shfl_xor(p, 16);
+            p += __shfl_xor(p, 32);
+            a = win - p;
+        }
+        if (g == 0 && ok) {
+            if (w_out) w_out[r] = a;
+            nrm += a * a;
+        }
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) acc[c] += v[c] * a;
+    }
+
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+        double s = acc[c];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        s += __shfl_xor(s, 8);
+        acc[c] = s;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
+    if (rl == 0) {
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+            if (c < cpg && c0 + c < m) red[wv][c0 + c] = acc[c];
+    }
+    if (lane == 0) red[wv][m] = nrm;
+    __syncthreads();
+    const int M1 = m + 1;
+    for (int c = threadIdx.x; c < M1; c += kBlock) {
+        double s = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s += red[w][c];
+        vals[c] = s;
+    }
+    __syncthreads();
+    if (!reduce_blocks(vals, M1, partials, counter, scratch, fin)) return;
+    if (h_out)
+        for (int c = threadIdx.x; c < m; c += kBlock) h_out[c] = fin[c];
+    if (threadIdx.x == 0) run_epilogue(mode, j, fin[m], h_in, ctl, h_sub, h_dia);
+}
+
+// Out(:, c0:c0+CB) = V(:, 0:K) Q(0:K, c0:c0+CB); thread per row, FP64 FMA.
+template <int CB, typename VT>
+__global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, const VT *__restrict__ V,
+                                                        long long ldv, const double *__restrict__ Q, int ldq,
+                                                        VT *__restrict__ Out, long long ldo, int c0,
+                                                        int res_local, double *r, double qlast,
+                                                        double beta, double *partials, unsigned *counter,
+                                                        Ctl *ctl)
+{
+    extern __shared__ double qs[]; // K x CB
+    __shared__ double red[kBlock / kWave];
+    __shared__ double vals[2];
+    __shared__ double scratch[kBlock];
+    __shared__ double fin[2];
+    for (int i = threadIdx.x; i < K * CB; i += kBlock) {
+        const int k = i / CB, c = i % CB;
+        qs[i] = (c0 + c < N) ? Q[(long long)(c0 + c) * ldq + k] : 0.0;
+    }
+    __syncthreads();
+    const double qrs = (res_local >= 0) ? qlast * ctl->rscale : 0.0;
+    double nrm = 0.0;
+    for (long long row = (long long)blockIdx.x * kBlock + threadIdx.x; row < n;
+         row += (long long)gridDim.x * kBlock) {
+        double acc[CB];
+#pragma unroll
+        for (int c = 0; c < CB; ++c) acc[c] = 0.0;
+        for (int k = 0; k < K; ++k) {
+            const double v = static_cast<double>(V[(long long)k * ldv + row]);
+#pragma unroll
+            for (int c = 0; c < CB; ++c) acc[c] += v * qs[k * CB + c];
+        }
+        double rk = 0.0;
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            if (c0 + c < N) Out[(long long)(c0 + c) * ldo + row] = static_cast<VT>(acc[c]);
+            if (c == res_local) rk = acc[c];
+        }
+        if (res_local >= 0) { // dsapps residual (HLanc.hpp:1605-1614)
+            const double t = qrs * r[row];
+            const double rr = t + beta * rk;
+            r[row] = rr;
+            nrm += rr * rr;
+        }
+    }
+    if (res_local < 0) return;
+    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s += red[w];
+        vals[0] = s;
+    }
+    __syncthreads();
+    if (!reduce_blocks(vals, 1, partials, counter, scratch, fin)) return;
+    if (threadIdx.x == 0) run_epilogue(EPI_RNORM, 0, fin[0], nullptr, ctl, nullptr, nullptr);
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Out^T(c, row) = sum_k Q(k, c) V(row, k) on v_mfma_f64_16x16x4f64.
+// A (16 x 4): lane l holds Q^T(ct*16 + l%16, k0 + l/16); B (4 x 16): lane l holds
+// V(r0 + l%16, k0 + l/16); D (16 x 16): lane l, element i is
+// Out(r0 + l%16, ct*16 + 4*(l/16) + i).  NT column tiles of 16 per launch.
+template <int NT>
+__global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, const double *__restrict__ V,
+                                                         long long ldv, const double *__restrict__ Q, int ldq,
+                                                         double *__restrict__ Out, long long ldo, int c0,
+                                                         int res_local, double *r, double qlast, double beta,
+                                                         double *partials, unsigned *counter, Ctl *ctl)
+{
+    extern __shared__ double qs[]; // Kp x (NT*16), row k
+    __shared__ double red[kBlock / kWave];
+    __shared__ double vals[2];
+    __shared__ double scratch[kBlock];
+    __shared__ double fin[2];
+    const int NC = NT * 16;
+    const int Kp = (K + 3) & ~3;
+    for (int i = threadIdx.x; i < Kp * NC; i += kBlock) {
+        const int k = i / NC, c = i % NC;
+        qs[i] = (k < K && c0 + c < N) ? Q[(long long)(c0 + c) * ldq + k] : 0.0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int lr = lane & 15, lk = lane >> 4;
+    const double qrs = (res_local >= 0) ? qlast * ctl->rscale : 0.0;
+    double nrm = 0.0;
+    const long long ntiles = (n + 15) / 16;
+    for (long long t = (long long)blockIdx.x * (kBlock / kWave) + wv; t < ntiles;
+         t += (long long)gridDim.x * (kBlock / kWave)) {
+        const long long r0 = t * 16;
+        const long long row = r0 + lr;
+        const bool ok = row < n;
+        dbl4 acc[NT];
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int k0 = 0; k0 < Kp; k0 += 4) {
+            const int k = k0 + lk;
+            const double b = (ok && k < K) ? V[(long long)k * ldv + row] : 0.0;
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) {
+                const double a = qs[k * NC + ct * 16 + lr];
+                acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[ct], 0, 0, 0);
+            }
+        }
+        double rk = 0.0;
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int cl = ct * 16 + 4 * lk + i;
+                if (ok && c0 + cl < N) Out[(long long)(c0 + cl) * ldo + row] = acc[ct][i];
+                if (cl == res_local) rk = acc[ct][i];
+            }
+        }
+        if (res_local >= 0) { // the lane holding column res_local for this row shares it
+            const int owner = ((res_local % 16) / 4) * 16 + lr;
+            const double rkv = __shfl(rk, owner);
+            if (lk == 0 && ok) {
+                const double tt = qrs * r[row];
+                const double rr = tt + beta * rkv;
+                r[row] = rr;
+                nrm += rr * rr;
+            }
+        }
+    }
+    if (res_local < 0) return;
+    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
+    if (lane == 0) red[wv] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s += red[w];
+        vals[0] = s;
+    }
+    __syncthreads();
+    if (!reduce_blocks(vals, 1, partials, counter, scratch, fin)) return;
+    if (threadIdx.x == 0) run_epilogue(EPI_RNORM, 0, fin[0], nullptr, ctl, nullptr, nullptr);
+}
+
+__host__ __device__ inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void start_vector_kernel(int n, uint64_t base, double *r)
+{
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        const uint64_t z = mix64(base + 0x9E3779B97F4A7C15ULL * (uint64_t)(i + 1));
+        r[i] = 2.0 * (static_cast<double>(z >> 11) * 0x1.0p-53) - 1.0;
+    }
+}
+
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void scale_copy_kernel(int n, const double *r, const Ctl *ctl, VT *v,
+                                                            double *vd)
+{
+    const double s = ctl->xscale;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
+        const double x = r[i] * s;
+        v[i] = static_cast<VT>(x);
+        if (vd) vd[i] = x;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void scale_rscale_kernel(int n, double *r, double q, const Ctl *ctl)
+{
+    const double s = q * ctl->rscale;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        r[i] *= s;
+}
+
+__global__ __launch_bounds__(kBlock) void cast_kernel(long long n, const double *in, float *out)
+{
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        out[i] = static_cast<float>(in[i]);
+}
+
+int grid_for(long long n, int per_block, int cap)
+{
+    long long g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return static_cast<int>(g);
+}
+
+template <typename VT>
+int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, double *w_out,
+                 const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia,
+                 int mode, int j, hipStream_t s)
+{
+    if (m < 0 || m > 4 * 32 || m > red.max_cols) {
+        set_error("CGS pass: m=%d outside [0, %d]", m, red.max_cols < 128 ? red.max_cols : 128);
+        return HL_ERR_ARG;
+    }
+    const int nb = cgs_blocks(n);
+    if (nb > red.max_blocks) {
+        set_error("CGS pass: reduction workspace too small");
+        return HL_ERR_STATE;
+    }
+    const int cpg = (m + kGroups - 1) / kGroups;
+#define HL_CGS(MC)                                                                                      \
+    cgs_kernel<MC, VT><<<nb, kBlock, 0, s>>>(n, m, V, ldv, w_in, w_out, h_in, h_out, red.partials, \
+                                             red.counter, ctl, h_sub, h_dia, mode, j)
+    if (cpg <= 2)
+        HL_CGS(2);
+    else if (cpg <= 4)
+        HL_CGS(4);
+    else if (cpg <= 8)
+        HL_CGS(8);
+    else if (cpg <= 16)
+        HL_CGS(16);
+    else if (cpg <= 24)
+        HL_CGS(24);
+    else
+        HL_CGS(32);
+#undef HL_CGS
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+}  // namespace
+
+int cgs_blocks(int n)
+{
+    const long long ntiles = (static_cast<long long>(n) + kTileRows - 1) / kTileRows;
+    return grid_for(ntiles, 16, 512);
+}
+
+int reduce_alloc(Reduce &r, int max_blocks, int max_cols)
+{
+    r.max_blocks = max_blocks;
+    r.max_cols = max_cols;
+    HL_TRY_HIP(hipMalloc(&r.partials, sizeof(double) * (size_t)max_blocks * (max_cols + 1)));
+    HL_TRY_HIP(hipMalloc(&r.counter, sizeof(unsigned) * 4));
+    HL_TRY_HIP(hipMemset(r.counter, 0, sizeof(unsigned) * 4));
+    return HL_OK;
+}
+
+void reduce_free(Reduce &r)
+{
+    if (r.partials) (void)hipFree(r.partials);
+    if (r.counter) (void)hipFree(r.counter);
+    r = Reduce{};
+}
+
+int launch_cgs(int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
+               const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia,
+               int mode, int j, hipStream_t s)
+{
+    return launch_cgs_t<double>(n, m, V, ldv, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
+}
+
+int launch_cgs_f32(int n, int m, const float *V, long long ldv, const double *w_in, double *w_out,
+                   const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia,
+                   int mode, int j, hipStream_t s)
+{
+    return launch_cgs_t<float>(n, m, V, ldv, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
+}
+
+int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStream_t s)
+{
+    return launch_cgs_t<double>(n, 0, static_cast<const double *>(nullptr), 0, r, nullptr, nullptr, nullptr,
+                                red, ctl, nullptr, nullptr, mode, 0, s);
+}
+
+namespace {
+template <typename VT>
+int launch_vq_fma(int n, int K, int N, const VT *V, long long ldv, const double *Q, int ldq, VT *Out,
+                  long long ldo, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
+                  hipStream_t s)
+{
+    constexpr int CB = 32;
+    const int nb = grid_for(n, kBlock * 4, 1024);
+    if (res_col >= 0 && nb > red.max_blocks) {
+        set_error("V.Q: reduction workspace too small");
+        return HL_ERR_STATE;
+    }
+    for (int c0 = 0; c0 < N; c0 += CB) {
+        const int rl = (res_col >= c0 && res_col < c0 + CB) ? res_col - c0 : -1;
+        const size_t lds = sizeof(double) * (size_t)K * CB;
+        vq_fma_kernel<CB, VT><<<nb, kBlock, lds, s>>>(n, K, N, V, ldv, Q, ldq, Out, ldo, c0, rl, r, qlast,
+                                                       beta, red.partials, red.counter, ctl);
+        HL_TRY_HIP(hipGetLastError());
+    }
+    return HL_OK;
+}
+}  // namespace
+
+int launch_vq(int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq, double *Out,
+              long long ldo, int use_mfma, int res_col, double *r, double qlast, double beta, Reduce &red,
+              Ctl *ctl, hipStream_t s)
+{
+    if (K <= 0 || N <= 0 || K > 128) {
+        set_error("V.Q: K=%d N=%d unsupported (K <= 128)", K, N);
+        return HL_ERR_ARG;
+    }
+    if (!use_mfma)
+        return launch_vq_fma<double>(n, K, N, V, ldv, Q, ldq, Out, ldo, res_col, r, qlast, beta, red, ctl, s);
+    const long long ntiles = (static_cast<long long>(n) + 15) / 16;
+    const int nb = grid_for(ntiles, 4 * 4, 1024);
+    if (res_col >= 0 && nb > red.max_blocks) {
+        set_error("V.Q: reduction workspace too small");
+        return HL_ERR_STATE;
+    }
+    const int Kp = (K + 3) & ~3;
+    // column tiles per launch: Q's K x 16*nt slice must fit 48 KB of LDS
+    int max_nt = 6144 / (Kp * 16);
+    max_nt = max_nt < 1 ? 1 : (max_nt > 4 ? 4 : max_nt);
+    const int chunk = 16 * max_nt;
+    for (int c0 = 0; c0 < N; c0 += chunk) {
+        const int width = N - c0 < chunk ? N - c0 : chunk;
+        const int nt = (width + 15) / 16;
+        const int rl = (res_col >= c0 && res_col < c0 + chunk) ? res_col - c0 : -1;
+        const size_t lds = sizeof(double) * (size_t)Kp * nt * 16;
+#define HL_VQM(NTV)                                                                                      \
+    vq_mfma_kernel<NTV><<<nb, kBlock, lds, s>>>(n, K, N, V, ldv, Q, ldq, Out, ldo, c0, rl, r, qlast, beta, \
+                                                red.partials, red.counter, ctl)
+        if (nt == 1)
+            HL_VQM(1);
+        else if (nt == 2)
+            HL_VQM(2);
+        else if (nt == 3)
+            HL_VQM(3);
+        else
+            HL_VQM(4);
+#undef HL_VQM
+        HL_TRY_HIP(hipGetLastError());
+    }
+    return HL_OK;
+}
+
+int launch_vq_f32(int n, int K, int N, const float *V, long long ldv, const double *Q, int ldq, float *Out,
+                  long long ldo, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
+                  hipStream_t s)
+{
+    if (K <= 0 || N <= 0 || K > 128) {
+        set_error("V.Q: K=%d N=%d unsupported (K <= 128)", K, N);
+        return HL_ERR_ARG;
+    }
+    return launch_vq_fma<float>(n, K, N, V, ldv, Q, ldq, Out, ldo, res_col, r, qlast, beta, red, ctl, s);
+}
+
+int launch_start_vector(int n, uint64_t seed, double *r, hipStream_t s)
+{
+    const uint64_t base = mix64(seed ^ 0x6A09E667F3BCC909ULL);
+    start_vector_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, base, r);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, hipStream_t s)
+{
+    scale_copy_kernel<double><<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, r, ctl, v, nullptr);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_scale_copy_f32(int n, const double *r, const Ctl *ctl, float *v, double *vd, hipStream_t s)
+{
+    scale_copy_kernel<float><<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, r, ctl, v, vd);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_scale_rscale(int n, double *r, double q, const Ctl *ctl, hipStream_t s)
+{
+    scale_rscale_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, r, q, ctl);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_cast_f32(long long n, const double *in, float *out, hipStream_t s)
+{
+    cast_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(n, in, out);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+}  // namespace hl

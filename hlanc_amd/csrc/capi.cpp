@@ -1,0 +1,288 @@
+// capi.cpp -- the extern "C" entry points declared in include/hlanc_capi.h.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "hl_internal.hpp"
+#include "irlm.hpp"
+
+struct hl_csr_s {
+    hl::CsrPlan plan;
+    std::mutex mu; // serializes host-staged applications (shared x/y staging)
+};
+
+namespace hl {
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int hip_status(hipError_t e, const char *what)
+{
+    set_error("%s: %s", what, hipGetErrorString(e));
+    return HL_ERR_HIP_BASE + static_cast<int>(e);
+}
+
+namespace {
+
+int check_device(int dev)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        set_error("no HIP device visible");
+        return HL_ERR_NO_DEVICE;
+    }
+    if (dev < 0 || dev >= count) {
+        set_error("device %d out of range (%d visible)", dev, count);
+        return HL_ERR_ARG;
+    }
+    return HL_OK;
+}
+
+// true when p is device memory of device dev (HLanc.hpp:539,545 probe).  Plain
+// malloc memory may report an error or an unregistered type: both mean host.
+bool is_device_ptr(const void *p, int dev)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice && a.device == dev;
+}
+
+int solve_common(hl::Solver &S, int dev, int n, int nev, int ncv, double *values, double *vectors, int ldv,
+                 const hl_options *opts, hl_info *info, bool host_op)
+{
+    hl_options o;
+    hl_options_default(&o);
+    if (opts) o = *opts;
+    hl_info inf;
+    std::memset(&inf, 0, sizeof(inf));
+    if (n <= 0 || nev <= 0 || ncv <= nev || ncv > n || ncv > 128 || ldv < n || !values || !vectors) {
+        set_error("ds_solver: need 0 < nev < ncv <= min(n, 128) and ldv >= n (n=%d nev=%d ncv=%d ldv=%d)", n,
+                  nev, ncv, ldv);
+        if (info) *info = inf;
+        return HL_ERR_ARG;
+    }
+    if (o.basis_prec != HL_PREC_F64) {
+        set_error("ds_solver: this entry point takes the FP64 basis (use hl_ds_solve_mixed)");
+        return HL_ERR_ARG;
+    }
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    if ((rc = S.ws.init(dev, n, ncv, o.basis_prec, host_op))) {
+        S.ws.release();
+        if (info) *info = inf;
+        return rc;
+    }
+    rc = S.run(nev, values, vectors, ldv, o, inf);
+    inf.ret = rc;
+    S.ws.release();
+    if (info) *info = inf;
+    return rc;
+}
+
+} // namespace
+} // namespace hl
+
+using namespace hl;
+
+extern "C" {
+
+const char *hl_last_error(void) { return g_err; }
+
+const char *hl_version(void) { return "1.0"; }
+
+int hl_device_count(int *count)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    if (count) *count = c;
+    return HL_OK;
+}
+
+void hl_options_default(hl_options *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->tol = 0.0;
+    o->maxiter = 200;
+    o->v0 = nullptr;
+    o->seed = 0;
+    o->basis_prec = HL_PREC_F64;
+    o->use_mfma = 1;
+}
+
+int hl_csr_create(int dev, int nrows, int ncols, int nnz, const int *row_ptr, const int *col,
+                  const double *val, int sym_lower, int val_prec, hl_csr_t *out)
+{
+    if (!out) return HL_ERR_ARG;
+    *out = nullptr;
+    if (nrows < 0 || ncols < 0 || (nrows > 0 && (!row_ptr || row_ptr[0] != 0 || row_ptr[nrows] != nnz)) ||
+        (nnz > 0 && (!col || !val))) {
+        set_error("hl_csr_create: inconsistent CSR (row_ptr[0] must be 0 and row_ptr[nrows] == nnz)");
+        return HL_ERR_ARG;
+    }
+    for (int r = 0; r < nrows; ++r)
+        if (row_ptr[r + 1] < row_ptr[r]) {
+            set_error("hl_csr_create: row_ptr decreases at row %d", r);
+            return HL_ERR_ARG;
+        }
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    hl_csr_s *h = new (std::nothrow) hl_csr_s;
+    if (!h) return HL_ERR_OOM;
+    h->plan.dev = dev;
+    rc = csr_plan_build(h->plan, nrows, ncols, row_ptr, col, val, sym_lower, val_prec);
+    if (rc) {
+        csr_plan_free(h->plan);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return HL_OK;
+}
+
+int hl_csr_destroy(hl_csr_t h)
+{
+    if (!h) return HL_OK;
+    DeviceGuard g(h->plan.dev);
+    (void)hipDeviceSynchronize();
+    csr_plan_free(h->plan);
+    delete h;
+    return HL_OK;
+}
+
+int hl_csr_info(hl_csr_t h, int *dev, int *nrows, int *ncols, long long *nnz_general)
+{
+    if (!h) return HL_ERR_ARG;
+    if (dev) *dev = h->plan.dev;
+    if (nrows) *nrows = h->plan.nrows;
+    if (ncols) *ncols = h->plan.ncols;
+    if (nnz_general) *nnz_general = h->plan.nnz;
+    return HL_OK;
+}
+
+int hl_csr_apply(hl_csr_t h, const double *x, double *y)
+{
+    if (!h || (h->plan.ncols > 0 && !x) || (h->plan.nrows > 0 && !y)) return HL_ERR_ARG;
+    std::lock_guard<std::mutex> lk(h->mu);
+    CsrPlan &p = h->plan;
+    DeviceGuard g(p.dev);
+    const bool xd = is_device_ptr(x, p.dev), yd = is_device_ptr(y, p.dev);
+    const double *xx = x;
+    if (!xd) {
+        HL_TRY_HIP(hipMemcpy(p.x_stage, x, sizeof(double) * p.ncols, hipMemcpyDefault));
+        xx = p.x_stage;
+    }
+    double *yy = yd ? y : p.y_stage;
+    int rc = launch_spmv(p, xx, nullptr, yy, nullptr, HL_PREC_F64, nullptr);
+    if (rc) return rc;
+    if (!yd) HL_TRY_HIP(hipMemcpy(y, yy, sizeof(double) * p.nrows, hipMemcpyDefault));
+    return HL_OK;
+}
+
+int hl_csr_spmv(hl_csr_t h, const double *x_dev, double *y_dev, void *stream)
+{
+    if (!h) return HL_ERR_ARG;
+    DeviceGuard g(h->plan.dev);
+    return launch_spmv(h->plan, x_dev, nullptr, y_dev, nullptr, HL_PREC_F64, static_cast<hipStream_t>(stream));
+}
+
+int hl_csr_spmv_f32val(hl_csr_t h, const double *x_dev, double *y_dev, void *stream)
+{
+    if (!h) return HL_ERR_ARG;
+    DeviceGuard g(h->plan.dev);
+    return launch_spmv(h->plan, x_dev, nullptr, y_dev, nullptr, HL_PREC_F32, static_cast<hipStream_t>(stream));
+}
+
+double hl_csr_spmv_bytes(hl_csr_t h, int val_prec)
+{
+    if (!h) return 0.0;
+    const double sa = val_prec == HL_PREC_F32 ? 4.0 : 8.0;
+    return (sa + 4.0) * static_cast<double>(h->plan.nnz) + 4.0 * (h->plan.nrows + 1.0) + 8.0 * h->plan.ncols +
+           8.0 * h->plan.nrows;
+}
+
+int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
+                const double *h_in, double *h_out, double *nrm2_out, void *stream)
+{
+    if (n < 0 || m < 0 || m > 128 || (m > 0 && (!V || ldv < n)) || !w_in) {
+        set_error("hl_cgs_pass: bad arguments");
+        return HL_ERR_ARG;
+    }
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Reduce red;
+    Ctl *ctl = nullptr;
+    double *hout = nullptr;
+    if ((rc = reduce_alloc(red, cgs_blocks(n), m > 0 ? m : 1))) return rc;
+    HL_TRY_HIP(hipMalloc(&ctl, sizeof(Ctl)));
+    HL_TRY_HIP(hipMalloc(&hout, sizeof(double) * (m > 0 ? m : 1)));
+    rc = launch_cgs(n, m, V, ldv, w_in, w_out, h_in, hout, red, ctl, nullptr, nullptr, EPI_NONE, 0, s);
+    if (!rc && h_out && m > 0)
+        rc = hipMemcpyAsync(h_out, hout, sizeof(double) * m, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    if (!rc && nrm2_out)
+        rc = hipMemcpyAsync(nrm2_out, &ctl->nrm2, sizeof(double), hipMemcpyDeviceToDevice, s) == hipSuccess
+                 ? 0
+                 : HL_ERR_HIP_BASE;
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(hout);
+    (void)hipFree(ctl);
+    reduce_free(red);
+    return rc;
+}
+
+int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq, double *Out,
+          long long ldo, int use_mfma, void *stream)
+{
+    if (n < 0 || K <= 0 || N <= 0 || K > 128 || !V || !Q || !Out || ldv < n || ldo < n || ldq < K) {
+        set_error("hl_vq: bad arguments");
+        return HL_ERR_ARG;
+    }
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    Reduce red;
+    return launch_vq(n, K, N, V, ldv, Q, ldq, Out, ldo, use_mfma, -1, nullptr, 0.0, 0.0, red, nullptr,
+                     static_cast<hipStream_t>(stream));
+}
+
+int hl_ds_solve_csr(hl_csr_t A, int nev, int ncv, double *values, double *vectors, int ldv,
+                    const hl_options *opts, hl_info *info)
+{
+    if (!A) return HL_ERR_ARG;
+    if (A->plan.nrows != A->plan.ncols) {
+        set_error("ds_solver: the operator must be square");
+        return HL_ERR_ARG;
+    }
+    hl::Solver S;
+    S.A = &A->plan;
+    return solve_common(S, A->plan.dev, A->plan.nrows, nev, ncv, values, vectors, ldv, opts, info, false);
+}
+
+int hl_ds_solve_host_op(int dev, int n, hl_host_op_fn op, void *ctx, int nev, int ncv, double *values,
+                        double *vectors, int ldv, const hl_options *opts, hl_info *info)
+{
+    if (!op) return HL_ERR_ARG;
+    hl::Solver S;
+    S.host_op = op;
+    S.host_ctx = ctx;
+    return solve_common(S, dev, n, nev, ncv, values, vectors, ldv, opts, info, true);
+}
+
+} // extern "C"

@@ -1,0 +1,166 @@
+// hl_internal.hpp -- shared definitions of libhlanc_hip.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "hlanc_capi.h"
+
+namespace hl {
+
+constexpr int kWave = 64;        // CDNA wavefront
+constexpr int kBlock = 256;      // threads per block in every kernel here
+constexpr double kDgks = 0.717;  // DGKS threshold (HLanc.hpp:1444,1451)
+constexpr double kEps = 1.1102230246251565e-16;    // dlamch("E") = 2^-53
+constexpr double kSafmin = 2.2250738585072014e-308; // dlamch("S")
+
+// ----------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+int hip_status(hipError_t e, const char *what);
+
+#define HL_TRY_HIP(call)                                           \
+    do {                                                           \
+        hipError_t hl_e_ = (call);                                 \
+        if (hl_e_ != hipSuccess) return ::hl::hip_status(hl_e_, #call); \
+    } while (0)
+
+// Restores the caller's current device on scope exit.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        ok = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// --------------------------------------------------- device-side solve state
+// Scalars of one Lanczos factorization, kept in device memory so that a whole
+// dsaitr sweep runs without host round trips.  Written only by the last block
+// of a reducing kernel ("epilogue") or by the host between sweeps.
+struct Ctl {
+    double rnorm;    // ||resid||, becomes beta_{j+1}
+    double wnorm;    // ||A v_j|| (DGKS reference norm)
+    double xscale;   // factor turning resid into v_{j+1}: 1/rnorm, 1, or 0
+    double rscale;   // 1, or 0 once DGKS declared an invariant subspace
+    int need;        // another DGKS correction is pending
+    int iter;        // DGKS failure count of the current step
+    long long cgs;   // CGS passes executed (statistics)
+    double nrm2;     // last reduced squared norm (standalone passes)
+    double pad[2];
+};
+
+// Epilogue modes of the reducing kernels.
+enum Epi : int {
+    EPI_NONE = 0,    // standalone pass: write h_out and ctl->nrm2 only
+    EPI_WNORM = 1,   // first CGS half (h = V^T w, ||w||) right after the SpMV
+    EPI_PASS1 = 2,   // w -= V h: alpha_j, beta_j, rnorm, DGKS decision
+    EPI_PASS2 = 3,   // first DGKS correction
+    EPI_PASS3 = 4,   // second DGKS correction (may zero the residual)
+    EPI_RNORM = 5,   // rnorm = ||resid|| after start vector or restart
+};
+
+// ------------------------------------------------------------ SpMV plan
+// One work item of the adaptive CSR SpMV.  tpr > 0: a "stream" block of whole
+// short rows [row0, row1) covering nnz [p0, p1), reduced with tpr lanes per
+// row.  tpr == 0: one chunk of a long row row0, nnz [p0, p1); nchunks > 1
+// means partial sums meet in slot [slot, slot + nchunks).
+struct SpmvBlk {
+    int row0, row1, p0, p1;
+    int tpr, chunk, nchunks, slot;
+};
+
+constexpr int kNnzPerBlock = 2048;  // stream block capacity (8 nnz per thread)
+constexpr int kRowsPerBlock = 256;
+constexpr int kLongChunk = 8192;    // nnz per chunk of a long row
+
+struct CsrPlan {
+    int dev = 0;
+    int nrows = 0, ncols = 0;
+    long long nnz = 0;        // general (expanded) nnz
+    int *row_ptr = nullptr;   // device
+    int *col = nullptr;       // device
+    double *val = nullptr;    // device FP64 values
+    float *val32 = nullptr;   // device FP32 values (optional)
+    SpmvBlk *blks = nullptr;  // device
+    int nblk = 0;
+    double *chunk_part = nullptr;
+    unsigned *chunk_cnt = nullptr;
+    int nslots = 0;
+    double *x_stage = nullptr, *y_stage = nullptr;  // for host-pointer calls
+};
+
+int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *col,
+                   const double *val, int sym_lower, int val_prec);
+void csr_plan_free(CsrPlan &p);
+
+// y = A (x * xscale) with optional fused write vout[r] = x[r] * xscale.
+// xscale == nullptr means 1.  val_prec selects the FP64 or FP32 value array.
+int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double *y, double *vout,
+                int val_prec, hipStream_t s);
+int launch_spmv_f32x(const CsrPlan &p, const float *x, const double *xscale, double *y, float *vout,
+                     hipStream_t s);
+
+// ------------------------------------------------------- basis kernels
+struct Reduce {
+    double *partials = nullptr;  // [nblocks_max * (ncv_max + 1)]
+    unsigned *counter = nullptr; // one counter, self-resetting
+    int max_blocks = 0;
+    int max_cols = 0;
+};
+int reduce_alloc(Reduce &r, int max_blocks, int max_cols);
+void reduce_free(Reduce &r);
+int cgs_blocks(int n);
+
+// One fused CGS pass over an FP64 basis (see hl_cgs_pass).  mode selects the
+// epilogue; j is the Lanczos step for the alpha/beta bookkeeping.
+int launch_cgs(int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
+               const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub,
+               double *h_dia, int mode, int j, hipStream_t s);
+// Same over an FP32 basis (FP64 accumulation, FP64 w).
+int launch_cgs_f32(int n, int m, const float *V, long long ldv, const double *w_in, double *w_out,
+                   const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub,
+                   double *h_dia, int mode, int j, hipStream_t s);
+
+// Out(:, 0:N) = V(:, 0:K) Q(0:K, 0:N); optionally fused residual update on
+// output column res_col: r = qlast * rscale * r + beta * Out(:, res_col) and
+// ctl->rnorm := ||r|| (EPI_RNORM).  res_col < 0 disables it.
+int launch_vq(int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq,
+              double *Out, long long ldo, int use_mfma, int res_col, double *r, double qlast,
+              double beta, Reduce &red, Ctl *ctl, hipStream_t s);
+int launch_vq_f32(int n, int K, int N, const float *V, long long ldv, const double *Q, int ldq,
+                  float *Out, long long ldo, int res_col, double *r, double qlast, double beta,
+                  Reduce &red, Ctl *ctl, hipStream_t s);
+
+// r = start vector (counter-based splitmix64, seed) or keep r; then ctl->rnorm.
+int launch_start_vector(int n, uint64_t seed, double *r, hipStream_t s);
+// v = r * ctl->xscale (host-operator path normalisation).
+int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, hipStream_t s);
+int launch_scale_copy_f32(int n, const double *r, const Ctl *ctl, float *v, double *vd,
+                          hipStream_t s);
+// r *= q * ctl->rscale (dsapps residual when beta_k = 0, HLanc.hpp:1605-1607).
+int launch_scale_rscale(int n, double *r, double q, const Ctl *ctl, hipStream_t s);
+// ||r||^2 -> epilogue (mode EPI_RNORM sets rnorm/xscale/rscale).
+int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStream_t s);
+// out = float(in)
+int launch_cast_f32(long long n, const double *in, float *out, hipStream_t s);
+
+// ------------------------------------------------------------ host dense
+// dlartg with MKL's sign convention (called at HLanc.hpp:1514,1536).
+void dlartg(double f, double g, double &c, double &s, double &r);
+// Symmetric tridiagonal eigen-decomposition by implicit-shift QL with
+// Wilkinson shifts; ascending eigenvalues, eigenvectors in z (ldz >= n).
+// Plays dsteqr("I") (HLanc.hpp:1691,1805).  Returns 0 or 1 (no convergence /
+// non-finite input).
+int tridiag_eig(int n, double *d, double *e, double *z, int ldz);
+// Householder QR of a (m x k, lda), LAPACK dgeqr2 storage (HLanc.hpp:1837),
+// then Q = H(0)...H(k-1) formed explicitly as m x m (ldq).
+void householder_q(int m, int k, double *a, int lda, double *tau, double *q, int ldq);
+
+}  // namespace hl

@@ -1,0 +1,273 @@
+// spmv.hip -- adaptive CSR SpMV for CDNA4 (gfx950).
+//
+// Replaces cusparseDcsrmv inside dcsrmv_operator_gpu::operator() (HLanc.hpp:553-567).
+// The matrix is cut on the host into work blocks by an nnz walk over row_ptr:
+//   * stream blocks: up to 256 whole short rows holding <= 2048 nnz.  The block
+//     streams val/col coalesced, multiplies by the gathered x into LDS, then
+//     reduces each row from LDS with tpr lanes per row (tpr = 1 is "a lane per
+//     short row", tpr = 64 "a wavefront per medium row");
+//   * chunk blocks: a row longer than 2048 nnz gets blocks of <= 8192 nnz each;
+//     the partial sums of a row are added in chunk order by whichever block
+//     finishes last (a deterministic split of heavy rows).
+// Every row sum is formed in a fixed order, so results are bitwise repeatable.
+#include <algorithm>
+#include <vector>
+
+#include "hl_internal.hpp"
+
+namespace hl {
+namespace {
+
+__device__ inline double block_sum_to_thread0(double v, double *wred)
+{
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) wred[threadIdx.x / kWave] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kBlock / kWave; ++w) t += wred[w];
+    }
+    return t;
+}
+
+template <typename AT, typename XT>
+__global__ __launch_bounds__(kBlock) void spmv_adaptive(
+    const SpmvBlk *__restrict__ blks, const int *__restrict__ rp, const int *__restrict__ col,
+    const AT *__restrict__ val, const XT *__restrict__ x, const double *__restrict__ xscale,
+    double *__restrict__ y, XT *__restrict__ vout, double *__restrict__ chunk_part,
+    unsigned *__restrict__ chunk_cnt)
+{
+    __shared__ double prod[kNnzPerBlock];
+    __shared__ double wred[kBlock / kWave];
+    const SpmvBlk b = blks[blockIdx.x];
+    const double s = xscale ? *xscale : 1.0;
+    const int tid = threadIdx.x;
+
+    if (b.tpr > 0) {
+        const int nnz = b.p1 - b.p0;
+        for (int i = tid; i < nnz; i += kBlock) {
+            const int p = b.p0 + i;
+            prod[i] = static_cast<double>(val[p]) * (static_cast<double>(x[col[p]]) * s);
+        }
+        if (vout) {
+            for (int r = b.row0 + tid; r < b.row1; r += kBlock)
+                vout[r] = static_cast<XT>(static_cast<double>(x[r]) * s);
+        }
+        __syncthreads();
+        const int tpr = b.tpr;
+        const int nrows = b.row1 - b.row0;
+        const int lr = tid / tpr, ll = tid & (tpr - 1);
+        const int rstep = kBlock / tpr;
+        for (int base = 0; base < nrows; base += rstep) {
+            const int rr = base + lr;
+            double acc = 0.0;
+            if (rr < nrows) {
+                const int r = b.row0 + rr;
+                const int a0 = rp[r] - b.p0, a1 = rp[r + 1] - b.p0;
+                for (int k = a0 + ll; k < a1; k += tpr) acc += prod[k];
+            }
+            for (int off = tpr >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+            if (rr < nrows && ll == 0) y[b.row0 + rr] = acc;
+        }
+        return;
+    }
+
+    // chunk of a long row
+    double acc = 0.0;
+    for (int p = b.p0 + tid; p < b.p1; p += kBlock)
+        acc += static_cast<double>(val[p]) * (static_cast<double>(x[col[p]]) * s);
+    const double tot = block_sum_to_thread0(acc, wred);
+    if (tid != 0) return;
+    if (vout && b.chunk == 0) vout[b.row0] = static_cast<XT>(static_cast<double>(x[b.row0]) * s);
+    if (b.nchunks == 1) {
+        y[b.row0] = tot;
+        return;
+    }
+    __hip_atomic_store(&chunk_part[b.slot + b.chunk], tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned prev = __hip_atomic_fetch_add(&chunk_cnt[b.slot], 1u, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    if (prev + 1 == static_cast<unsigned>(b.nchunks)) {
+        double t = 0.0;
+        for (int c = 0; c < b.nchunks; ++c)
+            t += __hip_atomic_load(&chunk_part[b.slot + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        y[b.row0] = t;
+        __hip_atomic_store(&chunk_cnt[b.slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <typename T>
+int dev_upload(T **dst, const T *src, size_t count)
+{
+    *dst = nullptr;
+    if (count == 0) count = 1;
+    HL_TRY_HIP(hipMalloc(reinterpret_cast<void **>(dst), count * sizeof(T)));
+    if (src) HL_TRY_HIP(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return HL_OK;
+}
+
+}  // namespace
+
+int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *col,
+                   const double *val, int sym_lower, int val_prec)
+{
+    if (nrows < 0 || ncols < 0 || (nrows > 0 && !rp)) {
+        set_error("hl_csr_create: bad shape");
+        return HL_ERR_ARG;
+    }
+    if (sym_lower && nrows != ncols) {
+        set_error("hl_csr_create: symmetric storage needs a square matrix");
+        return HL_ERR_ARG;
+    }
+    const long long nnz_in = nrows > 0 ? rp[nrows] : 0;
+    for (long long i = 0; i < nnz_in; ++i) {
+        if (col[i] < 0 || col[i] >= ncols) {
+            set_error("hl_csr_create: column index %d out of range at %lld", col[i], i);
+            return HL_ERR_ARG;
+        }
+    }
+    // Symmetric-lower storage expands to general storage with the per-row entry
+    // order that dcsr_matrix_t::save_bin produces (HLanc.hpp:339-364).
+    std::vector<int> grp, gcol;
+    std::vector<double> gval;
+    const int *R = rp, *C = col;
+    const double *A = val;
+    if (sym_lower) {
+        std::vector<long long> cnt(static_cast<size_t>(nrows) + 1, 0);
+        for (int r = 0; r < nrows; ++r)
+            for (int q = rp[r]; q < rp[r + 1]; ++q) {
+                cnt[r + 1]++;
+                if (col[q] != r) cnt[col[q] + 1]++;
+            }
+        for (int r = 0; r < nrows; ++r) cnt[r + 1] += cnt[r];
+        if (cnt[nrows] > 0x7fffffffLL) {
+            set_error("hl_csr_create: expanded nnz exceeds int32");
+            return HL_ERR_ARG;
+        }
+        grp.resize(static_cast<size_t>(nrows) + 1);
+        for (int r = 0; r <= nrows; ++r) grp[r] = static_cast<int>(cnt[r]);
+        gcol.resize(static_cast<size_t>(cnt[nrows]));
+        gval.resize(static_cast<size_t>(cnt[nrows]));
+        std::vector<long long> cur(cnt.begin(), cnt.end() - 1);
+        for (int r = 0; r < nrows; ++r)
+            for (int q = rp[r]; q < rp[r + 1]; ++q) {
+                const int c = col[q];
+                gcol[cur[r]] = c;
+                gval[cur[r]++] = val[q];
+                if (c != r) {
+                    gcol[cur[c]] = r;
+                    gval[cur[c]++] = val[q];
+                }
+            }
+        R = grp.data();
+        C = gcol.data();
+        A = gval.data();
+    }
+    const long long nnz = nrows > 0 ? R[nrows] : 0;
+
+    std::vector<SpmvBlk> blks;
+    int slots = 0;
+    for (int r = 0; r < nrows;) {
+        const int len = R[r + 1] - R[r];
+        if (len > kNnzPerBlock) {
+            const int nch = (len + kLongChunk - 1) / kLongChunk;
+            for (int c = 0; c < nch; ++c) {
+                SpmvBlk b{};
+                b.row0 = r;
+                b.row1 = r + 1;
+                b.p0 = R[r] + c * kLongChunk;
+                b.p1 = std::min(R[r + 1], b.p0 + kLongChunk);
+                b.tpr = 0;
+                b.chunk = c;
+                b.nchunks = nch;
+                b.slot = nch > 1 ? slots : 0;
+                blks.push_back(b);
+            }
+            if (nch > 1) slots += nch;
+            ++r;
+            continue;
+        }
+        const int r0 = r;
+        int nz = 0;
+        while (r < nrows && r - r0 < kRowsPerBlock) {
+            const int l = R[r + 1] - R[r];
+            if (l > kNnzPerBlock || nz + l > kNnzPerBlock) break;
+            nz += l;
+            ++r;
+        }
+        const int nr = r - r0;
+        int tpr = 1;
+        while (tpr < kWave && tpr * 2 * nr <= kBlock) tpr *= 2;
+        SpmvBlk b{};
+        b.row0 = r0;
+        b.row1 = r;
+        b.p0 = R[r0];
+        b.p1 = R[r];
+        b.tpr = tpr;
+        blks.push_back(b);
+    }
+
+    p.nrows = nrows;
+    p.ncols = ncols;
+    p.nnz = nnz;
+    p.nblk = static_cast<int>(blks.size());
+    p.nslots = slots;
+    int rc;
+    if ((rc = dev_upload(&p.row_ptr, R, static_cast<size_t>(nrows) + 1))) return rc;
+    if ((rc = dev_upload(&p.col, C, static_cast<size_t>(nnz)))) return rc;
+    if ((rc = dev_upload(&p.val, A, static_cast<size_t>(nnz)))) return rc;
+    if (val_prec == HL_PREC_F32) {
+        std::vector<float> v32(static_cast<size_t>(nnz));
+        for (long long i = 0; i < nnz; ++i) v32[i] = static_cast<float>(A[i]);
+        if ((rc = dev_upload(&p.val32, v32.data(), static_cast<size_t>(nnz)))) return rc;
+    }
+    if ((rc = dev_upload(&p.blks, blks.data(), blks.size()))) return rc;
+    if ((rc = dev_upload<double>(&p.chunk_part, nullptr, static_cast<size_t>(slots)))) return rc;
+    if ((rc = dev_upload<unsigned>(&p.chunk_cnt, nullptr, static_cast<size_t>(slots)))) return rc;
+    HL_TRY_HIP(hipMemset(p.chunk_cnt, 0, sizeof(unsigned) * std::max(slots, 1)));
+    if ((rc = dev_upload<double>(&p.x_stage, nullptr, static_cast<size_t>(ncols)))) return rc;
+    if ((rc = dev_upload<double>(&p.y_stage, nullptr, static_cast<size_t>(nrows)))) return rc;
+    return HL_OK;
+}
+
+void csr_plan_free(CsrPlan &p)
+{
+    void *ptrs[] = {p.row_ptr, p.col, p.val, p.val32, p.blks, p.chunk_part, p.chunk_cnt, p.x_stage, p.y_stage};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    p = CsrPlan{};
+}
+
+int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double *y, double *vout,
+                int val_prec, hipStream_t s)
+{
+    if (p.nblk == 0) return HL_OK;
+    if (val_prec == HL_PREC_F32) {
+        if (!p.val32) {
+            set_error("SpMV: FP32 values were not requested at creation");
+            return HL_ERR_STATE;
+        }
+        spmv_adaptive<float, double><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val32, x, xscale,
+                                                                y, vout, p.chunk_part, p.chunk_cnt);
+    } else {
+        spmv_adaptive<double, double><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val, x, xscale,
+                                                                 y, vout, p.chunk_part, p.chunk_cnt);
+    }
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_spmv_f32x(const CsrPlan &p, const float *x, const double *xscale, double *y, float *vout,
+                     hipStream_t s)
+{
+    if (p.nblk == 0) return HL_OK;
+    if (!p.val32) {
+        set_error("SpMV: FP32 values were not requested at creation");
+        return HL_ERR_STATE;
+    }
+    spmv_adaptive<float, float><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val32, x, xscale, y,
+                                                          vout, p.chunk_part, p.chunk_cnt);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+}  // namespace hl

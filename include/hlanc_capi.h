@@ -1,0 +1,171 @@
+/*
+ * hlanc_capi.h -- C ABI of libhlanc_hip.so, the MI355X (gfx950) implementation of
+ * HLanc's implicitly restarted Lanczos hot path.
+ *
+ * Plain C: pointers, sizes and int status codes; no C++ or torch types.  The
+ * drop-in C++ header include/HLanc.hpp is written on top of these entry points;
+ * the Python package hlanc_amd binds them with ctypes.  Each entry point names
+ * the reference interface it replaces (HLanc.hpp file:line).
+ *
+ * Status codes: HL_OK (0); HL_NOT_CONVERGED (1) is the reference's numerical
+ * "return 1" (HLanc.hpp:1692,1762,1791,1806,1838,1841); negative values are
+ * argument or runtime errors; HL_ERR_HIP_BASE + hipError_t for HIP failures.
+ * No entry point calls exit().
+ *
+ * Threading: every entry point is reentrant for distinct handles.  Calls on one
+ * operator handle from several host threads are serialized internally.  Device
+ * work runs on the stream passed in (0 = the default stream) or, for the solver
+ * entry points, on a stream owned by the solve.  The caller's current device is
+ * restored on return.
+ */
+#ifndef HLANC_CAPI_H
+#define HLANC_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HL_OK 0
+#define HL_NOT_CONVERGED 1
+#define HL_ERR_ARG (-1)
+#define HL_ERR_NO_DEVICE (-2)
+#define HL_ERR_OOM (-3)
+#define HL_ERR_IO (-4)
+#define HL_ERR_FORMAT (-5)
+#define HL_ERR_STATE (-6)
+#define HL_ERR_HIP_BASE 1000
+
+/* Storage precision of matrix values or of the Krylov basis. */
+#define HL_PREC_F64 0
+#define HL_PREC_F32 1
+
+typedef struct hl_csr_s *hl_csr_t;
+
+/* Text of the last error seen on this host thread (never NULL). */
+const char *hl_last_error(void);
+/* Library version "major.minor"; mirrors HLANC_VERSION_MAJOR/MINOR (HLanc.hpp:15-16). */
+const char *hl_version(void);
+/* Number of visible HIP devices (0 when none). */
+int hl_device_count(int *count);
+
+/* ---------------------------------------------------------------- matrices
+ * Host CSR in dcsr_matrix_t's layout (HLanc.hpp:74-100): zero-based int32
+ * row_ptr[nrows+1], col[nnz], double val[nnz]; sym = lower triangle stored.
+ * Memory returned by hl_matrix_* is owned by the library and released with
+ * hl_matrix_free. */
+typedef struct {
+    int nrows;
+    int ncols;
+    int nnz;
+    int sym;
+    int *row_ptr;
+    int *col;
+    double *val;
+} hl_matrix;
+
+/* dcsr_matrix_t(format, filename, is_symmetric) readers (HLanc.hpp:103-285):
+ * format "pp", "mm" or "bin".  Returns HL_ERR_IO / HL_ERR_FORMAT with a message
+ * where the reference prints and exits (HLanc.hpp:52-59). */
+int hl_matrix_read(const char *format, const char *filename, int is_symmetric, hl_matrix *out);
+/* save_pp / save_mm / save_bin (HLanc.hpp:287-392); "bin" expands symmetric
+ * storage to general as the reference does (HLanc.hpp:339-364). */
+int hl_matrix_write(const char *format, const char *filename, const hl_matrix *m);
+void hl_matrix_free(hl_matrix *m);
+
+/* ---------------------------------------------------------------- operator
+ * dcsrmv_operator_gpu(m, dev) (HLanc.hpp:493-514): uploads the CSR to device
+ * `dev`; symmetric-lower input is expanded to general storage on upload, which
+ * gives the (L + L^T - D) x semantics of HLanc.hpp:408-419 and 450-451.
+ * val_prec = HL_PREC_F32 additionally keeps an FP32 copy of the values for the
+ * mixed-precision solver.  Host arrays are only read during the call. */
+int hl_csr_create(int dev, int nrows, int ncols, int nnz, const int *row_ptr, const int *col,
+                  const double *val, int sym_lower, int val_prec, hl_csr_t *out);
+/* ~dcsrmv_operator_gpu (HLanc.hpp:516-528). */
+int hl_csr_destroy(hl_csr_t h);
+/* Shape queries. */
+int hl_csr_info(hl_csr_t h, int *dev, int *nrows, int *ncols, long long *nnz_general);
+
+/* dcsrmv_operator_gpu::operator()(x, y) (HLanc.hpp:530-572): y = A x.  x and y
+ * may each be host memory or device memory of the operator's device; host y
+ * means the result is complete on return, device y means the call is ordered
+ * on the default stream.  FP64 values. */
+int hl_csr_apply(hl_csr_t h, const double *x, double *y);
+/* Device-pointer form on an explicit stream (no pointer probing, no sync). */
+int hl_csr_spmv(hl_csr_t h, const double *x_dev, double *y_dev, void *stream);
+/* The FP32-value kernel (requires val_prec = HL_PREC_F32 at creation). */
+int hl_csr_spmv_f32val(hl_csr_t h, const double *x_dev, double *y_dev, void *stream);
+/* Algorithmic bytes of one SpMV: (s_A + 4) nnz + 4 (nrows + 1) + 8 ncols + 8 nrows. */
+double hl_csr_spmv_bytes(hl_csr_t h, int val_prec);
+
+/* ------------------------------------------------------ basis operations
+ * Device-pointer forms of mat_v_gpu_t's dense work, exposed for testing and for
+ * callers that keep their own basis.  V is column-major with leading dimension
+ * ldv (elements), FP64. */
+/* One classical Gram-Schmidt pass, mat_v_gpu_t::op (HLanc.hpp:1000-1024), fused:
+ *   w_out = w_in - V(:,0:m) h_in      (skipped when h_in == NULL: w_out = w_in)
+ *   h_out = V(:,0:m)^T w_out,  *nrm2_out = ||w_out||^2 (both device pointers).
+ * w_out may equal w_in or be NULL (not written). */
+int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const double *w_in,
+                double *w_out, const double *h_in, double *h_out, double *nrm2_out, void *stream);
+/* Out(:, 0:N) = V(:, 0:K) Q(0:K, 0:N): the V.Q product of dsapps and of the
+ * Ritz-vector step (HLanc.hpp:1029-1058, 1075-1107).  Out must not alias V.
+ * use_mfma = 1 selects the v_mfma_f64_16x16x4 kernel, 0 the FMA kernel. */
+int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq,
+          double *Out, long long ldo, int use_mfma, void *stream);
+
+/* ------------------------------------------------------------------ solver */
+
+/* y = A x on host memory, for arbitrary operators (HLanc.hpp:1424: the solver
+ * hands the operator host pointers of length n). */
+typedef void (*hl_host_op_fn)(void *ctx, const double *x, double *y);
+
+typedef struct {
+    double tol;        /* convergence tolerance; <= 0 -> machine epsilon (HLanc.hpp:1651) */
+    int maxiter;       /* restart cap; <= 0 -> 200 (HLanc.hpp:1682) */
+    const double *v0;  /* host start vector of length n, or NULL: seeded generator */
+    uint64_t seed;     /* generator seed when v0 == NULL */
+    int basis_prec;    /* HL_PREC_F64 (reference semantics) or HL_PREC_F32 (mixed) */
+    int use_mfma;      /* 1: V.Q on MFMA (default when the struct is zeroed: see below) */
+    int flags;         /* bit 0: do not use hipGraph replay for the Lanczos steps */
+    int reserved[5];
+} hl_options;
+
+typedef struct {
+    int ret;                /* status as returned */
+    int restarts;           /* restart iterations run */
+    int nconv;              /* converged Ritz values */
+    int pad;
+    long long spmv;         /* operator applications */
+    long long cgs_passes;   /* CGS passes incl. DGKS corrections */
+    double rnorm;           /* final residual norm */
+    double seconds;         /* wall time of the call */
+    double max_rel_resid;   /* max ||A v - l v|| / ||A||_est over returned pairs (csr path) */
+    double refine_shift;    /* mixed precision: largest Rayleigh-quotient change */
+} hl_info;
+
+/* Fills *o with the reference defaults (tol = eps, maxiter = 200, generator
+ * seed 0, FP64 basis, MFMA V.Q). */
+void hl_options_default(hl_options *o);
+
+/* ds_solver_gpu<dcsrmv_operator_gpu> (HLanc.hpp:1868-1872 driving 1649-1850):
+ * computes the nev largest-algebraic eigenpairs.  values (host, >= nev) gets
+ * them ascending; vectors (host, column-major ldv x ncv, ldv >= n) gets all ncv
+ * basis columns, the first nev being the unit Ritz vectors.  Both are written
+ * only on success.  The operator's SpMV runs on device pointers: no PCIe
+ * traffic per Lanczos step.  opts/info may be NULL. */
+int hl_ds_solve_csr(hl_csr_t A, int nev, int ncv, double *values, double *vectors, int ldv,
+                    const hl_options *opts, hl_info *info);
+
+/* ds_solver_gpu<any callable> (HLanc.hpp:1868-1872): the operator is a host
+ * callback called with host pointers, as the reference does (HLanc.hpp:1424). */
+int hl_ds_solve_host_op(int dev, int n, hl_host_op_fn op, void *ctx, int nev, int ncv,
+                        double *values, double *vectors, int ldv, const hl_options *opts,
+                        hl_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,68 @@
+"""GPU parity of the basis kernels (mat_v_gpu_t's dense work, HLanc.hpp:1000-1107)
+against float64 numpy: the fused CGS pass and the V.Q product (MFMA and FMA)."""
+import numpy as np
+import pytest
+
+import hlanc_amd as H
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+
+    return torch.tensor(np.asfortranarray(a).ravel(order="F"), device="cuda")
+
+
+@pytest.mark.parametrize("n", [1000, 65536 + 13])
+@pytest.mark.parametrize("m", [0, 1, 5, 30, 64, 96, 128])
+@pytest.mark.parametrize("with_h", [False, True])
+def test_cgs_pass_matches_numpy(n, m, with_h):
+    import torch
+
+    rng = np.random.default_rng(n + m)
+    ldv = n + 7
+    V = np.zeros((ldv, max(m, 1)))
+    V[:n, :m] = rng.uniform(-1, 1, (n, m))
+    w = rng.uniform(-1, 1, n)
+    h = rng.uniform(-1, 1, max(m, 1))
+    Vd, wd, hd = dev(V), torch.tensor(w, device="cuda"), torch.tensor(h, device="cuda")
+    wout = torch.empty_like(wd)
+    hout = torch.zeros(max(m, 1), dtype=torch.float64, device="cuda")
+    nrm = torch.zeros(1, dtype=torch.float64, device="cuda")
+    H.cgs_pass(0, n, m, Vd.data_ptr(), ldv, wd.data_ptr(), wout.data_ptr(), hd.data_ptr() if with_h else None,
+               hout.data_ptr(), nrm.data_ptr())
+    torch.cuda.synchronize()
+    Vn = V[:n, :m]
+    w_ref = w - Vn @ h[:m] if with_h else w
+    h_ref = Vn.T @ w_ref
+    scale = np.abs(Vn).sum(axis=1).max() * np.abs(h[:m]).max() if (with_h and m) else 1.0
+    np.testing.assert_allclose(wout.cpu().numpy(), w_ref, rtol=0, atol=1e-14 * max(scale, 1.0) * 4)
+    if m:
+        err = np.abs(hout.cpu().numpy()[:m] - h_ref)
+        assert np.all(err <= 1e-13 * (np.abs(Vn).T @ np.abs(w_ref)))
+    assert abs(nrm.item() - w_ref @ w_ref) <= 1e-13 * (w_ref @ w_ref)
+
+
+@pytest.mark.parametrize("mfma", [True, False])
+@pytest.mark.parametrize("n,K,N", [(1000, 12, 1), (1000, 12, 11), (4099, 30, 11), (4099, 30, 30),
+                                   (20000, 96, 33), (20000, 96, 96), (3001, 60, 64), (3001, 60, 65),
+                                   (513, 128, 128), (17, 7, 3)])
+def test_vq_matches_numpy(mfma, n, K, N):
+    import torch
+
+    rng = np.random.default_rng(K * 1000 + N)
+    ldv, ldo, ldq = n + 5, n + 3, K + 2
+    V = np.zeros((ldv, K))
+    V[:n] = rng.uniform(-1, 1, (n, K))
+    Q = np.zeros((ldq, N))
+    Q[:K] = rng.uniform(-1, 1, (K, N))
+    Vd, Qd = dev(V), dev(Q)
+    Od = torch.full((ldo * N,), np.nan, dtype=torch.float64, device="cuda")
+    H.vq(0, n, K, N, Vd.data_ptr(), ldv, Qd.data_ptr(), ldq, Od.data_ptr(), ldo, use_mfma=mfma)
+    torch.cuda.synchronize()
+    out = Od.cpu().numpy().reshape(N, ldo).T
+    ref = V[:n] @ Q[:K]
+    bound = 4 * K * 1.2e-16 * (np.abs(V[:n]) @ np.abs(Q[:K]))
+    assert np.all(np.abs(out[:n] - ref) <= bound + 1e-300)
+    assert np.all(np.isnan(out[n:]))  # padding rows untouched

@@ -1,0 +1,123 @@
+"""GPU parity of the adaptive CSR SpMV (dcsrmv_operator_gpu, HLanc.hpp:530-572)
+against the oracle's naive SpMV (HLanc.hpp:403-431) and against MKL outputs
+recorded in tests/golden/: ||y - y_ref|| <= 1e-14 ||y_ref||."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import hlanc_amd as H
+import oracle as O
+import refformats as R
+from hlanc_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-14
+
+
+def run_case(rp, col, val, ncols, sym, x=None, seed=0):
+    n = len(rp) - 1
+    rng = np.random.default_rng(seed)
+    if x is None:
+        x = rng.uniform(-1, 1, ncols)
+    m = H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, ncols=ncols, is_symmetric=sym)
+    op = H.dcsrmv_operator_gpu(m)
+    y = op(x)
+    y_ref = O.spmv(O.Csr(rp, col, val, ncols, sym), x)
+    scale = max(np.linalg.norm(y_ref), 1e-300)
+    assert np.linalg.norm(y - y_ref) <= TOL * scale, (n, np.linalg.norm(y - y_ref) / scale)
+    return op, x, y
+
+
+def test_golden_mkl_vectors(golden):
+    for tag, (rp, col, val, nc, sym) in (("simple", R.simple()), ("journals", R.journals())):
+        x, y_mkl = golden[f"spmv_{tag}_x"], golden[f"spmv_{tag}_y"]
+        op, _, y = run_case(rp, col, val, nc, sym, x=x)
+        assert np.linalg.norm(y - y_mkl) <= TOL * np.linalg.norm(y_mkl)
+    rp, col, val = golden["spmv_rand_rp"], golden["spmv_rand_col"], golden["spmv_rand_val"]
+    x, y_mkl = golden["spmv_rand_x"], golden["spmv_rand_y"]
+    op, _, y = run_case(rp, col, val, len(x), False, x=x)
+    assert np.linalg.norm(y - y_mkl) <= TOL * np.linalg.norm(y_mkl)
+
+
+@pytest.mark.parametrize("shape", ["lap256", "stencil20", "chunglu", "er"])
+def test_generated_matrices(shape):
+    if shape == "lap256":
+        rp, col, val = W.laplacian_2d(256)
+    elif shape == "stencil20":
+        rp, col, val = W.stencil27(20, seed=1)
+    elif shape == "chunglu":
+        rp, col, val = W.chung_lu_laplacian(1 << 15, seed=2)
+    else:
+        rp, col, val = W.erdos_renyi(5003, 17.0, seed=3)
+    run_case(rp, col, val, len(rp) - 1, False)
+
+
+def test_ragged_rows_heavy_rows_and_chunks():
+    # empty rows, a row of 2049 (just past one stream block), rows of 8192 and
+    # 8193 (one and two chunks), a row of 40000 (five chunks), n % 64 != 0
+    rng = np.random.default_rng(5)
+    n, ncols = 777, 50000
+    lens = rng.integers(0, 40, n)
+    lens[::9] = 0
+    lens[10], lens[20], lens[30], lens[40] = 2049, 8192, 8193, 40000
+    lens[-1] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = rng.integers(0, ncols, rp[-1]).astype(np.int32)  # unsorted, with duplicates
+    val = rng.uniform(-1, 1, rp[-1])
+    run_case(rp, col, val, ncols, False)
+
+
+def test_symmetric_lower_storage_matches_general():
+    rp, col, val = W.stencil27(12, seed=4)
+    A = sp.csr_matrix((val, col, rp))
+    L = sp.tril(A).tocsr()
+    op, x, y = run_case(L.indptr, L.indices, L.data, A.shape[0], True)
+    assert np.linalg.norm(y - A @ x) <= TOL * np.linalg.norm(A @ x)
+    assert op.nnz_general == A.nnz
+
+
+def test_device_pointers_and_determinism():
+    import torch
+
+    rp, col, val = W.chung_lu_laplacian(1 << 14, seed=6)
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    x = torch.rand(n, dtype=torch.float64, device="cuda") * 2 - 1
+    y1 = torch.empty(n, dtype=torch.float64, device="cuda")
+    y2 = torch.empty_like(y1)
+    op.spmv(x.data_ptr(), y1.data_ptr())
+    op.spmv(x.data_ptr(), y2.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    y_host = op(x.cpu().numpy())  # host-pointer path, same kernel
+    np.testing.assert_array_equal(y_host, y1.cpu().numpy())
+    y_dev = torch.empty_like(y1)
+    assert H.lib().hl_csr_apply(op.h, x.data_ptr(), y_dev.data_ptr()) == 0  # device pointers in place
+    torch.cuda.synchronize()
+    assert torch.equal(y_dev, y1)
+
+
+def test_fp32_values_kernel():
+    import torch
+
+    rp, col, val = W.stencil27(16, seed=7)
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val), fp32_values=True)
+    x = np.random.default_rng(1).uniform(-1, 1, n)
+    xd = torch.tensor(x, device="cuda")
+    yd = torch.empty_like(xd)
+    op.spmv(xd.data_ptr(), yd.data_ptr(), fp32_values=True)
+    torch.cuda.synchronize()
+    y_ref = O.spmv(O.Csr(rp, col, val.astype(np.float32).astype(np.float64)), x)
+    assert np.linalg.norm(yd.cpu().numpy() - y_ref) <= TOL * np.linalg.norm(y_ref)
+
+
+def test_mgpu_collapses_to_one_device_and_rejects_symmetric():
+    rp, col, val = W.laplacian_2d(40)
+    m = H.dcsr_matrix_t(row_ptr=rp, col=col, val=val)
+    op = H.dcsrmv_operator_mgpu(m)
+    x = np.random.default_rng(2).uniform(-1, 1, len(rp) - 1)
+    y_ref = O.spmv(O.Csr(rp, col, val), x)
+    assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
+    with pytest.raises(H.HLancError):
+        H.dcsrmv_operator_mgpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, is_symmetric=True))
