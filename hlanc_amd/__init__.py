@@ -54,6 +54,13 @@ class Info(ctypes.Structure):
                 ("refine_shift", ctypes.c_double)]
 
 
+class Profile(ctypes.Structure):
+    """hl_profile: per kernel class (0 SpMV, 1 CGS, 2 V.Q, 3 other) launches, seconds, bytes."""
+    _fields_ = [("launches", ctypes.c_longlong * 4), ("seconds", ctypes.c_double * 4), ("bytes", ctypes.c_double * 4)]
+
+
+HL_FLAG_PROFILE = 2
+
 _lib = None
 
 # (name, restype, argtypes) of every entry point of include/hlanc_capi.h
@@ -79,6 +86,7 @@ _SIGS = [
                              ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
                              ctypes.c_int, ctypes.c_void_p]),
     ("hl_options_default", None, [ctypes.POINTER(Options)]),
+    ("hl_profile_last", ctypes.c_int, [ctypes.POINTER(Profile)]),
     ("hl_ds_solve_csr", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                        ctypes.POINTER(Options), ctypes.POINTER(Info)]),
     ("hl_ds_solve_host_op", ctypes.c_int, [ctypes.c_int, ctypes.c_int, HOST_OP, ctypes.c_void_p, ctypes.c_int,

@@ -262,6 +262,13 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
                      static_cast<hipStream_t>(stream));
 }
 
+int hl_profile_last(hl_profile *out)
+{
+    if (!out) return HL_ERR_ARG;
+    *out = thread_profile();
+    return HL_OK;
+}
+
 int hl_ds_solve_csr(hl_csr_t A, int nev, int ncv, double *values, double *vectors, int ldv,
                     const hl_options *opts, hl_info *info)
 {

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "hlanc_capi.h"
 
@@ -150,6 +151,30 @@ int launch_scale_rscale(int n, double *r, double q, const Ctl *ctl, hipStream_t 
 int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStream_t s);
 // out = float(in)
 int launch_cast_f32(long long n, const double *in, float *out, hipStream_t s);
+
+// ------------------------------------------------------------ profiling
+enum KClass : int { K_SPMV = 0, K_CGS = 1, K_VQ = 2, K_MISC = 3, K_NCLASS = 4 };
+
+// HIP-event timing of individual launches on one stream (profiled solves only).
+struct Profiler {
+    bool on = false;
+    hl_profile tot{};
+    struct Rec {
+        int cls;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> pool;
+    std::vector<Rec> pending;
+    hipEvent_t open_ev = nullptr;
+
+    hipEvent_t take();
+    int begin(hipStream_t s);                       // record a start event
+    int end(hipStream_t s, int cls, double bytes);  // record the stop event
+    int flush();                                    // after a stream sync
+    void release();
+};
+hl_profile &thread_profile();
 
 // ------------------------------------------------------------ host dense
 // dlartg with MKL's sign convention (called at HLanc.hpp:1514,1536).

@@ -137,6 +137,75 @@ void restart_chase(int nev, int np, const double *shift, double *sub, double *di
 
 } // namespace
 
+// ------------------------------------------------------------------ Profiler
+
+hipEvent_t Profiler::take()
+{
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int Profiler::begin(hipStream_t s)
+{
+    if (!on) return HL_OK;
+    open_ev = take();
+    if (!open_ev) return HL_ERR_HIP_BASE;
+    HL_TRY_HIP(hipEventRecord(open_ev, s));
+    return HL_OK;
+}
+
+int Profiler::end(hipStream_t s, int cls, double bytes)
+{
+    if (!on) return HL_OK;
+    hipEvent_t b = take();
+    if (!b) return HL_ERR_HIP_BASE;
+    HL_TRY_HIP(hipEventRecord(b, s));
+    pending.push_back({cls, bytes, open_ev, b});
+    open_ev = nullptr;
+    return HL_OK;
+}
+
+int Profiler::flush()
+{
+    for (const Rec &r : pending) {
+        float ms = 0.0f;
+        HL_TRY_HIP(hipEventSynchronize(r.b));
+        HL_TRY_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        tot.launches[r.cls] += 1;
+        tot.seconds[r.cls] += 1e-3 * ms;
+        tot.bytes[r.cls] += r.bytes;
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    pending.clear();
+    return HL_OK;
+}
+
+void Profiler::release()
+{
+    for (const Rec &r : pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    pending.clear();
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+    if (open_ev) (void)hipEventDestroy(open_ev);
+    open_ev = nullptr;
+}
+
+hl_profile &thread_profile()
+{
+    static thread_local hl_profile p{};
+    return p;
+}
+
 // ------------------------------------------------------------------ Workspace
 
 Workspace::~Workspace() { release(); }
@@ -216,8 +285,14 @@ int Solver::apply_op(int j)
     Workspace &W = ws;
     ++spmv;
     if (A) {
-        return launch_spmv(*A, W.r, &W.ctl->xscale, W.w, W.V[W.cur] + (size_t)j * W.ldv, HL_PREC_F64,
-                           W.stream);
+        int rc = prof.begin(W.stream);
+        if (!rc) rc = launch_spmv(*A, W.r, &W.ctl->xscale, W.w, W.V[W.cur] + (size_t)j * W.ldv, HL_PREC_F64,
+                                  W.stream);
+        if (!rc) {
+            const double b = 12.0 * A->nnz + 4.0 * (A->nrows + 1) + 8.0 * A->ncols + 16.0 * A->nrows;
+            rc = prof.end(W.stream, K_SPMV, b);
+        }
+        return rc;
     }
     // host operator: v_j on the device, then x/y through pinned host buffers
     int rc = launch_scale_copy(W.n, W.r, W.ctl, W.V[W.cur] + (size_t)j * W.ldv, W.stream);
@@ -240,17 +315,31 @@ int Solver::step(int j)
     const double *V = W.V[W.cur];
     double *h0 = W.hbuf, *h1 = W.hbuf + W.ncv;
     const int m = j + 1;
-    if ((rc = launch_cgs(W.n, m, V, W.ldv, W.w, nullptr, nullptr, h0, W.red, W.ctl, W.hsub, W.hdia,
-                         EPI_WNORM, j, W.stream)))
-        return rc;
-    if ((rc = launch_cgs(W.n, m, V, W.ldv, W.w, W.r, h0, h1, W.red, W.ctl, W.hsub, W.hdia, EPI_PASS1, j,
+    if ((rc = cgs_step(m, V, W.w, nullptr, nullptr, h0, EPI_WNORM, j))) return rc;
+    if ((rc = cgs_step(m, V, W.w, W.r, h0, h1, EPI_PASS1, j))) return rc;
+    if ((rc = cgs_step(m, V, W.r, W.r, h1, h0, EPI_PASS2, j))) return rc;
+    return cgs_step(m, V, W.r, W.r, h0, nullptr, EPI_PASS3, j);
+}
+
+// One CGS launch.  Profiled solves decide on the host whether a DGKS
+// correction runs (so that only executed passes are timed); otherwise the
+// kernel itself exits early when the device-side decision says so.
+int Solver::cgs_step(int m, const double *V, const double *w_in, double *w_out, const double *h_in, double *h_out,
+                     int mode, int j)
+{
+    Workspace &W = ws;
+    if (prof.on && (mode == EPI_PASS2 || mode == EPI_PASS3)) {
+        HL_TRY_HIP(hipMemcpyAsync(W.ctl_h, W.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, W.stream));
+        HL_TRY_HIP(hipStreamSynchronize(W.stream));
+        if (!W.ctl_h->need) return HL_OK;
+    }
+    int rc = prof.begin(W.stream);
+    if (rc) return rc;
+    if ((rc = launch_cgs(W.n, m, V, W.ldv, w_in, w_out, h_in, h_out, W.red, W.ctl, W.hsub, W.hdia, mode, j,
                          W.stream)))
         return rc;
-    if ((rc = launch_cgs(W.n, m, V, W.ldv, W.r, W.r, h1, h0, W.red, W.ctl, W.hsub, W.hdia, EPI_PASS2, j,
-                         W.stream)))
-        return rc;
-    return launch_cgs(W.n, m, V, W.ldv, W.r, W.r, h0, nullptr, W.red, W.ctl, W.hsub, W.hdia, EPI_PASS3, j,
-                      W.stream);
+    const double bytes = 8.0 * W.n * (m + 1 + (w_out ? 1 : 0));
+    return prof.end(W.stream, K_CGS, bytes);
 }
 
 // dsaitr(k, np) and the read-back of the new alpha/beta entries and rnorm.
@@ -267,6 +356,10 @@ int Solver::extend(int k, int np)
                               W.stream));
     HL_TRY_HIP(hipMemcpyAsync(W.ctl_h, W.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, W.stream));
     HL_TRY_HIP(hipStreamSynchronize(W.stream));
+    if (prof.on) {
+        int rc = prof.flush();
+        if (rc) return rc;
+    }
     for (int i = 0; i < cnt; ++i) {
         sub[k + i] = W.hh[i];
         dia[k + i] = W.hh[W.ncv + i];
@@ -287,13 +380,16 @@ int Solver::apply_restart(int nev, const double *q)
     HL_TRY_HIP(hipMemcpyAsync(W.Qd, q, sizeof(double) * (size_t)ncv * N, hipMemcpyHostToDevice, W.stream));
     const double qlast = q[(size_t)(nev - 1) * ncv + ncv - 1];
     const int nxt = 1 - W.cur;
-    int rc;
+    int rc = prof.begin(W.stream);
+    if (rc) return rc;
     if (has_beta) {
         rc = launch_vq(W.n, ncv, N, W.V[W.cur], W.ldv, W.Qd, ncv, W.V[nxt], W.ldv, use_mfma, nev, W.r, qlast,
                        sub[nev], W.red, W.ctl, W.stream);
+        if (!rc) rc = prof.end(W.stream, K_VQ, 8.0 * W.n * (ncv + N + 2));
     } else {
         rc = launch_vq(W.n, ncv, N, W.V[W.cur], W.ldv, W.Qd, ncv, W.V[nxt], W.ldv, use_mfma, -1, nullptr, 0.0,
                        0.0, W.red, W.ctl, W.stream);
+        if (!rc) rc = prof.end(W.stream, K_VQ, 8.0 * W.n * (ncv + N));
         if (!rc) rc = launch_scale_rscale(W.n, W.r, qlast, W.ctl, W.stream);
         if (!rc) rc = launch_norm(W.n, W.r, W.red, W.ctl, EPI_RNORM, W.stream);
     }
@@ -314,6 +410,8 @@ int Solver::run(int nev0, double *values, double *vectors, int ldv_host, const h
     const int maxiter = o.maxiter > 0 ? o.maxiter : 200;
     const double eps23 = std::pow(kEps, 2.0 / 3.0);
     use_mfma = o.use_mfma;
+    prof.on = (o.flags & HL_FLAG_PROFILE) != 0;
+    prof.tot = hl_profile{};
     sub.assign(ncv, 0.0);
     dia.assign(ncv, 0.0);
 
@@ -421,9 +519,11 @@ int Solver::run(int nev0, double *values, double *vectors, int ldv_host, const h
         HL_TRY_HIP(hipMemcpyAsync(W.Qd, qf.data(), sizeof(double) * (size_t)ncv * ncv, hipMemcpyHostToDevice,
                                   W.stream));
         const int nxt = 1 - W.cur;
+        if ((rc = prof.begin(W.stream))) return rc;
         if ((rc = launch_vq(n, ncv, ncv, W.V[W.cur], W.ldv, W.Qd, ncv, W.V[nxt], W.ldv, use_mfma, -1, nullptr,
                             0.0, 0.0, W.red, W.ctl, W.stream)))
             return rc;
+        if ((rc = prof.end(W.stream, K_VQ, 16.0 * n * ncv))) return rc;
         // sync_host (HLanc.hpp:1063-1071): all ncv columns to the caller
         HL_TRY_HIP(hipMemcpy2DAsync(vectors, sizeof(double) * (size_t)ldv_host, W.V[nxt],
                                     sizeof(double) * (size_t)W.ldv, sizeof(double) * (size_t)n, ncv,
@@ -434,6 +534,12 @@ int Solver::run(int nev0, double *values, double *vectors, int ldv_host, const h
     }
 
 finish:
+    if (prof.on) {
+        (void)hipStreamSynchronize(W.stream);
+        (void)prof.flush();
+        thread_profile() = prof.tot;
+    }
+    prof.release();
     info.ret = ret;
     info.restarts = restarts;
     info.nconv = nconv;

@@ -37,6 +37,7 @@ struct Solver {
     double rnorm = 0.0;
     long long spmv = 0, cgs = 0;
     int restarts = 0;
+    Profiler prof;                       // HL_FLAG_PROFILE
 
     int run(int nev, double *values, double *vectors, int ldv_host, const hl_options &o, hl_info &info);
 
@@ -46,6 +47,8 @@ struct Solver {
     int step(int j);
     int extend(int k, int np);
     int apply_restart(int nev, const double *q);
+    int cgs_step(int m, const double *V, const double *w_in, double *w_out, const double *h_in, double *h_out,
+                 int mode, int j);
 };
 
 }  // namespace hl

@@ -146,6 +146,22 @@ typedef struct {
     double refine_shift;    /* mixed precision: largest Rayleigh-quotient change */
 } hl_info;
 
+/* hl_options.flags bits. */
+#define HL_FLAG_NO_GRAPH 1 /* reserved: launch the Lanczos steps one by one */
+#define HL_FLAG_PROFILE 2  /* time every kernel launch with HIP events (slower);
+                              the DGKS correction passes are then launched only
+                              when needed, decided on the host */
+
+/* Per-kernel-class totals of the last profiled solve on the calling thread:
+ * index 0 SpMV, 1 CGS passes, 2 V.Q GEMM, 3 other kernels.  bytes are
+ * algorithmic HBM bytes (each operand counted once per launch). */
+typedef struct {
+    long long launches[4];
+    double seconds[4];
+    double bytes[4];
+} hl_profile;
+int hl_profile_last(hl_profile *out);
+
 /* Fills *o with the reference defaults (tol = eps, maxiter = 200, generator
  * seed 0, FP64 basis, MFMA V.Q). */
 void hl_options_default(hl_options *o);
