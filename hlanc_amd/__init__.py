@@ -281,14 +281,15 @@ class SolveResult:
     info: Info
 
 
-def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mfma: bool = True,
+def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mfma: Optional[bool] = None,
             basis_prec: int = HL_PREC_F64, flags: int = 0) -> Options:
     o = Options()
     lib().hl_options_default(ctypes.byref(o))
     o.tol = tol
     o.maxiter = maxiter
     o.seed = seed
-    o.use_mfma = int(use_mfma)
+    if use_mfma is not None:  # None keeps the library default (hl_options_default)
+        o.use_mfma = int(use_mfma)
     o.basis_prec = basis_prec
     o.flags = flags
     if v0 is not None:
@@ -298,7 +299,7 @@ def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mf
 
 
 def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, v0=None,
-                  seed: int = 0, use_mfma: bool = True, ldv: Optional[int] = None) -> SolveResult:
+                  seed: int = 0, use_mfma: Optional[bool] = None, ldv: Optional[int] = None) -> SolveResult:
     """ds_solver_gpu (HLanc.hpp:1868-1872).  ``op`` is a dcsrmv_operator_gpu
     (device-resident path) or any callable ``op(x, y)`` on numpy arrays (host
     path, as the reference calls arbitrary operators with host pointers).

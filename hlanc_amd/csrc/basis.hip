@@ -275,8 +275,9 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // Out^T(c, row) = sum_k Q(k, c) V(row, k) on v_mfma_f64_16x16x4f64.
 // A (16 x 4): lane l holds Q^T(ct*16 + l%16, k0 + l/16); B (4 x 16): lane l holds
-// V(r0 + l%16, k0 + l/16); D (16 x 16): lane l, element i is
-// Out(r0 + l%16, ct*16 + 4*(l/16) + i).  NT column tiles of 16 per launch.
+// V(r0 + l%16, k0 + l/16); D (16 x 16): lane l, element i is D(l/16 + 4 i, l%16),
+// i.e. Out(r0 + l%16, ct*16 + l/16 + 4 i) (layout checked on the device by
+// tests/test_basis_gpu.py).  NT column tiles of 16 per launch.
 template <int NT>
 __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, const double *__restrict__ V,
                                                          long long ldv, const double *__restrict__ Q, int ldq,
@@ -324,13 +325,13 @@ __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, co
         for (int ct = 0; ct < NT; ++ct) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int cl = ct * 16 + 4 * lk + i;
+                const int cl = ct * 16 + lk + 4 * i;
                 if (ok && c0 + cl < N) Out[(long long)(c0 + cl) * ldo + row] = acc[ct][i];
                 if (cl == res_local) rk = acc[ct][i];
             }
         }
         if (res_local >= 0) { // the lane holding column res_local for this row shares it
-            const int owner = ((res_local % 16) / 4) * 16 + lr;
+            const int owner = ((res_local % 16) % 4) * 16 + lr;
             const double rkv = __shfl(rk, owner);
             if (lk == 0 && ok) {
                 const double tt = qrs * r[row];

@@ -128,7 +128,7 @@ typedef struct {
     const double *v0;  /* host start vector of length n, or NULL: seeded generator */
     uint64_t seed;     /* generator seed when v0 == NULL */
     int basis_prec;    /* HL_PREC_F64 (reference semantics) or HL_PREC_F32 (mixed) */
-    int use_mfma;      /* 1: V.Q on MFMA (default when the struct is zeroed: see below) */
+    int use_mfma;      /* 1: V.Q on the MFMA kernel, 0: FMA kernel (hl_options_default says which) */
     int flags;         /* bit 0: do not use hipGraph replay for the Lanczos steps */
     int reserved[5];
 } hl_options;

@@ -66,3 +66,22 @@ def test_vq_matches_numpy(mfma, n, K, N):
     bound = 4 * K * 1.2e-16 * (np.abs(V[:n]) @ np.abs(Q[:K]))
     assert np.all(np.abs(out[:n] - ref) <= bound + 1e-300)
     assert np.all(np.isnan(out[n:]))  # padding rows untouched
+
+
+@pytest.mark.parametrize("mfma", [True, False])
+def test_vq_identity_places_every_column(mfma):
+    # Q = I (K = N = 48): Out must reproduce V column by column; a wrong MFMA
+    # output layout shows up as permuted or missing columns.
+    import torch
+
+    n, K = 100, 48
+    V = np.arange(n * K, dtype=np.float64).reshape(K, n).T.copy() + 1.0
+    Q = np.eye(K)
+    Vd, Qd = dev(V), dev(Q)
+    Od = torch.zeros(n * K, dtype=torch.float64, device="cuda")
+    H.vq(0, n, K, K, Vd.data_ptr(), n, Qd.data_ptr(), K, Od.data_ptr(), n, use_mfma=mfma)
+    torch.cuda.synchronize()
+    out = Od.cpu().numpy().reshape(K, n).T
+    got = [int(round(out[0, c] - 1.0)) // n if out[0, c] else -1 for c in range(K)]
+    assert got == list(range(K)), got
+    np.testing.assert_array_equal(out, V)
