@@ -105,12 +105,23 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy 5 times (scale for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no collectives); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 5 x 1 GiB")
+        return
 
     import hlanc_amd as H
     from hlanc_amd import workloads as W

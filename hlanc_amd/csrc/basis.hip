@@ -127,7 +127,7 @@ __device__ inline bool reduce_blocks(const double *vals, int M1, double *partial
     return true;
 }
 
-template <int MAXC, typename VT>
+template <int MAXC, typename VT, bool TILED>
 __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__restrict__ V, long long ldv,
                                                      const double *w_in, double *w_out,
                                                      const double *__restrict__ h_in, double *h_out,
@@ -135,6 +135,7 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
                                                      double *h_sub, double *h_dia, int mode, int j)
 {
     if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return; // DGKS decided on the device
+    __shared__ double hs[kGroups * MAXC];
     __shared__ double red[kBlock / kWave][kGroups * MAXC + 1];
     __shared__ double vals[kGroups * MAXC + 1];
     __shared__ double scratch[kBlock];
@@ -146,31 +147,47 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
     const int rl = lane % kTileRows;
     const int cpg = (m + kGroups - 1) / kGroups;
     const int c0 = g * cpg;
+    const bool has_h = h_in != nullptr;
+    for (int c = threadIdx.x; c < kGroups * MAXC; c += kBlock) hs[c] = (has_h && c < m) ? h_in[c] : 0.0;
+    __syncthreads();
 
-    double hin[MAXC], acc[MAXC];
+    double acc[MAXC];
 #pragma unroll
-    for (int c = 0; c < MAXC; ++c) {
-        acc[c] = 0.0;
-        hin[c] = (h_in && c < cpg && c0 + c < m) ? h_in[c0 + c] : 0.0;
-    }
+    for (int c = 0; c < MAXC; ++c) acc[c] = 0.0;
     double nrm = 0.0;
     const long long ntiles = (n + kTileRows - 1) / kTileRows;
     const long long W = (long long)gridDim.x * (kBlock / kWave);
-    for (long long t = (long long)blockIdx.x * (kBlock / kWave) + wv; t < ntiles; t += W) {
-        const long long r = t * kTileRows + rl;
-        const bool ok = r < n;
-        const double win = ok ? w_in[r] : 0.0;
+    long long t = (long long)blockIdx.x * (kBlock / kWave) + wv;
+
+    // software pipeline: the next tile's V and w are in flight while this one is used
+    double vn[MAXC], wn;
+#define HL_CGS_FETCH(TT)                                                                         \
+    {                                                                                            \
+        const long long rr = (TT) * kTileRows + rl;                                              \
+        const bool okk = (TT) < ntiles && rr < n;                                                \
+        wn = okk ? w_in[rr] : 0.0;                                                               \
+        _Pragma("unroll") for (int c = 0; c < MAXC; ++c)                                         \
+        {                                                                                        \
+            const int colj = c0 + c;                                                             \
+            vn[c] = (okk && c < cpg && colj < m)                                                 \
+                        ? static_cast<double>(__builtin_nontemporal_load(&V[vpos<TILED>(rr, colj, ldv)])) \
+                        : 0.0;                                                                   \
+        }                                                                                        \
+    }
+    HL_CGS_FETCH(t);
+    for (; t < ntiles; t += W) {
         double v[MAXC];
 #pragma unroll
-        for (int c = 0; c < MAXC; ++c) {
-            const int colj = c0 + c;
-            v[c] = (ok && c < cpg && colj < m) ? static_cast<double>(V[(long long)colj * ldv + r]) : 0.0;
-        }
+        for (int c = 0; c < MAXC; ++c) v[c] = vn[c];
+        const double win = wn;
+        const long long r = t * kTileRows + rl;
+        const bool ok = r < n;
+        HL_CGS_FETCH(t + W);
         double a = win;
-        if (h_in) {
+        if (has_h) {
             double p = 0.0;
 #pragma unroll
-            for (int c = 0; c < MAXC; ++c) p += v[c] * hin[c];
+            for (int c = 0; c < MAXC; ++c) p += v[c] * hs[c0 + c];
             p += __shfl_xor(p, 16);
             p += __shfl_xor(p, 32);
             a = win - p;
@@ -182,6 +199,7 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) acc[c] += v[c] * a;
     }
+#undef HL_CGS_FETCH
 
 #pragma unroll
     for (int c = 0; c < MAXC; ++c) {
@@ -214,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
 }
 
 // Out(:, c0:c0+CB) = V(:, 0:K) Q(0:K, c0:c0+CB); thread per row, FP64 FMA.
-template <int CB, typename VT>
+template <int CB, typename VT, bool TIN, bool TOUT>
 __global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, const VT *__restrict__ V,
                                                         long long ldv, const double *__restrict__ Q, int ldq,
                                                         VT *__restrict__ Out, long long ldo, int c0,
@@ -240,14 +258,14 @@ __global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, con
 #pragma unroll
         for (int c = 0; c < CB; ++c) acc[c] = 0.0;
         for (int k = 0; k < K; ++k) {
-            const double v = static_cast<double>(V[(long long)k * ldv + row]);
+            const double v = static_cast<double>(V[vpos<TIN>(row, k, ldv)]);
 #pragma unroll
             for (int c = 0; c < CB; ++c) acc[c] += v * qs[k * CB + c];
         }
         double rk = 0.0;
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
-            if (c0 + c < N) Out[(long long)(c0 + c) * ldo + row] = static_cast<VT>(acc[c]);
+            if (c0 + c < N) Out[vpos<TOUT>(row, c0 + c, ldo)] = static_cast<VT>(acc[c]);
             if (c == res_local) rk = acc[c];
         }
         if (res_local >= 0) { // dsapps residual (HLanc.hpp:1605-1614)
@@ -278,7 +296,7 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // V(r0 + l%16, k0 + l/16); D (16 x 16): lane l, element i is D(l/16 + 4 i, l%16),
 // i.e. Out(r0 + l%16, ct*16 + l/16 + 4 i) (layout checked on the device by
 // tests/test_basis_gpu.py).  NT column tiles of 16 per launch.
-template <int NT>
+template <int NT, bool TIN, bool TOUT>
 __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, const double *__restrict__ V,
                                                          long long ldv, const double *__restrict__ Q, int ldq,
                                                          double *__restrict__ Out, long long ldo, int c0,
@@ -313,7 +331,7 @@ __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, co
         for (int ct = 0; ct < NT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
         for (int k0 = 0; k0 < Kp; k0 += 4) {
             const int k = k0 + lk;
-            const double b = (ok && k < K) ? V[(long long)k * ldv + row] : 0.0;
+            const double b = (ok && k < K) ? V[vpos<TIN>(row, k, ldv)] : 0.0;
 #pragma unroll
             for (int ct = 0; ct < NT; ++ct) {
                 const double a = qs[k * NC + ct * 16 + lr];
@@ -326,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, co
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int cl = ct * 16 + lk + 4 * i;
-                if (ok && c0 + cl < N) Out[(long long)(c0 + cl) * ldo + row] = acc[ct][i];
+                if (ok && c0 + cl < N) Out[vpos<TOUT>(row, c0 + cl, ldo)] = acc[ct][i];
                 if (cl == res_local) rk = acc[ct][i];
             }
         }
@@ -370,15 +388,16 @@ __global__ __launch_bounds__(kBlock) void start_vector_kernel(int n, uint64_t ba
     }
 }
 
+// v = r * xscale into a contiguous vector and/or a basis column.
 template <typename VT>
-__global__ __launch_bounds__(kBlock) void scale_copy_kernel(int n, const double *r, const Ctl *ctl, VT *v,
-                                                            double *vd)
+__global__ __launch_bounds__(kBlock) void scale_copy_kernel(int n, const double *r, const Ctl *ctl, double *v,
+                                                            VT *col, long long ld, int j, int tiled)
 {
     const double s = ctl->xscale;
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock) {
         const double x = r[i] * s;
-        v[i] = static_cast<VT>(x);
-        if (vd) vd[i] = x;
+        if (v) v[i] = x;
+        if (col) col[tiled ? vpos<true>(i, j, ld) : vpos<false>(i, j, ld)] = static_cast<VT>(x);
     }
 }
 
@@ -403,7 +422,7 @@ int grid_for(long long n, int per_block, int cap)
     return static_cast<int>(g);
 }
 
-template <typename VT>
+template <typename VT, bool TILED>
 int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, double *w_out,
                  const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia,
                  int mode, int j, hipStream_t s)
@@ -418,9 +437,9 @@ int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, d
         return HL_ERR_STATE;
     }
     const int cpg = (m + kGroups - 1) / kGroups;
-#define HL_CGS(MC)                                                                                      \
-    cgs_kernel<MC, VT><<<nb, kBlock, 0, s>>>(n, m, V, ldv, w_in, w_out, h_in, h_out, red.partials, \
-                                             red.counter, ctl, h_sub, h_dia, mode, j)
+#define HL_CGS(MC)                                                                                             \
+    cgs_kernel<MC, VT, TILED><<<nb, kBlock, 0, s>>>(n, m, V, ldv, w_in, w_out, h_in, h_out, red.partials,    \
+                                                    red.counter, ctl, h_sub, h_dia, mode, j)
     if (cpg <= 2)
         HL_CGS(2);
     else if (cpg <= 4)
@@ -438,12 +457,71 @@ int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, d
     return HL_OK;
 }
 
+template <typename VT, bool TIN, bool TOUT>
+int launch_vq_fma(int n, int K, int N, const VT *V, long long ldv, const double *Q, int ldq, VT *Out,
+                  long long ldo, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
+                  hipStream_t s)
+{
+    constexpr int CB = 32;
+    const int nb = grid_for(n, kBlock * 4, 1024);
+    if (res_col >= 0 && nb > red.max_blocks) {
+        set_error("V.Q: reduction workspace too small");
+        return HL_ERR_STATE;
+    }
+    for (int c0 = 0; c0 < N; c0 += CB) {
+        const int rl = (res_col >= c0 && res_col < c0 + CB) ? res_col - c0 : -1;
+        const size_t lds = sizeof(double) * (size_t)K * CB;
+        vq_fma_kernel<CB, VT, TIN, TOUT><<<nb, kBlock, lds, s>>>(n, K, N, V, ldv, Q, ldq, Out, ldo, c0, rl, r,
+                                                                  qlast, beta, red.partials, red.counter, ctl);
+        HL_TRY_HIP(hipGetLastError());
+    }
+    return HL_OK;
+}
+
+template <bool TIN, bool TOUT>
+int launch_vq_mfma(int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq, double *Out,
+                   long long ldo, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
+                   hipStream_t s)
+{
+    const long long ntiles = (static_cast<long long>(n) + 15) / 16;
+    const int nb = grid_for(ntiles, 4 * 4, 1024);
+    if (res_col >= 0 && nb > red.max_blocks) {
+        set_error("V.Q: reduction workspace too small");
+        return HL_ERR_STATE;
+    }
+    const int Kp = (K + 3) & ~3;
+    // column tiles per launch: Q's K x 16*nt slice must fit 48 KB of LDS
+    int max_nt = 6144 / (Kp * 16);
+    max_nt = max_nt < 1 ? 1 : (max_nt > 4 ? 4 : max_nt);
+    const int chunk = 16 * max_nt;
+    for (int c0 = 0; c0 < N; c0 += chunk) {
+        const int width = N - c0 < chunk ? N - c0 : chunk;
+        const int nt = (width + 15) / 16;
+        const int rl = (res_col >= c0 && res_col < c0 + chunk) ? res_col - c0 : -1;
+        const size_t lds = sizeof(double) * (size_t)Kp * nt * 16;
+#define HL_VQM(NTV)                                                                                          \
+    vq_mfma_kernel<NTV, TIN, TOUT><<<nb, kBlock, lds, s>>>(n, K, N, V, ldv, Q, ldq, Out, ldo, c0, rl, r, qlast, \
+                                                           beta, red.partials, red.counter, ctl)
+        if (nt == 1)
+            HL_VQM(1);
+        else if (nt == 2)
+            HL_VQM(2);
+        else if (nt == 3)
+            HL_VQM(3);
+        else
+            HL_VQM(4);
+#undef HL_VQM
+        HL_TRY_HIP(hipGetLastError());
+    }
+    return HL_OK;
+}
+
 }  // namespace
 
 int cgs_blocks(int n)
 {
     const long long ntiles = (static_cast<long long>(n) + kTileRows - 1) / kTileRows;
-    return grid_for(ntiles, 16, 512);
+    return grid_for(ntiles, 16, 1024);
 }
 
 int reduce_alloc(Reduce &r, int max_blocks, int max_cols)
@@ -463,101 +541,62 @@ void reduce_free(Reduce &r)
     r = Reduce{};
 }
 
-int launch_cgs(int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
-               const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia,
-               int mode, int j, hipStream_t s)
+int launch_cgs(int n, int m, const double *V, VLayout L, const double *w_in, double *w_out, const double *h_in,
+               double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia, int mode, int j, hipStream_t s)
 {
-    return launch_cgs_t<double>(n, m, V, ldv, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
+    if (L.tiled)
+        return launch_cgs_t<double, true>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
+    return launch_cgs_t<double, false>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
 }
 
-int launch_cgs_f32(int n, int m, const float *V, long long ldv, const double *w_in, double *w_out,
-                   const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia,
-                   int mode, int j, hipStream_t s)
+int launch_cgs_f32(int n, int m, const float *V, VLayout L, const double *w_in, double *w_out, const double *h_in,
+                   double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia, int mode, int j,
+                   hipStream_t s)
 {
-    return launch_cgs_t<float>(n, m, V, ldv, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
+    if (L.tiled)
+        return launch_cgs_t<float, true>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
+    return launch_cgs_t<float, false>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
 }
 
 int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStream_t s)
 {
-    return launch_cgs_t<double>(n, 0, static_cast<const double *>(nullptr), 0, r, nullptr, nullptr, nullptr,
-                                red, ctl, nullptr, nullptr, mode, 0, s);
+    return launch_cgs_t<double, false>(n, 0, static_cast<const double *>(nullptr), 0, r, nullptr, nullptr, nullptr,
+                                       red, ctl, nullptr, nullptr, mode, 0, s);
 }
 
-namespace {
-template <typename VT>
-int launch_vq_fma(int n, int K, int N, const VT *V, long long ldv, const double *Q, int ldq, VT *Out,
-                  long long ldo, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
-                  hipStream_t s)
-{
-    constexpr int CB = 32;
-    const int nb = grid_for(n, kBlock * 4, 1024);
-    if (res_col >= 0 && nb > red.max_blocks) {
-        set_error("V.Q: reduction workspace too small");
-        return HL_ERR_STATE;
-    }
-    for (int c0 = 0; c0 < N; c0 += CB) {
-        const int rl = (res_col >= c0 && res_col < c0 + CB) ? res_col - c0 : -1;
-        const size_t lds = sizeof(double) * (size_t)K * CB;
-        vq_fma_kernel<CB, VT><<<nb, kBlock, lds, s>>>(n, K, N, V, ldv, Q, ldq, Out, ldo, c0, rl, r, qlast,
-                                                       beta, red.partials, red.counter, ctl);
-        HL_TRY_HIP(hipGetLastError());
-    }
-    return HL_OK;
-}
-}  // namespace
-
-int launch_vq(int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq, double *Out,
-              long long ldo, int use_mfma, int res_col, double *r, double qlast, double beta, Reduce &red,
-              Ctl *ctl, hipStream_t s)
+int launch_vq(int n, int K, int N, const double *V, VLayout Lin, const double *Q, int ldq, double *Out,
+              VLayout Lout, int use_mfma, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
+              hipStream_t s)
 {
     if (K <= 0 || N <= 0 || K > 128) {
         set_error("V.Q: K=%d N=%d unsupported (K <= 128)", K, N);
         return HL_ERR_ARG;
     }
-    if (!use_mfma)
-        return launch_vq_fma<double>(n, K, N, V, ldv, Q, ldq, Out, ldo, res_col, r, qlast, beta, red, ctl, s);
-    const long long ntiles = (static_cast<long long>(n) + 15) / 16;
-    const int nb = grid_for(ntiles, 4 * 4, 1024);
-    if (res_col >= 0 && nb > red.max_blocks) {
-        set_error("V.Q: reduction workspace too small");
-        return HL_ERR_STATE;
-    }
-    const int Kp = (K + 3) & ~3;
-    // column tiles per launch: Q's K x 16*nt slice must fit 48 KB of LDS
-    int max_nt = 6144 / (Kp * 16);
-    max_nt = max_nt < 1 ? 1 : (max_nt > 4 ? 4 : max_nt);
-    const int chunk = 16 * max_nt;
-    for (int c0 = 0; c0 < N; c0 += chunk) {
-        const int width = N - c0 < chunk ? N - c0 : chunk;
-        const int nt = (width + 15) / 16;
-        const int rl = (res_col >= c0 && res_col < c0 + chunk) ? res_col - c0 : -1;
-        const size_t lds = sizeof(double) * (size_t)Kp * nt * 16;
-#define HL_VQM(NTV)                                                                                      \
-    vq_mfma_kernel<NTV><<<nb, kBlock, lds, s>>>(n, K, N, V, ldv, Q, ldq, Out, ldo, c0, rl, r, qlast, beta, \
-                                                red.partials, red.counter, ctl)
-        if (nt == 1)
-            HL_VQM(1);
-        else if (nt == 2)
-            HL_VQM(2);
-        else if (nt == 3)
-            HL_VQM(3);
-        else
-            HL_VQM(4);
-#undef HL_VQM
-        HL_TRY_HIP(hipGetLastError());
-    }
-    return HL_OK;
+#define HL_VQ_DISPATCH(TI, TO)                                                                                   \
+    return use_mfma ? launch_vq_mfma<TI, TO>(n, K, N, V, Lin.ld, Q, ldq, Out, Lout.ld, res_col, r, qlast, beta, \
+                                             red, ctl, s)                                                       \
+                    : launch_vq_fma<double, TI, TO>(n, K, N, V, Lin.ld, Q, ldq, Out, Lout.ld, res_col, r, qlast, \
+                                                    beta, red, ctl, s)
+    if (Lin.tiled && Lout.tiled) HL_VQ_DISPATCH(true, true);
+    if (Lin.tiled) HL_VQ_DISPATCH(true, false);
+    if (Lout.tiled) HL_VQ_DISPATCH(false, true);
+    HL_VQ_DISPATCH(false, false);
+#undef HL_VQ_DISPATCH
 }
 
-int launch_vq_f32(int n, int K, int N, const float *V, long long ldv, const double *Q, int ldq, float *Out,
-                  long long ldo, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
+int launch_vq_f32(int n, int K, int N, const float *V, VLayout Lin, const double *Q, int ldq, float *Out,
+                  VLayout Lout, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
                   hipStream_t s)
 {
     if (K <= 0 || N <= 0 || K > 128) {
         set_error("V.Q: K=%d N=%d unsupported (K <= 128)", K, N);
         return HL_ERR_ARG;
     }
-    return launch_vq_fma<float>(n, K, N, V, ldv, Q, ldq, Out, ldo, res_col, r, qlast, beta, red, ctl, s);
+    if (Lin.tiled && Lout.tiled)
+        return launch_vq_fma<float, true, true>(n, K, N, V, Lin.ld, Q, ldq, Out, Lout.ld, res_col, r, qlast, beta,
+                                                red, ctl, s);
+    return launch_vq_fma<float, false, false>(n, K, N, V, Lin.ld, Q, ldq, Out, Lout.ld, res_col, r, qlast, beta,
+                                              red, ctl, s);
 }
 
 int launch_start_vector(int n, uint64_t seed, double *r, hipStream_t s)
@@ -568,16 +607,10 @@ int launch_start_vector(int n, uint64_t seed, double *r, hipStream_t s)
     return HL_OK;
 }
 
-int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, hipStream_t s)
+int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn col, hipStream_t s)
 {
-    scale_copy_kernel<double><<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, r, ctl, v, nullptr);
-    HL_TRY_HIP(hipGetLastError());
-    return HL_OK;
-}
-
-int launch_scale_copy_f32(int n, const double *r, const Ctl *ctl, float *v, double *vd, hipStream_t s)
-{
-    scale_copy_kernel<float><<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, r, ctl, v, vd);
+    scale_copy_kernel<double><<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, r, ctl, v, col.base, col.L.ld, col.j,
+                                                                            col.L.tiled ? 1 : 0);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

@@ -188,7 +188,7 @@ int hl_csr_apply(hl_csr_t h, const double *x, double *y)
         xx = p.x_stage;
     }
     double *yy = yd ? y : p.y_stage;
-    int rc = launch_spmv(p, xx, nullptr, yy, nullptr, HL_PREC_F64, nullptr);
+    int rc = launch_spmv(p, xx, nullptr, yy, VColumn{}, HL_PREC_F64, nullptr);
     if (rc) return rc;
     if (!yd) HL_TRY_HIP(hipMemcpy(y, yy, sizeof(double) * p.nrows, hipMemcpyDefault));
     return HL_OK;
@@ -198,14 +198,14 @@ int hl_csr_spmv(hl_csr_t h, const double *x_dev, double *y_dev, void *stream)
 {
     if (!h) return HL_ERR_ARG;
     DeviceGuard g(h->plan.dev);
-    return launch_spmv(h->plan, x_dev, nullptr, y_dev, nullptr, HL_PREC_F64, static_cast<hipStream_t>(stream));
+    return launch_spmv(h->plan, x_dev, nullptr, y_dev, VColumn{}, HL_PREC_F64, static_cast<hipStream_t>(stream));
 }
 
 int hl_csr_spmv_f32val(hl_csr_t h, const double *x_dev, double *y_dev, void *stream)
 {
     if (!h) return HL_ERR_ARG;
     DeviceGuard g(h->plan.dev);
-    return launch_spmv(h->plan, x_dev, nullptr, y_dev, nullptr, HL_PREC_F32, static_cast<hipStream_t>(stream));
+    return launch_spmv(h->plan, x_dev, nullptr, y_dev, VColumn{}, HL_PREC_F32, static_cast<hipStream_t>(stream));
 }
 
 double hl_csr_spmv_bytes(hl_csr_t h, int val_prec)
@@ -233,7 +233,8 @@ int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const dou
     if ((rc = reduce_alloc(red, cgs_blocks(n), m > 0 ? m : 1))) return rc;
     HL_TRY_HIP(hipMalloc(&ctl, sizeof(Ctl)));
     HL_TRY_HIP(hipMalloc(&hout, sizeof(double) * (m > 0 ? m : 1)));
-    rc = launch_cgs(n, m, V, ldv, w_in, w_out, h_in, hout, red, ctl, nullptr, nullptr, EPI_NONE, 0, s);
+    rc = launch_cgs(n, m, V, VLayout{ldv, false}, w_in, w_out, h_in, hout, red, ctl, nullptr, nullptr, EPI_NONE, 0,
+                    s);
     if (!rc && h_out && m > 0)
         rc = hipMemcpyAsync(h_out, hout, sizeof(double) * m, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
     if (!rc && nrm2_out)
@@ -258,8 +259,8 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
     if (rc) return rc;
     DeviceGuard g(dev);
     Reduce red;
-    return launch_vq(n, K, N, V, ldv, Q, ldq, Out, ldo, use_mfma, -1, nullptr, 0.0, 0.0, red, nullptr,
-                     static_cast<hipStream_t>(stream));
+    return launch_vq(n, K, N, V, VLayout{ldv, false}, Q, ldq, Out, VLayout{ldo, false}, use_mfma, -1, nullptr, 0.0,
+                     0.0, red, nullptr, static_cast<hipStream_t>(stream));
 }
 
 int hl_profile_last(hl_profile *out)

@@ -67,6 +67,26 @@ enum Epi : int {
     EPI_RNORM = 5,   // rnorm = ||resid|| after start vector or restart
 };
 
+// ------------------------------------------------------------ basis layout
+// Storage of a basis V (n rows): column-major with leading dimension ld, or
+// "tiled": blocks of 16 consecutive rows, each block holding ld columns of 16
+// contiguous doubles, so a 16-row slice of V(:, 0:m) is one contiguous run.
+struct VLayout {
+    long long ld = 0;
+    bool tiled = false;
+};
+template <bool TILED>
+__host__ __device__ inline long long vpos(long long r, long long c, long long ld)
+{
+    return TILED ? (((r >> 4) * ld + c) << 4) + (r & 15) : c * ld + r;
+}
+// One column j of a basis, as a write target (base == nullptr: none).
+struct VColumn {
+    double *base = nullptr;
+    VLayout L;
+    int j = 0;
+};
+
 // ------------------------------------------------------------ SpMV plan
 // One work item of the adaptive CSR SpMV.  tpr > 0: a "stream" block of whole
 // short rows [row0, row1) covering nnz [p0, p1), reduced with tpr lanes per
@@ -101,12 +121,10 @@ int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *c
                    const double *val, int sym_lower, int val_prec);
 void csr_plan_free(CsrPlan &p);
 
-// y = A (x * xscale) with optional fused write vout[r] = x[r] * xscale.
+// y = A (x * xscale) with the optional fused write vcol(r) = x[r] * xscale.
 // xscale == nullptr means 1.  val_prec selects the FP64 or FP32 value array.
-int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double *y, double *vout,
+int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double *y, VColumn vcol,
                 int val_prec, hipStream_t s);
-int launch_spmv_f32x(const CsrPlan &p, const float *x, const double *xscale, double *y, float *vout,
-                     hipStream_t s);
 
 // ------------------------------------------------------- basis kernels
 struct Reduce {
@@ -121,30 +139,29 @@ int cgs_blocks(int n);
 
 // One fused CGS pass over an FP64 basis (see hl_cgs_pass).  mode selects the
 // epilogue; j is the Lanczos step for the alpha/beta bookkeeping.
-int launch_cgs(int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
+int launch_cgs(int n, int m, const double *V, VLayout L, const double *w_in, double *w_out,
                const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub,
                double *h_dia, int mode, int j, hipStream_t s);
 // Same over an FP32 basis (FP64 accumulation, FP64 w).
-int launch_cgs_f32(int n, int m, const float *V, long long ldv, const double *w_in, double *w_out,
+int launch_cgs_f32(int n, int m, const float *V, VLayout L, const double *w_in, double *w_out,
                    const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub,
                    double *h_dia, int mode, int j, hipStream_t s);
 
 // Out(:, 0:N) = V(:, 0:K) Q(0:K, 0:N); optionally fused residual update on
 // output column res_col: r = qlast * rscale * r + beta * Out(:, res_col) and
 // ctl->rnorm := ||r|| (EPI_RNORM).  res_col < 0 disables it.
-int launch_vq(int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq,
-              double *Out, long long ldo, int use_mfma, int res_col, double *r, double qlast,
+int launch_vq(int n, int K, int N, const double *V, VLayout Lin, const double *Q, int ldq,
+              double *Out, VLayout Lout, int use_mfma, int res_col, double *r, double qlast,
               double beta, Reduce &red, Ctl *ctl, hipStream_t s);
-int launch_vq_f32(int n, int K, int N, const float *V, long long ldv, const double *Q, int ldq,
-                  float *Out, long long ldo, int res_col, double *r, double qlast, double beta,
+int launch_vq_f32(int n, int K, int N, const float *V, VLayout Lin, const double *Q, int ldq,
+                  float *Out, VLayout Lout, int res_col, double *r, double qlast, double beta,
                   Reduce &red, Ctl *ctl, hipStream_t s);
 
 // r = start vector (counter-based splitmix64, seed) or keep r; then ctl->rnorm.
 int launch_start_vector(int n, uint64_t seed, double *r, hipStream_t s);
-// v = r * ctl->xscale (host-operator path normalisation).
-int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, hipStream_t s);
-int launch_scale_copy_f32(int n, const double *r, const Ctl *ctl, float *v, double *vd,
-                          hipStream_t s);
+// v = r * ctl->xscale into a contiguous vector v (may be null) and the basis
+// column col (host-operator path normalisation).
+int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn col, hipStream_t s);
 // r *= q * ctl->rscale (dsapps residual when beta_k = 0, HLanc.hpp:1605-1607).
 int launch_scale_rscale(int n, double *r, double q, const Ctl *ctl, hipStream_t s);
 // ||r||^2 -> epilogue (mode EPI_RNORM sets rnorm/xscale/rscale).

@@ -212,7 +212,7 @@ Workspace::~Workspace() { release(); }
 
 void Workspace::release()
 {
-    void *ptrs[] = {V[0], V[1], Vf[0], Vf[1], r, w, hsub, hdia, hbuf, Qd, ctl, vd_tmp};
+    void *ptrs[] = {V[0], V[1], Vf[0], Vf[1], r, w, hsub, hdia, hbuf, Qd, ctl, vd_tmp, xv};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctl_h) (void)hipHostFree(ctl_h);
@@ -223,7 +223,7 @@ void Workspace::release()
     if (stream) (void)hipStreamDestroy(stream);
     V[0] = V[1] = nullptr;
     Vf[0] = Vf[1] = nullptr;
-    r = w = hsub = hdia = hbuf = Qd = vd_tmp = nullptr;
+    r = w = hsub = hdia = hbuf = Qd = vd_tmp = xv = nullptr;
     ctl = nullptr;
     ctl_h = nullptr;
     hh = xh = yh = nullptr;
@@ -257,6 +257,7 @@ int Workspace::init(int dev_, int n_, int ncv_, int prec_, bool host_op)
     HL_TRY_HIP(hipHostMalloc(&ctl_h, sizeof(Ctl), hipHostMallocDefault));
     HL_TRY_HIP(hipHostMalloc(&hh, sizeof(double) * 2 * ncv, hipHostMallocDefault));
     if (host_op) {
+        HL_TRY_HIP(hipMalloc(&xv, sizeof(double) * (size_t)ldv));
         HL_TRY_HIP(hipHostMalloc(&xh, sizeof(double) * (size_t)n, hipHostMallocDefault));
         HL_TRY_HIP(hipHostMalloc(&yh, sizeof(double) * (size_t)n, hipHostMallocDefault));
     }
@@ -286,7 +287,7 @@ int Solver::apply_op(int j)
     ++spmv;
     if (A) {
         int rc = prof.begin(W.stream);
-        if (!rc) rc = launch_spmv(*A, W.r, &W.ctl->xscale, W.w, W.V[W.cur] + (size_t)j * W.ldv, HL_PREC_F64,
+        if (!rc) rc = launch_spmv(*A, W.r, &W.ctl->xscale, W.w, VColumn{W.V[W.cur], W.vl(), j}, HL_PREC_F64,
                                   W.stream);
         if (!rc) {
             const double b = 12.0 * A->nnz + 4.0 * (A->nrows + 1) + 8.0 * A->ncols + 16.0 * A->nrows;
@@ -295,10 +296,9 @@ int Solver::apply_op(int j)
         return rc;
     }
     // host operator: v_j on the device, then x/y through pinned host buffers
-    int rc = launch_scale_copy(W.n, W.r, W.ctl, W.V[W.cur] + (size_t)j * W.ldv, W.stream);
+    int rc = launch_scale_copy(W.n, W.r, W.ctl, W.xv, VColumn{W.V[W.cur], W.vl(), j}, W.stream);
     if (rc) return rc;
-    HL_TRY_HIP(hipMemcpyAsync(W.xh, W.V[W.cur] + (size_t)j * W.ldv, sizeof(double) * W.n,
-                              hipMemcpyDeviceToHost, W.stream));
+    HL_TRY_HIP(hipMemcpyAsync(W.xh, W.xv, sizeof(double) * W.n, hipMemcpyDeviceToHost, W.stream));
     HL_TRY_HIP(hipStreamSynchronize(W.stream));
     host_op(host_ctx, W.xh, W.yh);
     HL_TRY_HIP(hipMemcpyAsync(W.w, W.yh, sizeof(double) * W.n, hipMemcpyHostToDevice, W.stream));
@@ -335,7 +335,7 @@ int Solver::cgs_step(int m, const double *V, const double *w_in, double *w_out, 
     }
     int rc = prof.begin(W.stream);
     if (rc) return rc;
-    if ((rc = launch_cgs(W.n, m, V, W.ldv, w_in, w_out, h_in, h_out, W.red, W.ctl, W.hsub, W.hdia, mode, j,
+    if ((rc = launch_cgs(W.n, m, V, W.vl(), w_in, w_out, h_in, h_out, W.red, W.ctl, W.hsub, W.hdia, mode, j,
                          W.stream)))
         return rc;
     const double bytes = 8.0 * W.n * (m + 1 + (w_out ? 1 : 0));
@@ -383,11 +383,11 @@ int Solver::apply_restart(int nev, const double *q)
     int rc = prof.begin(W.stream);
     if (rc) return rc;
     if (has_beta) {
-        rc = launch_vq(W.n, ncv, N, W.V[W.cur], W.ldv, W.Qd, ncv, W.V[nxt], W.ldv, use_mfma, nev, W.r, qlast,
+        rc = launch_vq(W.n, ncv, N, W.V[W.cur], W.vl(), W.Qd, ncv, W.V[nxt], W.vl(), use_mfma, nev, W.r, qlast,
                        sub[nev], W.red, W.ctl, W.stream);
         if (!rc) rc = prof.end(W.stream, K_VQ, 8.0 * W.n * (ncv + N + 2));
     } else {
-        rc = launch_vq(W.n, ncv, N, W.V[W.cur], W.ldv, W.Qd, ncv, W.V[nxt], W.ldv, use_mfma, -1, nullptr, 0.0,
+        rc = launch_vq(W.n, ncv, N, W.V[W.cur], W.vl(), W.Qd, ncv, W.V[nxt], W.vl(), use_mfma, -1, nullptr, 0.0,
                        0.0, W.red, W.ctl, W.stream);
         if (!rc) rc = prof.end(W.stream, K_VQ, 8.0 * W.n * (ncv + N));
         if (!rc) rc = launch_scale_rscale(W.n, W.r, qlast, W.ctl, W.stream);
@@ -520,8 +520,8 @@ int Solver::run(int nev0, double *values, double *vectors, int ldv_host, const h
                                   W.stream));
         const int nxt = 1 - W.cur;
         if ((rc = prof.begin(W.stream))) return rc;
-        if ((rc = launch_vq(n, ncv, ncv, W.V[W.cur], W.ldv, W.Qd, ncv, W.V[nxt], W.ldv, use_mfma, -1, nullptr,
-                            0.0, 0.0, W.red, W.ctl, W.stream)))
+        if ((rc = launch_vq(n, ncv, ncv, W.V[W.cur], W.vl(), W.Qd, ncv, W.V[nxt], VLayout{W.ldv, false}, use_mfma,
+                            -1, nullptr, 0.0, 0.0, W.red, W.ctl, W.stream)))
             return rc;
         if ((rc = prof.end(W.stream, K_VQ, 16.0 * n * ncv))) return rc;
         // sync_host (HLanc.hpp:1063-1071): all ncv columns to the caller

@@ -17,11 +17,14 @@ struct Workspace {
     int cur = 0;
     double *r = nullptr, *w = nullptr;  // residual and A v_j
     double *hsub = nullptr, *hdia = nullptr, *hbuf = nullptr, *Qd = nullptr, *vd_tmp = nullptr;
+    double *xv = nullptr;               // contiguous v_j for host operators
     Ctl *ctl = nullptr;
     Ctl *ctl_h = nullptr;               // pinned mirrors
     double *hh = nullptr, *xh = nullptr, *yh = nullptr;
     Reduce red;
 
+    // The ping-pong bases are tiled (16-row blocks of ncv columns).
+    VLayout vl() const { return VLayout{ncv, true}; }
     ~Workspace();
     int init(int dev, int n, int ncv, int prec, bool host_op);
     void release();

@@ -30,12 +30,17 @@ __device__ inline double block_sum_to_thread0(double v, double *wred)
     return t;
 }
 
+__device__ inline long long vcol_pos(long long r, long long ld, int j, bool tiled)
+{
+    return tiled ? vpos<true>(r, j, ld) : vpos<false>(r, j, ld);
+}
+
 template <typename AT, typename XT>
 __global__ __launch_bounds__(kBlock) void spmv_adaptive(
     const SpmvBlk *__restrict__ blks, const int *__restrict__ rp, const int *__restrict__ col,
     const AT *__restrict__ val, const XT *__restrict__ x, const double *__restrict__ xscale,
-    double *__restrict__ y, XT *__restrict__ vout, double *__restrict__ chunk_part,
-    unsigned *__restrict__ chunk_cnt)
+    double *__restrict__ y, XT *__restrict__ vout, long long vld, int vj, bool vtiled,
+    double *__restrict__ chunk_part, unsigned *__restrict__ chunk_cnt)
 {
     __shared__ double prod[kNnzPerBlock];
     __shared__ double wred[kBlock / kWave];
@@ -51,7 +56,7 @@ __global__ __launch_bounds__(kBlock) void spmv_adaptive(
         }
         if (vout) {
             for (int r = b.row0 + tid; r < b.row1; r += kBlock)
-                vout[r] = static_cast<XT>(static_cast<double>(x[r]) * s);
+                vout[vcol_pos(r, vld, vj, vtiled)] = static_cast<XT>(static_cast<double>(x[r]) * s);
         }
         __syncthreads();
         const int tpr = b.tpr;
@@ -78,7 +83,8 @@ __global__ __launch_bounds__(kBlock) void spmv_adaptive(
         acc += static_cast<double>(val[p]) * (static_cast<double>(x[col[p]]) * s);
     const double tot = block_sum_to_thread0(acc, wred);
     if (tid != 0) return;
-    if (vout && b.chunk == 0) vout[b.row0] = static_cast<XT>(static_cast<double>(x[b.row0]) * s);
+    if (vout && b.chunk == 0)
+        vout[vcol_pos(b.row0, vld, vj, vtiled)] = static_cast<XT>(static_cast<double>(x[b.row0]) * s);
     if (b.nchunks == 1) {
         y[b.row0] = tot;
         return;
@@ -237,7 +243,7 @@ void csr_plan_free(CsrPlan &p)
     p = CsrPlan{};
 }
 
-int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double *y, double *vout,
+int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double *y, VColumn vcol,
                 int val_prec, hipStream_t s)
 {
     if (p.nblk == 0) return HL_OK;
@@ -246,26 +252,14 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
             set_error("SpMV: FP32 values were not requested at creation");
             return HL_ERR_STATE;
         }
-        spmv_adaptive<float, double><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val32, x, xscale,
-                                                                y, vout, p.chunk_part, p.chunk_cnt);
+        spmv_adaptive<float, double><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val32, x, xscale, y,
+                                                                vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled,
+                                                                p.chunk_part, p.chunk_cnt);
     } else {
-        spmv_adaptive<double, double><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val, x, xscale,
-                                                                 y, vout, p.chunk_part, p.chunk_cnt);
+        spmv_adaptive<double, double><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val, x, xscale, y,
+                                                                 vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled,
+                                                                 p.chunk_part, p.chunk_cnt);
     }
-    HL_TRY_HIP(hipGetLastError());
-    return HL_OK;
-}
-
-int launch_spmv_f32x(const CsrPlan &p, const float *x, const double *xscale, double *y, float *vout,
-                     hipStream_t s)
-{
-    if (p.nblk == 0) return HL_OK;
-    if (!p.val32) {
-        set_error("SpMV: FP32 values were not requested at creation");
-        return HL_ERR_STATE;
-    }
-    spmv_adaptive<float, float><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val32, x, xscale, y,
-                                                          vout, p.chunk_part, p.chunk_cnt);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }
