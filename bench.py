@@ -167,11 +167,11 @@ def main():
                                  ctypes.byref(o), ctypes.byref(info))
     pr = H.Profile()
     H.lib().hl_profile_last(ctypes.byref(pr))
-    names = ["spmv", "cgs", "vq", "other"]
+    names = H.PROFILE_CLASSES
     classes = {names[k]: {"launches": int(pr.launches[k]), "seconds": pr.seconds[k], "bytes": pr.bytes[k],
                           "GBps": (pr.bytes[k] / pr.seconds[k] / 1e9) if pr.seconds[k] > 0 else None}
-               for k in range(4)}
-    dom = max(range(3), key=lambda k: pr.seconds[k])
+               for k in range(len(names)) if pr.launches[k] > 0}
+    dom = max(range(len(names)), key=lambda k: pr.seconds[k])
     log(f"profile (ret {rc}): " + ", ".join(f"{k} {v['launches']}x {v['seconds']:.3f}s {v['GBps'] or 0:.0f} GB/s"
                                            for k, v in classes.items()))
     achieved = pr.bytes[dom] / pr.seconds[dom] / 1e9

@@ -55,11 +55,14 @@ class Info(ctypes.Structure):
 
 
 class Profile(ctypes.Structure):
-    """hl_profile: per kernel class (0 SpMV, 1 CGS, 2 V.Q, 3 other) launches, seconds, bytes."""
-    _fields_ = [("launches", ctypes.c_longlong * 4), ("seconds", ctypes.c_double * 4), ("bytes", ctypes.c_double * 4)]
+    """hl_profile: per kernel class (0 SpMV, 1 CGS, 2 V.Q, 3 other, 4 fused SpMV + first CGS half)
+    launches, seconds, algorithmic bytes."""
+    _fields_ = [("launches", ctypes.c_longlong * 6), ("seconds", ctypes.c_double * 6), ("bytes", ctypes.c_double * 6)]
 
 
+PROFILE_CLASSES = ["spmv", "cgs", "vq", "other", "spmv_proj"]
 HL_FLAG_PROFILE = 2
+HL_FLAG_NO_FUSE = 4
 
 _lib = None
 
@@ -85,6 +88,10 @@ _SIGS = [
     ("hl_vq", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                              ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
                              ctypes.c_int, ctypes.c_void_p]),
+    ("hl_tune_set", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    ("hl_tune_get", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    ("hl_bench_cgs", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_double)]),
     ("hl_options_default", None, [ctypes.POINTER(Options)]),
     ("hl_profile_last", ctypes.c_int, [ctypes.POINTER(Profile)]),
     ("hl_ds_solve_csr", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
@@ -299,7 +306,8 @@ def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mf
 
 
 def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, v0=None,
-                  seed: int = 0, use_mfma: Optional[bool] = None, ldv: Optional[int] = None) -> SolveResult:
+                  seed: int = 0, use_mfma: Optional[bool] = None, ldv: Optional[int] = None,
+                  flags: int = 0) -> SolveResult:
     """ds_solver_gpu (HLanc.hpp:1868-1872).  ``op`` is a dcsrmv_operator_gpu
     (device-resident path) or any callable ``op(x, y)`` on numpy arrays (host
     path, as the reference calls arbitrary operators with host pointers).
@@ -308,7 +316,7 @@ def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 
     ldv = n if ldv is None else ldv
     values = np.zeros(ncv)
     vectors = np.zeros((ncv, ldv)).T  # column-major ldv x ncv
-    o = options(tol, maxiter, v0, seed, use_mfma)
+    o = options(tol, maxiter, v0, seed, use_mfma, flags=flags)
     info = Info()
     if isinstance(op, dcsrmv_operator_gpu):
         if op.nrows != n:
@@ -338,3 +346,22 @@ def cgs_pass(dev, n, m, V_ptr, ldv, w_in_ptr, w_out_ptr, h_in_ptr, h_out_ptr, nr
 def vq(dev, n, K, N, V_ptr, ldv, Q_ptr, ldq, Out_ptr, ldo, use_mfma=True, stream=0):
     """Out = V Q on device pointers (hl_vq)."""
     _check(lib().hl_vq(dev, n, K, N, V_ptr, ldv, Q_ptr, ldq, Out_ptr, ldo, int(use_mfma), stream))
+
+
+# ------------------------------------------------------------------ tuning
+
+def tune_set(key: str, value: int) -> None:
+    _check(lib().hl_tune_set(key.encode(), int(value)))
+
+
+def tune_get(key: str) -> int:
+    v = ctypes.c_int()
+    _check(lib().hl_tune_get(key.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def bench_cgs(n: int, m: int, tiled: bool = True, reps: int = 20, dev: int = 0) -> float:
+    """Seconds per launch of one fused CGS pass over an n x m FP64 basis."""
+    t = ctypes.c_double()
+    _check(lib().hl_bench_cgs(dev, n, m, int(tiled), reps, ctypes.byref(t)))
+    return t.value

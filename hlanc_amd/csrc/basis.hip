@@ -16,118 +16,24 @@
 //     update r = qlast r + beta Out(:, k) with ||r||.
 #include <cmath>
 
+#include "hl_device.hpp"
 #include "hl_internal.hpp"
 
 namespace hl {
 namespace {
+using namespace dev;
 
-constexpr int kGroups = 4;                 // column groups per wave
-constexpr int kTileRows = kWave / kGroups; // 16 rows per wave tile
-
-__device__ inline void run_epilogue(int mode, int j, double nrm2, const double *h_in, Ctl *ctl,
-                                    double *h_sub, double *h_dia)
+template <typename T, bool NT>
+__device__ __forceinline__ T ld_stream(const T *p)
 {
-    switch (mode) {
-    case EPI_NONE:
-        ctl->nrm2 = nrm2;
-        break;
-    case EPI_WNORM:
-        ctl->wnorm = sqrt(nrm2);
-        break;
-    case EPI_PASS1: // HLanc.hpp:1431-1440
-        h_dia[j] = h_in[j];
-        h_sub[j] = (j == 0) ? 0.0 : ctl->rnorm;
-        ctl->rnorm = sqrt(nrm2);
-        ctl->need = ctl->rnorm < kDgks * ctl->wnorm;
-        ctl->iter = 0;
-        ctl->cgs += 1;
-        break;
-    case EPI_PASS2: { // HLanc.hpp:1444-1458, first correction
-        h_dia[j] += h_in[j];
-        const double r1 = sqrt(nrm2);
-        if (r1 > kDgks * ctl->rnorm) {
-            ctl->rnorm = r1;
-            ctl->need = 0;
-        } else {
-            ctl->rnorm = r1;
-            ctl->iter = 1;
-            ctl->need = r1 < kDgks * ctl->wnorm;
-        }
-        ctl->cgs += 1;
-        break;
-    }
-    case EPI_PASS3: { // second correction; failure zeroes resid (HLanc.hpp:1459-1463)
-        h_dia[j] += h_in[j];
-        const double r1 = sqrt(nrm2);
-        if (r1 > kDgks * ctl->rnorm) {
-            ctl->rnorm = r1;
-        } else {
-            ctl->rnorm = 0.0;
-            ctl->rscale = 0.0;
-        }
-        ctl->need = 0;
-        ctl->cgs += 1;
-        break;
-    }
-    case EPI_RNORM:
-        ctl->rnorm = sqrt(nrm2);
-        ctl->rscale = 1.0;
-        break;
-    default:
-        break;
-    }
-    if (mode >= EPI_PASS1) // STEP 2's scale for v_{j+1} (HLanc.hpp:1412-1419)
-        ctl->xscale = (ctl->rnorm > kSafmin ? 1.0 / ctl->rnorm : 1.0) * ctl->rscale;
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
 }
 
-// Block partials of M1 values (already in LDS vals[0..M1)) -> global partials;
-// the last block to arrive reduces all of them in block order into fin[0..M1)
-// (LDS) and returns true.
-__device__ inline bool reduce_blocks(const double *vals, int M1, double *partials, unsigned *counter,
-                                     double *scratch /* LDS kBlock */, double *fin /* LDS >= M1 */)
-{
-    __shared__ int is_last;
-    const int tid = threadIdx.x;
-    for (int c = tid; c < M1; c += kBlock)
-        __hip_atomic_store(&partials[(long long)blockIdx.x * M1 + c], vals[c], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) {
-        const unsigned prev =
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev + 1 == gridDim.x);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    const int nb = gridDim.x;
-    const int nsplit = kBlock / M1 > 0 ? kBlock / M1 : 1;
-    for (int c0 = 0; c0 < M1; c0 += kBlock) {
-        const int span = (M1 - c0) < kBlock ? (M1 - c0) : kBlock;
-        const int ns = span == M1 ? nsplit : 1;
-        const int c = c0 + tid % span;
-        const int part = tid / span;
-        double s = 0.0;
-        if (part < ns) {
-            for (int b = part; b < nb; b += ns)
-                s += __hip_atomic_load(&partials[(long long)b * M1 + c], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-        }
-        scratch[tid] = s;
-        __syncthreads();
-        if (tid < span) {
-            double t = 0.0;
-            for (int q = 0; q < ns; ++q) t += scratch[q * span + tid];
-            fin[c0 + tid] = t;
-        }
-        __syncthreads();
-    }
-    if (tid == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
-}
-
-template <int MAXC, typename VT, bool TILED>
+// FL bit 0: non-temporal basis loads; bit 1: software prefetch of the next tile.
+template <int MAXC, typename VT, bool TILED, int FL>
 __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__restrict__ V, long long ldv,
                                                      const double *w_in, double *w_out,
                                                      const double *__restrict__ h_in, double *h_out,
@@ -170,19 +76,21 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
         {                                                                                        \
             const int colj = c0 + c;                                                             \
             vn[c] = (okk && c < cpg && colj < m)                                                 \
-                        ? static_cast<double>(__builtin_nontemporal_load(&V[vpos<TILED>(rr, colj, ldv)])) \
+                        ? static_cast<double>(ld_stream<VT, (FL & 1) != 0>(&V[vpos<TILED>(rr, colj, ldv)])) \
                         : 0.0;                                                                   \
         }                                                                                        \
     }
-    HL_CGS_FETCH(t);
+    constexpr bool PF = (FL & 2) != 0;
+    if (PF) HL_CGS_FETCH(t);
     for (; t < ntiles; t += W) {
+        if (!PF) HL_CGS_FETCH(t);
         double v[MAXC];
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) v[c] = vn[c];
         const double win = wn;
         const long long r = t * kTileRows + rl;
         const bool ok = r < n;
-        HL_CGS_FETCH(t + W);
+        if (PF) HL_CGS_FETCH(t + W);
         double a = win;
         if (has_h) {
             double p = 0.0;
@@ -437,22 +345,34 @@ int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, d
         return HL_ERR_STATE;
     }
     const int cpg = (m + kGroups - 1) / kGroups;
-#define HL_CGS(MC)                                                                                             \
-    cgs_kernel<MC, VT, TILED><<<nb, kBlock, 0, s>>>(n, m, V, ldv, w_in, w_out, h_in, h_out, red.partials,    \
-                                                    red.counter, ctl, h_sub, h_dia, mode, j)
-    if (cpg <= 2)
+    const int fl = tuning().cgs_flags & 3;
+#define HL_CGS1(MC, F)                                                                                         \
+    cgs_kernel<MC, VT, TILED, F><<<nb, kBlock, 0, s>>>(n, m, V, ldv, w_in, w_out, h_in, h_out, red.partials, \
+                                                       red.counter, ctl, h_sub, h_dia, mode, j)
+#define HL_CGS(MC)              \
+    if (fl == 0)                \
+        HL_CGS1(MC, 0);         \
+    else if (fl == 1)           \
+        HL_CGS1(MC, 1);         \
+    else if (fl == 2)           \
+        HL_CGS1(MC, 2);         \
+    else                        \
+        HL_CGS1(MC, 3)
+    if (cpg <= 2) {
         HL_CGS(2);
-    else if (cpg <= 4)
+    } else if (cpg <= 4) {
         HL_CGS(4);
-    else if (cpg <= 8)
+    } else if (cpg <= 8) {
         HL_CGS(8);
-    else if (cpg <= 16)
+    } else if (cpg <= 16) {
         HL_CGS(16);
-    else if (cpg <= 24)
+    } else if (cpg <= 24) {
         HL_CGS(24);
-    else
+    } else {
         HL_CGS(32);
+    }
 #undef HL_CGS
+#undef HL_CGS1
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }
@@ -521,7 +441,13 @@ int launch_vq_mfma(int n, int K, int N, const double *V, long long ldv, const do
 int cgs_blocks(int n)
 {
     const long long ntiles = (static_cast<long long>(n) + kTileRows - 1) / kTileRows;
-    return grid_for(ntiles, 16, 1024);
+    return grid_for(ntiles, 16, tuning().cgs_blocks_max);
+}
+
+Tuning &tuning()
+{
+    static Tuning t;
+    return t;
 }
 
 int reduce_alloc(Reduce &r, int max_blocks, int max_cols)

@@ -216,6 +216,32 @@ double hl_csr_spmv_bytes(hl_csr_t h, int val_prec)
            8.0 * h->plan.nrows;
 }
 
+namespace {
+// Reduction workspace of the standalone basis entry points, one per host
+// thread and device, kept for the life of the process so that the calls are
+// asynchronous and cheap to repeat.
+struct PassWs {
+    Reduce red;
+    Ctl *ctl = nullptr;
+    double *hout = nullptr;
+};
+int pass_ws(int dev, PassWs *&out)
+{
+    static thread_local PassWs ws[16];
+    if (dev < 0 || dev >= 16) return HL_ERR_ARG;
+    PassWs &w = ws[dev];
+    if (!w.ctl) {
+        int rc = reduce_alloc(w.red, kMaxReduceBlocks, 128);
+        if (rc) return rc;
+        HL_TRY_HIP(hipMalloc(&w.ctl, sizeof(Ctl)));
+        HL_TRY_HIP(hipMemset(w.ctl, 0, sizeof(Ctl)));
+        HL_TRY_HIP(hipMalloc(&w.hout, sizeof(double) * 128));
+    }
+    out = &w;
+    return HL_OK;
+}
+} // namespace
+
 int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
                 const double *h_in, double *h_out, double *nrm2_out, void *stream)
 {
@@ -227,25 +253,89 @@ int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const dou
     if (rc) return rc;
     DeviceGuard g(dev);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    Reduce red;
-    Ctl *ctl = nullptr;
-    double *hout = nullptr;
-    if ((rc = reduce_alloc(red, cgs_blocks(n), m > 0 ? m : 1))) return rc;
-    HL_TRY_HIP(hipMalloc(&ctl, sizeof(Ctl)));
-    HL_TRY_HIP(hipMalloc(&hout, sizeof(double) * (m > 0 ? m : 1)));
-    rc = launch_cgs(n, m, V, VLayout{ldv, false}, w_in, w_out, h_in, hout, red, ctl, nullptr, nullptr, EPI_NONE, 0,
-                    s);
-    if (!rc && h_out && m > 0)
-        rc = hipMemcpyAsync(h_out, hout, sizeof(double) * m, hipMemcpyDeviceToDevice, s) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
-    if (!rc && nrm2_out)
-        rc = hipMemcpyAsync(nrm2_out, &ctl->nrm2, sizeof(double), hipMemcpyDeviceToDevice, s) == hipSuccess
-                 ? 0
-                 : HL_ERR_HIP_BASE;
-    (void)hipStreamSynchronize(s);
-    (void)hipFree(hout);
-    (void)hipFree(ctl);
-    reduce_free(red);
+    PassWs *ws = nullptr;
+    if ((rc = pass_ws(dev, ws))) return rc;
+    rc = launch_cgs(n, m, V, VLayout{ldv, false}, w_in, w_out, h_in, ws->hout, ws->red, ws->ctl, nullptr, nullptr,
+                    EPI_NONE, 0, s);
+    if (rc) return rc;
+    if (h_out && m > 0)
+        HL_TRY_HIP(hipMemcpyAsync(h_out, ws->hout, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+    if (nrm2_out) HL_TRY_HIP(hipMemcpyAsync(nrm2_out, &ws->ctl->nrm2, sizeof(double), hipMemcpyDeviceToDevice, s));
+    return HL_OK;
+}
+
+int hl_bench_cgs(int dev, int n, int m, int tiled, int reps, double *seconds)
+{
+    if (n <= 0 || m <= 0 || m > 128 || reps <= 0 || !seconds) return HL_ERR_ARG;
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    PassWs *ws = nullptr;
+    if ((rc = pass_ws(dev, ws))) return rc;
+    const long long ld = ((long long)n + 63) / 64 * 64;
+    double *V = nullptr, *w = nullptr, *w2 = nullptr, *h = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    HL_TRY_HIP(hipMalloc(&V, sizeof(double) * ld * m));
+    HL_TRY_HIP(hipMalloc(&w, sizeof(double) * ld));
+    HL_TRY_HIP(hipMalloc(&w2, sizeof(double) * ld));
+    HL_TRY_HIP(hipMalloc(&h, sizeof(double) * 128));
+    HL_TRY_HIP(hipMemset(V, 0, sizeof(double) * ld * m));
+    HL_TRY_HIP(hipMemset(w, 0, sizeof(double) * ld));
+    HL_TRY_HIP(hipMemset(h, 0, sizeof(double) * 128));
+    HL_TRY_HIP(hipEventCreate(&a));
+    HL_TRY_HIP(hipEventCreate(&b));
+    const VLayout L = tiled ? VLayout{m, true} : VLayout{ld, false};
+    for (int i = 0; i < 3 && !rc; ++i)
+        rc = launch_cgs(n, m, V, L, w, w2, h, ws->hout, ws->red, ws->ctl, nullptr, nullptr, EPI_NONE, 0, nullptr);
+    if (!rc) rc = hipEventRecord(a, nullptr) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    for (int i = 0; i < reps && !rc; ++i)
+        rc = launch_cgs(n, m, V, L, w, w2, h, ws->hout, ws->red, ws->ctl, nullptr, nullptr, EPI_NONE, 0, nullptr);
+    float ms = 0.0f;
+    if (!rc) rc = hipEventRecord(b, nullptr) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    if (!rc) rc = hipEventSynchronize(b) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    if (!rc) rc = hipEventElapsedTime(&ms, a, b) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    *seconds = 1e-3 * ms / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(V);
+    (void)hipFree(w);
+    (void)hipFree(w2);
+    (void)hipFree(h);
     return rc;
+}
+
+int hl_tune_set(const char *key, int value)
+{
+    if (!key) return HL_ERR_ARG;
+    Tuning &t = tuning();
+    if (!std::strcmp(key, "cgs_blocks_max") && value >= 1 && value <= kMaxReduceBlocks)
+        t.cgs_blocks_max = value;
+    else if (!std::strcmp(key, "cgs_flags") && value >= 0 && value <= 3)
+        t.cgs_flags = value;
+    else if (!std::strcmp(key, "basis_tiled") && (value == 0 || value == 1))
+        t.basis_tiled = value;
+    else {
+        set_error("hl_tune_set: unknown key or value out of range: %s=%d", key, value);
+        return HL_ERR_ARG;
+    }
+    return HL_OK;
+}
+
+int hl_tune_get(const char *key, int *value)
+{
+    if (!key || !value) return HL_ERR_ARG;
+    const Tuning &t = tuning();
+    if (!std::strcmp(key, "cgs_blocks_max"))
+        *value = t.cgs_blocks_max;
+    else if (!std::strcmp(key, "cgs_flags"))
+        *value = t.cgs_flags;
+    else if (!std::strcmp(key, "basis_tiled"))
+        *value = t.basis_tiled;
+    else {
+        set_error("hl_tune_get: unknown key %s", key);
+        return HL_ERR_ARG;
+    }
+    return HL_OK;
 }
 
 int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq, double *Out,

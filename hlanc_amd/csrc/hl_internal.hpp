@@ -67,6 +67,16 @@ enum Epi : int {
     EPI_RNORM = 5,   // rnorm = ||resid|| after start vector or restart
 };
 
+// ------------------------------------------------------------ tuning
+// Process-wide kernel parameters (hl_tune_set); fixed while solves run.
+struct Tuning {
+    int cgs_blocks_max = 1024; // grid cap of the CGS passes (<= kMaxReduceBlocks)
+    int cgs_flags = 3;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile
+    int basis_tiled = 1;       // solver basis layout: 1 tiled, 0 column-major
+};
+Tuning &tuning();
+constexpr int kMaxReduceBlocks = 2048;
+
 // ------------------------------------------------------------ basis layout
 // Storage of a basis V (n rows): column-major with leading dimension ld, or
 // "tiled": blocks of 16 consecutive rows, each block holding ld columns of 16
@@ -100,6 +110,8 @@ struct SpmvBlk {
 constexpr int kNnzPerBlock = 2048;  // stream block capacity (8 nnz per thread)
 constexpr int kRowsPerBlock = 256;
 constexpr int kLongChunk = 8192;    // nnz per chunk of a long row
+constexpr int kPersistBlocks = 1024; // grid of the fused SpMV + projection kernel
+constexpr double kProjRowBytes = 256.0; // cost weight of projecting one row (~32 FP64 columns)
 
 struct CsrPlan {
     int dev = 0;
@@ -115,6 +127,10 @@ struct CsrPlan {
     unsigned *chunk_cnt = nullptr;
     int nslots = 0;
     double *x_stage = nullptr, *y_stage = nullptr;  // for host-pointer calls
+    int *pfirst = nullptr;    // device [npers+1]: item ranges of the persistent fused kernel
+    int npers = 0;
+    int *long_rows = nullptr; // device: rows split into chunk items, ascending
+    int nlong = 0;
 };
 
 int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *col,
@@ -127,6 +143,12 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
                 int val_prec, hipStream_t s);
 
 // ------------------------------------------------------- basis kernels
+struct Reduce;
+// Fused head of a Lanczos step on a tiled basis: y = A (x s), V(:, j) = x s,
+// h_out = V(:, 0:j+1)^T y, ctl->wnorm = ||y|| (EPI_WNORM).
+int launch_spmv_project(const CsrPlan &p, const double *x, const double *xscale, double *y, double *V, VLayout L,
+                        int j, double *h_out, Reduce &red, Ctl *ctl, hipStream_t s);
+
 struct Reduce {
     double *partials = nullptr;  // [nblocks_max * (ncv_max + 1)]
     unsigned *counter = nullptr; // one counter, self-resetting
@@ -170,7 +192,7 @@ int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStre
 int launch_cast_f32(long long n, const double *in, float *out, hipStream_t s);
 
 // ------------------------------------------------------------ profiling
-enum KClass : int { K_SPMV = 0, K_CGS = 1, K_VQ = 2, K_MISC = 3, K_NCLASS = 4 };
+enum KClass : int { K_SPMV = 0, K_CGS = 1, K_VQ = 2, K_MISC = 3, K_SPMV_PROJ = 4, K_NCLASS = 6 };
 
 // HIP-event timing of individual launches on one stream (profiled solves only).
 struct Profiler {

@@ -237,6 +237,7 @@ int Workspace::init(int dev_, int n_, int ncv_, int prec_, bool host_op)
     ncv = ncv_;
     prec = prec_;
     ldv = ((long long)n + 63) / 64 * 64; // 512-byte aligned columns
+    tiled = tuning().basis_tiled != 0;
     HL_TRY_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     const size_t vbytes = sizeof(double) * (size_t)ldv * ncv;
     if (prec == HL_PREC_F32) {
@@ -261,7 +262,7 @@ int Workspace::init(int dev_, int n_, int ncv_, int prec_, bool host_op)
         HL_TRY_HIP(hipHostMalloc(&xh, sizeof(double) * (size_t)n, hipHostMallocDefault));
         HL_TRY_HIP(hipHostMalloc(&yh, sizeof(double) * (size_t)n, hipHostMallocDefault));
     }
-    const int nb = std::max(cgs_blocks(n), 1024);
+    const int nb = kMaxReduceBlocks;
     int rc = reduce_alloc(red, nb, ncv);
     if (rc) return rc;
     HL_TRY_HIP(hipMemsetAsync(hsub, 0, sizeof(double) * ncv, stream));
@@ -310,12 +311,22 @@ int Solver::apply_op(int j)
 int Solver::step(int j)
 {
     Workspace &W = ws;
-    int rc = apply_op(j);
-    if (rc) return rc;
     const double *V = W.V[W.cur];
     double *h0 = W.hbuf, *h1 = W.hbuf + W.ncv;
     const int m = j + 1;
-    if ((rc = cgs_step(m, V, W.w, nullptr, nullptr, h0, EPI_WNORM, j))) return rc;
+    int rc;
+    if (A && fuse && W.tiled) { // SpMV and the first CGS half in one persistent kernel
+        ++spmv;
+        if ((rc = prof.begin(W.stream))) return rc;
+        if ((rc = launch_spmv_project(*A, W.r, &W.ctl->xscale, W.w, W.V[W.cur], W.vl(), j, h0, W.red, W.ctl,
+                                      W.stream)))
+            return rc;
+        const double b = 12.0 * A->nnz + 4.0 * (A->nrows + 1) + 8.0 * A->ncols + 16.0 * A->nrows + 8.0 * W.n * j;
+        if ((rc = prof.end(W.stream, K_SPMV_PROJ, b))) return rc;
+    } else {
+        if ((rc = apply_op(j))) return rc;
+        if ((rc = cgs_step(m, V, W.w, nullptr, nullptr, h0, EPI_WNORM, j))) return rc;
+    }
     if ((rc = cgs_step(m, V, W.w, W.r, h0, h1, EPI_PASS1, j))) return rc;
     if ((rc = cgs_step(m, V, W.r, W.r, h1, h0, EPI_PASS2, j))) return rc;
     return cgs_step(m, V, W.r, W.r, h0, nullptr, EPI_PASS3, j);
@@ -410,6 +421,7 @@ int Solver::run(int nev0, double *values, double *vectors, int ldv_host, const h
     const int maxiter = o.maxiter > 0 ? o.maxiter : 200;
     const double eps23 = std::pow(kEps, 2.0 / 3.0);
     use_mfma = o.use_mfma;
+    fuse = (o.flags & HL_FLAG_NO_FUSE) == 0;
     prof.on = (o.flags & HL_FLAG_PROFILE) != 0;
     prof.tot = hl_profile{};
     sub.assign(ncv, 0.0);

@@ -24,7 +24,8 @@ struct Workspace {
     Reduce red;
 
     // The ping-pong bases are tiled (16-row blocks of ncv columns).
-    VLayout vl() const { return VLayout{ncv, true}; }
+    bool tiled = true;                  // fixed at init from tuning().basis_tiled
+    VLayout vl() const { return tiled ? VLayout{ncv, true} : VLayout{ldv, false}; }
     ~Workspace();
     int init(int dev, int n, int ncv, int prec, bool host_op);
     void release();
@@ -36,6 +37,7 @@ struct Solver {
     hl_host_op_fn host_op = nullptr;     // host callback
     void *host_ctx = nullptr;
     int use_mfma = 1;
+    bool fuse = true;                    // fused SpMV + first CGS half (CSR operator)
     std::vector<double> sub, dia;        // host copy of T: sub[j] = beta_j, dia[j] = alpha_j
     double rnorm = 0.0;
     long long spmv = 0, cgs = 0;

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "hl_device.hpp"
 #include "hl_internal.hpp"
 
 namespace hl {
@@ -99,6 +100,168 @@ __global__ __launch_bounds__(kBlock) void spmv_adaptive(
         y[b.row0] = t;
         __hip_atomic_store(&chunk_cnt[b.slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// Fused Lanczos step head: y = A (x s) (the SpMV), V(:, j) = x s, and the first
+// CGS half over the tiled basis, h = V(:, 0:j+1)^T y and ||y||^2, reduced
+// deterministically with the EPI_WNORM epilogue.  Persistent: block b handles
+// the work items [pfirst[b], pfirst[b+1]); each stream item projects its own
+// rows right after computing them (while they are in LDS), so y and the basis
+// rows of that item are read once.  Rows split into chunk items are projected
+// by the last block, in the fixed order of long_rows.
+template <int MAXC, typename AT>
+__global__ __launch_bounds__(kBlock) void spmv_project_kernel(
+    const SpmvBlk *__restrict__ blks, const int *__restrict__ pfirst, const int *__restrict__ rp,
+    const int *__restrict__ col, const AT *__restrict__ val, const double *__restrict__ x,
+    const double *__restrict__ xscale, double *__restrict__ y, double *__restrict__ V, long long vld, int j,
+    double *__restrict__ chunk_part, unsigned *__restrict__ chunk_cnt, const int *__restrict__ long_rows,
+    int nlong, double *__restrict__ h_out, double *partials, unsigned *counter, Ctl *ctl)
+{
+    __shared__ double prod[kNnzPerBlock];
+    __shared__ double ys[kRowsPerBlock], xs[kRowsPerBlock];
+    __shared__ double wred[kBlock / kWave];
+    __shared__ double red[kBlock / kWave][dev::kGroups * MAXC + 1];
+    __shared__ double vals[dev::kGroups * MAXC + 1];
+    __shared__ double scratch[kBlock];
+    __shared__ double fin[dev::kGroups * MAXC + 1];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wv = tid / kWave;
+    const int g = lane / dev::kTileRows;
+    const int rl = lane % dev::kTileRows;
+    const int m = j + 1;
+    const int cpg = (m + dev::kGroups - 1) / dev::kGroups;
+    const int c0 = g * cpg;
+    const double s = *xscale;
+    double acc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) acc[c] = 0.0;
+    double nrm = 0.0;
+
+    for (int it = pfirst[blockIdx.x]; it < pfirst[blockIdx.x + 1]; ++it) {
+        const SpmvBlk b = blks[it];
+        if (b.tpr > 0) {
+            const int nnz = b.p1 - b.p0;
+            for (int i = tid; i < nnz; i += kBlock) {
+                const int p = b.p0 + i;
+                prod[i] = static_cast<double>(__builtin_nontemporal_load(&val[p])) *
+                          (x[__builtin_nontemporal_load(&col[p])] * s);
+            }
+            for (int r = b.row0 + tid; r < b.row1; r += kBlock) {
+                const double xv = x[r] * s;
+                V[vpos<true>(r, j, vld)] = xv;
+                xs[r - b.row0] = xv;
+            }
+            __syncthreads();
+            const int tpr = b.tpr;
+            const int nrows = b.row1 - b.row0;
+            const int lr = tid / tpr, ll = tid & (tpr - 1);
+            const int rstep = kBlock / tpr;
+            for (int base = 0; base < nrows; base += rstep) {
+                const int rr = base + lr;
+                double a = 0.0;
+                if (rr < nrows) {
+                    const int r = b.row0 + rr;
+                    const int a0 = rp[r] - b.p0, a1 = rp[r + 1] - b.p0;
+                    for (int k = a0 + ll; k < a1; k += tpr) a += prod[k];
+                }
+                for (int off = tpr >> 1; off > 0; off >>= 1) a += __shfl_xor(a, off);
+                if (rr < nrows && ll == 0) {
+                    y[b.row0 + rr] = a;
+                    ys[rr] = a;
+                }
+            }
+            __syncthreads();
+            // project this item's rows: 16-row slices per wave, 4 column groups
+            for (int q = wv * dev::kTileRows; q < nrows; q += kBlock / dev::kGroups) {
+                const int rr = q + rl;
+                const bool ok = rr < nrows;
+                const double a = ok ? ys[rr] : 0.0;
+                const long long r = b.row0 + rr;
+#pragma unroll
+                for (int c = 0; c < MAXC; ++c) {
+                    const int colj = c0 + c;
+                    if (ok && c < cpg && colj < m) {
+                        const double v = (colj == j) ? xs[rr]
+                                                     : __builtin_nontemporal_load(&V[vpos<true>(r, colj, vld)]);
+                        acc[c] += v * a;
+                    }
+                }
+                if (g == 0 && ok) nrm += a * a;
+            }
+            __syncthreads();
+            continue;
+        }
+        // chunk of a long row: y only (projected by the last block)
+        double a = 0.0;
+        for (int p = b.p0 + tid; p < b.p1; p += kBlock)
+            a += static_cast<double>(__builtin_nontemporal_load(&val[p])) *
+                 (x[__builtin_nontemporal_load(&col[p])] * s);
+        for (int off = kWave / 2; off > 0; off >>= 1) a += __shfl_xor(a, off);
+        if (lane == 0) wred[wv] = a;
+        __syncthreads();
+        if (tid == 0) {
+            double tot = 0.0;
+            for (int w = 0; w < kBlock / kWave; ++w) tot += wred[w];
+            if (b.chunk == 0) V[vpos<true>(b.row0, j, vld)] = x[b.row0] * s;
+            if (b.nchunks == 1) {
+                y[b.row0] = tot;
+            } else {
+                __hip_atomic_store(&chunk_part[b.slot + b.chunk], tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned prev =
+                    __hip_atomic_fetch_add(&chunk_cnt[b.slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                if (prev + 1 == static_cast<unsigned>(b.nchunks)) {
+                    double t = 0.0;
+                    for (int c = 0; c < b.nchunks; ++c)
+                        t += __hip_atomic_load(&chunk_part[b.slot + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    y[b.row0] = t;
+                    __hip_atomic_store(&chunk_cnt[b.slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+        double t = acc[c];
+        t += __shfl_xor(t, 1);
+        t += __shfl_xor(t, 2);
+        t += __shfl_xor(t, 4);
+        t += __shfl_xor(t, 8);
+        acc[c] = t;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
+    if (rl == 0) {
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+            if (c < cpg && c0 + c < m) red[wv][c0 + c] = acc[c];
+    }
+    if (lane == 0) red[wv][m] = nrm;
+    __syncthreads();
+    for (int c = tid; c <= m; c += kBlock) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += red[w][c];
+        vals[c] = t;
+    }
+    __syncthreads();
+    if (!dev::reduce_blocks(vals, m + 1, partials, counter, scratch, fin)) return;
+    // long rows, in list order (their y was completed by other blocks)
+    for (int c = tid; c <= m; c += kBlock) {
+        double t = fin[c];
+        for (int q = 0; q < nlong; ++q) {
+            const int r = long_rows[q];
+            const double a = __hip_atomic_load(&y[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c == m)
+                t += a * a;
+            else
+                t += ((c == j) ? x[r] * s : V[vpos<true>(r, c, vld)]) * a;
+        }
+        fin[c] = t;
+    }
+    __syncthreads();
+    for (int c = tid; c < m; c += kBlock) h_out[c] = fin[c];
+    if (tid == 0) dev::run_epilogue(EPI_WNORM, j, fin[m], nullptr, ctl, nullptr, nullptr);
 }
 
 template <typename T>
@@ -232,12 +395,39 @@ int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *c
     HL_TRY_HIP(hipMemset(p.chunk_cnt, 0, sizeof(unsigned) * std::max(slots, 1)));
     if ((rc = dev_upload<double>(&p.x_stage, nullptr, static_cast<size_t>(ncols)))) return rc;
     if ((rc = dev_upload<double>(&p.y_stage, nullptr, static_cast<size_t>(nrows)))) return rc;
+
+    // Persistent partition for the fused SpMV + projection kernel: contiguous
+    // item ranges of about equal cost (matrix bytes + basis rows to project).
+    std::vector<int> long_rows;
+    std::vector<double> cost(blks.size());
+    double tot = 0.0;
+    for (size_t i = 0; i < blks.size(); ++i) {
+        const SpmvBlk &b = blks[i];
+        if (b.tpr == 0 && b.chunk == 0) long_rows.push_back(b.row0);
+        cost[i] = 12.0 * (b.p1 - b.p0) + (b.tpr > 0 ? kProjRowBytes * (b.row1 - b.row0) : 0.0);
+        tot += cost[i];
+    }
+    const int P = static_cast<int>(std::min<size_t>(kPersistBlocks, std::max<size_t>(blks.size(), 1)));
+    std::vector<int> first(P + 1, 0);
+    double acc = 0.0;
+    int part = 1;
+    for (size_t i = 0; i < blks.size() && part < P; ++i) {
+        acc += cost[i];
+        while (part < P && acc >= tot * part / P) first[part++] = static_cast<int>(i + 1);
+    }
+    while (part < P) first[part++] = static_cast<int>(blks.size());
+    first[P] = static_cast<int>(blks.size());
+    p.npers = P;
+    p.nlong = static_cast<int>(long_rows.size());
+    if ((rc = dev_upload(&p.pfirst, first.data(), first.size()))) return rc;
+    if ((rc = dev_upload(&p.long_rows, long_rows.data(), long_rows.size()))) return rc;
     return HL_OK;
 }
 
 void csr_plan_free(CsrPlan &p)
 {
-    void *ptrs[] = {p.row_ptr, p.col, p.val, p.val32, p.blks, p.chunk_part, p.chunk_cnt, p.x_stage, p.y_stage};
+    void *ptrs[] = {p.row_ptr, p.col, p.val, p.val32, p.blks, p.chunk_part, p.chunk_cnt, p.x_stage, p.y_stage,
+                    p.pfirst, p.long_rows};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p = CsrPlan{};
@@ -260,6 +450,42 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
                                                                  vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled,
                                                                  p.chunk_part, p.chunk_cnt);
     }
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+}  // namespace hl
+
+namespace hl {
+
+int launch_spmv_project(const CsrPlan &p, const double *x, const double *xscale, double *y, double *V, VLayout L,
+                        int j, double *h_out, Reduce &red, Ctl *ctl, hipStream_t s)
+{
+    const int m = j + 1;
+    if (!L.tiled || m > 128 || m > red.max_cols || p.npers > red.max_blocks) {
+        set_error("fused SpMV/projection: unsupported shape (m=%d)", m);
+        return HL_ERR_ARG;
+    }
+    if (p.nblk == 0) return HL_OK;
+    const int cpg = (m + dev::kGroups - 1) / dev::kGroups;
+#define HL_SP(MC)                                                                                              \
+    spmv_project_kernel<MC, double><<<p.npers, kBlock, 0, s>>>(p.blks, p.pfirst, p.row_ptr, p.col, p.val, x,  \
+                                                               xscale, y, V, L.ld, j, p.chunk_part, p.chunk_cnt, \
+                                                               p.long_rows, p.nlong, h_out, red.partials,     \
+                                                               red.counter, ctl)
+    if (cpg <= 2)
+        HL_SP(2);
+    else if (cpg <= 4)
+        HL_SP(4);
+    else if (cpg <= 8)
+        HL_SP(8);
+    else if (cpg <= 16)
+        HL_SP(16);
+    else if (cpg <= 24)
+        HL_SP(24);
+    else
+        HL_SP(32);
+#undef HL_SP
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

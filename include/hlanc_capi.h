@@ -107,7 +107,9 @@ double hl_csr_spmv_bytes(hl_csr_t h, int val_prec);
 /* One classical Gram-Schmidt pass, mat_v_gpu_t::op (HLanc.hpp:1000-1024), fused:
  *   w_out = w_in - V(:,0:m) h_in      (skipped when h_in == NULL: w_out = w_in)
  *   h_out = V(:,0:m)^T w_out,  *nrm2_out = ||w_out||^2 (both device pointers).
- * w_out may equal w_in or be NULL (not written). */
+ * w_out may equal w_in or be NULL (not written).  Asynchronous on `stream`;
+ * calls from one host thread share a reduction workspace, so calls on
+ * different streams of one thread must not overlap. */
 int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const double *w_in,
                 double *w_out, const double *h_in, double *h_out, double *nrm2_out, void *stream);
 /* Out(:, 0:N) = V(:, 0:K) Q(0:K, 0:N): the V.Q product of dsapps and of the
@@ -115,6 +117,19 @@ int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const dou
  * use_mfma = 1 selects the v_mfma_f64_16x16x4 kernel, 0 the FMA kernel. */
 int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq,
           double *Out, long long ldo, int use_mfma, void *stream);
+
+/* ---------------------------------------------------------------- tuning
+ * Process-wide kernel parameters, set before solves start (not thread-safe
+ * against running solves).  Keys: "cgs_blocks_max" (grid cap of the CGS
+ * passes, 1..2048), "cgs_flags" (bit 0 non-temporal basis loads, bit 1
+ * prefetch of the next tile), "basis_tiled" (1: 16-row tiled basis, 0:
+ * column-major).  No counterpart in the reference (its kernels are fixed). */
+int hl_tune_set(const char *key, int value);
+int hl_tune_get(const char *key, int *value);
+/* Average seconds per launch of one fused CGS pass (w -= V h; h' = V^T w) over
+ * an n x m FP64 basis, layout tiled (1) or column-major (0), timed with HIP
+ * events over `reps` launches on the default stream (own buffers). */
+int hl_bench_cgs(int dev, int n, int m, int tiled, int reps, double *seconds);
 
 /* ------------------------------------------------------------------ solver */
 
@@ -151,14 +166,16 @@ typedef struct {
 #define HL_FLAG_PROFILE 2  /* time every kernel launch with HIP events (slower);
                               the DGKS correction passes are then launched only
                               when needed, decided on the host */
+#define HL_FLAG_NO_FUSE 4  /* run the SpMV and the first CGS half as two kernels */
 
 /* Per-kernel-class totals of the last profiled solve on the calling thread:
- * index 0 SpMV, 1 CGS passes, 2 V.Q GEMM, 3 other kernels.  bytes are
- * algorithmic HBM bytes (each operand counted once per launch). */
+ * index 0 SpMV, 1 CGS passes, 2 V.Q GEMM, 3 other kernels, 4 fused SpMV +
+ * first CGS half, 5 unused.  bytes are algorithmic HBM bytes (each operand
+ * counted once per launch). */
 typedef struct {
-    long long launches[4];
-    double seconds[4];
-    double bytes[4];
+    long long launches[6];
+    double seconds[6];
+    double bytes[6];
 } hl_profile;
 int hl_profile_last(hl_profile *out);
 
