@@ -82,7 +82,7 @@ int solve_common(hl::Solver &S, int dev, int n, int nev, int ncv, double *values
     int rc = check_device(dev);
     if (rc) return rc;
     DeviceGuard g(dev);
-    if ((rc = S.ws.init(dev, n, ncv, o.basis_prec, host_op))) {
+    if ((rc = S.ws.init(dev, n, ncv, o.basis_prec, host_op, S.A ? S.A->nblk : 0))) {
         S.ws.release();
         if (info) *info = inf;
         return rc;

@@ -75,10 +75,8 @@ __device__ __forceinline__ bool reduce_blocks(const double *vals, int M1, double
 {
     __shared__ int is_last;
     const int tid = threadIdx.x;
-    for (int c = tid; c < M1; c += kBlock)
-        __hip_atomic_store(&partials[(long long)blockIdx.x * M1 + c], vals[c], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
+    for (int c = tid; c < M1; c += kBlock) partials[(long long)blockIdx.x * M1 + c] = vals[c];
+    __threadfence(); // release: this block's partials before its arrival is counted
     __syncthreads();
     if (tid == 0) {
         const unsigned prev =
@@ -97,9 +95,8 @@ __device__ __forceinline__ bool reduce_blocks(const double *vals, int M1, double
         const int part = tid / span;
         double s = 0.0;
         if (part < ns) {
-            for (int b = part; b < nb; b += ns)
-                s += __hip_atomic_load(&partials[(long long)b * M1 + c], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+            // plain loads: the acquire fence above made the other blocks' partials visible
+            for (int b = part; b < nb; b += ns) s += partials[(long long)b * M1 + c];
         }
         scratch[tid] = s;
         __syncthreads();

@@ -110,8 +110,6 @@ struct SpmvBlk {
 constexpr int kNnzPerBlock = 2048;  // stream block capacity (8 nnz per thread)
 constexpr int kRowsPerBlock = 256;
 constexpr int kLongChunk = 8192;    // nnz per chunk of a long row
-constexpr int kPersistBlocks = 1024; // grid of the fused SpMV + projection kernel
-constexpr double kProjRowBytes = 256.0; // cost weight of projecting one row (~32 FP64 columns)
 
 struct CsrPlan {
     int dev = 0;
@@ -127,8 +125,6 @@ struct CsrPlan {
     unsigned *chunk_cnt = nullptr;
     int nslots = 0;
     double *x_stage = nullptr, *y_stage = nullptr;  // for host-pointer calls
-    int *pfirst = nullptr;    // device [npers+1]: item ranges of the persistent fused kernel
-    int npers = 0;
     int *long_rows = nullptr; // device: rows split into chunk items, ascending
     int nlong = 0;
 };
@@ -145,9 +141,10 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
 // ------------------------------------------------------- basis kernels
 struct Reduce;
 // Fused head of a Lanczos step on a tiled basis: y = A (x s), V(:, j) = x s,
-// h_out = V(:, 0:j+1)^T y, ctl->wnorm = ||y|| (EPI_WNORM).
+// h_out = V(:, 0:j+1)^T y, ctl->wnorm = ||y|| (EPI_WNORM).  ipart holds
+// nblk x (j+2) partial sums, fin j+2 doubles; red.counter is used (self-resetting).
 int launch_spmv_project(const CsrPlan &p, const double *x, const double *xscale, double *y, double *V, VLayout L,
-                        int j, double *h_out, Reduce &red, Ctl *ctl, hipStream_t s);
+                        int j, double *h_out, double *ipart, double *fin, Reduce &red, Ctl *ctl, hipStream_t s);
 
 struct Reduce {
     double *partials = nullptr;  // [nblocks_max * (ncv_max + 1)]

@@ -212,7 +212,7 @@ Workspace::~Workspace() { release(); }
 
 void Workspace::release()
 {
-    void *ptrs[] = {V[0], V[1], Vf[0], Vf[1], r, w, hsub, hdia, hbuf, Qd, ctl, vd_tmp, xv};
+    void *ptrs[] = {V[0], V[1], Vf[0], Vf[1], r, w, hsub, hdia, hbuf, Qd, ctl, vd_tmp, xv, ipart, fin};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctl_h) (void)hipHostFree(ctl_h);
@@ -223,14 +223,14 @@ void Workspace::release()
     if (stream) (void)hipStreamDestroy(stream);
     V[0] = V[1] = nullptr;
     Vf[0] = Vf[1] = nullptr;
-    r = w = hsub = hdia = hbuf = Qd = vd_tmp = xv = nullptr;
+    r = w = hsub = hdia = hbuf = Qd = vd_tmp = xv = ipart = fin = nullptr;
     ctl = nullptr;
     ctl_h = nullptr;
     hh = xh = yh = nullptr;
     stream = nullptr;
 }
 
-int Workspace::init(int dev_, int n_, int ncv_, int prec_, bool host_op)
+int Workspace::init(int dev_, int n_, int ncv_, int prec_, bool host_op, int items)
 {
     dev = dev_;
     n = n_;
@@ -255,6 +255,10 @@ int Workspace::init(int dev_, int n_, int ncv_, int prec_, bool host_op)
     HL_TRY_HIP(hipMalloc(&hbuf, sizeof(double) * 2 * ncv));
     HL_TRY_HIP(hipMalloc(&Qd, sizeof(double) * (size_t)ncv * ncv));
     HL_TRY_HIP(hipMalloc(&ctl, sizeof(Ctl)));
+    if (items > 0) { // partial sums of the fused SpMV + projection kernel
+        HL_TRY_HIP(hipMalloc(&ipart, sizeof(double) * (size_t)items * (ncv + 1)));
+        HL_TRY_HIP(hipMalloc(&fin, sizeof(double) * (ncv + 1)));
+    }
     HL_TRY_HIP(hipHostMalloc(&ctl_h, sizeof(Ctl), hipHostMallocDefault));
     HL_TRY_HIP(hipHostMalloc(&hh, sizeof(double) * 2 * ncv, hipHostMallocDefault));
     if (host_op) {
@@ -318,8 +322,8 @@ int Solver::step(int j)
     if (A && fuse && W.tiled) { // SpMV and the first CGS half in one persistent kernel
         ++spmv;
         if ((rc = prof.begin(W.stream))) return rc;
-        if ((rc = launch_spmv_project(*A, W.r, &W.ctl->xscale, W.w, W.V[W.cur], W.vl(), j, h0, W.red, W.ctl,
-                                      W.stream)))
+        if ((rc = launch_spmv_project(*A, W.r, &W.ctl->xscale, W.w, W.V[W.cur], W.vl(), j, h0, W.ipart, W.fin,
+                                      W.red, W.ctl, W.stream)))
             return rc;
         const double b = 12.0 * A->nnz + 4.0 * (A->nrows + 1) + 8.0 * A->ncols + 16.0 * A->nrows + 8.0 * W.n * j;
         if ((rc = prof.end(W.stream, K_SPMV_PROJ, b))) return rc;

@@ -18,6 +18,7 @@ struct Workspace {
     double *r = nullptr, *w = nullptr;  // residual and A v_j
     double *hsub = nullptr, *hdia = nullptr, *hbuf = nullptr, *Qd = nullptr, *vd_tmp = nullptr;
     double *xv = nullptr;               // contiguous v_j for host operators
+    double *ipart = nullptr, *fin = nullptr; // fused SpMV + projection partial sums
     Ctl *ctl = nullptr;
     Ctl *ctl_h = nullptr;               // pinned mirrors
     double *hh = nullptr, *xh = nullptr, *yh = nullptr;
@@ -27,7 +28,7 @@ struct Workspace {
     bool tiled = true;                  // fixed at init from tuning().basis_tiled
     VLayout vl() const { return tiled ? VLayout{ncv, true} : VLayout{ldv, false}; }
     ~Workspace();
-    int init(int dev, int n, int ncv, int prec, bool host_op);
+    int init(int dev, int n, int ncv, int prec, bool host_op, int items);
     void release();
 };
 
