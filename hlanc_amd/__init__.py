@@ -92,6 +92,8 @@ _SIGS = [
     ("hl_tune_get", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
     ("hl_bench_cgs", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double)]),
+    ("hl_bench_stream", ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_double)]),
     ("hl_options_default", None, [ctypes.POINTER(Options)]),
     ("hl_profile_last", ctypes.c_int, [ctypes.POINTER(Profile)]),
     ("hl_ds_solve_csr", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
@@ -364,4 +366,11 @@ def bench_cgs(n: int, m: int, tiled: bool = True, reps: int = 20, dev: int = 0) 
     """Seconds per launch of one fused CGS pass over an n x m FP64 basis."""
     t = ctypes.c_double()
     _check(lib().hl_bench_cgs(dev, n, m, int(tiled), reps, ctypes.byref(t)))
+    return t.value
+
+
+def bench_stream(nbytes: int = 4 << 30, reps: int = 10, dev: int = 0) -> float:
+    """Seconds per read-only streaming pass over nbytes of HBM."""
+    t = ctypes.c_double()
+    _check(lib().hl_bench_stream(dev, int(nbytes), reps, ctypes.byref(t)))
     return t.value

@@ -304,14 +304,46 @@ int hl_bench_cgs(int dev, int n, int m, int tiled, int reps, double *seconds)
     return rc;
 }
 
+int hl_bench_stream(int dev, long long bytes, int reps, double *seconds)
+{
+    if (bytes < 1024 || reps <= 0 || !seconds) return HL_ERR_ARG;
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    const long long n = bytes / 16 * 2;
+    const int blocks = 2048;
+    double *a = nullptr, *out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HL_TRY_HIP(hipMalloc(&a, sizeof(double) * n));
+    HL_TRY_HIP(hipMalloc(&out, sizeof(double) * blocks * (kBlock / kWave)));
+    HL_TRY_HIP(hipMemset(a, 0, sizeof(double) * n));
+    HL_TRY_HIP(hipEventCreate(&e0));
+    HL_TRY_HIP(hipEventCreate(&e1));
+    rc = launch_stream_read(a, n, out, blocks, nullptr);
+    if (!rc) rc = hipEventRecord(e0, nullptr) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    for (int i = 0; i < reps && !rc; ++i) rc = launch_stream_read(a, n, out, blocks, nullptr);
+    float ms = 0.0f;
+    if (!rc) rc = hipEventRecord(e1, nullptr) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    if (!rc) rc = hipEventElapsedTime(&ms, e0, e1) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    *seconds = 1e-3 * ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(out);
+    return rc;
+}
+
 int hl_tune_set(const char *key, int value)
 {
     if (!key) return HL_ERR_ARG;
     Tuning &t = tuning();
     if (!std::strcmp(key, "cgs_blocks_max") && value >= 1 && value <= kMaxReduceBlocks)
         t.cgs_blocks_max = value;
-    else if (!std::strcmp(key, "cgs_flags") && value >= 0 && value <= 3)
+    else if (!std::strcmp(key, "cgs_flags") && value >= 0 && value <= 7)
         t.cgs_flags = value;
+    else if (!std::strcmp(key, "fuse_head") && (value == 0 || value == 1))
+        t.fuse_head = value;
     else if (!std::strcmp(key, "basis_tiled") && (value == 0 || value == 1))
         t.basis_tiled = value;
     else {
@@ -331,6 +363,8 @@ int hl_tune_get(const char *key, int *value)
         *value = t.cgs_flags;
     else if (!std::strcmp(key, "basis_tiled"))
         *value = t.basis_tiled;
+    else if (!std::strcmp(key, "fuse_head"))
+        *value = t.fuse_head;
     else {
         set_error("hl_tune_get: unknown key %s", key);
         return HL_ERR_ARG;

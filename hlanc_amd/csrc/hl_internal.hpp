@@ -70,8 +70,10 @@ enum Epi : int {
 // ------------------------------------------------------------ tuning
 // Process-wide kernel parameters (hl_tune_set); fixed while solves run.
 struct Tuning {
-    int cgs_blocks_max = 1024; // grid cap of the CGS passes (<= kMaxReduceBlocks)
-    int cgs_flags = 3;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile
+    int cgs_blocks_max = 512;  // grid cap of the CGS passes (<= kMaxReduceBlocks)
+    int cgs_flags = 1;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile,
+                               // bit 2: 16-byte loads (tiled FP64 basis)
+    int fuse_head = 0;         // fused SpMV + first CGS half (1) or two kernels (0)
     int basis_tiled = 1;       // solver basis layout: 1 tiled, 0 column-major
 };
 Tuning &tuning();
@@ -185,6 +187,8 @@ int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn
 int launch_scale_rscale(int n, double *r, double q, const Ctl *ctl, hipStream_t s);
 // ||r||^2 -> epilogue (mode EPI_RNORM sets rnorm/xscale/rscale).
 int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStream_t s);
+// HBM read probe: sums n doubles (n even, 16-byte aligned) into out[blocks*4].
+int launch_stream_read(const double *a, long long n, double *out, int blocks, hipStream_t s);
 // out = float(in)
 int launch_cast_f32(long long n, const double *in, float *out, hipStream_t s);
 

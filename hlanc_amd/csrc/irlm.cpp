@@ -425,7 +425,7 @@ int Solver::run(int nev0, double *values, double *vectors, int ldv_host, const h
     const int maxiter = o.maxiter > 0 ? o.maxiter : 200;
     const double eps23 = std::pow(kEps, 2.0 / 3.0);
     use_mfma = o.use_mfma;
-    fuse = (o.flags & HL_FLAG_NO_FUSE) == 0;
+    fuse = tuning().fuse_head != 0 && (o.flags & HL_FLAG_NO_FUSE) == 0;
     prof.on = (o.flags & HL_FLAG_PROFILE) != 0;
     prof.tot = hl_profile{};
     sub.assign(ncv, 0.0);

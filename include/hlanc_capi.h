@@ -122,14 +122,19 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
  * Process-wide kernel parameters, set before solves start (not thread-safe
  * against running solves).  Keys: "cgs_blocks_max" (grid cap of the CGS
  * passes, 1..2048), "cgs_flags" (bit 0 non-temporal basis loads, bit 1
- * prefetch of the next tile), "basis_tiled" (1: 16-row tiled basis, 0:
- * column-major).  No counterpart in the reference (its kernels are fixed). */
+ * prefetch of the next tile, bit 2 16-byte loads on the tiled basis),
+ * "basis_tiled" (1: 16-row tiled basis, 0: column-major), "fuse_head" (1:
+ * SpMV and the first CGS half in one kernel).  No counterpart in the
+ * reference (its kernels are fixed). */
 int hl_tune_set(const char *key, int value);
 int hl_tune_get(const char *key, int *value);
 /* Average seconds per launch of one fused CGS pass (w -= V h; h' = V^T w) over
  * an n x m FP64 basis, layout tiled (1) or column-major (0), timed with HIP
  * events over `reps` launches on the default stream (own buffers). */
 int hl_bench_cgs(int dev, int n, int m, int tiled, int reps, double *seconds);
+/* Average seconds per pass of a read-only 16-byte-load stream over `bytes`
+ * bytes of HBM (own buffer): the read rate this device reaches. */
+int hl_bench_stream(int dev, long long bytes, int reps, double *seconds);
 
 /* ------------------------------------------------------------------ solver */
 

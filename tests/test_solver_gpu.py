@@ -207,8 +207,13 @@ def test_fused_and_unfused_step_agree(gen):
         nev, ncv = 6, 20
     n = len(rp) - 1
     op = op_for(rp, col, val)
-    a = H.ds_solver_gpu(n, nev, ncv, op, tol=1e-10, maxiter=2000)
-    b = H.ds_solver_gpu(n, nev, ncv, op, tol=1e-10, maxiter=2000, flags=H.HL_FLAG_NO_FUSE)
+    old = H.tune_get("fuse_head")
+    try:
+        H.tune_set("fuse_head", 1)
+        a = H.ds_solver_gpu(n, nev, ncv, op, tol=1e-10, maxiter=2000)
+        b = H.ds_solver_gpu(n, nev, ncv, op, tol=1e-10, maxiter=2000, flags=H.HL_FLAG_NO_FUSE)
+    finally:
+        H.tune_set("fuse_head", old)
     o = O.ds_solver(O.Csr(rp, col, val), nev, ncv, tol=1e-10, maxiter=2000)
     assert a.ret == b.ret == o.ret == 0
     np.testing.assert_allclose(a.values, o.values, rtol=1e-10)
@@ -219,3 +224,20 @@ def test_fused_and_unfused_step_agree(gen):
         for i in range(nev):
             v = r.vectors[:, i]
             assert np.linalg.norm(A @ v - r.values[i] * v) <= 1e-10 * nrm
+
+
+@pytest.mark.parametrize("flags,fuse", [(0, 0), (1, 0), (2, 0), (3, 0), (5, 0), (1, 1), (5, 1)])
+def test_kernel_variants_agree_with_oracle(flags, fuse):
+    # every tunable CGS variant and the fused step head compute the same solve
+    rp, col, val = W.stencil27(12, seed=11)
+    old = (H.tune_get("cgs_flags"), H.tune_get("fuse_head"))
+    try:
+        H.tune_set("cgs_flags", flags)
+        H.tune_set("fuse_head", fuse)
+        r = H.ds_solver_gpu(1728, 8, 24, op_for(rp, col, val), tol=1e-10, maxiter=1000)
+    finally:
+        H.tune_set("cgs_flags", old[0])
+        H.tune_set("fuse_head", old[1])
+    o = O.ds_solver(O.Csr(rp, col, val), 8, 24, tol=1e-10, maxiter=1000)
+    assert r.ret == 0 and o.ret == 0
+    np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
