@@ -134,10 +134,42 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
         vals[c] = s;
     }
     __syncthreads();
+    if constexpr ((FL & 8) != 0) { // split: partials only, cgs_finalize_kernel finishes
+        for (int c = threadIdx.x; c < M1; c += kBlock) partials[(long long)c * gridDim.x + blockIdx.x] = vals[c];
+        return;
+    }
     if (!reduce_blocks(vals, M1, partials, counter, scratch, fin)) return;
     if (h_out)
         for (int c = threadIdx.x; c < m; c += kBlock) h_out[c] = fin[c];
     if (threadIdx.x == 0) run_epilogue(mode, j, fin[m], h_in, ctl, h_sub, h_dia);
+}
+
+// Second kernel of a split CGS pass: block c adds column c of the column-major
+// block partials in block order; block m (the squared norm) runs the epilogue.
+// Columns whose block starts after the epilogue cleared ctl->need are not
+// written, and no later kernel reads them.
+__global__ __launch_bounds__(kBlock) void cgs_finalize_kernel(int nb, int m, const double *__restrict__ partials,
+                                                              const double *h_in, double *h_out, Ctl *ctl,
+                                                              double *h_sub, double *h_dia, int mode, int j)
+{
+    if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return;
+    __shared__ double sred[kBlock];
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x;
+    double t = 0.0;
+    for (int b = tid; b < nb; b += kBlock) t += partials[(long long)c * nb + b];
+    sred[tid] = t;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) sred[tid] += sred[tid + w];
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    if (c < m) {
+        if (h_out) h_out[c] = sred[0];
+    } else {
+        run_epilogue(mode, j, sred[0], h_in, ctl, h_sub, h_dia);
+    }
 }
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
@@ -501,7 +533,7 @@ int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, d
         }
     }
     const int cpg = (m + kGroups - 1) / kGroups;
-    const int fl = tuning().cgs_flags & 3;
+    const int fl = (tuning().cgs_flags & 8) ? 9 : (tuning().cgs_flags & 3);
 #define HL_CGS1(MC, F)                                                                                         \
     cgs_kernel<MC, VT, TILED, F><<<nb, kBlock, 0, s>>>(n, m, V, ldv, w_in, w_out, h_in, h_out, red.partials, \
                                                        red.counter, ctl, h_sub, h_dia, mode, j)
@@ -512,8 +544,10 @@ int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, d
         HL_CGS1(MC, 1);         \
     else if (fl == 2)           \
         HL_CGS1(MC, 2);         \
+    else if (fl == 3)           \
+        HL_CGS1(MC, 3);         \
     else                        \
-        HL_CGS1(MC, 3)
+        HL_CGS1(MC, 9)
     if (cpg <= 2) {
         HL_CGS(2);
     } else if (cpg <= 4) {
@@ -530,6 +564,10 @@ int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, d
 #undef HL_CGS
 #undef HL_CGS1
     HL_TRY_HIP(hipGetLastError());
+    if (fl == 9) {
+        cgs_finalize_kernel<<<m + 1, kBlock, 0, s>>>(nb, m, red.partials, h_in, h_out, ctl, h_sub, h_dia, mode, j);
+        HL_TRY_HIP(hipGetLastError());
+    }
     return HL_OK;
 }
 

@@ -340,7 +340,7 @@ int hl_tune_set(const char *key, int value)
     Tuning &t = tuning();
     if (!std::strcmp(key, "cgs_blocks_max") && value >= 1 && value <= kMaxReduceBlocks)
         t.cgs_blocks_max = value;
-    else if (!std::strcmp(key, "cgs_flags") && value >= 0 && value <= 7)
+    else if (!std::strcmp(key, "cgs_flags") && value >= 0 && value <= 15)
         t.cgs_flags = value;
     else if (!std::strcmp(key, "fuse_head") && (value == 0 || value == 1))
         t.fuse_head = value;

@@ -72,7 +72,8 @@ enum Epi : int {
 struct Tuning {
     int cgs_blocks_max = 512;  // grid cap of the CGS passes (<= kMaxReduceBlocks)
     int cgs_flags = 1;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile,
-                               // bit 2: 16-byte loads (tiled FP64 basis)
+                               // bit 2: 16-byte loads (tiled FP64 basis),
+                               // bit 3: split (partials kernel + finalize kernel)
     int fuse_head = 0;         // fused SpMV + first CGS half (1) or two kernels (0)
     int basis_tiled = 1;       // solver basis layout: 1 tiled, 0 column-major
 };

@@ -122,7 +122,8 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
  * Process-wide kernel parameters, set before solves start (not thread-safe
  * against running solves).  Keys: "cgs_blocks_max" (grid cap of the CGS
  * passes, 1..2048), "cgs_flags" (bit 0 non-temporal basis loads, bit 1
- * prefetch of the next tile, bit 2 16-byte loads on the tiled basis),
+ * prefetch of the next tile, bit 2 16-byte loads on the tiled basis, bit 3
+ * finish each pass in a second kernel instead of a last-block reduction),
  * "basis_tiled" (1: 16-row tiled basis, 0: column-major), "fuse_head" (1:
  * SpMV and the first CGS half in one kernel).  No counterpart in the
  * reference (its kernels are fixed). */

@@ -226,7 +226,7 @@ def test_fused_and_unfused_step_agree(gen):
             assert np.linalg.norm(A @ v - r.values[i] * v) <= 1e-10 * nrm
 
 
-@pytest.mark.parametrize("flags,fuse", [(0, 0), (1, 0), (2, 0), (3, 0), (5, 0), (1, 1), (5, 1)])
+@pytest.mark.parametrize("flags,fuse", [(0, 0), (1, 0), (2, 0), (3, 0), (5, 0), (9, 0), (1, 1), (9, 1)])
 def test_kernel_variants_agree_with_oracle(flags, fuse):
     # every tunable CGS variant and the fused step head compute the same solve
     rp, col, val = W.stencil27(12, seed=11)
