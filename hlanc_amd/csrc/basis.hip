@@ -309,8 +309,6 @@ __global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, con
     extern __shared__ double qs[]; // K x CB
     __shared__ double red[kBlock / kWave];
     __shared__ double vals[2];
-    __shared__ double scratch[kBlock];
-    __shared__ double fin[2];
     for (int i = threadIdx.x; i < K * CB; i += kBlock) {
         const int k = i / CB, c = i % CB;
         qs[i] = (c0 + c < N) ? Q[(long long)(c0 + c) * ldq + k] : 0.0;
@@ -351,8 +349,8 @@ __global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, con
         vals[0] = s;
     }
     __syncthreads();
-    if (!reduce_blocks(vals, 1, partials, counter, scratch, fin)) return;
-    if (threadIdx.x == 0) run_epilogue(EPI_RNORM, 0, fin[0], nullptr, ctl, nullptr, nullptr);
+    // block partial of ||r||^2; cgs_finalize_kernel (one block, m = 0) adds them
+    if (threadIdx.x == 0) partials[blockIdx.x] = vals[0];
 }
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -372,8 +370,6 @@ __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, co
     extern __shared__ double qs[]; // Kp x (NT*16), row k
     __shared__ double red[kBlock / kWave];
     __shared__ double vals[2];
-    __shared__ double scratch[kBlock];
-    __shared__ double fin[2];
     const int NC = NT * 16;
     const int Kp = (K + 3) & ~3;
     for (int i = threadIdx.x; i < Kp * NC; i += kBlock) {
@@ -435,8 +431,8 @@ __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, co
         vals[0] = s;
     }
     __syncthreads();
-    if (!reduce_blocks(vals, 1, partials, counter, scratch, fin)) return;
-    if (threadIdx.x == 0) run_epilogue(EPI_RNORM, 0, fin[0], nullptr, ctl, nullptr, nullptr);
+    // block partial of ||r||^2; cgs_finalize_kernel (one block, m = 0) adds them
+    if (threadIdx.x == 0) partials[blockIdx.x] = vals[0];
 }
 
 __host__ __device__ inline uint64_t mix64(uint64_t z)
@@ -589,6 +585,11 @@ int launch_vq_fma(int n, int K, int N, const VT *V, long long ldv, const double 
                                                                   qlast, beta, red.partials, red.counter, ctl);
         HL_TRY_HIP(hipGetLastError());
     }
+    if (res_col >= 0) {
+        cgs_finalize_kernel<<<1, kBlock, 0, s>>>(nb, 0, red.partials, nullptr, nullptr, ctl, nullptr, nullptr,
+                                                 EPI_RNORM, 0);
+        HL_TRY_HIP(hipGetLastError());
+    }
     return HL_OK;
 }
 
@@ -625,6 +626,11 @@ int launch_vq_mfma(int n, int K, int N, const double *V, long long ldv, const do
         else
             HL_VQM(4);
 #undef HL_VQM
+        HL_TRY_HIP(hipGetLastError());
+    }
+    if (res_col >= 0) {
+        cgs_finalize_kernel<<<1, kBlock, 0, s>>>(nb, 0, red.partials, nullptr, nullptr, ctl, nullptr, nullptr,
+                                                 EPI_RNORM, 0);
         HL_TRY_HIP(hipGetLastError());
     }
     return HL_OK;

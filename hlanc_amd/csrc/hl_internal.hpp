@@ -70,15 +70,15 @@ enum Epi : int {
 // ------------------------------------------------------------ tuning
 // Process-wide kernel parameters (hl_tune_set); fixed while solves run.
 struct Tuning {
-    int cgs_blocks_max = 512;  // grid cap of the CGS passes (<= kMaxReduceBlocks)
-    int cgs_flags = 1;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile,
+    int cgs_blocks_max = 2048; // grid cap of the CGS passes (<= kMaxReduceBlocks)
+    int cgs_flags = 9;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile,
                                // bit 2: 16-byte loads (tiled FP64 basis),
                                // bit 3: split (partials kernel + finalize kernel)
     int fuse_head = 0;         // fused SpMV + first CGS half (1) or two kernels (0)
     int basis_tiled = 1;       // solver basis layout: 1 tiled, 0 column-major
 };
 Tuning &tuning();
-constexpr int kMaxReduceBlocks = 2048;
+constexpr int kMaxReduceBlocks = 4096;
 
 // ------------------------------------------------------------ basis layout
 // Storage of a basis V (n rows): column-major with leading dimension ld, or
