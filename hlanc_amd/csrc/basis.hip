@@ -33,13 +33,14 @@ __device__ __forceinline__ T ld_stream(const T *p)
         return *p;
 }
 
-// FL bit 0: non-temporal basis loads; bit 1: software prefetch of the next tile.
+// Block `bid` of an `nb`-block CGS pass.  FL bit 0: non-temporal basis loads;
+// bit 1: software prefetch of the next tile; bit 3: split (write this block's
+// partial sums and leave the rest to cgs_finalize_kernel).
 template <int MAXC, typename VT, bool TILED, int FL>
-__global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__restrict__ V, long long ldv,
-                                                     const double *w_in, double *w_out,
-                                                     const double *__restrict__ h_in, double *h_out,
-                                                     double *partials, unsigned *counter, Ctl *ctl,
-                                                     double *h_sub, double *h_dia, int mode, int j)
+__device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const VT *__restrict__ V, long long ldv,
+                                          const double *w_in, double *w_out, const double *__restrict__ h_in,
+                                          double *h_out, double *partials, unsigned *counter, Ctl *ctl,
+                                          double *h_sub, double *h_dia, int mode, int j)
 {
     if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return; // DGKS decided on the device
     __shared__ double hs[kGroups * MAXC];
@@ -63,8 +64,8 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
     for (int c = 0; c < MAXC; ++c) acc[c] = 0.0;
     double nrm = 0.0;
     const long long ntiles = (n + kTileRows - 1) / kTileRows;
-    const long long W = (long long)gridDim.x * (kBlock / kWave);
-    long long t = (long long)blockIdx.x * (kBlock / kWave) + wv;
+    const long long W = (long long)nb * (kBlock / kWave);
+    long long t = (long long)bid * (kBlock / kWave) + wv;
 
     // software pipeline: the next tile's V and w are in flight while this one is used
     double vn[MAXC], wn;
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
     }
     __syncthreads();
     if constexpr ((FL & 8) != 0) { // split: partials only, cgs_finalize_kernel finishes
-        for (int c = threadIdx.x; c < M1; c += kBlock) partials[(long long)c * gridDim.x + blockIdx.x] = vals[c];
+        for (int c = threadIdx.x; c < M1; c += kBlock) partials[(long long)c * nb + bid] = vals[c];
         return;
     }
     if (!reduce_blocks(vals, M1, partials, counter, scratch, fin)) return;
@@ -144,17 +145,27 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
     if (threadIdx.x == 0) run_epilogue(mode, j, fin[m], h_in, ctl, h_sub, h_dia);
 }
 
+template <int MAXC, typename VT, bool TILED, int FL>
+__global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__restrict__ V, long long ldv,
+                                                     const double *w_in, double *w_out,
+                                                     const double *__restrict__ h_in, double *h_out,
+                                                     double *partials, unsigned *counter, Ctl *ctl,
+                                                     double *h_sub, double *h_dia, int mode, int j)
+{
+    cgs_block<MAXC, VT, TILED, FL>(blockIdx.x, gridDim.x, n, m, V, ldv, w_in, w_out, h_in, h_out, partials, counter,
+                                   ctl, h_sub, h_dia, mode, j);
+}
+
 // Second kernel of a split CGS pass: block c adds column c of the column-major
 // block partials in block order; block m (the squared norm) runs the epilogue.
 // Columns whose block starts after the epilogue cleared ctl->need are not
 // written, and no later kernel reads them.
-__global__ __launch_bounds__(kBlock) void cgs_finalize_kernel(int nb, int m, const double *__restrict__ partials,
-                                                              const double *h_in, double *h_out, Ctl *ctl,
-                                                              double *h_sub, double *h_dia, int mode, int j)
+__device__ __forceinline__ void cgs_finalize_block(int c, int nb, int m, const double *__restrict__ partials,
+                                                   const double *h_in, double *h_out, Ctl *ctl, double *h_sub,
+                                                   double *h_dia, int mode, int j)
 {
     if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return;
     __shared__ double sred[kBlock];
-    const int c = blockIdx.x;
     const int tid = threadIdx.x;
     double t = 0.0;
     for (int b = tid; b < nb; b += kBlock) t += partials[(long long)c * nb + b];
@@ -170,6 +181,74 @@ __global__ __launch_bounds__(kBlock) void cgs_finalize_kernel(int nb, int m, con
     } else {
         run_epilogue(mode, j, sred[0], h_in, ctl, h_sub, h_dia);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void cgs_finalize_kernel(int nb, int m, const double *__restrict__ partials,
+                                                              const double *h_in, double *h_out, Ctl *ctl,
+                                                              double *h_sub, double *h_dia, int mode, int j)
+{
+    cgs_finalize_block(blockIdx.x, nb, m, partials, h_in, h_out, ctl, h_sub, h_dia, mode, j);
+}
+
+// Largest i < ns with first[i * stride] <= b: the segment owning block b
+// (segments are listed in launch order; empty ones are skipped over).
+__device__ __forceinline__ int seg_find(const int *first, int stride, int ns, int b)
+{
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid * stride] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Pointers of CGS pass `mode` of a Lanczos step (the same sequence Solver::step
+// launches for one solve).
+struct PassPtrs {
+    const double *w_in;
+    double *w_out;
+    const double *h_in;
+    double *h_out;
+};
+__device__ __forceinline__ PassPtrs pass_ptrs(const SegMember &mb, int mode)
+{
+    switch (mode) {
+    case EPI_WNORM:
+        return {mb.w, nullptr, nullptr, mb.h0};
+    case EPI_PASS1:
+        return {mb.w, mb.r, mb.h0, mb.h1};
+    case EPI_PASS2:
+        return {mb.r, mb.r, mb.h1, mb.h0};
+    default:
+        return {mb.r, mb.r, mb.h0, nullptr};
+    }
+}
+
+// One CGS pass of every listed solve in a single launch (split form).
+template <int MAXC>
+__global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__restrict__ M,
+                                                         const SegStep *__restrict__ S, int ns, int mode)
+{
+    const int i = seg_find(&S[0].cgs0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
+    const SegStep st = S[i];
+    const SegMember mb = M[st.mem];
+    const PassPtrs p = pass_ptrs(mb, mode);
+    cgs_block<MAXC, double, true, 9>(blockIdx.x - st.cgs0, mb.nb, mb.n, st.j + 1, mb.V, mb.vld, p.w_in, p.w_out,
+                                     p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub, mb.hdia, mode, st.j);
+}
+
+__global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMember *__restrict__ M,
+                                                                  const SegStep *__restrict__ S, int ns, int mode)
+{
+    const int i = seg_find(&S[0].fin0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
+    const SegStep st = S[i];
+    const SegMember mb = M[st.mem];
+    const PassPtrs p = pass_ptrs(mb, mode);
+    cgs_finalize_block(blockIdx.x - st.fin0, mb.nb, st.j + 1, mb.partials, p.h_in, p.h_out, mb.ctl, mb.hsub, mb.hdia,
+                       mode, st.j);
 }
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
@@ -744,6 +823,30 @@ int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn
 int launch_scale_rscale(int n, double *r, double q, const Ctl *ctl, hipStream_t s)
 {
     scale_rscale_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, r, q, ctl);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total,
+                   int fin_blocks_total, int mode, hipStream_t s)
+{
+    if (ns <= 0) return HL_OK;
+#define HL_SEG(MC) cgs_seg_kernel<MC><<<cgs_blocks_total, kBlock, 0, s>>>(M, S, ns, mode)
+    if (max_cpg <= 2)
+        HL_SEG(2);
+    else if (max_cpg <= 4)
+        HL_SEG(4);
+    else if (max_cpg <= 8)
+        HL_SEG(8);
+    else if (max_cpg <= 16)
+        HL_SEG(16);
+    else if (max_cpg <= 24)
+        HL_SEG(24);
+    else
+        HL_SEG(32);
+#undef HL_SEG
+    HL_TRY_HIP(hipGetLastError());
+    cgs_finalize_seg_kernel<<<fin_blocks_total, kBlock, 0, s>>>(M, S, ns, mode);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

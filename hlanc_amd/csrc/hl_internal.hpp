@@ -141,6 +141,34 @@ void csr_plan_free(CsrPlan &p);
 int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double *y, VColumn vcol,
                 int val_prec, hipStream_t s);
 
+// ------------------------------------------------------- segmented launches
+// Several solves share one launch per kernel: each solve is a segment of the
+// grid, and every block computes exactly what it computes when the solve runs
+// alone (same block count, same partial-sum order), so results are bitwise the
+// same alone and in a batch.
+struct SegMember {               // per solve, fixed during a sweep
+    const SpmvBlk *blks;         // CSR operator (null for host operators)
+    const int *rp, *col;
+    const double *val;
+    double *chunk_part;
+    unsigned *chunk_cnt;
+    double *V;                   // tiled basis in use
+    long long vld;
+    double *r, *w, *h0, *h1, *hsub, *hdia, *partials;
+    Ctl *ctl;
+    int n, nb;                   // rows; CGS blocks = cgs_blocks(n)
+};
+struct SegStep {                 // per solve and Lanczos step
+    int mem;                     // index into the SegMember table
+    int j;                       // step index (projects onto j + 1 columns)
+    int spmv0;                   // first block in the SpMV launch
+    int cgs0;                    // first block in the CGS launch
+    int fin0;                    // first block in the finalize launch
+};
+int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total,
+                   int fin_blocks_total, int mode, hipStream_t s);
+int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s);
+
 // ------------------------------------------------------- basis kernels
 struct Reduce;
 // Fused head of a Lanczos step on a tiled basis: y = A (x s), V(:, j) = x s,

@@ -36,16 +36,16 @@ __device__ inline long long vcol_pos(long long r, long long ld, int j, bool tile
     return tiled ? vpos<true>(r, j, ld) : vpos<false>(r, j, ld);
 }
 
+// One work item of the SpMV (one block).
 template <typename AT, typename XT>
-__global__ __launch_bounds__(kBlock) void spmv_adaptive(
-    const SpmvBlk *__restrict__ blks, const int *__restrict__ rp, const int *__restrict__ col,
-    const AT *__restrict__ val, const XT *__restrict__ x, const double *__restrict__ xscale,
-    double *__restrict__ y, XT *__restrict__ vout, long long vld, int vj, bool vtiled,
-    double *__restrict__ chunk_part, unsigned *__restrict__ chunk_cnt)
+__device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict__ rp, const int *__restrict__ col,
+                                          const AT *__restrict__ val, const XT *__restrict__ x,
+                                          const double *__restrict__ xscale, double *__restrict__ y,
+                                          XT *__restrict__ vout, long long vld, int vj, bool vtiled,
+                                          double *__restrict__ chunk_part, unsigned *__restrict__ chunk_cnt)
 {
     __shared__ double prod[kNnzPerBlock];
     __shared__ double wred[kBlock / kWave];
-    const SpmvBlk b = blks[blockIdx.x];
     const double s = xscale ? *xscale : 1.0;
     const int tid = threadIdx.x;
 
@@ -100,6 +100,40 @@ __global__ __launch_bounds__(kBlock) void spmv_adaptive(
         y[b.row0] = t;
         __hip_atomic_store(&chunk_cnt[b.slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+template <typename AT, typename XT>
+__global__ __launch_bounds__(kBlock) void spmv_adaptive(
+    const SpmvBlk *__restrict__ blks, const int *__restrict__ rp, const int *__restrict__ col,
+    const AT *__restrict__ val, const XT *__restrict__ x, const double *__restrict__ xscale,
+    double *__restrict__ y, XT *__restrict__ vout, long long vld, int vj, bool vtiled,
+    double *__restrict__ chunk_part, unsigned *__restrict__ chunk_cnt)
+{
+    spmv_item<AT, XT>(blks[blockIdx.x], rp, col, val, x, xscale, y, vout, vld, vj, vtiled, chunk_part, chunk_cnt);
+}
+
+__device__ __forceinline__ int seg_find_spmv(const SegStep *S, int ns, int b)
+{
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S[mid].spmv0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// The SpMV step of every listed solve in one launch: y = A (r s), V(:, j) = r s.
+__global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__restrict__ M,
+                                                          const SegStep *__restrict__ S, int ns)
+{
+    const int i = seg_find_spmv(S, ns, blockIdx.x);
+    const SegStep st = S[i];
+    const SegMember mb = M[st.mem];
+    spmv_item<double, double>(mb.blks[blockIdx.x - st.spmv0], mb.rp, mb.col, mb.val, mb.r, &mb.ctl->xscale, mb.w,
+                              mb.V, mb.vld, st.j, true, mb.chunk_part, mb.chunk_cnt);
 }
 
 // Fused Lanczos step head: y = A (x s) (the SpMV), V(:, j) = x s, and the
@@ -484,6 +518,18 @@ int launch_spmv_project(const CsrPlan &p, const double *x, const double *xscale,
     HL_TRY_HIP(hipGetLastError());
     project_finalize_kernel<<<m + 1, kBlock, 0, s>>>(p.nblk, m, ipart, p.long_rows, p.nlong, x, xscale, y, V, L.ld,
                                                      j, h_out, fin, red.counter, ctl);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+}  // namespace hl
+
+namespace hl {
+
+int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s)
+{
+    if (ns <= 0 || blocks_total <= 0) return HL_OK;
+    spmv_seg_kernel<<<blocks_total, kBlock, 0, s>>>(M, S, ns);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }
