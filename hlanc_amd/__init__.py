@@ -55,14 +55,21 @@ class Info(ctypes.Structure):
 
 
 class Profile(ctypes.Structure):
-    """hl_profile: per kernel class (0 SpMV, 1 CGS, 2 V.Q, 3 other, 4 fused SpMV + first CGS half)
+    """hl_profile: per kernel class (0 SpMV, 1 CGS, 2 V.Q, 3 other; 4, 5 unused)
     launches, seconds, algorithmic bytes."""
     _fields_ = [("launches", ctypes.c_longlong * 6), ("seconds", ctypes.c_double * 6), ("bytes", ctypes.c_double * 6)]
 
 
-PROFILE_CLASSES = ["spmv", "cgs", "vq", "other", "spmv_proj"]
+PROFILE_CLASSES = ["spmv", "cgs", "vq", "other"]
 HL_FLAG_PROFILE = 2
-HL_FLAG_NO_FUSE = 4
+
+
+class SolveItem(ctypes.Structure):
+    """hl_solve_item: one solve of hl_ds_solve_batch."""
+    _fields_ = [("A", ctypes.c_void_p), ("nev", ctypes.c_int), ("ncv", ctypes.c_int), ("values", _dp),
+                ("vectors", _dp), ("ldv", ctypes.c_int), ("pad0", ctypes.c_int), ("opts", Options),
+                ("arrival", ctypes.c_double), ("info", Info), ("status", ctypes.c_int), ("pad1", ctypes.c_int),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
 
 _lib = None
 
@@ -101,6 +108,8 @@ _SIGS = [
     ("hl_ds_solve_host_op", ctypes.c_int, [ctypes.c_int, ctypes.c_int, HOST_OP, ctypes.c_void_p, ctypes.c_int,
                                            ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.POINTER(Options),
                                            ctypes.POINTER(Info)]),
+    ("hl_ds_solve_batch", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(SolveItem), ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double]),
 ]
 
 
@@ -336,6 +345,68 @@ def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 
                                        ctypes.byref(o), ctypes.byref(info))
     _check(rc, allow=(HL_OK, HL_NOT_CONVERGED))
     return SolveResult(rc, values[:nev].copy(), np.asarray(vectors[:n]), info)
+
+
+@dataclass
+class BatchResult:
+    status: int
+    values: np.ndarray
+    vectors: Optional[np.ndarray]
+    info: Info
+    t_admit: float
+    t_done: float
+    arrival: float
+
+    @property
+    def latency(self) -> float:
+        """Seconds from arrival to completion."""
+        return self.t_done - self.arrival
+
+
+def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, seeds=None,
+                   v0s=None, use_mfma: Optional[bool] = None, vectors: bool = True, arrivals=None,
+                   max_active: int = 0, hbm_budget: float = 0.0, flags: int = 0):
+    """Many solves on one device with continuous batching (hl_ds_solve_batch).
+
+    ``ops`` are dcsrmv_operator_gpu on ``dev``; nev, ncv, tol and maxiter are
+    one value for all solves or one per solve.  ``arrivals`` (seconds after the
+    call, non-decreasing) delays admission, as a serving loop sees requests
+    arrive.  Each result is bitwise what ds_solver_gpu returns for the same
+    operator and options."""
+    k = len(ops)
+    per = lambda x, i: x[i] if isinstance(x, (list, tuple, np.ndarray)) else x  # noqa: E731
+    items = (SolveItem * k)()
+    keep = []
+    for i, op in enumerate(ops):
+        if not isinstance(op, dcsrmv_operator_gpu):
+            raise HLancError(-1, "ds_solve_batch takes dcsrmv_operator_gpu operators")
+        n = op.nrows
+        nev_i, ncv_i = int(per(nev, i)), int(per(ncv, i))
+        vals = np.zeros(max(ncv_i, 1))
+        vecs = np.zeros((max(ncv_i, 1), n)).T if vectors else None
+        seed = 0 if seeds is None else int(seeds[i])
+        v0 = None if v0s is None else v0s[i]
+        o = options(float(per(tol, i)), int(per(maxiter, i)), v0, seed, use_mfma, flags=flags)
+        it = items[i]
+        it.A = op.h
+        it.nev = nev_i
+        it.ncv = ncv_i
+        it.values = _d(vals)
+        it.vectors = vecs.ctypes.data_as(_dp) if vecs is not None else None
+        it.ldv = n
+        it.opts = o
+        it.arrival = 0.0 if arrivals is None else float(arrivals[i])
+        keep.append((vals, vecs, o))
+    rc = lib().hl_ds_solve_batch(dev, items, k, int(max_active), float(hbm_budget))
+    _check(rc)
+    out = []
+    for i in range(k):
+        it = items[i]
+        vals, vecs, _ = keep[i]
+        info = Info.from_buffer_copy(it.info)
+        out.append(BatchResult(it.status, vals[:it.nev].copy(), None if vecs is None else np.asarray(vecs),
+                               info, it.t_admit, it.t_done, it.arrival))
+    return out
 
 
 # ------------------------------------------------------- basis operations

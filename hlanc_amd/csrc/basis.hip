@@ -253,115 +253,6 @@ __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMembe
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
 
-// Same pass over a tiled FP64 basis with 16-byte loads: 8 column groups of 8
-// lanes, lane q of a group holding rows 2q and 2q+1 of the 16-row tile, so a
-// load instruction reads eight 128-byte column segments of one tile.
-template <int MAXC8, bool NT>
-__global__ __launch_bounds__(kBlock) void cgs_wide_kernel(int n, int m, const double *__restrict__ V, long long ldv,
-                                                          const double *w_in, double *w_out,
-                                                          const double *__restrict__ h_in, double *h_out,
-                                                          double *partials, unsigned *counter, Ctl *ctl,
-                                                          double *h_sub, double *h_dia, int mode, int j)
-{
-    if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return;
-    constexpr int G = 8;
-    __shared__ double hs[G * MAXC8];
-    __shared__ double red[kBlock / kWave][G * MAXC8 + 1];
-    __shared__ double vals[G * MAXC8 + 1];
-    __shared__ double scratch[kBlock];
-    __shared__ double fin[G * MAXC8 + 1];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
-    const int g = lane >> 3;
-    const int q = lane & 7;
-    const int cpg = (m + G - 1) / G;
-    const int c0 = g * cpg;
-    const bool has_h = h_in != nullptr;
-    for (int c = threadIdx.x; c < G * MAXC8; c += kBlock) hs[c] = (has_h && c < m) ? h_in[c] : 0.0;
-    __syncthreads();
-    double acc[MAXC8];
-#pragma unroll
-    for (int c = 0; c < MAXC8; ++c) acc[c] = 0.0;
-    double nrm = 0.0;
-    const long long ntiles = (n + 15) / 16;
-    const long long W = (long long)gridDim.x * (kBlock / kWave);
-    for (long long t = (long long)blockIdx.x * (kBlock / kWave) + wv; t < ntiles; t += W) {
-        const long long r0 = t * 16 + 2 * q;
-        const bool ok0 = r0 < n, ok1 = r0 + 1 < n;
-        dv2 wv2 = dv2{0.0, 0.0};
-        if (ok0) {
-            wv2 = *reinterpret_cast<const dv2 *>(&w_in[r0]);
-            if (!ok1) wv2.y = 0.0;
-        }
-        dv2 v[MAXC8];
-#pragma unroll
-        for (int c = 0; c < MAXC8; ++c) {
-            const int colj = c0 + c;
-            v[c] = dv2{0.0, 0.0};
-            if (ok0 && c < cpg && colj < m) {
-                const dv2 *ptr = reinterpret_cast<const dv2 *>(&V[((t * ldv + colj) << 4) + 2 * q]);
-                v[c] = NT ? __builtin_nontemporal_load(ptr) : *ptr;
-                if (!ok1) v[c].y = 0.0;
-            }
-        }
-        double a0 = wv2.x, a1 = wv2.y;
-        if (has_h) {
-            double p0 = 0.0, p1 = 0.0;
-#pragma unroll
-            for (int c = 0; c < MAXC8; ++c) {
-                p0 += v[c].x * hs[c0 + c];
-                p1 += v[c].y * hs[c0 + c];
-            }
-            p0 += __shfl_xor(p0, 8);
-            p1 += __shfl_xor(p1, 8);
-            p0 += __shfl_xor(p0, 16);
-            p1 += __shfl_xor(p1, 16);
-            p0 += __shfl_xor(p0, 32);
-            p1 += __shfl_xor(p1, 32);
-            a0 -= p0;
-            a1 -= p1;
-        }
-        if (g == 0 && ok0) {
-            if (w_out) {
-                if (ok1)
-                    *reinterpret_cast<dv2 *>(&w_out[r0]) = dv2{a0, a1};
-                else
-                    w_out[r0] = a0;
-            }
-            nrm += a0 * a0 + a1 * a1;
-        }
-#pragma unroll
-        for (int c = 0; c < MAXC8; ++c) acc[c] += v[c].x * a0 + v[c].y * a1;
-    }
-#pragma unroll
-    for (int c = 0; c < MAXC8; ++c) {
-        double t = acc[c];
-        t += __shfl_xor(t, 1);
-        t += __shfl_xor(t, 2);
-        t += __shfl_xor(t, 4);
-        acc[c] = t;
-    }
-    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
-    if (q == 0) {
-#pragma unroll
-        for (int c = 0; c < MAXC8; ++c)
-            if (c < cpg && c0 + c < m) red[wv][c0 + c] = acc[c];
-    }
-    if (lane == 0) red[wv][m] = nrm;
-    __syncthreads();
-    const int M1 = m + 1;
-    for (int c = threadIdx.x; c < M1; c += kBlock) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) t += red[w][c];
-        vals[c] = t;
-    }
-    __syncthreads();
-    if (!reduce_blocks(vals, M1, partials, counter, scratch, fin)) return;
-    if (h_out)
-        for (int c = threadIdx.x; c < m; c += kBlock) h_out[c] = fin[c];
-    if (threadIdx.x == 0) run_epilogue(mode, j, fin[m], h_in, ctl, h_sub, h_dia);
-}
-
 // Read-only streaming probe: every block sums a contiguous share with 16-byte
 // loads; measures the HBM read rate the box reaches.
 __global__ __launch_bounds__(kBlock) void stream_read_kernel(const dv2 *__restrict__ a, long long n2,
@@ -576,36 +467,6 @@ int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, d
     if (nb > red.max_blocks) {
         set_error("CGS pass: reduction workspace too small");
         return HL_ERR_STATE;
-    }
-    if constexpr (TILED && std::is_same<VT, double>::value) {
-        if (tuning().cgs_flags & 4) { // 16-byte-load variant
-            const int cpg8 = (m + 7) / 8;
-            const bool nt = (tuning().cgs_flags & 1) != 0;
-#define HL_CGSW(MC)                                                                                            \
-    {                                                                                                          \
-        if (nt)                                                                                                \
-            cgs_wide_kernel<MC, true><<<nb, kBlock, 0, s>>>(n, m, V, ldv, w_in, w_out, h_in, h_out,            \
-                                                            red.partials, red.counter, ctl, h_sub, h_dia,      \
-                                                            mode, j);                                          \
-        else                                                                                                   \
-            cgs_wide_kernel<MC, false><<<nb, kBlock, 0, s>>>(n, m, V, ldv, w_in, w_out, h_in, h_out,           \
-                                                             red.partials, red.counter, ctl, h_sub, h_dia,     \
-                                                             mode, j);                                         \
-    }
-            if (cpg8 <= 2)
-                HL_CGSW(2)
-            else if (cpg8 <= 4)
-                HL_CGSW(4)
-            else if (cpg8 <= 8)
-                HL_CGSW(8)
-            else if (cpg8 <= 12)
-                HL_CGSW(12)
-            else
-                HL_CGSW(16)
-#undef HL_CGSW
-            HL_TRY_HIP(hipGetLastError());
-            return HL_OK;
-        }
     }
     const int cpg = (m + kGroups - 1) / kGroups;
     const int fl = (tuning().cgs_flags & 8) ? 9 : (tuning().cgs_flags & 3);

@@ -61,37 +61,56 @@ bool is_device_ptr(const void *p, int dev)
     return a.type == hipMemoryTypeDevice && a.device == dev;
 }
 
-int solve_common(hl::Solver &S, int dev, int n, int nev, int ncv, double *values, double *vectors, int ldv,
-                 const hl_options *opts, hl_info *info, bool host_op)
+// Argument rules of ds_solver (HLanc.hpp:1649-1660) plus this library's
+// limits (ncv <= 128: the CGS kernels' column capacity).
+bool solve_args_ok(int n, int nev, int ncv, int ldv, const double *values, bool need_vectors,
+                   const double *vectors, const hl_options &o, const char *who)
+{
+    if (n <= 0 || nev <= 0 || ncv <= nev || ncv > n || ncv > 128 || !values ||
+        (need_vectors && !vectors) || (vectors && ldv < n)) {
+        set_error("%s: need 0 < nev < ncv <= min(n, 128) and ldv >= n (n=%d nev=%d ncv=%d ldv=%d)", who, n, nev,
+                  ncv, ldv);
+        return false;
+    }
+    if (o.basis_prec != HL_PREC_F64) {
+        set_error("%s: this entry point takes the FP64 basis (basis_prec = HL_PREC_F64)", who);
+        return false;
+    }
+    return true;
+}
+
+// One solve is an engine with one member.
+int solve_common(hl::Member &m, int dev, int n, int nev, int ncv, double *values, double *vectors, int ldv,
+                 const hl_options *opts, hl_info *info)
 {
     hl_options o;
     hl_options_default(&o);
     if (opts) o = *opts;
     hl_info inf;
     std::memset(&inf, 0, sizeof(inf));
-    if (n <= 0 || nev <= 0 || ncv <= nev || ncv > n || ncv > 128 || ldv < n || !values || !vectors) {
-        set_error("ds_solver: need 0 < nev < ncv <= min(n, 128) and ldv >= n (n=%d nev=%d ncv=%d ldv=%d)", n,
-                  nev, ncv, ldv);
-        if (info) *info = inf;
-        return HL_ERR_ARG;
-    }
-    if (o.basis_prec != HL_PREC_F64) {
-        set_error("ds_solver: this entry point takes the FP64 basis (use hl_ds_solve_mixed)");
-        return HL_ERR_ARG;
-    }
+    if (info) *info = inf;
+    if (!solve_args_ok(n, nev, ncv, ldv, values, true, vectors, o, "ds_solver")) return HL_ERR_ARG;
     int rc = check_device(dev);
     if (rc) return rc;
     DeviceGuard g(dev);
-    if ((rc = S.ws.init(dev, n, ncv, o.basis_prec, host_op, S.A ? S.A->nblk : 0))) {
-        S.ws.release();
-        if (info) *info = inf;
-        return rc;
-    }
-    rc = S.run(nev, values, vectors, ldv, o, inf);
-    inf.ret = rc;
-    S.ws.release();
-    if (info) *info = inf;
-    return rc;
+    m.dev = dev;
+    m.n = n;
+    m.nev0 = nev;
+    m.ncv = ncv;
+    m.opts = o;
+    m.values = values;
+    m.vectors = vectors;
+    m.ldv_host = ldv;
+    hl::Engine eng;
+    if ((rc = eng.init(dev))) return rc;
+    eng.prof.on = (o.flags & HL_FLAG_PROFILE) != 0;
+    std::vector<hl::Member *> q{&m};
+    rc = eng.run(q, 1, 0.0);
+    if (!rc) rc = m.err;
+    if (eng.prof.on && !rc) thread_profile() = eng.prof.tot;
+    if (rc) return rc;
+    if (info) *info = m.info;
+    return m.ret;
 }
 
 } // namespace
@@ -340,12 +359,8 @@ int hl_tune_set(const char *key, int value)
     Tuning &t = tuning();
     if (!std::strcmp(key, "cgs_blocks_max") && value >= 1 && value <= kMaxReduceBlocks)
         t.cgs_blocks_max = value;
-    else if (!std::strcmp(key, "cgs_flags") && value >= 0 && value <= 15)
+    else if (!std::strcmp(key, "cgs_flags") && value >= 0 && value <= 15 && !(value & 4))
         t.cgs_flags = value;
-    else if (!std::strcmp(key, "fuse_head") && (value == 0 || value == 1))
-        t.fuse_head = value;
-    else if (!std::strcmp(key, "basis_tiled") && (value == 0 || value == 1))
-        t.basis_tiled = value;
     else {
         set_error("hl_tune_set: unknown key or value out of range: %s=%d", key, value);
         return HL_ERR_ARG;
@@ -361,10 +376,6 @@ int hl_tune_get(const char *key, int *value)
         *value = t.cgs_blocks_max;
     else if (!std::strcmp(key, "cgs_flags"))
         *value = t.cgs_flags;
-    else if (!std::strcmp(key, "basis_tiled"))
-        *value = t.basis_tiled;
-    else if (!std::strcmp(key, "fuse_head"))
-        *value = t.fuse_head;
     else {
         set_error("hl_tune_get: unknown key %s", key);
         return HL_ERR_ARG;
@@ -402,19 +413,79 @@ int hl_ds_solve_csr(hl_csr_t A, int nev, int ncv, double *values, double *vector
         set_error("ds_solver: the operator must be square");
         return HL_ERR_ARG;
     }
-    hl::Solver S;
-    S.A = &A->plan;
-    return solve_common(S, A->plan.dev, A->plan.nrows, nev, ncv, values, vectors, ldv, opts, info, false);
+    hl::Member m;
+    m.A = &A->plan;
+    return solve_common(m, A->plan.dev, A->plan.nrows, nev, ncv, values, vectors, ldv, opts, info);
 }
 
 int hl_ds_solve_host_op(int dev, int n, hl_host_op_fn op, void *ctx, int nev, int ncv, double *values,
                         double *vectors, int ldv, const hl_options *opts, hl_info *info)
 {
     if (!op) return HL_ERR_ARG;
-    hl::Solver S;
-    S.host_op = op;
-    S.host_ctx = ctx;
-    return solve_common(S, dev, n, nev, ncv, values, vectors, ldv, opts, info, true);
+    hl::Member m;
+    m.host_op = op;
+    m.host_ctx = ctx;
+    return solve_common(m, dev, n, nev, ncv, values, vectors, ldv, opts, info);
+}
+
+int hl_ds_solve_batch(int dev, hl_solve_item *items, int count, int max_active, double hbm_budget)
+{
+    if (count < 0 || (count > 0 && !items)) return HL_ERR_ARG;
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    std::vector<hl::Member> ms(static_cast<size_t>(count));
+    std::vector<hl::Member *> queue;
+    bool profile = false;
+    double last_arrival = 0.0;
+    for (int i = 0; i < count; ++i) {
+        hl_solve_item &it = items[i];
+        std::memset(&it.info, 0, sizeof(it.info));
+        it.status = HL_ERR_ARG;
+        it.t_admit = it.t_done = -1.0;
+        const hl::CsrPlan *p = it.A ? &it.A->plan : nullptr;
+        if (!p || p->dev != dev || p->nrows != p->ncols) {
+            set_error("hl_ds_solve_batch: item %d needs a square operator on device %d", i, dev);
+            continue;
+        }
+        if (!solve_args_ok(p->nrows, it.nev, it.ncv, it.ldv, it.values, false, it.vectors, it.opts,
+                           "hl_ds_solve_batch") ||
+            it.arrival < last_arrival) {
+            if (it.arrival < last_arrival) set_error("hl_ds_solve_batch: arrivals must not decrease (item %d)", i);
+            continue;
+        }
+        last_arrival = it.arrival;
+        hl::Member &m = ms[i];
+        m.A = p;
+        m.dev = dev;
+        m.n = p->nrows;
+        m.nev0 = it.nev;
+        m.ncv = it.ncv;
+        m.opts = it.opts;
+        m.values = it.values;
+        m.vectors = it.vectors;
+        m.ldv_host = it.ldv;
+        m.arrival = it.arrival;
+        profile |= (it.opts.flags & HL_FLAG_PROFILE) != 0;
+        queue.push_back(&m);
+    }
+    hl::Engine eng;
+    if ((rc = eng.init(dev))) return rc;
+    eng.prof.on = profile;
+    rc = eng.run(queue, max_active, hbm_budget);
+    if (profile && !rc) thread_profile() = eng.prof.tot;
+    for (int i = 0; i < count; ++i) {
+        const hl::Member &m = ms[i];
+        if (!m.A) continue;
+        hl_solve_item &it = items[i];
+        it.status = m.err ? m.err : (m.phase == hl::Member::DONE ? m.ret : HL_ERR_ARG);
+        it.info = m.info;
+        it.info.ret = it.status;
+        it.t_admit = m.t_admit;
+        it.t_done = m.t_done;
+    }
+    return rc;
 }
 
 } // extern "C"
+

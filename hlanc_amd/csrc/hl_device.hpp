@@ -30,6 +30,8 @@ __device__ __forceinline__ void run_epilogue(int mode, int j, double nrm2, const
         ctl->cgs += 1;
         break;
     case EPI_PASS2: { // HLanc.hpp:1444-1458, first correction
+        ctl->p2 += 1;
+        ctl->m2sum += j + 1;
         h_dia[j] += h_in[j];
         const double r1 = sqrt(nrm2);
         if (r1 > kDgks * ctl->rnorm) {
@@ -44,6 +46,8 @@ __device__ __forceinline__ void run_epilogue(int mode, int j, double nrm2, const
         break;
     }
     case EPI_PASS3: { // second correction; failure zeroes resid (HLanc.hpp:1459-1463)
+        ctl->p3 += 1;
+        ctl->m3sum += j + 1;
         h_dia[j] += h_in[j];
         const double r1 = sqrt(nrm2);
         if (r1 > kDgks * ctl->rnorm) {

@@ -54,7 +54,8 @@ struct Ctl {
     int iter;        // DGKS failure count of the current step
     long long cgs;   // CGS passes executed (statistics)
     double nrm2;     // last reduced squared norm (standalone passes)
-    double pad[2];
+    long long p2, p3;       // DGKS corrections executed (first, second)
+    long long m2sum, m3sum; // sum of their basis widths (for byte counts)
 };
 
 // Epilogue modes of the reducing kernels.
@@ -68,14 +69,13 @@ enum Epi : int {
 };
 
 // ------------------------------------------------------------ tuning
-// Process-wide kernel parameters (hl_tune_set); fixed while solves run.
+// Process-wide kernel parameters (hl_tune_set); fixed while solves run.  The
+// solver always runs the split, tiled FP64 CGS pass; cgs_flags selects the
+// variant of the standalone pass (hl_cgs_pass, hl_bench_cgs) only.
 struct Tuning {
     int cgs_blocks_max = 2048; // grid cap of the CGS passes (<= kMaxReduceBlocks)
     int cgs_flags = 9;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile,
-                               // bit 2: 16-byte loads (tiled FP64 basis),
-                               // bit 3: split (partials kernel + finalize kernel)
-    int fuse_head = 0;         // fused SpMV + first CGS half (1) or two kernels (0)
-    int basis_tiled = 1;       // solver basis layout: 1 tiled, 0 column-major
+                               // bit 3: split (partials kernel + finalize kernel); bit 2 unused
 };
 Tuning &tuning();
 constexpr int kMaxReduceBlocks = 4096;
@@ -128,8 +128,6 @@ struct CsrPlan {
     unsigned *chunk_cnt = nullptr;
     int nslots = 0;
     double *x_stage = nullptr, *y_stage = nullptr;  // for host-pointer calls
-    int *long_rows = nullptr; // device: rows split into chunk items, ascending
-    int nlong = 0;
 };
 
 int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *col,
@@ -170,13 +168,6 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
 int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s);
 
 // ------------------------------------------------------- basis kernels
-struct Reduce;
-// Fused head of a Lanczos step on a tiled basis: y = A (x s), V(:, j) = x s,
-// h_out = V(:, 0:j+1)^T y, ctl->wnorm = ||y|| (EPI_WNORM).  ipart holds
-// nblk x (j+2) partial sums, fin j+2 doubles; red.counter is used (self-resetting).
-int launch_spmv_project(const CsrPlan &p, const double *x, const double *xscale, double *y, double *V, VLayout L,
-                        int j, double *h_out, double *ipart, double *fin, Reduce &red, Ctl *ctl, hipStream_t s);
-
 struct Reduce {
     double *partials = nullptr;  // [nblocks_max * (ncv_max + 1)]
     unsigned *counter = nullptr; // one counter, self-resetting
@@ -222,7 +213,7 @@ int launch_stream_read(const double *a, long long n, double *out, int blocks, hi
 int launch_cast_f32(long long n, const double *in, float *out, hipStream_t s);
 
 // ------------------------------------------------------------ profiling
-enum KClass : int { K_SPMV = 0, K_CGS = 1, K_VQ = 2, K_MISC = 3, K_SPMV_PROJ = 4, K_NCLASS = 6 };
+enum KClass : int { K_SPMV = 0, K_CGS = 1, K_VQ = 2, K_MISC = 3, K_NCLASS = 6 };
 
 // HIP-event timing of individual launches on one stream (profiled solves only).
 struct Profiler {

@@ -136,184 +136,6 @@ __global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__res
                               mb.V, mb.vld, st.j, true, mb.chunk_part, mb.chunk_cnt);
 }
 
-// Fused Lanczos step head: y = A (x s) (the SpMV), V(:, j) = x s, and the
-// partial sums of the first CGS half, V(:, 0:j+1)^T y and ||y||^2, for the rows
-// of one work item per block: the item's y is projected while it is in LDS, so
-// y and those basis rows are read once.  Each block writes its m + 1 partial
-// sums to column-major ipart[c * nitems + item] (zeros for chunk items);
-// project_finalize_kernel adds them in item order.
-template <int MAXC, typename AT>
-__global__ __launch_bounds__(kBlock) void spmv_project_kernel(
-    const SpmvBlk *__restrict__ blks, int nitems, const int *__restrict__ rp, const int *__restrict__ col,
-    const AT *__restrict__ val, const double *__restrict__ x, const double *__restrict__ xscale,
-    double *__restrict__ y, double *__restrict__ V, long long vld, int j, double *__restrict__ chunk_part,
-    unsigned *__restrict__ chunk_cnt, double *__restrict__ ipart)
-{
-    __shared__ double prod[kNnzPerBlock];
-    __shared__ double ys[kRowsPerBlock], xs[kRowsPerBlock];
-    __shared__ double wred[kBlock / kWave];
-    __shared__ double red[kBlock / kWave][dev::kGroups * MAXC + 1];
-    const int tid = threadIdx.x;
-    const int lane = tid & (kWave - 1);
-    const int wv = tid / kWave;
-    const int m = j + 1;
-    const double s = *xscale;
-    const SpmvBlk b = blks[blockIdx.x];
-
-    if (b.tpr == 0) { // chunk of a long row: y only (its projection is added by the finalize kernel)
-        for (int c = tid; c <= m; c += kBlock) ipart[(long long)c * nitems + blockIdx.x] = 0.0;
-        double a = 0.0;
-        for (int p = b.p0 + tid; p < b.p1; p += kBlock)
-            a += static_cast<double>(__builtin_nontemporal_load(&val[p])) *
-                 (x[__builtin_nontemporal_load(&col[p])] * s);
-        for (int off = kWave / 2; off > 0; off >>= 1) a += __shfl_xor(a, off);
-        if (lane == 0) wred[wv] = a;
-        __syncthreads();
-        if (tid != 0) return;
-        double tot = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) tot += wred[w];
-        if (b.chunk == 0) V[vpos<true>(b.row0, j, vld)] = x[b.row0] * s;
-        if (b.nchunks == 1) {
-            y[b.row0] = tot;
-            return;
-        }
-        __hip_atomic_store(&chunk_part[b.slot + b.chunk], tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned prev =
-            __hip_atomic_fetch_add(&chunk_cnt[b.slot], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev + 1 == static_cast<unsigned>(b.nchunks)) {
-            double t = 0.0;
-            for (int c = 0; c < b.nchunks; ++c)
-                t += __hip_atomic_load(&chunk_part[b.slot + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            y[b.row0] = t;
-            __hip_atomic_store(&chunk_cnt[b.slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        return;
-    }
-
-    const int nnz = b.p1 - b.p0;
-    for (int i = tid; i < nnz; i += kBlock) {
-        const int p = b.p0 + i;
-        prod[i] = static_cast<double>(__builtin_nontemporal_load(&val[p])) *
-                  (x[__builtin_nontemporal_load(&col[p])] * s);
-    }
-    for (int r = b.row0 + tid; r < b.row1; r += kBlock) {
-        const double xv = x[r] * s;
-        V[vpos<true>(r, j, vld)] = xv;
-        xs[r - b.row0] = xv;
-    }
-    __syncthreads();
-    const int tpr = b.tpr;
-    const int nrows = b.row1 - b.row0;
-    {
-        const int lr = tid / tpr, ll = tid & (tpr - 1);
-        const int rstep = kBlock / tpr;
-        for (int base = 0; base < nrows; base += rstep) {
-            const int rr = base + lr;
-            double a = 0.0;
-            if (rr < nrows) {
-                const int r = b.row0 + rr;
-                const int a0 = rp[r] - b.p0, a1 = rp[r + 1] - b.p0;
-                for (int k = a0 + ll; k < a1; k += tpr) a += prod[k];
-            }
-            for (int off = tpr >> 1; off > 0; off >>= 1) a += __shfl_xor(a, off);
-            if (rr < nrows && ll == 0) {
-                y[b.row0 + rr] = a;
-                ys[rr] = a;
-            }
-        }
-    }
-    __syncthreads();
-    // project this item's rows: 16-row slices per wave, 4 column groups
-    const int g = lane / dev::kTileRows;
-    const int rl = lane % dev::kTileRows;
-    const int cpg = (m + dev::kGroups - 1) / dev::kGroups;
-    const int c0 = g * cpg;
-    double acc[MAXC];
-#pragma unroll
-    for (int c = 0; c < MAXC; ++c) acc[c] = 0.0;
-    double nrm = 0.0;
-    for (int q = wv * dev::kTileRows; q < nrows; q += kBlock / dev::kGroups) {
-        const int rr = q + rl;
-        const bool ok = rr < nrows;
-        const double a = ok ? ys[rr] : 0.0;
-        const long long r = b.row0 + rr;
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c) {
-            const int colj = c0 + c;
-            if (ok && c < cpg && colj < m) {
-                const double v = (colj == j) ? xs[rr] : __builtin_nontemporal_load(&V[vpos<true>(r, colj, vld)]);
-                acc[c] += v * a;
-            }
-        }
-        if (g == 0 && ok) nrm += a * a;
-    }
-#pragma unroll
-    for (int c = 0; c < MAXC; ++c) {
-        double t = acc[c];
-        t += __shfl_xor(t, 1);
-        t += __shfl_xor(t, 2);
-        t += __shfl_xor(t, 4);
-        t += __shfl_xor(t, 8);
-        acc[c] = t;
-    }
-    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
-    if (rl == 0) {
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c)
-            if (c < cpg && c0 + c < m) red[wv][c0 + c] = acc[c];
-    }
-    if (lane == 0) red[wv][m] = nrm;
-    __syncthreads();
-    for (int c = tid; c <= m; c += kBlock) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) t += red[w][c];
-        ipart[(long long)c * nitems + blockIdx.x] = t;
-    }
-}
-
-// One block per column c of [h ; ||y||^2]: adds the items' partial sums in item
-// order, then the rows split into chunk items in long_rows order, and the last
-// column block to finish runs the EPI_WNORM epilogue.
-__global__ __launch_bounds__(kBlock) void project_finalize_kernel(
-    int nitems, int m, const double *__restrict__ ipart, const int *__restrict__ long_rows, int nlong,
-    const double *__restrict__ x, const double *__restrict__ xscale, const double *__restrict__ y,
-    const double *__restrict__ V, long long vld, int j, double *__restrict__ h_out, double *fin, unsigned *counter,
-    Ctl *ctl)
-{
-    __shared__ double sred[kBlock];
-    __shared__ int is_last;
-    const int c = blockIdx.x;
-    const int tid = threadIdx.x;
-    const double *pc = ipart + (long long)c * nitems;
-    double t = 0.0;
-    for (int i = tid; i < nitems; i += kBlock) t += pc[i];
-    sred[tid] = t;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (tid < w) sred[tid] += sred[tid + w];
-        __syncthreads();
-    }
-    if (tid == 0) {
-        double tot = sred[0];
-        const double s = *xscale;
-        for (int q = 0; q < nlong; ++q) {
-            const int r = long_rows[q];
-            const double a = y[r];
-            tot += (c == m) ? a * a : ((c == j) ? x[r] * s : V[vpos<true>(r, c, vld)]) * a;
-        }
-        fin[c] = tot;
-        if (c < m) h_out[c] = tot;
-        __threadfence();
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev + 1 == gridDim.x);
-    }
-    __syncthreads();
-    if (!is_last || tid != 0) return;
-    __threadfence();
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    dev::run_epilogue(EPI_WNORM, j, fin[m], nullptr, ctl, nullptr, nullptr);
-}
-
 template <typename T>
 int dev_upload(T **dst, const T *src, size_t count)
 {
@@ -445,19 +267,12 @@ int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *c
     HL_TRY_HIP(hipMemset(p.chunk_cnt, 0, sizeof(unsigned) * std::max(slots, 1)));
     if ((rc = dev_upload<double>(&p.x_stage, nullptr, static_cast<size_t>(ncols)))) return rc;
     if ((rc = dev_upload<double>(&p.y_stage, nullptr, static_cast<size_t>(nrows)))) return rc;
-
-    std::vector<int> long_rows; // rows split into chunk items, projected by the finalize kernel
-    for (const SpmvBlk &b : blks)
-        if (b.tpr == 0 && b.chunk == 0) long_rows.push_back(b.row0);
-    p.nlong = static_cast<int>(long_rows.size());
-    if ((rc = dev_upload(&p.long_rows, long_rows.data(), long_rows.size()))) return rc;
     return HL_OK;
 }
 
 void csr_plan_free(CsrPlan &p)
 {
-    void *ptrs[] = {p.row_ptr, p.col, p.val, p.val32, p.blks, p.chunk_part, p.chunk_cnt, p.x_stage, p.y_stage,
-                    p.long_rows};
+    void *ptrs[] = {p.row_ptr, p.col, p.val, p.val32, p.blks, p.chunk_part, p.chunk_cnt, p.x_stage, p.y_stage};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p = CsrPlan{};
@@ -480,44 +295,6 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
                                                                  vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled,
                                                                  p.chunk_part, p.chunk_cnt);
     }
-    HL_TRY_HIP(hipGetLastError());
-    return HL_OK;
-}
-
-}  // namespace hl
-
-namespace hl {
-
-int launch_spmv_project(const CsrPlan &p, const double *x, const double *xscale, double *y, double *V, VLayout L,
-                        int j, double *h_out, double *ipart, double *fin, Reduce &red, Ctl *ctl, hipStream_t s)
-{
-    const int m = j + 1;
-    if (!L.tiled || m > 128) {
-        set_error("fused SpMV/projection: unsupported shape (m=%d)", m);
-        return HL_ERR_ARG;
-    }
-    if (p.nblk == 0) return HL_OK;
-    const int cpg = (m + dev::kGroups - 1) / dev::kGroups;
-#define HL_SP(MC)                                                                                            \
-    spmv_project_kernel<MC, double><<<p.nblk, kBlock, 0, s>>>(p.blks, p.nblk, p.row_ptr, p.col, p.val, x,  \
-                                                              xscale, y, V, L.ld, j, p.chunk_part,         \
-                                                              p.chunk_cnt, ipart)
-    if (cpg <= 2)
-        HL_SP(2);
-    else if (cpg <= 4)
-        HL_SP(4);
-    else if (cpg <= 8)
-        HL_SP(8);
-    else if (cpg <= 16)
-        HL_SP(16);
-    else if (cpg <= 24)
-        HL_SP(24);
-    else
-        HL_SP(32);
-#undef HL_SP
-    HL_TRY_HIP(hipGetLastError());
-    project_finalize_kernel<<<m + 1, kBlock, 0, s>>>(p.nblk, m, ipart, p.long_rows, p.nlong, x, xscale, y, V, L.ld,
-                                                     j, h_out, fin, red.counter, ctl);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

@@ -122,10 +122,10 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
  * Process-wide kernel parameters, set before solves start (not thread-safe
  * against running solves).  Keys: "cgs_blocks_max" (grid cap of the CGS
  * passes, 1..2048), "cgs_flags" (bit 0 non-temporal basis loads, bit 1
- * prefetch of the next tile, bit 2 16-byte loads on the tiled basis, bit 3
- * finish each pass in a second kernel instead of a last-block reduction),
- * "basis_tiled" (1: 16-row tiled basis, 0: column-major), "fuse_head" (1:
- * SpMV and the first CGS half in one kernel).  No counterpart in the
+ * prefetch of the next tile, bit 3 finish each pass in a second kernel
+ * instead of a last-block reduction; bit 2 is refused).
+ * The solvers always run the split pass over a 16-row tiled basis; cgs_flags
+ * selects the variant of hl_cgs_pass and hl_bench_cgs.  No counterpart in the
  * reference (its kernels are fixed). */
 int hl_tune_set(const char *key, int value);
 int hl_tune_get(const char *key, int *value);
@@ -169,15 +169,13 @@ typedef struct {
 
 /* hl_options.flags bits. */
 #define HL_FLAG_NO_GRAPH 1 /* reserved: launch the Lanczos steps one by one */
-#define HL_FLAG_PROFILE 2  /* time every kernel launch with HIP events (slower);
-                              the DGKS correction passes are then launched only
-                              when needed, decided on the host */
-#define HL_FLAG_NO_FUSE 4  /* run the SpMV and the first CGS half as two kernels */
+#define HL_FLAG_PROFILE 2  /* time every SpMV, CGS and V.Q launch with HIP events */
 
-/* Per-kernel-class totals of the last profiled solve on the calling thread:
- * index 0 SpMV, 1 CGS passes, 2 V.Q GEMM, 3 other kernels, 4 fused SpMV +
- * first CGS half, 5 unused.  bytes are algorithmic HBM bytes (each operand
- * counted once per launch). */
+/* Per-kernel-class totals of the last profiled solve (or batch) on the calling
+ * thread: index 0 SpMV, 1 CGS passes, 2 V.Q GEMM, 3 other kernels, 4 and 5
+ * unused.  bytes are algorithmic HBM bytes (each operand counted once per
+ * launch; a DGKS correction pass counts only where it ran, from the device's
+ * own counters). */
 typedef struct {
     long long launches[6];
     double seconds[6];
@@ -197,6 +195,39 @@ void hl_options_default(hl_options *o);
  * traffic per Lanczos step.  opts/info may be NULL. */
 int hl_ds_solve_csr(hl_csr_t A, int nev, int ncv, double *values, double *vectors, int ldv,
                     const hl_options *opts, hl_info *info);
+
+/* One solve of a batch (hl_ds_solve_batch). */
+typedef struct {
+    hl_csr_t A;          /* square operator on the batch's device */
+    int nev, ncv;        /* as hl_ds_solve_csr */
+    double *values;      /* host, >= nev: ascending eigenvalues on success */
+    double *vectors;     /* host, column-major ldv x ncv as hl_ds_solve_csr, or
+                            NULL: values only (the Ritz-vector GEMM is skipped) */
+    int ldv;
+    int pad0;
+    hl_options opts;     /* basis_prec must be HL_PREC_F64 */
+    double arrival;      /* seconds after the call starts; the solve is admitted
+                            no earlier (0: at once); must not decrease along the
+                            array */
+    hl_info info;        /* out */
+    int status;          /* out: HL_OK, HL_NOT_CONVERGED or an error code */
+    int pad1;
+    double t_admit;      /* out: seconds after the call starts */
+    double t_done;       /* out: seconds after the call starts */
+} hl_solve_item;
+
+/* Many eigensolves on one device with continuous batching.  No counterpart in
+ * the reference, which solves one operator per ds_solver call: every solve in
+ * flight advances in lockstep, one segmented SpMV launch and four segmented
+ * CGS passes per Lanczos step serving all of them; solves are admitted in
+ * array order at restart boundaries, once their arrival time has passed,
+ * while fewer than max_active (<= 0: no cap) are in flight and their
+ * workspaces fit in hbm_budget bytes (<= 0: no cap; one solve is always
+ * admitted).  A workspace is allocated on admission and freed when its solve
+ * ends.  Each item's results are bitwise those of hl_ds_solve_csr on the same
+ * operator and options.  Returns HL_OK once every valid item has run (see each
+ * item's status), or the HIP error that stopped the batch. */
+int hl_ds_solve_batch(int dev, hl_solve_item *items, int count, int max_active, double hbm_budget);
 
 /* ds_solver_gpu<any callable> (HLanc.hpp:1868-1872): the operator is a host
  * callback called with host pointers, as the reference does (HLanc.hpp:1424). */

@@ -44,6 +44,41 @@ def test_cgs_pass_matches_numpy(n, m, with_h):
     assert abs(nrm.item() - w_ref @ w_ref) <= 1e-13 * (w_ref @ w_ref)
 
 
+@pytest.mark.parametrize("flags", [0, 1, 2, 3, 8, 9, 11])
+def test_cgs_pass_variants_match_numpy(flags):
+    # every tunable variant of the standalone pass (non-temporal loads,
+    # prefetch, split finalize) computes the same pass
+    import torch
+
+    n, m = 70001, 45
+    rng = np.random.default_rng(flags)
+    V = rng.uniform(-1, 1, (n, m))
+    w = rng.uniform(-1, 1, n)
+    h = rng.uniform(-1, 1, m)
+    Vd, wd, hd = dev(V), torch.tensor(w, device="cuda"), torch.tensor(h, device="cuda")
+    wout = torch.empty_like(wd)
+    hout = torch.zeros(m, dtype=torch.float64, device="cuda")
+    nrm = torch.zeros(1, dtype=torch.float64, device="cuda")
+    old = H.tune_get("cgs_flags")
+    try:
+        H.tune_set("cgs_flags", flags)
+        H.cgs_pass(0, n, m, Vd.data_ptr(), n, wd.data_ptr(), wout.data_ptr(), hd.data_ptr(), hout.data_ptr(),
+                   nrm.data_ptr())
+        torch.cuda.synchronize()
+    finally:
+        H.tune_set("cgs_flags", old)
+    w_ref = w - V @ h
+    h_ref = V.T @ w_ref
+    np.testing.assert_allclose(wout.cpu().numpy(), w_ref, rtol=0, atol=1e-13 * np.abs(V).sum(axis=1).max())
+    assert np.all(np.abs(hout.cpu().numpy() - h_ref) <= 1e-13 * (np.abs(V).T @ np.abs(w_ref)))
+    assert abs(nrm.item() - w_ref @ w_ref) <= 1e-13 * (w_ref @ w_ref)
+
+
+def test_cgs_flag_bit2_is_refused():
+    with pytest.raises(H.HLancError):
+        H.tune_set("cgs_flags", 4)
+
+
 @pytest.mark.parametrize("mfma", [True, False])
 @pytest.mark.parametrize("n,K,N", [(1000, 12, 1), (1000, 12, 11), (4099, 30, 11), (4099, 30, 30),
                                    (20000, 96, 33), (20000, 96, 96), (3001, 60, 64), (3001, 60, 65),

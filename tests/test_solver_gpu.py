@@ -183,61 +183,53 @@ def test_vq_backends_agree(mfma):
     np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
 
 
-@pytest.mark.parametrize("gen", ["stencil", "powerlaw", "arrow"])
-def test_fused_and_unfused_step_agree(gen):
-    # the fused SpMV + first CGS half and the two-kernel form of a Lanczos step
+def heavy_row_matrix(gen):
     if gen == "stencil":
         rp, col, val = W.stencil27(13, seed=8)
-        nev, ncv = 8, 24
-    elif gen == "powerlaw":  # rows of 2k-7k nnz: single-chunk long rows, projected by the last block
+        return rp, col, val, 8, 24
+    if gen == "powerlaw":  # rows of 2k-7k nnz: single-chunk long rows
         rp, col, val = W.chung_lu_laplacian(1 << 15, max_degree=20000, seed=4)
-        nev, ncv = 6, 20
-    else:  # an arrow matrix: two rows of > 8192 nnz, split into several chunks
-        rng = np.random.default_rng(6)
-        n = 30000
-        cols = np.arange(1, n)
-        far = np.setdiff1d(rng.choice(n, 9001, replace=False), [0, n // 2])[:9000]
-        rows = np.concatenate([np.zeros(n - 1, int), cols, [n // 2] * 9000, far])
-        cc = np.concatenate([cols, np.zeros(n - 1, int), far, [n // 2] * 9000])
-        vv = np.concatenate([np.full(2 * (n - 1), 0.01), np.full(18000, 0.02)])
-        A = sp.coo_matrix((vv, (rows, cc)), shape=(n, n)).tocsr() + sp.diags(rng.uniform(0, 1, n))
-        A.sum_duplicates()
-        A.sort_indices()
-        rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
-        nev, ncv = 6, 20
+        return rp, col, val, 6, 20
+    # an arrow matrix: two rows of > 8192 nnz, split into several chunks
+    rng = np.random.default_rng(6)
+    n = 30000
+    cols = np.arange(1, n)
+    far = np.setdiff1d(rng.choice(n, 9001, replace=False), [0, n // 2])[:9000]
+    rows = np.concatenate([np.zeros(n - 1, int), cols, [n // 2] * 9000, far])
+    cc = np.concatenate([cols, np.zeros(n - 1, int), far, [n // 2] * 9000])
+    vv = np.concatenate([np.full(2 * (n - 1), 0.01), np.full(18000, 0.02)])
+    A = sp.coo_matrix((vv, (rows, cc)), shape=(n, n)).tocsr() + sp.diags(rng.uniform(0, 1, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data, 6, 20
+
+
+@pytest.mark.parametrize("gen", ["stencil", "powerlaw", "arrow"])
+def test_heavy_rows_match_oracle(gen):
+    # stream items, single-chunk long rows and multi-chunk long rows in a solve
+    rp, col, val, nev, ncv = heavy_row_matrix(gen)
     n = len(rp) - 1
-    op = op_for(rp, col, val)
-    old = H.tune_get("fuse_head")
-    try:
-        H.tune_set("fuse_head", 1)
-        a = H.ds_solver_gpu(n, nev, ncv, op, tol=1e-10, maxiter=2000)
-        b = H.ds_solver_gpu(n, nev, ncv, op, tol=1e-10, maxiter=2000, flags=H.HL_FLAG_NO_FUSE)
-    finally:
-        H.tune_set("fuse_head", old)
+    r = H.ds_solver_gpu(n, nev, ncv, op_for(rp, col, val), tol=1e-10, maxiter=2000)
     o = O.ds_solver(O.Csr(rp, col, val), nev, ncv, tol=1e-10, maxiter=2000)
-    assert a.ret == b.ret == o.ret == 0
-    np.testing.assert_allclose(a.values, o.values, rtol=1e-10)
-    np.testing.assert_allclose(b.values, o.values, rtol=1e-10)
+    assert r.ret == o.ret == 0
+    np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
     A = sp.csr_matrix((val, col, rp))
     nrm = np.abs(o.values).max()
-    for r in (a, b):
-        for i in range(nev):
-            v = r.vectors[:, i]
-            assert np.linalg.norm(A @ v - r.values[i] * v) <= 1e-10 * nrm
+    for i in range(nev):
+        v = r.vectors[:, i]
+        assert np.linalg.norm(A @ v - r.values[i] * v) <= 1e-10 * nrm
 
 
-@pytest.mark.parametrize("flags,fuse", [(0, 0), (1, 0), (2, 0), (3, 0), (5, 0), (9, 0), (1, 1), (9, 1)])
-def test_kernel_variants_agree_with_oracle(flags, fuse):
-    # every tunable CGS variant and the fused step head compute the same solve
+@pytest.mark.parametrize("blocks", [1, 7, 256, 2048, 4096])
+def test_cgs_grid_cap_does_not_change_the_solve_beyond_rounding(blocks):
+    # the CGS grid (and with it the partial-sum order) follows cgs_blocks_max
     rp, col, val = W.stencil27(12, seed=11)
-    old = (H.tune_get("cgs_flags"), H.tune_get("fuse_head"))
+    old = H.tune_get("cgs_blocks_max")
     try:
-        H.tune_set("cgs_flags", flags)
-        H.tune_set("fuse_head", fuse)
+        H.tune_set("cgs_blocks_max", blocks)
         r = H.ds_solver_gpu(1728, 8, 24, op_for(rp, col, val), tol=1e-10, maxiter=1000)
     finally:
-        H.tune_set("cgs_flags", old[0])
-        H.tune_set("fuse_head", old[1])
+        H.tune_set("cgs_blocks_max", old)
     o = O.ds_solver(O.Csr(rp, col, val), 8, 24, tol=1e-10, maxiter=1000)
     assert r.ret == 0 and o.ret == 0
     np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
