@@ -55,12 +55,13 @@ class Info(ctypes.Structure):
 
 
 class Profile(ctypes.Structure):
-    """hl_profile: per kernel class (0 SpMV, 1 CGS, 2 V.Q, 3 other; 4, 5 unused)
-    launches, seconds, algorithmic bytes."""
+    """hl_profile: per kernel class (0 SpMV, 1 CGS passes, 2 V.Q, 3 other kernels,
+    4 DGKS correction passes, 5 host work between sweeps) launches, seconds,
+    algorithmic bytes."""
     _fields_ = [("launches", ctypes.c_longlong * 6), ("seconds", ctypes.c_double * 6), ("bytes", ctypes.c_double * 6)]
 
 
-PROFILE_CLASSES = ["spmv", "cgs", "vq", "other"]
+PROFILE_CLASSES = ["spmv", "cgs", "vq", "other", "dgks", "host"]
 HL_FLAG_PROFILE = 2
 
 

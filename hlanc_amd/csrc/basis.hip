@@ -227,17 +227,19 @@ __device__ __forceinline__ PassPtrs pass_ptrs(const SegMember &mb, int mode)
     }
 }
 
-// One CGS pass of every listed solve in a single launch (split form).
-template <int MAXC>
+// One CGS pass of every listed solve in a single launch (split form).  The
+// pass (MODE) is a template argument so that profiles tell the always-run
+// passes (EPI_WNORM, EPI_PASS1) from the DGKS corrections.
+template <int MAXC, int MODE>
 __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__restrict__ M,
-                                                         const SegStep *__restrict__ S, int ns, int mode)
+                                                         const SegStep *__restrict__ S, int ns)
 {
     const int i = seg_find(&S[0].cgs0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
     const SegStep st = S[i];
     const SegMember mb = M[st.mem];
-    const PassPtrs p = pass_ptrs(mb, mode);
+    const PassPtrs p = pass_ptrs(mb, MODE);
     cgs_block<MAXC, double, true, 9>(blockIdx.x - st.cgs0, mb.nb, mb.n, st.j + 1, mb.V, mb.vld, p.w_in, p.w_out,
-                                     p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub, mb.hdia, mode, st.j);
+                                     p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub, mb.hdia, MODE, st.j);
 }
 
 __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMember *__restrict__ M,
@@ -321,6 +323,83 @@ __global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, con
     __syncthreads();
     // block partial of ||r||^2; cgs_finalize_kernel (one block, m = 0) adds them
     if (threadIdx.x == 0) partials[blockIdx.x] = vals[0];
+}
+
+// The restart V.Q of every listed solve in one launch: Out = V Q(:, 0:N) on
+// tiled bases (ld K), then the dsapps residual update
+// r = Q(ncv-1, nev-1) rscale r + beta Out(:, res_col) (res_col < 0: beta term
+// absent) and each block's partial ||r||^2 in partials[block].
+template <int CB>
+__global__ __launch_bounds__(kBlock) void vq_seg_kernel(const SegVQ *__restrict__ T, int nt)
+{
+    extern __shared__ double qs[]; // K x CB
+    __shared__ double red[kBlock / kWave];
+    const int i = seg_find(&T[0].blk0, sizeof(SegVQ) / sizeof(int), nt, blockIdx.x);
+    const SegVQ t = T[i];
+    const int bid = blockIdx.x - t.blk0;
+    const double qrs = t.qlast * t.ctl->rscale;
+    double nrm = 0.0;
+    for (int c0 = 0; c0 < t.N; c0 += CB) {
+        __syncthreads(); // the previous chunk's readers of qs are done
+        for (int x = threadIdx.x; x < t.K * CB; x += kBlock) {
+            const int k = x / CB, c = x % CB;
+            qs[x] = (c0 + c < t.N) ? t.Q[(long long)(c0 + c) * t.K + k] : 0.0;
+        }
+        __syncthreads();
+        const int rl = (t.res_col >= c0 && t.res_col < c0 + CB) ? t.res_col - c0 : -1;
+        const bool upd = t.res_col >= 0 ? rl >= 0 : c0 == 0;
+        for (long long row = (long long)bid * kBlock + threadIdx.x; row < t.n; row += (long long)t.nb * kBlock) {
+            double acc[CB];
+#pragma unroll
+            for (int c = 0; c < CB; ++c) acc[c] = 0.0;
+            for (int k = 0; k < t.K; ++k) {
+                const double v = t.V[vpos<true>(row, k, t.K)];
+#pragma unroll
+                for (int c = 0; c < CB; ++c) acc[c] += v * qs[k * CB + c];
+            }
+            double rk = 0.0;
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                if (c0 + c < t.N) t.Out[vpos<true>(row, c0 + c, t.K)] = acc[c];
+                if (c == rl) rk = acc[c];
+            }
+            if (upd) { // dsapps residual (HLanc.hpp:1605-1614)
+                const double rr = qrs * t.r[row] + t.beta * rk;
+                t.r[row] = rr;
+                nrm += rr * rr;
+            }
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s += red[w];
+        t.partials[bid] = s;
+    }
+}
+
+// One block per listed solve: rnorm = sqrt(sum of its V.Q block partials).
+__global__ __launch_bounds__(kBlock) void vq_seg_finalize_kernel(const SegVQ *__restrict__ T)
+{
+    const SegVQ t = T[blockIdx.x];
+    cgs_finalize_block(0, t.nb, 0, t.partials, nullptr, nullptr, t.ctl, nullptr, nullptr, EPI_RNORM, 0);
+}
+
+// One block per member: hsub, hdia (ncv each) and the Ctl words, packed at
+// out + member * stride, so that one copy brings every member's state home.
+__global__ __launch_bounds__(kBlock) void seg_readback_kernel(const SegMember *__restrict__ M, int stride,
+                                                              double *__restrict__ out)
+{
+    const SegMember mb = M[blockIdx.x];
+    double *o = out + (long long)blockIdx.x * stride;
+    for (int x = threadIdx.x; x < mb.ncv; x += kBlock) {
+        o[x] = mb.hsub[x];
+        o[mb.ncv + x] = mb.hdia[x];
+    }
+    const double *c = reinterpret_cast<const double *>(mb.ctl);
+    for (int x = threadIdx.x; x < kCtlWords; x += kBlock) o[2 * mb.ncv + x] = c[x];
 }
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -688,11 +767,11 @@ int launch_scale_rscale(int n, double *r, double q, const Ctl *ctl, hipStream_t 
     return HL_OK;
 }
 
-int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total,
-                   int fin_blocks_total, int mode, hipStream_t s)
+namespace {
+template <int MODE>
+void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg, int blocks, hipStream_t s)
 {
-    if (ns <= 0) return HL_OK;
-#define HL_SEG(MC) cgs_seg_kernel<MC><<<cgs_blocks_total, kBlock, 0, s>>>(M, S, ns, mode)
+#define HL_SEG(MC) cgs_seg_kernel<MC, MODE><<<blocks, kBlock, 0, s>>>(M, S, ns)
     if (max_cpg <= 2)
         HL_SEG(2);
     else if (max_cpg <= 4)
@@ -706,8 +785,74 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
     else
         HL_SEG(32);
 #undef HL_SEG
+}
+} // namespace
+
+int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total, int mode,
+                   hipStream_t s)
+{
+    if (ns <= 0) return HL_OK;
+    if (max_cpg > 32) {
+        set_error("CGS pass: more than 128 columns");
+        return HL_ERR_ARG;
+    }
+    switch (mode) {
+    case EPI_WNORM:
+        cgs_seg_dispatch<EPI_WNORM>(M, S, ns, max_cpg, cgs_blocks_total, s);
+        break;
+    case EPI_PASS1:
+        cgs_seg_dispatch<EPI_PASS1>(M, S, ns, max_cpg, cgs_blocks_total, s);
+        break;
+    case EPI_PASS2:
+        cgs_seg_dispatch<EPI_PASS2>(M, S, ns, max_cpg, cgs_blocks_total, s);
+        break;
+    case EPI_PASS3:
+        cgs_seg_dispatch<EPI_PASS3>(M, S, ns, max_cpg, cgs_blocks_total, s);
+        break;
+    default:
+        set_error("CGS pass: bad mode %d", mode);
+        return HL_ERR_ARG;
+    }
     HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fin_blocks_total, int mode,
+                            hipStream_t s)
+{
+    if (ns <= 0) return HL_OK;
     cgs_finalize_seg_kernel<<<fin_blocks_total, kBlock, 0, s>>>(M, S, ns, mode);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int vq_seg_blocks(int n) { return grid_for(n, kBlock * 4, 1024); }
+
+int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, hipStream_t s)
+{
+    if (nt <= 0) return HL_OK;
+    if (max_K > 128 || max_K <= 0) {
+        set_error("V.Q: K=%d unsupported (K <= 128)", max_K);
+        return HL_ERR_ARG;
+    }
+#define HL_VQS(CB) vq_seg_kernel<CB><<<blocks_total, kBlock, sizeof(double) * (size_t)max_K * (CB), s>>>(T, nt)
+    if (max_N <= 8)
+        HL_VQS(8);
+    else if (max_N <= 16)
+        HL_VQS(16);
+    else
+        HL_VQS(32);
+#undef HL_VQS
+    HL_TRY_HIP(hipGetLastError());
+    vq_seg_finalize_kernel<<<nt, kBlock, 0, s>>>(T);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_seg_readback(const SegMember *M, int nm, int stride, double *out, hipStream_t s)
+{
+    if (nm <= 0) return HL_OK;
+    seg_readback_kernel<<<nm, kBlock, 0, s>>>(M, stride, out);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

@@ -8,18 +8,25 @@
 // passes (each a partials kernel and a per-column finalize kernel):
 //   * the basis V, the residual and the alpha/beta recurrence stay in HBM; the
 //     DGKS decisions are taken on the device (a correction pass that is not
-//     needed exits at once) and the host reads back 2 ncv doubles per sweep;
-//   * the ncv x ncv work stays on the host in FP64;
-//   * V Q of dsapps and of the Ritz-vector step is one GEMM into a ping-pong
-//     copy of V (no in-place column shuffling).
+//     needed exits at once);
+//   * after a sweep one kernel packs every member's alpha/beta and Ctl and one
+//     copy brings them home; the members' ncv x ncv work runs on host threads
+//     in FP64;
+//   * V Q of dsapps for every restarting member is one segmented GEMM launch
+//     into the other ping-pong basis (no in-place column shuffling), and the
+//     Ritz-vector step is one GEMM per finishing member.
 // A solve run alone is an engine with one member; every block of a segmented
 // launch computes what it computes in a one-member launch, so a member's
 // results are bitwise the same alone and in any batch.  Return values and the
 // default tol / restart cap are the reference's.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <thread>
 
 #include "hl_device.hpp"
@@ -49,15 +56,17 @@ int Member::start(hipStream_t s)
     dia.assign(ncv, 0.0);
     ritz.assign(ncv, 0.0);
     bounds.assign(ncv, 0.0);
+    zlast.assign(ncv, 0.0);
     q.assign((size_t)ncv * ncv, 0.0);
     e.assign(ncv, 0.0);
     work.assign(ncv, 0.0);
     ritz_saved.clear();
     bounds_saved.clear();
-    spmv = cgs = p2 = p3 = m2sum = m3sum = 0;
+    spmv = cgs = p2 = p3 = m2sum = m3sum = d_m23 = 0;
     nconv = 0;
     rnorm = 0.0;
     ret = HL_NOT_CONVERGED;
+    act = ACT_NONE;
     std::memset(W.ctl_h, 0, sizeof(Ctl));
     W.ctl_h->rscale = 1.0;
     W.ctl_h->xscale = 1.0;
@@ -95,63 +104,47 @@ int Member::host_apply(int j, hipStream_t s)
     return HL_OK;
 }
 
-// Device half of dsapps (HLanc.hpp:1591-1614): V := V Q(:, 0:nev) into the other
-// ping-pong buffer and resid := Q(ncv-1, nev-1) resid + beta V Q(:, nev).  The
-// host q stays untouched until the engine's next stream synchronisation.
-int Member::restart(hipStream_t s)
+// The read-back of a sweep: entries [sweep_k, ncv) of the tridiagonal (the rest
+// is the host's own, as the restart left it) and the device counters.
+void Member::absorb(const double *rb)
 {
-    Workspace &W = ws;
-    const bool has_beta = sub[nev] > 0.0;
-    const int N = has_beta ? nev + 1 : nev;
-    HL_TRY_HIP(hipMemcpyAsync(W.Qd, q.data(), sizeof(double) * (size_t)ncv * N, hipMemcpyHostToDevice, s));
-    const double qlast = q[(size_t)(nev - 1) * ncv + ncv - 1];
-    const int nxt = 1 - W.cur;
-    int rc = prof->begin(s);
-    if (rc) return rc;
-    if (has_beta) {
-        rc = launch_vq(n, ncv, N, W.V[W.cur], W.vl(), W.Qd, ncv, W.V[nxt], W.vl(), opts.use_mfma, nev, W.r, qlast,
-                       sub[nev], W.red, W.ctl, s);
-        if (!rc) rc = prof->end(s, K_VQ, 8.0 * n * (ncv + N + 2));
-    } else {
-        rc = launch_vq(n, ncv, N, W.V[W.cur], W.vl(), W.Qd, ncv, W.V[nxt], W.vl(), opts.use_mfma, -1, nullptr, 0.0,
-                       0.0, W.red, W.ctl, s);
-        if (!rc) rc = prof->end(s, K_VQ, 8.0 * n * (ncv + N));
-        if (!rc) rc = launch_scale_rscale(n, W.r, qlast, W.ctl, s);
-        if (!rc) rc = launch_norm(n, W.r, W.red, W.ctl, EPI_RNORM, s);
+    for (int i = sweep_k; i < ncv; ++i) {
+        sub[i] = rb[i];
+        dia[i] = rb[ncv + i];
     }
-    if (rc) return rc;
-    W.cur = nxt;
-    return HL_OK;
+    Ctl c;
+    std::memcpy(&c, rb + 2 * ncv, sizeof(Ctl));
+    rnorm = c.rnorm;
+    cgs = c.cgs;
+    d_m23 = (c.m2sum - m2sum) + 2 * (c.p2 - p2) + (c.m3sum - m3sum) + 2 * (c.p3 - p3);
+    p2 = c.p2;
+    p3 = c.p3;
+    m2sum = c.m2sum;
+    m3sum = c.m3sum;
+    spmv += sweep_np;
 }
 
 // The rest of ds_solver after a sweep (HLanc.hpp:1687-1736): Ritz values and
-// bounds, convergence, nev adaptation, exact-shift restart; or the end.
-int Member::after_sweep(hipStream_t s)
+// bounds, convergence, nev adaptation, exact-shift chase; or the end.
+void Member::decide(double *qslot)
 {
-    const int k = sweep_k, cnt = ncv - k;
-    for (int i = 0; i < cnt; ++i) {
-        sub[k + i] = ws.hh[i];
-        dia[k + i] = ws.hh[ncv + i];
-    }
-    rnorm = ws.ctl_h->rnorm;
-    cgs = ws.ctl_h->cgs;
-    p2 = ws.ctl_h->p2;
-    p3 = ws.ctl_h->p3;
-    m2sum = ws.ctl_h->m2sum;
-    m3sum = ws.ctl_h->m3sum;
-    spmv += sweep_np;
+    act = ACT_NONE;
     if (stage == 0) { // the initial dsaitr(0, nev0) is followed directly by the main loop
         stage = 1;
         restarts = iter;
         sweep_k = nev;
         sweep_np = np;
-        return HL_OK;
+        act = ACT_SWEEP;
+        return;
     }
     std::copy(dia.begin(), dia.end(), ritz.begin());
     for (int i = 0; i + 1 < ncv; ++i) e[i] = sub[i + 1];
-    if (tridiag_eig(ncv, ritz.data(), e.data(), q.data(), ncv)) // dsteqr failure (HLanc.hpp:1692)
-        return finish(s, false);
-    for (int i = 0; i < ncv; ++i) bounds[i] = rnorm * std::fabs(q[(size_t)i * ncv + ncv - 1]);
+    // dsteqr("I") (HLanc.hpp:1691); only Q's last row is used here (1694-1696)
+    if (tridiag_eig_last(ncv, ritz.data(), e.data(), zlast.data())) {
+        finish_host(qslot, false); // dsteqr failure (HLanc.hpp:1692)
+        return;
+    }
+    for (int i = 0; i < ncv; ++i) bounds[i] = rnorm * std::fabs(zlast[i]);
     ritz_saved = ritz;
     bounds_saved = bounds;
     const int np0 = ncv - nev0;
@@ -166,7 +159,10 @@ int Member::after_sweep(hipStream_t s)
             ++nev;
         }
     }
-    if (nconv >= nev0 || np == 0) return finish(s, true);
+    if (nconv >= nev0 || np == 0) {
+        finish_host(qslot, true);
+        return;
+    }
     const int nevbef = nev; // HLanc.hpp:1719-1730
     nev += std::min(nconv, np / 2);
     if (nev == 1 && ncv >= 6)
@@ -176,90 +172,87 @@ int Member::after_sweep(hipStream_t s)
     np = ncv - nev;
     if (nevbef < nev) sort_wanted(nev, np, ritz.data(), bounds.data());
     restart_chase(nev, np, ritz.data(), sub.data(), dia.data(), q.data());
-    int rc = restart(s);
-    if (rc) return rc;
+    // device half of dsapps (HLanc.hpp:1591-1614): V := V Q(:, 0:nev) and
+    // resid := Q(ncv-1, nev-1) resid + beta V Q(:, nev)
+    const bool has_beta = sub[nev] > 0.0;
+    vq_N = has_beta ? nev + 1 : nev;
+    vq_res = has_beta ? nev : -1;
+    vq_beta = has_beta ? sub[nev] : 0.0;
+    vq_qlast = q[(size_t)(nev - 1) * ncv + ncv - 1];
+    std::copy(q.begin(), q.begin() + (size_t)ncv * vq_N, qslot);
     ++iter;
     restarts = iter;
-    if (iter >= maxiter) return finish(s, true);
+    if (iter >= maxiter) { // the restart would feed no further sweep: the solve ends unconverged
+        finish_host(qslot, true);
+        return;
+    }
     sweep_k = nev;
     sweep_np = np;
-    return HL_OK;
+    act = ACT_RESTART;
 }
 
-// Ritz vectors and outputs (HLanc.hpp:1762-1843).  vectors_ok is false after a
-// tridiagonal-solver failure, which ends the solve without them.
-int Member::finish(hipStream_t s, bool vectors_ok)
+// Ritz values and the Ritz-vector Q (HLanc.hpp:1762-1840) into qslot (ncv x
+// ncv), or the failure return.  vectors_ok is false after a tridiagonal-solver
+// failure, which ends the solve without them.
+void Member::finish_host(double *qslot, bool vectors_ok)
 {
-    Workspace &W = ws;
-    const double eps23 = std::pow(kEps, 2.0 / 3.0);
-    phase = DONE;
+    act = ACT_FINISH;
+    with_vectors = false;
     ret = HL_NOT_CONVERGED;
-    if (vectors_ok && nconv >= nev0) {
-        const int np0 = ncv - nev0;
-        std::vector<double> idx(ncv);
-        std::vector<char> select(ncv, 0);
-        for (int j = 0; j < ncv; ++j) idx[j] = j + 1;
-        std::vector<double> rz = ritz_saved;
-        sort_wanted(nev0, np0, rz.data(), idx.data());
-        int reord = 0, numcnv = 0;
-        for (int j = 0; j < ncv; ++j) {
-            const double temp1 = std::max(eps23, std::fabs(rz[ncv - j - 1]));
-            const int jj = static_cast<int>(idx[ncv - j - 1]);
-            if (numcnv < nconv && bounds_saved[jj - 1] <= tol * temp1) {
-                select[jj - 1] = 1;
-                ++numcnv;
-                if (jj > nev0) reord = 1;
-            }
-        }
-        std::vector<double> ihd(dia), ihb(ncv, 0.0), iq((size_t)ncv * ncv), tau(ncv);
-        for (int i = 0; i + 1 < ncv; ++i) ihb[i] = sub[i + 1];
-        if (numcnv == nconv && !tridiag_eig(ncv, ihd.data(), ihb.data(), iq.data(), ncv)) {
-            if (reord) { // selected columns to the front (HLanc.hpp:1808-1831)
-                int left = 0, right = ncv - 1;
-                for (;;) {
-                    while (select[left]) ++left;
-                    while (!select[right]) --right;
-                    if (left > right) break;
-                    std::swap(ihd[left], ihd[right]);
-                    for (int i = 0; i < ncv; ++i)
-                        std::swap(iq[(size_t)left * ncv + i], iq[(size_t)right * ncv + i]);
-                    ++left;
-                    --right;
-                }
-            }
-            // dsesrt (HLanc.hpp:1626-1647): ascending values, Q columns alongside
-            std::vector<int> order(nconv);
-            for (int i = 0; i < nconv; ++i) order[i] = i;
-            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ihd[a] < ihd[b]; });
-            std::vector<double> vals(nconv), qs((size_t)ncv * nconv);
-            for (int i = 0; i < nconv; ++i) {
-                vals[i] = ihd[order[i]];
-                std::copy(iq.begin() + (size_t)order[i] * ncv, iq.begin() + (size_t)(order[i] + 1) * ncv,
-                          qs.begin() + (size_t)i * ncv);
-            }
-            // dgeqr2 + dorm2r("R","N") (HLanc.hpp:1837-1840) as one V Q_full GEMM
-            qf.assign((size_t)ncv * ncv, 0.0);
-            householder_q(ncv, nconv, qs.data(), ncv, tau.data(), qf.data(), ncv);
-            if (vectors) {
-                HL_TRY_HIP(hipMemcpyAsync(W.Qd, qf.data(), sizeof(double) * (size_t)ncv * ncv,
-                                          hipMemcpyHostToDevice, s));
-                const int nxt = 1 - W.cur;
-                int rc = prof->begin(s);
-                if (!rc)
-                    rc = launch_vq(n, ncv, ncv, W.V[W.cur], W.vl(), W.Qd, ncv, W.V[nxt], VLayout{W.ldv, false},
-                                   opts.use_mfma, -1, nullptr, 0.0, 0.0, W.red, W.ctl, s);
-                if (!rc) rc = prof->end(s, K_VQ, 16.0 * n * ncv);
-                if (rc) return rc;
-                // sync_host (HLanc.hpp:1063-1071): all ncv columns to the caller
-                HL_TRY_HIP(hipMemcpy2DAsync(vectors, sizeof(double) * (size_t)ldv_host, W.V[nxt],
-                                            sizeof(double) * (size_t)W.ldv, sizeof(double) * (size_t)n, ncv,
-                                            hipMemcpyDeviceToHost, s));
-                HL_TRY_HIP(hipStreamSynchronize(s));
-            }
-            std::copy(vals.begin(), vals.end(), values);
-            ret = HL_OK;
+    if (!vectors_ok || nconv < nev0) return; // HLanc.hpp:1762
+    const double eps23 = std::pow(kEps, 2.0 / 3.0);
+    const int np0 = ncv - nev0;
+    std::vector<double> idx(ncv);
+    std::vector<char> select(ncv, 0);
+    for (int j = 0; j < ncv; ++j) idx[j] = j + 1;
+    std::vector<double> rz = ritz_saved;
+    sort_wanted(nev0, np0, rz.data(), idx.data());
+    int reord = 0, numcnv = 0;
+    for (int j = 0; j < ncv; ++j) {
+        const double temp1 = std::max(eps23, std::fabs(rz[ncv - j - 1]));
+        const int jj = static_cast<int>(idx[ncv - j - 1]);
+        if (numcnv < nconv && bounds_saved[jj - 1] <= tol * temp1) {
+            select[jj - 1] = 1;
+            ++numcnv;
+            if (jj > nev0) reord = 1;
         }
     }
+    if (numcnv != nconv) return; // HLanc.hpp:1791
+    std::vector<double> ihd(dia), ihb(ncv, 0.0), iq((size_t)ncv * ncv), tau(ncv);
+    for (int i = 0; i + 1 < ncv; ++i) ihb[i] = sub[i + 1];
+    if (tridiag_eig(ncv, ihd.data(), ihb.data(), iq.data(), ncv)) return; // HLanc.hpp:1805-1806
+    if (reord) { // selected columns to the front (HLanc.hpp:1808-1831)
+        int left = 0, right = ncv - 1;
+        for (;;) {
+            while (select[left]) ++left;
+            while (!select[right]) --right;
+            if (left > right) break;
+            std::swap(ihd[left], ihd[right]);
+            for (int i = 0; i < ncv; ++i) std::swap(iq[(size_t)left * ncv + i], iq[(size_t)right * ncv + i]);
+            ++left;
+            --right;
+        }
+    }
+    // dsesrt (HLanc.hpp:1626-1647): ascending values, Q columns alongside
+    std::vector<int> order(nconv);
+    for (int i = 0; i < nconv; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ihd[a] < ihd[b]; });
+    vals.assign(nconv, 0.0);
+    std::vector<double> qs((size_t)ncv * nconv);
+    for (int i = 0; i < nconv; ++i) {
+        vals[i] = ihd[order[i]];
+        std::copy(iq.begin() + (size_t)order[i] * ncv, iq.begin() + (size_t)(order[i] + 1) * ncv,
+                  qs.begin() + (size_t)i * ncv);
+    }
+    // dgeqr2 + dorm2r("R","N") (HLanc.hpp:1837-1840) as one V Q_full GEMM
+    householder_q(ncv, nconv, qs.data(), ncv, tau.data(), qslot, ncv);
+    with_vectors = vectors != nullptr;
+    ret = HL_OK;
+}
+
+void Member::complete()
+{
+    if (ret == HL_OK) std::copy(vals.begin(), vals.end(), values);
     info.ret = ret;
     info.restarts = restarts;
     info.nconv = nconv;
@@ -269,63 +262,137 @@ int Member::finish(hipStream_t s, bool vectors_ok)
     info.seconds = now_s() - t0;
     info.max_rel_resid = -1.0;
     info.refine_shift = 0.0;
-    return HL_OK;
+    phase = DONE;
 }
 
 // ------------------------------------------------------------------ Engine
+
+// Host threads for the members' restart logic: a parallel loop over members,
+// the calling thread taking part.
+struct Engine::Pool {
+    std::vector<std::thread> th;
+    std::mutex mu;
+    std::condition_variable go, done;
+    std::function<void(int)> job;
+    int count = 0, busy = 0;
+    long gen = 0;
+    bool stop = false;
+    std::atomic<int> next{0};
+
+    explicit Pool(int nt)
+    {
+        for (int i = 0; i < nt; ++i) th.emplace_back([this] { work(); });
+    }
+    ~Pool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        go.notify_all();
+        for (std::thread &t : th) t.join();
+    }
+    void work()
+    {
+        long seen = 0;
+        for (;;) {
+            std::function<void(int)> f;
+            int cnt;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                go.wait(lk, [&] { return stop || gen != seen; });
+                if (stop) return;
+                seen = gen;
+                f = job;
+                cnt = count;
+                ++busy;
+            }
+            for (int i; (i = next.fetch_add(1)) < cnt;) f(i);
+            std::lock_guard<std::mutex> lk(mu);
+            if (--busy == 0) done.notify_all();
+        }
+    }
+    void run(int n, const std::function<void(int)> &f)
+    {
+        if (n < 16 || th.empty()) {
+            for (int i = 0; i < n; ++i) f(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            job = f;
+            count = n;
+            next.store(0);
+            ++gen;
+        }
+        go.notify_all();
+        for (int i; (i = next.fetch_add(1)) < n;) f(i);
+        std::unique_lock<std::mutex> lk(mu);
+        done.wait(lk, [&] { return busy == 0; });
+    }
+};
 
 int Engine::init(int dev_)
 {
     dev = dev_;
     HL_TRY_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const unsigned hw = std::thread::hardware_concurrency();
+    pool = new Pool(static_cast<int>(std::min(15u, hw > 1 ? hw - 1 : 0u)));
     return HL_OK;
 }
 
 void Engine::release()
 {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (dM) (void)hipFree(dM);
-    if (dS) (void)hipFree(dS);
-    if (hM) (void)hipHostFree(hM);
-    if (hS) (void)hipHostFree(hS);
+    void *d[] = {dM, dS, dV, dRB, dQ};
+    for (void *p : d)
+        if (p) (void)hipFree(p);
+    void *h[] = {hM, hS, hV, hRB, hQ};
+    for (void *p : h)
+        if (p) (void)hipHostFree(p);
+    dM = hM = nullptr;
+    dS = hS = nullptr;
+    dV = hV = nullptr;
+    dRB = hRB = dQ = hQ = nullptr;
+    capM = capS = capV = capRB = capQ = 0;
     prof.release();
     if (stream) (void)hipStreamDestroy(stream);
-    dM = nullptr;
-    dS = nullptr;
-    hM = nullptr;
-    hS = nullptr;
     stream = nullptr;
-    capM = capS = 0;
+    delete pool;
+    pool = nullptr;
 }
 
-int Engine::grow(size_t nm, size_t ns)
+namespace {
+template <typename T>
+int grow_pair(T *&d, T *&h, size_t &cap, size_t need)
 {
-    if (nm > capM) {
-        HL_TRY_HIP(hipStreamSynchronize(stream));
-        if (dM) (void)hipFree(dM);
-        if (hM) (void)hipHostFree(hM);
-        dM = nullptr;
-        hM = nullptr;
-        capM = std::max(nm, 2 * capM);
-        HL_TRY_HIP(hipMalloc(&dM, sizeof(SegMember) * capM));
-        HL_TRY_HIP(hipHostMalloc(&hM, sizeof(SegMember) * capM, hipHostMallocDefault));
-    }
-    if (ns > capS) {
-        HL_TRY_HIP(hipStreamSynchronize(stream));
-        if (dS) (void)hipFree(dS);
-        if (hS) (void)hipHostFree(hS);
-        dS = nullptr;
-        hS = nullptr;
-        capS = std::max(ns, 2 * capS);
-        HL_TRY_HIP(hipMalloc(&dS, sizeof(SegStep) * capS));
-        HL_TRY_HIP(hipHostMalloc(&hS, sizeof(SegStep) * capS, hipHostMallocDefault));
-    }
+    if (need <= cap) return HL_OK;
+    if (d) (void)hipFree(d);
+    if (h) (void)hipHostFree(h);
+    d = nullptr;
+    h = nullptr;
+    cap = std::max(need, 2 * cap);
+    HL_TRY_HIP(hipMalloc(&d, sizeof(T) * cap));
+    HL_TRY_HIP(hipHostMalloc(&h, sizeof(T) * cap, hipHostMallocDefault));
     return HL_OK;
+}
+} // namespace
+
+int Engine::grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd)
+{
+    HL_TRY_HIP(hipStreamSynchronize(stream)); // the pinned tables are rewritten every sweep
+    int rc;
+    if ((rc = grow_pair(dM, hM, capM, nm))) return rc;
+    if ((rc = grow_pair(dS, hS, capS, ns))) return rc;
+    if ((rc = grow_pair(dV, hV, capV, nv))) return rc;
+    if ((rc = grow_pair(dRB, hRB, capRB, rb))) return rc;
+    return grow_pair(dQ, hQ, capQ, qd);
 }
 
 int Engine::sweep(std::vector<Member *> &ms)
 {
     int rc;
+    const double th0 = now_s();
     for (Member *m : ms)
         if (m->phase == Member::NEW && (rc = m->start(stream))) return rc;
     std::vector<Member *> act;
@@ -334,11 +401,15 @@ int Engine::sweep(std::vector<Member *> &ms)
     if (act.empty()) return HL_OK;
 
     // member table and all steps of the sweep, uploaded once
-    int maxsteps = 0;
-    for (Member *m : act) maxsteps = std::max(maxsteps, m->sweep_np);
+    int maxsteps = 0, ncvmax = 0;
+    for (Member *m : act) {
+        maxsteps = std::max(maxsteps, m->sweep_np);
+        ncvmax = std::max(ncvmax, m->ncv);
+    }
     const size_t nm = act.size();
-    if ((rc = grow(nm, nm * (size_t)std::max(maxsteps, 1)))) return rc;
-    HL_TRY_HIP(hipStreamSynchronize(stream)); // the pinned tables are rewritten every sweep
+    const int stride = 2 * ncvmax + kCtlWords;
+    if ((rc = grow(nm, nm * (size_t)std::max(maxsteps, 1), nm, nm * (size_t)stride, nm * (size_t)ncvmax * ncvmax)))
+        return rc;
     for (size_t i = 0; i < nm; ++i) {
         Member &m = *act[i];
         Workspace &W = m.ws;
@@ -361,6 +432,7 @@ int Engine::sweep(std::vector<Member *> &ms)
         M.ctl = W.ctl;
         M.n = m.n;
         M.nb = cgs_blocks(m.n);
+        M.ncv = m.ncv;
     }
     std::vector<int> off(maxsteps + 1, 0), nspmv(maxsteps, 0), ncgs(maxsteps, 0), nfin(maxsteps, 0),
         maxcpg(maxsteps, 0);
@@ -385,6 +457,7 @@ int Engine::sweep(std::vector<Member *> &ms)
     off[maxsteps] = static_cast<int>(ns);
     HL_TRY_HIP(hipMemcpyAsync(dM, hM, sizeof(SegMember) * nm, hipMemcpyHostToDevice, stream));
     HL_TRY_HIP(hipMemcpyAsync(dS, hS, sizeof(SegStep) * ns, hipMemcpyHostToDevice, stream));
+    double host_s = now_s() - th0; // host time of this sweep (profiled runs report it)
 
     const int modes[4] = {EPI_WNORM, EPI_PASS1, EPI_PASS2, EPI_PASS3};
     for (int st = 0; st < maxsteps; ++st) {
@@ -409,30 +482,99 @@ int Engine::sweep(std::vector<Member *> &ms)
                     bytes += m.cgs_bytes_pass(hS[q].j + 1) - (p == 0 ? 8.0 * m.n : 0.0);
                 }
             if ((rc = prof.begin(stream))) return rc;
-            if ((rc = launch_cgs_seg(dM, S, cnt, maxcpg[st], ncgs[st], nfin[st], modes[p], stream))) return rc;
-            if ((rc = prof.end(stream, K_CGS, bytes))) return rc;
+            if ((rc = launch_cgs_seg(dM, S, cnt, maxcpg[st], ncgs[st], modes[p], stream))) return rc;
+            if ((rc = prof.end(stream, p < 2 ? K_CGS : K_DGKS, bytes))) return rc;
+            if ((rc = prof.begin(stream))) return rc;
+            if ((rc = launch_cgs_seg_finalize(dM, S, cnt, nfin[st], modes[p], stream))) return rc;
+            if ((rc = prof.end(stream, K_MISC, 0.0))) return rc;
         }
     }
-    // the new alpha/beta entries and the residual norm of every member
-    for (Member *m : act) {
-        Workspace &W = m->ws;
-        const int k = m->sweep_k, cnt = m->ncv - k;
-        HL_TRY_HIP(hipMemcpyAsync(W.hh, W.hsub + k, sizeof(double) * cnt, hipMemcpyDeviceToHost, stream));
-        HL_TRY_HIP(hipMemcpyAsync(W.hh + m->ncv, W.hdia + k, sizeof(double) * cnt, hipMemcpyDeviceToHost, stream));
-        HL_TRY_HIP(hipMemcpyAsync(W.ctl_h, W.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+    // every member's alpha/beta entries and counters in one copy
+    if ((rc = launch_seg_readback(dM, static_cast<int>(nm), stride, dRB, stream))) return rc;
+    HL_TRY_HIP(hipMemcpyAsync(hRB, dRB, sizeof(double) * nm * stride, hipMemcpyDeviceToHost, stream));
+    HL_TRY_HIP(hipStreamSynchronize(stream));
+    const double th1 = now_s();
+    if (prof.on && (rc = prof.flush())) return rc;
+    for (size_t i = 0; i < nm; ++i) {
+        act[i]->absorb(hRB + i * stride);
+        if (prof.on) prof.tot.bytes[K_DGKS] += 8.0 * act[i]->n * (double)act[i]->d_m23;
+    }
+    rc = after_sweep(act);
+    if (prof.on) {
+        host_s += now_s() - th1;
+        prof.tot.launches[K_HOST] += 1;
+        prof.tot.seconds[K_HOST] += host_s;
+    }
+    return rc;
+}
+
+// Restart logic of every member on host threads, then one segmented V.Q launch
+// for the restarting members and a V.Q GEMM plus copy-out per finishing one.
+int Engine::after_sweep(std::vector<Member *> &act)
+{
+    const size_t nm = act.size();
+    int ncvmax = 0;
+    for (Member *m : act) ncvmax = std::max(ncvmax, m->ncv);
+    const size_t qstride = (size_t)ncvmax * ncvmax;
+    pool->run(static_cast<int>(nm), [&](int i) { act[i]->decide(hQ + (size_t)i * qstride); });
+
+    int nv = 0, vblocks = 0, maxK = 0, maxN = 0, nfinish = 0;
+    double bytes = 0.0;
+    for (size_t i = 0; i < nm; ++i) {
+        Member &m = *act[i];
+        if (m.act == Member::ACT_FINISH) ++nfinish;
+        if (m.act != Member::ACT_RESTART) continue;
+        Workspace &W = m.ws;
+        SegVQ &t = hV[nv++];
+        t.V = W.V[W.cur];
+        t.Out = W.V[1 - W.cur];
+        t.Q = dQ + i * qstride;
+        t.r = W.r;
+        t.ctl = W.ctl;
+        t.partials = W.red.partials;
+        t.qlast = m.vq_qlast;
+        t.beta = m.vq_beta;
+        t.n = m.n;
+        t.K = m.ncv;
+        t.N = m.vq_N;
+        t.res_col = m.vq_res;
+        t.nb = vq_seg_blocks(m.n);
+        t.blk0 = vblocks;
+        vblocks += t.nb;
+        maxK = std::max(maxK, t.K);
+        maxN = std::max(maxN, t.N);
+        bytes += 8.0 * m.n * (t.K + t.N + 2);
+    }
+    int rc;
+    if (nv > 0 || nfinish > 0)
+        HL_TRY_HIP(hipMemcpyAsync(dQ, hQ, sizeof(double) * nm * qstride, hipMemcpyHostToDevice, stream));
+    if (nv > 0) {
+        HL_TRY_HIP(hipMemcpyAsync(dV, hV, sizeof(SegVQ) * nv, hipMemcpyHostToDevice, stream));
+        if ((rc = prof.begin(stream))) return rc;
+        if ((rc = launch_vq_seg(dV, nv, vblocks, maxK, maxN, stream))) return rc;
+        if ((rc = prof.end(stream, K_VQ, bytes))) return rc;
+        for (Member *m : act)
+            if (m->act == Member::ACT_RESTART) m->ws.cur = 1 - m->ws.cur;
+    }
+    if (nfinish == 0) return HL_OK;
+    for (size_t i = 0; i < nm; ++i) {
+        Member &m = *act[i];
+        if (m.act != Member::ACT_FINISH || !m.with_vectors) continue;
+        Workspace &W = m.ws;
+        const int nxt = 1 - W.cur;
+        if ((rc = prof.begin(stream))) return rc;
+        if ((rc = launch_vq(m.n, m.ncv, m.ncv, W.V[W.cur], W.vl(), dQ + i * qstride, m.ncv, W.V[nxt],
+                            VLayout{W.ldv, false}, m.opts.use_mfma, -1, nullptr, 0.0, 0.0, W.red, W.ctl, stream)))
+            return rc;
+        if ((rc = prof.end(stream, K_VQ, 16.0 * m.n * m.ncv))) return rc;
+        // sync_host (HLanc.hpp:1063-1071): all ncv columns to the caller
+        HL_TRY_HIP(hipMemcpy2DAsync(m.vectors, sizeof(double) * (size_t)m.ldv_host, W.V[nxt],
+                                    sizeof(double) * (size_t)W.ldv, sizeof(double) * (size_t)m.n, m.ncv,
+                                    hipMemcpyDeviceToHost, stream));
     }
     HL_TRY_HIP(hipStreamSynchronize(stream));
-    if (prof.on) {
-        if ((rc = prof.flush())) return rc;
-        for (Member *m : act) {
-            const Ctl &c = *m->ws.ctl_h;
-            prof.tot.bytes[K_CGS] += 8.0 * m->n *
-                                     ((double)(c.m2sum - m->m2sum) + 2.0 * (double)(c.p2 - m->p2) +
-                                      (double)(c.m3sum - m->m3sum) + 2.0 * (double)(c.p3 - m->p3));
-        }
-    }
     for (Member *m : act)
-        if ((rc = m->after_sweep(stream))) return rc;
+        if (m->act == Member::ACT_FINISH) m->complete();
     return HL_OK;
 }
 
@@ -444,8 +586,7 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
     double in_use = 0.0;
     int rc = HL_OK;
     auto need = [](const Member *m) {
-        return Workspace::bytes_needed(m->n, m->ncv, m->opts.basis_prec, m->A == nullptr,
-                                       m->A ? m->A->nslots : 0);
+        return Workspace::bytes_needed(m->n, m->ncv, m->A == nullptr, m->A ? m->A->nslots : 0);
     };
     while (next < queue.size() || !active.empty()) {
         // admission at the sweep boundary, in queue order
@@ -458,11 +599,9 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
                (max_active <= 0 || (int)active.size() < max_active) &&
                (hbm_budget <= 0.0 || active.empty() || in_use + need(queue[next]) <= hbm_budget)) {
             Member *m = queue[next++];
-            m->prof = &prof;
             m->t_admit = t;
             m->phase = Member::NEW;
-            if ((m->err = m->ws.init(dev, m->n, m->ncv, m->opts.basis_prec, m->A == nullptr,
-                                     m->A ? m->A->nslots : 0))) {
+            if ((m->err = m->ws.init(dev, m->n, m->ncv, m->A == nullptr, m->A ? m->A->nslots : 0, stream))) {
                 m->ws.release();
                 m->phase = Member::DONE;
                 m->t_done = now_s() - t_start;

@@ -58,6 +58,9 @@ struct Ctl {
     long long m2sum, m3sum; // sum of their basis widths (for byte counts)
 };
 
+static_assert(sizeof(Ctl) % sizeof(double) == 0, "Ctl is read back as doubles");
+constexpr int kCtlWords = sizeof(Ctl) / sizeof(double);
+
 // Epilogue modes of the reducing kernels.
 enum Epi : int {
     EPI_NONE = 0,    // standalone pass: write h_out and ctl->nrm2 only
@@ -155,6 +158,7 @@ struct SegMember {               // per solve, fixed during a sweep
     double *r, *w, *h0, *h1, *hsub, *hdia, *partials;
     Ctl *ctl;
     int n, nb;                   // rows; CGS blocks = cgs_blocks(n)
+    int ncv;                     // basis width (read-back size)
 };
 struct SegStep {                 // per solve and Lanczos step
     int mem;                     // index into the SegMember table
@@ -163,9 +167,31 @@ struct SegStep {                 // per solve and Lanczos step
     int cgs0;                    // first block in the CGS launch
     int fin0;                    // first block in the finalize launch
 };
-int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total,
-                   int fin_blocks_total, int mode, hipStream_t s);
+// Partials kernel of CGS pass `mode` (EPI_WNORM .. EPI_PASS3) for every listed
+// step, then the per-column finalize kernel (j + 2 blocks per step).
+int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total, int mode,
+                   hipStream_t s);
+int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fin_blocks_total, int mode,
+                            hipStream_t s);
 int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s);
+
+// Restart V.Q of one solve within a segmented launch (tiled bases, ld K).
+struct SegVQ {
+    const double *V;             // input basis
+    double *Out;                 // output basis (the other ping-pong buffer)
+    const double *Q;             // K x N, column-major, ld K
+    double *r;                   // residual, updated in place
+    Ctl *ctl;                    // rnorm / xscale for the next sweep
+    double *partials;            // nb block partials of ||r||^2
+    double qlast, beta;          // dsapps residual coefficients
+    int n, K, N, res_col;        // res_col < 0: no beta term
+    int nb, blk0;                // blocks (vq_seg_blocks(n)); first block in the launch
+};
+int vq_seg_blocks(int n);
+int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, hipStream_t s);
+// Packs hsub, hdia (ncv each) and the Ctl words of every member at
+// out + member * stride (stride >= 2 ncv + kCtlWords).
+int launch_seg_readback(const SegMember *M, int nm, int stride, double *out, hipStream_t s);
 
 // ------------------------------------------------------- basis kernels
 struct Reduce {
@@ -213,7 +239,11 @@ int launch_stream_read(const double *a, long long n, double *out, int blocks, hi
 int launch_cast_f32(long long n, const double *in, float *out, hipStream_t s);
 
 // ------------------------------------------------------------ profiling
-enum KClass : int { K_SPMV = 0, K_CGS = 1, K_VQ = 2, K_MISC = 3, K_NCLASS = 6 };
+// Profile classes: SpMV; CGS partials kernels of the two passes every step
+// runs; V.Q; finalize and other small kernels; CGS partials kernels of the
+// DGKS corrections (launched every step, exiting at once when not needed);
+// host work of the engine between sweeps (one "launch" per sweep).
+enum KClass : int { K_SPMV = 0, K_CGS = 1, K_VQ = 2, K_MISC = 3, K_DGKS = 4, K_HOST = 5, K_NCLASS = 6 };
 
 // HIP-event timing of individual launches on one stream (profiled solves only).
 struct Profiler {
@@ -244,6 +274,10 @@ void dlartg(double f, double g, double &c, double &s, double &r);
 // Plays dsteqr("I") (HLanc.hpp:1691,1805).  Returns 0 or 1 (no convergence /
 // non-finite input).
 int tridiag_eig(int n, double *d, double *e, double *z, int ldz);
+// Same eigenvalues, and only the last row of the eigenvector matrix
+// (zlast[i] = Z(n-1, i)): what the restart loop needs (HLanc.hpp:1691-1696),
+// in O(n^2); bitwise equal to the corresponding entries of tridiag_eig.
+int tridiag_eig_last(int n, double *d, double *e, double *zlast);
 // Householder QR of a (m x k, lda), LAPACK dgeqr2 storage (HLanc.hpp:1837),
 // then Q = H(0)...H(k-1) formed explicitly as m x m (ldq).
 void householder_q(int m, int k, double *a, int lda, double *tau, double *q, int ldq);

@@ -201,74 +201,96 @@ Workspace::~Workspace() { release(); }
 
 void Workspace::release()
 {
-    void *ptrs[] = {V[0], V[1], Vf[0], Vf[1], r, w, hsub, hdia, hbuf, Qd, ctl, vd_tmp, xv, chunk_part, chunk_cnt};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
-    if (ctl_h) (void)hipHostFree(ctl_h);
-    if (hh) (void)hipHostFree(hh);
-    if (xh) (void)hipHostFree(xh);
-    if (yh) (void)hipHostFree(yh);
-    reduce_free(red);
+    if (blob) (void)hipFree(blob);
+    if (hblob) (void)hipHostFree(hblob);
+    blob = hblob = nullptr;
     V[0] = V[1] = nullptr;
-    Vf[0] = Vf[1] = nullptr;
-    r = w = hsub = hdia = hbuf = Qd = vd_tmp = xv = chunk_part = nullptr;
+    r = w = hsub = hdia = hbuf = xv = chunk_part = nullptr;
     chunk_cnt = nullptr;
     ctl = nullptr;
     ctl_h = nullptr;
-    hh = xh = yh = nullptr;
+    xh = yh = nullptr;
+    red = Reduce{};
 }
 
 namespace {
 long long padded_rows(int n) { return ((long long)n + 63) / 64 * 64; } // 512-byte aligned columns
+
+// Offsets of the workspace's pieces in its device allocation (256-byte aligned).
+struct Carve {
+    size_t off = 0;
+    size_t take(size_t bytes)
+    {
+        const size_t at = off;
+        off += (bytes + 255) / 256 * 256;
+        return at;
+    }
+};
+
+struct WsLayout {
+    size_t V0, V1, r, w, hsub, hdia, hbuf, ctl, xv, cpart, ccnt, part, cnt, total;
+    WsLayout(int n, int ncv, bool host_op, int nslots)
+    {
+        const size_t ldv = static_cast<size_t>(padded_rows(n));
+        Carve c;
+        V0 = c.take(8 * ldv * ncv);
+        V1 = c.take(8 * ldv * ncv);
+        r = c.take(8 * ldv);
+        w = c.take(8 * ldv);
+        hsub = c.take(8 * (size_t)ncv);
+        hdia = c.take(8 * (size_t)ncv);
+        hbuf = c.take(16 * (size_t)ncv);
+        ctl = c.take(sizeof(Ctl));
+        xv = host_op ? c.take(8 * ldv) : 0;
+        cpart = nslots > 0 ? c.take(8 * (size_t)nslots) : 0;
+        ccnt = nslots > 0 ? c.take(4 * (size_t)nslots) : 0;
+        part = c.take(8 * (size_t)kMaxReduceBlocks * (ncv + 1));
+        cnt = c.take(16);
+        total = c.off;
+    }
+};
 } // namespace
 
-double Workspace::bytes_needed(int n, int ncv, int prec, bool host_op, int nslots)
+double Workspace::bytes_needed(int n, int ncv, bool host_op, int nslots)
 {
-    const double ldv = static_cast<double>(padded_rows(n));
-    const double vb = 8.0 * ldv * ncv;
-    double b = prec == HL_PREC_F32 ? vb + vb : 2.0 * vb;
-    b += 8.0 * ldv * (host_op ? 3 : 2) + 8.0 * (4.0 * ncv + (double)ncv * ncv) + sizeof(Ctl);
-    b += 8.0 * (double)kMaxReduceBlocks * (ncv + 1) + 16.0 + 12.0 * nslots;
-    return b;
+    return static_cast<double>(WsLayout(n, ncv, host_op, nslots).total);
 }
 
-int Workspace::init(int dev_, int n_, int ncv_, int prec_, bool host_op, int nslots)
+int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, hipStream_t s)
 {
     dev = dev_;
     n = n_;
     ncv = ncv_;
-    prec = prec_;
     ldv = padded_rows(n);
-    tiled = true;
-    const size_t vbytes = sizeof(double) * (size_t)ldv * ncv;
-    if (prec == HL_PREC_F32) {
-        HL_TRY_HIP(hipMalloc(&Vf[0], vbytes / 2));
-        HL_TRY_HIP(hipMalloc(&Vf[1], vbytes / 2));
-        HL_TRY_HIP(hipMalloc(&vd_tmp, vbytes)); // FP64 staging of the final basis
-    } else {
-        HL_TRY_HIP(hipMalloc(&V[0], vbytes));
-        HL_TRY_HIP(hipMalloc(&V[1], vbytes));
-    }
-    HL_TRY_HIP(hipMalloc(&r, sizeof(double) * (size_t)ldv));
-    HL_TRY_HIP(hipMalloc(&w, sizeof(double) * (size_t)ldv));
-    HL_TRY_HIP(hipMalloc(&hsub, sizeof(double) * ncv));
-    HL_TRY_HIP(hipMalloc(&hdia, sizeof(double) * ncv));
-    HL_TRY_HIP(hipMalloc(&hbuf, sizeof(double) * 2 * ncv));
-    HL_TRY_HIP(hipMalloc(&Qd, sizeof(double) * (size_t)ncv * ncv));
-    HL_TRY_HIP(hipMalloc(&ctl, sizeof(Ctl)));
-    HL_TRY_HIP(hipHostMalloc(&ctl_h, sizeof(Ctl), hipHostMallocDefault));
-    HL_TRY_HIP(hipHostMalloc(&hh, sizeof(double) * 2 * ncv, hipHostMallocDefault));
+    cur = 0;
+    const WsLayout L(n, ncv, host_op, nslots);
+    HL_TRY_HIP(hipMalloc(&blob, L.total));
+    char *b = static_cast<char *>(blob);
+    V[0] = reinterpret_cast<double *>(b + L.V0);
+    V[1] = reinterpret_cast<double *>(b + L.V1);
+    r = reinterpret_cast<double *>(b + L.r);
+    w = reinterpret_cast<double *>(b + L.w);
+    hsub = reinterpret_cast<double *>(b + L.hsub);
+    hdia = reinterpret_cast<double *>(b + L.hdia);
+    hbuf = reinterpret_cast<double *>(b + L.hbuf);
+    ctl = reinterpret_cast<Ctl *>(b + L.ctl);
+    xv = host_op ? reinterpret_cast<double *>(b + L.xv) : nullptr;
+    chunk_part = nslots > 0 ? reinterpret_cast<double *>(b + L.cpart) : nullptr;
+    chunk_cnt = nslots > 0 ? reinterpret_cast<unsigned *>(b + L.ccnt) : nullptr;
+    red.partials = reinterpret_cast<double *>(b + L.part);
+    red.counter = reinterpret_cast<unsigned *>(b + L.cnt);
+    red.max_blocks = kMaxReduceBlocks;
+    red.max_cols = ncv;
+    if (chunk_cnt) HL_TRY_HIP(hipMemsetAsync(chunk_cnt, 0, sizeof(unsigned) * (size_t)nslots, s));
+    HL_TRY_HIP(hipMemsetAsync(red.counter, 0, 16, s));
+    const size_t hbytes = sizeof(Ctl) + (host_op ? 16 * (size_t)n : 0);
+    HL_TRY_HIP(hipHostMalloc(&hblob, hbytes, hipHostMallocDefault));
+    ctl_h = static_cast<Ctl *>(hblob);
     if (host_op) {
-        HL_TRY_HIP(hipMalloc(&xv, sizeof(double) * (size_t)ldv));
-        HL_TRY_HIP(hipHostMalloc(&xh, sizeof(double) * (size_t)n, hipHostMallocDefault));
-        HL_TRY_HIP(hipHostMalloc(&yh, sizeof(double) * (size_t)n, hipHostMallocDefault));
+        xh = reinterpret_cast<double *>(static_cast<char *>(hblob) + sizeof(Ctl));
+        yh = xh + n;
     }
-    if (nslots > 0) {
-        HL_TRY_HIP(hipMalloc(&chunk_part, sizeof(double) * (size_t)nslots));
-        HL_TRY_HIP(hipMalloc(&chunk_cnt, sizeof(unsigned) * (size_t)nslots));
-        HL_TRY_HIP(hipMemset(chunk_cnt, 0, sizeof(unsigned) * (size_t)nslots));
-    }
-    return reduce_alloc(red, kMaxReduceBlocks, ncv);
+    return HL_OK;
 }
 
 } // namespace hl

@@ -7,30 +7,30 @@
 
 namespace hl {
 
-// Device workspace of one solve (freed on every return path).
+// Device workspace of one solve: one device allocation and one pinned host
+// allocation, carved up (freed on every return path).
 struct Workspace {
-    int dev = 0, n = 0, ncv = 0, prec = HL_PREC_F64;
-    long long ldv = 0;               // column-major leading dimension, multiple of 64
-    bool tiled = true;               // the solver's basis is always tiled
-    double *V[2] = {nullptr, nullptr};  // FP64 basis, ping-pong for V Q
-    float *Vf[2] = {nullptr, nullptr};  // FP32 basis (mixed precision)
+    int dev = 0, n = 0, ncv = 0;
+    long long ldv = 0;                  // column-major leading dimension, multiple of 64
+    double *V[2] = {nullptr, nullptr};  // FP64 tiled basis, ping-pong for V Q
     int cur = 0;
     double *r = nullptr, *w = nullptr;  // residual and A v_j
-    double *hsub = nullptr, *hdia = nullptr, *hbuf = nullptr, *Qd = nullptr, *vd_tmp = nullptr;
+    double *hsub = nullptr, *hdia = nullptr, *hbuf = nullptr;
     double *xv = nullptr;               // contiguous v_j for host operators
     double *chunk_part = nullptr;       // this solve's long-row partial sums (nslots)
     unsigned *chunk_cnt = nullptr;      // and arrival counters, so that solves of
                                         // one operator can share an SpMV launch
     Ctl *ctl = nullptr;
-    Ctl *ctl_h = nullptr;               // pinned mirrors
-    double *hh = nullptr, *xh = nullptr, *yh = nullptr;
-    Reduce red;
+    Ctl *ctl_h = nullptr;               // pinned
+    double *xh = nullptr, *yh = nullptr; // pinned, host operators only
+    Reduce red;                         // block partial sums (carved, not owned)
+    void *blob = nullptr, *hblob = nullptr;
 
-    // The ping-pong bases are tiled (16-row blocks of ncv columns) by default.
-    VLayout vl() const { return tiled ? VLayout{ncv, true} : VLayout{ldv, false}; }
+    // The ping-pong bases are tiled: 16-row blocks of ncv columns.
+    VLayout vl() const { return VLayout{ncv, true}; }
     ~Workspace();
-    int init(int dev, int n, int ncv, int prec, bool host_op, int nslots);
-    static double bytes_needed(int n, int ncv, int prec, bool host_op, int nslots); // device bytes of init
+    int init(int dev, int n, int ncv, bool host_op, int nslots, hipStream_t s); // zeroing on s
+    static double bytes_needed(int n, int ncv, bool host_op, int nslots); // device bytes of init
     void release();
 };
 
@@ -41,8 +41,10 @@ void restart_chase(int nev, int np, const double *shift, double *sub, double *di
 
 // One eigensolve: the host half of ds_solver (HLanc.hpp:1649-1850) as a state
 // machine that the engine drives sweep by sweep.  A sweep is one dsaitr call,
-// Lanczos steps [sweep_k, sweep_k + sweep_np); between sweeps the member reads
-// back its new alpha/beta entries and runs the restart logic on the host.
+// Lanczos steps [sweep_k, sweep_k + sweep_np).  After a sweep the engine hands
+// the member its read-back alpha/beta entries and Ctl (absorb), the member
+// decides on the host what comes next (decide: restart logic, restart Q or
+// Ritz-vector Q), and the engine enqueues the device work of all members.
 struct Member {
     // inputs
     const CsrPlan *A = nullptr;          // device operator, or
@@ -54,46 +56,57 @@ struct Member {
     int ldv_host = 0;
     double arrival = 0.0;                // seconds after the engine's start; admitted no earlier
     double t_admit = 0.0, t_done = 0.0;  // seconds after the engine's start
-    Profiler *prof = nullptr;            // the engine's (set on admission)
     // device state
     Workspace ws;
     // host state
     enum Phase { NEW, ACTIVE, DONE } phase = NEW;
+    enum Act { ACT_NONE, ACT_SWEEP, ACT_RESTART, ACT_FINISH } act = ACT_NONE;
     int stage = 0;                       // 0: initial sweep pending, 1: main loop
     int sweep_k = 0, sweep_np = 0;
     int nev = 0, np = 0, nconv = 0, iter = 0, restarts = 0, maxiter = 200;
     double tol = 0.0, rnorm = 0.0, t0 = 0.0;
     long long spmv = 0, cgs = 0, p2 = 0, p3 = 0, m2sum = 0, m3sum = 0;
+    long long d_m23 = 0;                 // DGKS basis columns read in the last sweep (profile bytes)
     int ret = HL_NOT_CONVERGED;
     int err = HL_OK;                     // a HIP or allocation failure of this member
     hl_info info{};
-    std::vector<double> sub, dia, ritz, bounds, q, e, work, ritz_saved, bounds_saved, qf;
+    // pending device work chosen by decide()
+    int vq_N = 0, vq_res = -1;           // restart: Q columns and the beta column
+    double vq_qlast = 0.0, vq_beta = 0.0;
+    bool with_vectors = false;           // finish: Ritz vectors to the caller
+    std::vector<double> sub, dia, ritz, bounds, zlast, q, e, work, ritz_saved, bounds_saved, vals;
 
     int start(hipStream_t s);            // start vector, norm, first sweep [0, nev0)
-    int after_sweep(hipStream_t s);      // restart logic; next sweep, or finish
+    void absorb(const double *rb);       // hsub, hdia (ncv each) and the Ctl words
+    // Host-only (thread-safe across members): restart logic; writes the restart
+    // Q (ncv x vq_N) or the Ritz-vector Q (ncv x ncv) to qslot.
+    void decide(double *qslot);
     int host_apply(int j, hipStream_t s); // SpMV of a host operator for step j
     double cgs_bytes_pass(int m) const;  // algorithmic bytes of one CGS pass
+    void complete();                     // outputs and info once the engine synchronised
 
   private:
-    int restart(hipStream_t s);
-    int finish(hipStream_t s, bool vectors_ok);
+    void finish_host(double *qslot, bool vectors_ok);
 };
 
 // Runs members to completion, all active members advancing in lockstep: one
 // segmented SpMV launch and four segmented CGS passes per Lanczos step serve
-// every member that has a step left in its current sweep.
+// every member that has a step left in its current sweep; one segmented V.Q
+// launch restarts every member that restarts after the sweep.
 struct Engine {
     int dev = 0;
     hipStream_t stream = nullptr;
     Profiler prof;
-    // device tables
-    SegMember *dM = nullptr;
-    SegStep *dS = nullptr;
-    size_t capM = 0, capS = 0;
-    // pinned staging
-    SegMember *hM = nullptr;
-    SegStep *hS = nullptr;
-    size_t hcapM = 0, hcapS = 0;
+    // launch tables (device) and their pinned staging
+    SegMember *dM = nullptr, *hM = nullptr;
+    SegStep *dS = nullptr, *hS = nullptr;
+    SegVQ *dV = nullptr, *hV = nullptr;
+    size_t capM = 0, capS = 0, capV = 0;
+    // read-back and Q arenas (device + pinned)
+    double *dRB = nullptr, *hRB = nullptr, *dQ = nullptr, *hQ = nullptr;
+    size_t capRB = 0, capQ = 0;
+    struct Pool;
+    Pool *pool = nullptr;                // host threads for decide()
 
     double t_start = 0.0;                // steady-clock seconds at run()
     int init(int dev);
@@ -110,7 +123,8 @@ struct Engine {
     int run(std::vector<Member *> &queue, int max_active, double hbm_budget);
 
   private:
-    int grow(size_t nm, size_t ns);
+    int grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd);
+    int after_sweep(std::vector<Member *> &act);
 };
 
 }  // namespace hl

@@ -42,11 +42,19 @@ void dlartg(double f, double g, double &c, double &s, double &r)
     }
 }
 
-int tridiag_eig(int n, double *d, double *e, double *z, int ldz)
+namespace {
+// Implicit QL.  FULL: z is n x n (ldz), the eigenvectors.  !FULL: z holds only
+// the last row of the eigenvector matrix (z[i] = Z(n-1, i)); the rotations and
+// the eigenvalues are the same, so both forms agree bit for bit there.
+template <bool FULL>
+int tridiag_ql(int n, double *d, double *e, double *z, int ldz)
 {
     for (int i = 0; i < n; ++i) {
         if (!std::isfinite(d[i]) || (i + 1 < n && !std::isfinite(e[i]))) return 1;
-        for (int k = 0; k < n; ++k) z[(size_t)i * ldz + k] = (i == k) ? 1.0 : 0.0;
+        if (FULL)
+            for (int k = 0; k < n; ++k) z[(size_t)i * ldz + k] = (i == k) ? 1.0 : 0.0;
+        else
+            z[i] = (i == n - 1) ? 1.0 : 0.0;
     }
     if (n == 0) return 0;
     std::vector<double> off(n, 0.0); // off[i] couples rows i and i+1
@@ -86,11 +94,17 @@ int tridiag_eig(int n, double *d, double *e, double *z, int ldz)
                 p = s * r;
                 d[i + 1] = g + p;
                 g = c * r - b;
-                double *zi = z + (size_t)i * ldz, *zj = z + (size_t)(i + 1) * ldz;
-                for (int k = 0; k < n; ++k) {
-                    const double t = zj[k];
-                    zj[k] = s * zi[k] + c * t;
-                    zi[k] = c * zi[k] - s * t;
+                if (FULL) {
+                    double *zi = z + (size_t)i * ldz, *zj = z + (size_t)(i + 1) * ldz;
+                    for (int k = 0; k < n; ++k) {
+                        const double t = zj[k];
+                        zj[k] = s * zi[k] + c * t;
+                        zi[k] = c * zi[k] - s * t;
+                    }
+                } else {
+                    const double t = z[i + 1];
+                    z[i + 1] = s * z[i] + c * t;
+                    z[i] = c * z[i] - s * t;
                 }
             }
             if (underflow) continue;
@@ -106,7 +120,10 @@ int tridiag_eig(int n, double *d, double *e, double *z, int ldz)
             if (d[k] < d[b]) b = k;
         if (b != i) {
             std::swap(d[i], d[b]);
-            for (int k = 0; k < n; ++k) std::swap(z[(size_t)i * ldz + k], z[(size_t)b * ldz + k]);
+            if (FULL)
+                for (int k = 0; k < n; ++k) std::swap(z[(size_t)i * ldz + k], z[(size_t)b * ldz + k]);
+            else
+                std::swap(z[i], z[b]);
         }
     }
     for (int i = 0; i + 1 < n; ++i) e[i] = 0.0;
@@ -114,6 +131,11 @@ int tridiag_eig(int n, double *d, double *e, double *z, int ldz)
         if (!std::isfinite(d[i])) return 1;
     return 0;
 }
+} // namespace
+
+int tridiag_eig(int n, double *d, double *e, double *z, int ldz) { return tridiag_ql<true>(n, d, e, z, ldz); }
+
+int tridiag_eig_last(int n, double *d, double *e, double *zlast) { return tridiag_ql<false>(n, d, e, zlast, 1); }
 
 void householder_q(int m, int k, double *a, int lda, double *tau, double *q, int ldq)
 {

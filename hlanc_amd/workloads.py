@@ -12,6 +12,14 @@
   nnz = (3m - 2)^3.
 * ``erdos_renyi(n, deg, seed)``: symmetric random graph with mean degree
   ``deg``, weights U(-1,1), diagonal shifted to make it positive definite.
+* ``stencil5(m, seed)``: 5-point stencil on an m x m grid with the coefficient
+  rule of ``stencil27``.
+* ``batch_mix(count, seed)``: ``count`` instances with n log-uniform in
+  [1e3, 2e5]: a quarter 2-D 5-point and a quarter 3-D 27-point stencils, half
+  Erdos-Renyi graphs with degree uniform in 8..32.
+* ``serve_mix(count, seed)``: a pool of request matrices with n log-uniform in
+  [4e3, 2e6]: a quarter 27-point stencils, a quarter Chung-Lu Laplacians, half
+  Erdos-Renyi graphs, by seed.
 
 Each returns ``(row_ptr, col, val)`` as numpy arrays of a general (both
 triangles stored) CSR matrix with sorted column indices.
@@ -124,3 +132,59 @@ def erdos_renyi(n: int, deg: float, seed: int = 0):
     A.sum_duplicates()
     shift = np.abs(A).sum(axis=1).A.ravel().max() + 1.0
     return _csr(A + sp.eye(n) * shift)
+
+
+def stencil5(m: int, seed: int = 0):
+    rng = np.random.default_rng(seed)
+    n = m * m
+    idx = np.arange(n, dtype=np.int64).reshape(m, m)
+    rows, cols, vals = [], [], []
+    for a, b in ((idx[:-1, :].ravel(), idx[1:, :].ravel()), (idx[:, :-1].ravel(), idx[:, 1:].ravel())):
+        v = -(1.0 + 0.1 * rng.random(a.size))
+        rows += [a, b]
+        cols += [b, a]
+        vals += [v, v]
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    v = np.concatenate(vals)
+    diag = np.bincount(r, weights=np.abs(v), minlength=n) + 1e-3
+    r = np.concatenate([r, np.arange(n)])
+    c = np.concatenate([c, np.arange(n)])
+    v = np.concatenate([v, diag])
+    return _csr(sp.coo_matrix((v, (r, c)), shape=(n, n)))
+
+
+def mix_instance(kind: str, n: float, seed: int, deg: int = 16):
+    """One instance of the batch / serving mixes: kind 'st5', 'st27', 'er' or
+    'cl' (Chung-Lu normalized Laplacian)."""
+    if kind == "st5":
+        return stencil5(max(2, int(round(np.sqrt(n)))), seed)
+    if kind == "st27":
+        return stencil27(max(2, int(round(np.cbrt(n)))), seed)
+    if kind == "cl":
+        return chung_lu_laplacian(int(round(n)), seed=seed)
+    return erdos_renyi(int(round(n)), deg, seed)
+
+
+def batch_mix(count: int = 512, seed: int = 0, nmin: float = 1e3, nmax: float = 2e5):
+    """[(kind, (row_ptr, col, val))] of the batched uneven workload."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count))
+    degs = rng.integers(8, 33, count)
+    out = []
+    for i in range(count):
+        kind = ("st5", "er", "st27", "er")[i % 4]
+        out.append((kind, mix_instance(kind, ns[i], seed * 100003 + i, int(degs[i]))))
+    return out
+
+
+def serve_mix(count: int = 32, seed: int = 0, nmin: float = 4e3, nmax: float = 2e6):
+    """[(kind, (row_ptr, col, val))]: the request matrices of the serving stream."""
+    rng = np.random.default_rng(seed + 7)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count))
+    degs = rng.integers(8, 33, count)
+    out = []
+    for i in range(count):
+        kind = ("st27", "er", "cl", "er")[i % 4]
+        out.append((kind, mix_instance(kind, ns[i], seed * 100019 + i, int(degs[i]))))
+    return out

@@ -172,10 +172,14 @@ typedef struct {
 #define HL_FLAG_PROFILE 2  /* time every SpMV, CGS and V.Q launch with HIP events */
 
 /* Per-kernel-class totals of the last profiled solve (or batch) on the calling
- * thread: index 0 SpMV, 1 CGS passes, 2 V.Q GEMM, 3 other kernels, 4 and 5
- * unused.  bytes are algorithmic HBM bytes (each operand counted once per
- * launch; a DGKS correction pass counts only where it ran, from the device's
- * own counters). */
+ * thread: index 0 SpMV, 1 the two CGS passes of every Lanczos step (partials
+ * kernels), 2 V.Q GEMM, 3 other kernels (CGS finalize, norms), 4 the DGKS
+ * correction passes (partials kernels; launched every step, exiting at once
+ * when not needed), 5 the engine's host work between sweeps (launches = sweeps,
+ * seconds = host time: restart logic, launch tables, starts).  bytes are
+ * algorithmic HBM bytes (each operand
+ * counted once per launch; a correction pass counts only where it ran, from
+ * the device's own counters). */
 typedef struct {
     long long launches[6];
     double seconds[6];
