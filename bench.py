@@ -342,6 +342,12 @@ def main():
     log(f"profile (ret {rc}): " + ", ".join(f"{k} {v['launches']}x {v['seconds']:.3f}s {v['GBps'] or 0:.0f} GB/s"
                                            for k, v in classes.items()))
     achieved = pr.bytes[dom] / pr.seconds[dom] / 1e9
+    traffic = None  # measured HBM bytes per dispatch of the dominant kernel (rocprofv3 PMC runs)
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        rec = json.load(open(tpath)).get(args.workload, {}).get(names[dom])
+        if rec:
+            traffic = rec["fetch_bytes"] + rec["write_bytes"]
 
     t_spmv = spmv_timing(op, n)
     spmv_bytes = op.spmv_bytes()
@@ -378,7 +384,7 @@ def main():
                  "frac_of_peak": spmv_gbps / HBM_PEAK_GBPS},
         "kernel_classes": classes,
         "roofline": {"kernel": names[dom], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

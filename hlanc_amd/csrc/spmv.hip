@@ -51,9 +51,28 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
 
     if (b.tpr > 0) {
         const int nnz = b.p1 - b.p0;
-        for (int i = tid; i < nnz; i += kBlock) {
-            const int p = b.p0 + i;
-            prod[i] = static_cast<double>(val[p]) * (static_cast<double>(x[col[p]]) * s);
+        // all of this thread's (<= 8) column indices and values in flight at
+        // once, then all of its x gathers: the matrix is streamed once
+        // (non-temporal), so L2 keeps x
+        constexpr int kPer = kNnzPerBlock / kBlock;
+        int ci[kPer];
+        AT av[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = tid + k * kBlock;
+            if (i < nnz) {
+                ci[k] = __builtin_nontemporal_load(&col[b.p0 + i]);
+                av[k] = __builtin_nontemporal_load(&val[b.p0 + i]);
+            }
+        }
+        XT xv[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (tid + k * kBlock < nnz) xv[k] = x[ci[k]];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = tid + k * kBlock;
+            if (i < nnz) prod[i] = static_cast<double>(av[k]) * (static_cast<double>(xv[k]) * s);
         }
         if (vout) {
             for (int r = b.row0 + tid; r < b.row1; r += kBlock)
@@ -78,10 +97,12 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
         return;
     }
 
-    // chunk of a long row
+    // chunk of a long row (loads of later terms may run ahead; the sum keeps its order)
     double acc = 0.0;
+#pragma unroll 4
     for (int p = b.p0 + tid; p < b.p1; p += kBlock)
-        acc += static_cast<double>(val[p]) * (static_cast<double>(x[col[p]]) * s);
+        acc += static_cast<double>(__builtin_nontemporal_load(&val[p])) *
+               (static_cast<double>(x[__builtin_nontemporal_load(&col[p])]) * s);
     const double tot = block_sum_to_thread0(acc, wred);
     if (tid != 0) return;
     if (vout && b.chunk == 0)

@@ -3,6 +3,11 @@ import sys
 
 import pytest
 
+# PyTorch-ROCm ships its own HIP runtime with the same soname as /opt/rocm's.
+# Whichever loads first serves the whole process, and torch fails to find the
+# GPU when the library under test loaded /opt/rocm's first, so torch goes first.
+import torch  # noqa: F401,E402
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "tests")):
     if p not in sys.path:
