@@ -48,7 +48,8 @@ class Options(ctypes.Structure):
 
 
 class Info(ctypes.Structure):
-    _fields_ = [("ret", ctypes.c_int), ("restarts", ctypes.c_int), ("nconv", ctypes.c_int), ("pad", ctypes.c_int),
+    _fields_ = [("ret", ctypes.c_int), ("restarts", ctypes.c_int), ("nconv", ctypes.c_int),
+                ("graph_sweeps", ctypes.c_int),
                 ("spmv", ctypes.c_longlong), ("cgs_passes", ctypes.c_longlong), ("rnorm", ctypes.c_double),
                 ("seconds", ctypes.c_double), ("max_rel_resid", ctypes.c_double),
                 ("refine_shift", ctypes.c_double)]
@@ -62,6 +63,7 @@ class Profile(ctypes.Structure):
 
 
 PROFILE_CLASSES = ["spmv", "cgs", "vq", "other", "dgks", "host"]
+HL_FLAG_NO_GRAPH = 1
 HL_FLAG_PROFILE = 2
 
 

@@ -104,6 +104,7 @@ int solve_common(hl::Member &m, int dev, int n, int nev, int ncv, double *values
     hl::Engine eng;
     if ((rc = eng.init(dev))) return rc;
     eng.prof.on = (o.flags & HL_FLAG_PROFILE) != 0;
+    eng.graphs = (o.flags & HL_FLAG_NO_GRAPH) == 0;
     std::vector<hl::Member *> q{&m};
     rc = eng.run(q, 1, 0.0);
     if (!rc) rc = m.err;
@@ -436,7 +437,7 @@ int hl_ds_solve_batch(int dev, hl_solve_item *items, int count, int max_active, 
     DeviceGuard g(dev);
     std::vector<hl::Member> ms(static_cast<size_t>(count));
     std::vector<hl::Member *> queue;
-    bool profile = false;
+    bool profile = false, graphs = true;
     double last_arrival = 0.0;
     for (int i = 0; i < count; ++i) {
         hl_solve_item &it = items[i];
@@ -467,11 +468,13 @@ int hl_ds_solve_batch(int dev, hl_solve_item *items, int count, int max_active, 
         m.ldv_host = it.ldv;
         m.arrival = it.arrival;
         profile |= (it.opts.flags & HL_FLAG_PROFILE) != 0;
+        graphs &= (it.opts.flags & HL_FLAG_NO_GRAPH) == 0;
         queue.push_back(&m);
     }
     hl::Engine eng;
     if ((rc = eng.init(dev))) return rc;
     eng.prof.on = profile;
+    eng.graphs = graphs;
     rc = eng.run(queue, max_active, hbm_budget);
     if (profile && !rc) thread_profile() = eng.prof.tot;
     for (int i = 0; i < count; ++i) {

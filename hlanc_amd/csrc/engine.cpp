@@ -63,6 +63,7 @@ int Member::start(hipStream_t s)
     ritz_saved.clear();
     bounds_saved.clear();
     spmv = cgs = p2 = p3 = m2sum = m3sum = d_m23 = 0;
+    graph_sweeps = 0;
     nconv = 0;
     rnorm = 0.0;
     ret = HL_NOT_CONVERGED;
@@ -262,6 +263,7 @@ void Member::complete()
     info.seconds = now_s() - t0;
     info.max_rel_resid = -1.0;
     info.refine_shift = 0.0;
+    info.graph_sweeps = graph_sweeps;
     phase = DONE;
 }
 
@@ -341,9 +343,17 @@ int Engine::init(int dev_)
     return HL_OK;
 }
 
+void Engine::drop_graphs()
+{
+    for (GraphEntry &g : gcache) (void)hipGraphExecDestroy(g.exec);
+    gcache.clear();
+    gseen.clear();
+}
+
 void Engine::release()
 {
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
     void *d[] = {dM, dS, dV, dRB, dQ};
     for (void *p : d)
         if (p) (void)hipFree(p);
@@ -381,6 +391,7 @@ int grow_pair(T *&d, T *&h, size_t &cap, size_t need)
 int Engine::grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd)
 {
     HL_TRY_HIP(hipStreamSynchronize(stream)); // the pinned tables are rewritten every sweep
+    if (nm > capM || ns > capS) drop_graphs(); // captured launches name the old tables
     int rc;
     if ((rc = grow_pair(dM, hM, capM, nm))) return rc;
     if ((rc = grow_pair(dS, hS, capS, ns))) return rc;
@@ -460,33 +471,96 @@ int Engine::sweep(std::vector<Member *> &ms)
     double host_s = now_s() - th0; // host time of this sweep (profiled runs report it)
 
     const int modes[4] = {EPI_WNORM, EPI_PASS1, EPI_PASS2, EPI_PASS3};
-    for (int st = 0; st < maxsteps; ++st) {
-        const SegStep *S = dS + off[st];
-        const int cnt = off[st + 1] - off[st];
-        double bytes = 0.0;
-        for (int q = off[st]; q < off[st + 1]; ++q) {
-            Member &m = *act[hS[q].mem];
-            if (!m.A && (rc = m.host_apply(hS[q].j, stream))) return rc;
-            if (m.A) bytes += 12.0 * m.A->nnz + 4.0 * (m.A->nrows + 1) + 8.0 * m.A->ncols + 16.0 * m.A->nrows;
+    bool any_host = false;
+    for (Member *m : act) any_host |= (m->A == nullptr);
+    auto launch_steps = [&]() -> int {
+        int r;
+        for (int st = 0; st < maxsteps; ++st) {
+            const SegStep *S = dS + off[st];
+            const int cnt = off[st + 1] - off[st];
+            double bytes = 0.0;
+            for (int q = off[st]; q < off[st + 1]; ++q) {
+                Member &m = *act[hS[q].mem];
+                if (!m.A && (r = m.host_apply(hS[q].j, stream))) return r;
+                if (m.A) bytes += 12.0 * m.A->nnz + 4.0 * (m.A->nrows + 1) + 8.0 * m.A->ncols + 16.0 * m.A->nrows;
+            }
+            if (nspmv[st] > 0) {
+                if ((r = prof.begin(stream))) return r;
+                if ((r = launch_spmv_seg(dM, S, cnt, nspmv[st], stream))) return r;
+                if ((r = prof.end(stream, K_SPMV, bytes))) return r;
+            }
+            for (int p = 0; p < 4; ++p) {
+                bytes = 0.0;
+                if (p < 2) // executed DGKS corrections are counted from the device counters
+                    for (int q = off[st]; q < off[st + 1]; ++q) {
+                        const Member &m = *act[hS[q].mem];
+                        bytes += m.cgs_bytes_pass(hS[q].j + 1) - (p == 0 ? 8.0 * m.n : 0.0);
+                    }
+                if ((r = prof.begin(stream))) return r;
+                if ((r = launch_cgs_seg(dM, S, cnt, maxcpg[st], ncgs[st], modes[p], stream))) return r;
+                if ((r = prof.end(stream, p < 2 ? K_CGS : K_DGKS, bytes))) return r;
+                if ((r = prof.begin(stream))) return r;
+                if ((r = launch_cgs_seg_finalize(dM, S, cnt, nfin[st], modes[p], stream))) return r;
+                if ((r = prof.end(stream, K_MISC, 0.0))) return r;
+            }
         }
-        if (nspmv[st] > 0) {
-            if ((rc = prof.begin(stream))) return rc;
-            if ((rc = launch_spmv_seg(dM, S, cnt, nspmv[st], stream))) return rc;
-            if ((rc = prof.end(stream, K_SPMV, bytes))) return rc;
+        return HL_OK;
+    };
+    if (!graphs || prof.on || any_host) {
+        if ((rc = launch_steps())) return rc;
+    } else {
+        // the sweep's shape: table addresses and every step's segment and grid sizes
+        std::vector<long long> key;
+        key.reserve(2 + 6 * (size_t)maxsteps);
+        key.push_back(reinterpret_cast<long long>(dM));
+        key.push_back(reinterpret_cast<long long>(dS));
+        for (int st = 0; st < maxsteps; ++st) {
+            key.push_back(off[st]);
+            key.push_back(off[st + 1] - off[st]);
+            key.push_back(nspmv[st]);
+            key.push_back(ncgs[st]);
+            key.push_back(nfin[st]);
+            key.push_back(maxcpg[st]);
         }
-        for (int p = 0; p < 4; ++p) {
-            bytes = 0.0;
-            if (p < 2) // executed DGKS corrections are counted from the device counters
-                for (int q = off[st]; q < off[st + 1]; ++q) {
-                    const Member &m = *act[hS[q].mem];
-                    bytes += m.cgs_bytes_pass(hS[q].j + 1) - (p == 0 ? 8.0 * m.n : 0.0);
-                }
-            if ((rc = prof.begin(stream))) return rc;
-            if ((rc = launch_cgs_seg(dM, S, cnt, maxcpg[st], ncgs[st], modes[p], stream))) return rc;
-            if ((rc = prof.end(stream, p < 2 ? K_CGS : K_DGKS, bytes))) return rc;
-            if ((rc = prof.begin(stream))) return rc;
-            if ((rc = launch_cgs_seg_finalize(dM, S, cnt, nfin[st], modes[p], stream))) return rc;
-            if ((rc = prof.end(stream, K_MISC, 0.0))) return rc;
+        ++gclock;
+        GraphEntry *hit = nullptr;
+        for (GraphEntry &g : gcache)
+            if (g.key == key) hit = &g;
+        if (!hit && std::find(gseen.begin(), gseen.end(), key) != gseen.end()) {
+            // second sighting of this shape: capture it once, replay from now on
+            hipGraph_t graph = nullptr;
+            HL_TRY_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            rc = launch_steps();
+            hipError_t ce = hipStreamEndCapture(stream, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            if (ce != hipSuccess) return hip_status(ce, "hipStreamEndCapture");
+            hipGraphExec_t exec = nullptr;
+            ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ce != hipSuccess) return hip_status(ce, "hipGraphInstantiate");
+            constexpr size_t kMaxGraphs = 16;
+            if (gcache.size() >= kMaxGraphs) { // evict the least recently used
+                auto lru = std::min_element(gcache.begin(), gcache.end(),
+                                            [](const GraphEntry &a, const GraphEntry &b) { return a.last < b.last; });
+                (void)hipGraphExecDestroy(lru->exec);
+                gcache.erase(lru);
+            }
+            gcache.push_back(GraphEntry{key, exec, gclock});
+            hit = &gcache.back();
+            ++graph_captures;
+        }
+        if (hit) {
+            hit->last = gclock;
+            HL_TRY_HIP(hipGraphLaunch(hit->exec, stream));
+            ++graph_launches;
+            for (Member *m : act) ++m->graph_sweeps;
+        } else {
+            if (gseen.size() >= 64) gseen.erase(gseen.begin());
+            gseen.push_back(key);
+            if ((rc = launch_steps())) return rc;
         }
     }
     // every member's alpha/beta entries and counters in one copy

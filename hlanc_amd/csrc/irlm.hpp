@@ -67,6 +67,7 @@ struct Member {
     double tol = 0.0, rnorm = 0.0, t0 = 0.0;
     long long spmv = 0, cgs = 0, p2 = 0, p3 = 0, m2sum = 0, m3sum = 0;
     long long d_m23 = 0;                 // DGKS basis columns read in the last sweep (profile bytes)
+    int graph_sweeps = 0;                // sweeps replayed from a captured graph
     int ret = HL_NOT_CONVERGED;
     int err = HL_OK;                     // a HIP or allocation failure of this member
     hl_info info{};
@@ -107,6 +108,20 @@ struct Engine {
     size_t capRB = 0, capQ = 0;
     struct Pool;
     Pool *pool = nullptr;                // host threads for decide()
+    // Sweep launch sequences replayed as HIP graphs: a sweep's launches depend
+    // only on the table addresses and per-step grid shapes, so a shape seen
+    // twice is captured once and replayed after that (no profiling, no host
+    // operators, not with HL_FLAG_NO_GRAPH).
+    bool graphs = true;
+    struct GraphEntry {
+        std::vector<long long> key;
+        hipGraphExec_t exec;
+        long long last;
+    };
+    std::vector<GraphEntry> gcache;
+    std::vector<std::vector<long long>> gseen; // shapes met once
+    long long gclock = 0;
+    long long graph_launches = 0, graph_captures = 0;
 
     double t_start = 0.0;                // steady-clock seconds at run()
     int init(int dev);
@@ -125,6 +140,7 @@ struct Engine {
   private:
     int grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd);
     int after_sweep(std::vector<Member *> &act);
+    void drop_graphs();
 };
 
 }  // namespace hl

@@ -158,7 +158,7 @@ typedef struct {
     int ret;                /* status as returned */
     int restarts;           /* restart iterations run */
     int nconv;              /* converged Ritz values */
-    int pad;
+    int graph_sweeps;       /* dsaitr sweeps replayed from a captured HIP graph */
     long long spmv;         /* operator applications */
     long long cgs_passes;   /* CGS passes incl. DGKS corrections */
     double rnorm;           /* final residual norm */
@@ -168,7 +168,9 @@ typedef struct {
 } hl_info;
 
 /* hl_options.flags bits. */
-#define HL_FLAG_NO_GRAPH 1 /* reserved: launch the Lanczos steps one by one */
+#define HL_FLAG_NO_GRAPH 1 /* launch every sweep's kernels one by one; by default a sweep
+                              shape met a second time is captured as a HIP graph and
+                              replayed (same kernels, same results) */
 #define HL_FLAG_PROFILE 2  /* time every SpMV, CGS and V.Q launch with HIP events */
 
 /* Per-kernel-class totals of the last profiled solve (or batch) on the calling

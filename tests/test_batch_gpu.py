@@ -166,3 +166,12 @@ def test_profiled_batch_counts_every_class(mixed):
     assert pr.launches[0] >= steps and pr.launches[1] >= 2 * steps and pr.launches[4] >= 2 * steps
     assert pr.launches[3] >= 4 * steps
     assert pr.bytes[0] > 0 and pr.bytes[1] > 0 and pr.bytes[4] > 0 and pr.launches[2] > 0
+
+
+def test_batch_with_and_without_graphs(mixed):
+    mats, ops, alone = mixed
+    a = run_batch(ops)
+    b = run_batch(ops, flags=H.HL_FLAG_NO_GRAPH)
+    assert_same(a, alone)
+    assert_same(b, alone)
+    assert all(r.info.graph_sweeps == 0 for r in b)

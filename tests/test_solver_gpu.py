@@ -233,3 +233,19 @@ def test_cgs_grid_cap_does_not_change_the_solve_beyond_rounding(blocks):
     o = O.ds_solver(O.Csr(rp, col, val), 8, 24, tol=1e-10, maxiter=1000)
     assert r.ret == 0 and o.ret == 0
     np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
+
+
+def test_graph_replay_matches_launch_by_launch():
+    # hundreds of restarts whose sweep shapes repeat: replayed HIP graphs must
+    # give the same bits as launching every kernel on its own
+    rp, col, val = W.laplacian_2d(48)
+    n = len(rp) - 1
+    op = op_for(rp, col, val)
+    a = H.ds_solver_gpu(n, 8, 24, op, tol=1e-10, maxiter=3000)
+    b = H.ds_solver_gpu(n, 8, 24, op, tol=1e-10, maxiter=3000, flags=H.HL_FLAG_NO_GRAPH)
+    assert a.ret == b.ret == 0
+    assert a.info.graph_sweeps > 0 and b.info.graph_sweeps == 0
+    assert a.info.restarts == b.info.restarts
+    np.testing.assert_array_equal(a.values, b.values)
+    np.testing.assert_array_equal(a.vectors, b.vectors)
+    np.testing.assert_allclose(a.values, W.laplacian_2d_eigs(48)[-8:], rtol=1e-10)
