@@ -262,6 +262,7 @@ def main():
     ap.add_argument("--pool", type=int, default=32, help="serve: distinct matrices")
     ap.add_argument("--seed", type=int, default=0, help="serve: arrival stream seed")
     ap.add_argument("--profile", action="store_true", help="batch: one more profiled batched pass")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV work-item shape")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -300,6 +301,11 @@ def main():
     log(f"workload {args.workload}: n={n} nnz={len(val)} generated in {time.perf_counter() - t0:.1f} s")
     op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
     torch.cuda.init()
+    items = None
+    if not args.no_autotune:
+        nc, rc_, t_best = op.autotune(reps=10)
+        items = {"nnz_cap": nc, "rows_cap": rc_}
+        log(f"SpMV autotune: items of <= {nc} nnz / <= {rc_} rows, {t_best * 1e6:.1f} us per launch")
 
     for i in range(args.warmup):
         r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter)
@@ -373,7 +379,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded generator)",
         "config": {"workload": label, "n": n, "nnz": int(len(val)), "nev": nev, "ncv": ncv, "tol": args.tol,
-                   "which": "largest algebraic", "basis": "fp64"},
+                   "which": "largest algebraic", "basis": "fp64", "spmv_items": items},
         "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
         "max_rel_resid": resid,
         "ritz_orthogonality": ortho,

@@ -92,6 +92,8 @@ _SIGS = [
     ("hl_csr_spmv", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("hl_csr_spmv_f32val", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("hl_csr_spmv_bytes", ctypes.c_double, [ctypes.c_void_p, ctypes.c_int]),
+    ("hl_csr_set_items", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("hl_csr_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip, ctypes.POINTER(ctypes.c_double)]),
     ("hl_cgs_pass", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p]),
@@ -275,6 +277,17 @@ class dcsrmv_operator_gpu:
 
     def spmv_bytes(self, fp32_values: bool = False) -> float:
         return lib().hl_csr_spmv_bytes(self.h, HL_PREC_F32 if fp32_values else HL_PREC_F64)
+
+    def set_items(self, nnz_cap: int, rows_cap: int) -> None:
+        """Work-item shape of the SpMV (hl_csr_set_items)."""
+        _check(lib().hl_csr_set_items(self.h, int(nnz_cap), int(rows_cap)))
+
+    def autotune(self, reps: int = 20):
+        """Pick the fastest work-item shape (hl_csr_autotune); returns
+        (nnz_cap, rows_cap, seconds per SpMV)."""
+        nc, rc_, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().hl_csr_autotune(self.h, int(reps), ctypes.byref(nc), ctypes.byref(rc_), ctypes.byref(t)))
+        return nc.value, rc_.value, t.value
 
 
 class dcsrmv_operator_mgpu(dcsrmv_operator_gpu):

@@ -237,6 +237,77 @@ double hl_csr_spmv_bytes(hl_csr_t h, int val_prec)
 }
 
 namespace {
+int csr_items(hl_csr_t h, int nnz_cap, int rows_cap)
+{
+    std::vector<int> R(static_cast<size_t>(h->plan.nrows) + 1);
+    HL_TRY_HIP(hipMemcpy(R.data(), h->plan.row_ptr, sizeof(int) * R.size(), hipMemcpyDeviceToHost));
+    return plan_items(h->plan, R.data(), nnz_cap, rows_cap);
+}
+} // namespace
+
+int hl_csr_set_items(hl_csr_t h, int nnz_cap, int rows_cap)
+{
+    if (!h) return HL_ERR_ARG;
+    DeviceGuard g(h->plan.dev);
+    std::lock_guard<std::mutex> lk(h->mu);
+    return csr_items(h, nnz_cap, rows_cap);
+}
+
+int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *seconds)
+{
+    if (!h || reps <= 0) return HL_ERR_ARG;
+    DeviceGuard g(h->plan.dev);
+    std::lock_guard<std::mutex> lk(h->mu);
+    CsrPlan &p = h->plan;
+    double *x = nullptr, *y = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc = HL_OK, best_n = kNnzPerBlock, best_r = kRowsPerBlock;
+    double best_t = 1e30;
+    auto cleanup = [&]() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        if (x) (void)hipFree(x);
+        if (y) (void)hipFree(y);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&x, sizeof(double) * std::max(p.ncols, 1))) != hipSuccess ||
+        (e = hipMalloc(&y, sizeof(double) * std::max(p.nrows, 1))) != hipSuccess ||
+        (e = hipMemset(x, 0, sizeof(double) * std::max(p.ncols, 1))) != hipSuccess ||
+        (e = hipEventCreate(&a)) != hipSuccess || (e = hipEventCreate(&b)) != hipSuccess) {
+        cleanup();
+        return hip_status(e, "hl_csr_autotune");
+    }
+    const int ncap[] = {512, 1024, 2048};
+    const int rcap[] = {64, 128, 256, 512, 1024, 2048};
+    for (int nc : ncap)
+        for (int rcp : rcap) {
+            if ((rc = csr_items(h, nc, rcp))) break;
+            for (int i = 0; i < 3 && !rc; ++i) rc = launch_spmv(p, x, nullptr, y, VColumn{}, HL_PREC_F64, nullptr);
+            if (rc) break;
+            float ms = 0.0f;
+            if ((e = hipEventRecord(a, nullptr)) != hipSuccess) break;
+            for (int i = 0; i < reps && !rc; ++i) rc = launch_spmv(p, x, nullptr, y, VColumn{}, HL_PREC_F64, nullptr);
+            if (rc || (e = hipEventRecord(b, nullptr)) != hipSuccess || (e = hipEventSynchronize(b)) != hipSuccess ||
+                (e = hipEventElapsedTime(&ms, a, b)) != hipSuccess)
+                break;
+            const double t = 1e-3 * ms / reps;
+            if (t < best_t) {
+                best_t = t;
+                best_n = nc;
+                best_r = rcp;
+            }
+        }
+    if (!rc && e != hipSuccess) rc = hip_status(e, "hl_csr_autotune");
+    cleanup();
+    if (rc) return rc;
+    if ((rc = csr_items(h, best_n, best_r))) return rc;
+    if (nnz_cap) *nnz_cap = best_n;
+    if (rows_cap) *rows_cap = best_r;
+    if (seconds) *seconds = best_t;
+    return HL_OK;
+}
+
+namespace {
 // Reduction workspace of the standalone basis entry points, one per host
 // thread and device, kept for the life of the process so that the calls are
 // asynchronous and cheap to repeat.

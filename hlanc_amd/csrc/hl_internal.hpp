@@ -113,8 +113,9 @@ struct SpmvBlk {
     int tpr, chunk, nchunks, slot;
 };
 
-constexpr int kNnzPerBlock = 2048;  // stream block capacity (8 nnz per thread)
-constexpr int kRowsPerBlock = 256;
+constexpr int kNnzPerBlock = 2048;  // stream item capacity (8 nnz per thread), default and maximum
+constexpr int kRowsPerBlock = 256;  // default rows per stream item
+constexpr int kMaxRowsPerBlock = 2048;
 constexpr int kLongChunk = 8192;    // nnz per chunk of a long row
 
 struct CsrPlan {
@@ -131,11 +132,14 @@ struct CsrPlan {
     unsigned *chunk_cnt = nullptr;
     int nslots = 0;
     double *x_stage = nullptr, *y_stage = nullptr;  // for host-pointer calls
+    int nnz_cap = kNnzPerBlock, rows_cap = kRowsPerBlock; // work-item shape in use
 };
 
 int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *col,
                    const double *val, int sym_lower, int val_prec);
 void csr_plan_free(CsrPlan &p);
+// Re-cuts the work items (see hl_csr_set_items); R is a host copy of row_ptr.
+int plan_items(CsrPlan &p, const int *R, int nnz_cap, int rows_cap);
 
 // y = A (x * xscale) with the optional fused write vcol(r) = x[r] * xscale.
 // xscale == nullptr means 1.  val_prec selects the FP64 or FP32 value array.

@@ -226,11 +226,40 @@ int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *c
     }
     const long long nnz = nrows > 0 ? R[nrows] : 0;
 
+    p.nrows = nrows;
+    p.ncols = ncols;
+    p.nnz = nnz;
+    int rc;
+    if ((rc = dev_upload(&p.row_ptr, R, static_cast<size_t>(nrows) + 1))) return rc;
+    if ((rc = dev_upload(&p.col, C, static_cast<size_t>(nnz)))) return rc;
+    if ((rc = dev_upload(&p.val, A, static_cast<size_t>(nnz)))) return rc;
+    if (val_prec == HL_PREC_F32) {
+        std::vector<float> v32(static_cast<size_t>(nnz));
+        for (long long i = 0; i < nnz; ++i) v32[i] = static_cast<float>(A[i]);
+        if ((rc = dev_upload(&p.val32, v32.data(), static_cast<size_t>(nnz)))) return rc;
+    }
+    if ((rc = dev_upload<double>(&p.x_stage, nullptr, static_cast<size_t>(ncols)))) return rc;
+    if ((rc = dev_upload<double>(&p.y_stage, nullptr, static_cast<size_t>(nrows)))) return rc;
+    return plan_items(p, R, kNnzPerBlock, kRowsPerBlock);
+}
+
+// Cuts the rows into work items by an nnz walk over row_ptr (host copy R):
+// stream items of <= rows_cap whole rows holding <= nnz_cap nnz, and chunk items
+// of <= kLongChunk nnz for every row longer than nnz_cap.  Replaces the plan's
+// items and long-row slots.
+int plan_items(CsrPlan &p, const int *R, int nnz_cap, int rows_cap)
+{
+    if (nnz_cap < kBlock || nnz_cap > kNnzPerBlock || rows_cap < 1 || rows_cap > kMaxRowsPerBlock) {
+        set_error("SpMV items: nnz cap %d (256..%d) or rows cap %d (1..%d) out of range", nnz_cap, kNnzPerBlock,
+                  rows_cap, kMaxRowsPerBlock);
+        return HL_ERR_ARG;
+    }
+    const int nrows = p.nrows;
     std::vector<SpmvBlk> blks;
     int slots = 0;
     for (int r = 0; r < nrows;) {
         const int len = R[r + 1] - R[r];
-        if (len > kNnzPerBlock) {
+        if (len > nnz_cap) {
             const int nch = (len + kLongChunk - 1) / kLongChunk;
             for (int c = 0; c < nch; ++c) {
                 SpmvBlk b{};
@@ -250,9 +279,9 @@ int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *c
         }
         const int r0 = r;
         int nz = 0;
-        while (r < nrows && r - r0 < kRowsPerBlock) {
+        while (r < nrows && r - r0 < rows_cap) {
             const int l = R[r + 1] - R[r];
-            if (l > kNnzPerBlock || nz + l > kNnzPerBlock) break;
+            if (l > nnz_cap || nz + l > nnz_cap) break;
             nz += l;
             ++r;
         }
@@ -267,27 +296,22 @@ int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *c
         b.tpr = tpr;
         blks.push_back(b);
     }
-
-    p.nrows = nrows;
-    p.ncols = ncols;
-    p.nnz = nnz;
+    void *old[] = {p.blks, p.chunk_part, p.chunk_cnt};
+    for (void *q : old)
+        if (q) (void)hipFree(q);
+    p.blks = nullptr;
+    p.chunk_part = nullptr;
+    p.chunk_cnt = nullptr;
     p.nblk = static_cast<int>(blks.size());
     p.nslots = slots;
+    p.nnz_cap = nnz_cap;
+    p.rows_cap = rows_cap;
     int rc;
-    if ((rc = dev_upload(&p.row_ptr, R, static_cast<size_t>(nrows) + 1))) return rc;
-    if ((rc = dev_upload(&p.col, C, static_cast<size_t>(nnz)))) return rc;
-    if ((rc = dev_upload(&p.val, A, static_cast<size_t>(nnz)))) return rc;
-    if (val_prec == HL_PREC_F32) {
-        std::vector<float> v32(static_cast<size_t>(nnz));
-        for (long long i = 0; i < nnz; ++i) v32[i] = static_cast<float>(A[i]);
-        if ((rc = dev_upload(&p.val32, v32.data(), static_cast<size_t>(nnz)))) return rc;
-    }
     if ((rc = dev_upload(&p.blks, blks.data(), blks.size()))) return rc;
     if ((rc = dev_upload<double>(&p.chunk_part, nullptr, static_cast<size_t>(slots)))) return rc;
     if ((rc = dev_upload<unsigned>(&p.chunk_cnt, nullptr, static_cast<size_t>(slots)))) return rc;
     HL_TRY_HIP(hipMemset(p.chunk_cnt, 0, sizeof(unsigned) * std::max(slots, 1)));
-    if ((rc = dev_upload<double>(&p.x_stage, nullptr, static_cast<size_t>(ncols)))) return rc;
-    if ((rc = dev_upload<double>(&p.y_stage, nullptr, static_cast<size_t>(nrows)))) return rc;
+    HL_TRY_HIP(hipDeviceSynchronize());
     return HL_OK;
 }
 

@@ -99,6 +99,18 @@ int hl_csr_spmv(hl_csr_t h, const double *x_dev, double *y_dev, void *stream);
 int hl_csr_spmv_f32val(hl_csr_t h, const double *x_dev, double *y_dev, void *stream);
 /* Algorithmic bytes of one SpMV: (s_A + 4) nnz + 4 (nrows + 1) + 8 ncols + 8 nrows. */
 double hl_csr_spmv_bytes(hl_csr_t h, int val_prec);
+/* Work-item shape of the operator's SpMV: stream items hold <= rows_cap whole
+ * rows (1..2048) with <= nnz_cap nonzeros (256..2048); longer rows are split
+ * into chunks.  The default is 2048 / 256.  Changing it changes the order of
+ * the row sums (results move at rounding level) and must not overlap solves
+ * that use the operator.  No counterpart in the reference (cuSPARSE picks its
+ * own schedule). */
+int hl_csr_set_items(hl_csr_t h, int nnz_cap, int rows_cap);
+/* Autotuner: times `reps` SpMV launches for every shape of a small grid
+ * (nnz_cap in {512, 1024, 2048} x rows_cap in {64 .. 2048}) with HIP events on
+ * the null stream, keeps the fastest and reports it (outputs may be NULL).
+ * Same rules as hl_csr_set_items. */
+int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *seconds);
 
 /* ------------------------------------------------------ basis operations
  * Device-pointer forms of mat_v_gpu_t's dense work, exposed for testing and for

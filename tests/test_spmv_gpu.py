@@ -121,3 +121,59 @@ def test_mgpu_collapses_to_one_device_and_rejects_symmetric():
     assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
     with pytest.raises(H.HLancError):
         H.dcsrmv_operator_mgpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, is_symmetric=True))
+
+
+def ragged_matrix(seed=5):
+    """Empty rows, rows just past the item capacity, multi-chunk rows."""
+    rng = np.random.default_rng(seed)
+    n, ncols = 900, 30000
+    lens = rng.integers(0, 40, n)
+    lens[[3, 100, 101, 500]] = [0, 2049, 8193, 20000]
+    lens[[7, 8, 9]] = [513, 1025, 300]
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = np.concatenate([np.sort(rng.choice(ncols, l, replace=False)) for l in lens]).astype(np.int32)
+    val = rng.uniform(-1, 1, rp[-1])
+    return rp, col, val, ncols
+
+
+@pytest.mark.parametrize("nnz_cap", [256, 512, 1024, 2048])
+@pytest.mark.parametrize("rows_cap", [1, 64, 256, 2048])
+def test_every_item_shape_matches_oracle(nnz_cap, rows_cap):
+    # work-item shapes the autotuner may pick: rows longer than nnz_cap become
+    # chunks, short-row matrices get up to 2048 rows per item
+    rp, col, val, ncols = ragged_matrix()
+    m = H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, ncols=ncols)
+    op = H.dcsrmv_operator_gpu(m)
+    op.set_items(nnz_cap, rows_cap)
+    x = np.random.default_rng(1).uniform(-1, 1, ncols)
+    y_ref = O.spmv(O.Csr(rp, col, val, ncols, False), x)
+    assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
+    rp2, col2, val2 = W.laplacian_2d(100)  # 5 nonzeros per row: row-capped items
+    op2 = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp2, col=col2, val=val2))
+    op2.set_items(nnz_cap, rows_cap)
+    x2 = np.random.default_rng(2).uniform(-1, 1, 10000)
+    y2 = O.spmv(O.Csr(rp2, col2, val2, 10000, False), x2)
+    assert np.linalg.norm(op2(x2) - y2) <= TOL * np.linalg.norm(y2)
+
+
+def test_bad_item_shapes_are_refused():
+    rp, col, val = W.laplacian_2d(10)
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    for nc, rc in ((255, 64), (4096, 64), (1024, 0), (1024, 4096)):
+        with pytest.raises(H.HLancError):
+            op.set_items(nc, rc)
+
+
+def test_autotune_then_solve_matches_oracle():
+    rp, col, val = W.stencil27(14, seed=3)
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    nc, rc, t = op.autotune(reps=5)
+    assert nc in (512, 1024, 2048) and rc in (64, 128, 256, 512, 1024, 2048) and t > 0
+    x = np.random.default_rng(4).uniform(-1, 1, n)
+    y_ref = O.spmv(O.Csr(rp, col, val, n, False), x)
+    assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
+    r = H.ds_solver_gpu(n, 8, 24, op, tol=1e-10, maxiter=1000)
+    o = O.ds_solver(O.Csr(rp, col, val), 8, 24, tol=1e-10, maxiter=1000)
+    assert r.ret == 0 and o.ret == 0
+    np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
