@@ -512,19 +512,6 @@ __global__ __launch_bounds__(kBlock) void scale_copy_kernel(int n, const double 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void scale_rscale_kernel(int n, double *r, double q, const Ctl *ctl)
-{
-    const double s = q * ctl->rscale;
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-        r[i] *= s;
-}
-
-__global__ __launch_bounds__(kBlock) void cast_kernel(long long n, const double *in, float *out)
-{
-    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-        out[i] = static_cast<float>(in[i]);
-}
-
 int grid_for(long long n, int per_block, int cap)
 {
     long long g = (n + per_block - 1) / per_block;
@@ -679,28 +666,12 @@ int reduce_alloc(Reduce &r, int max_blocks, int max_cols)
     return HL_OK;
 }
 
-void reduce_free(Reduce &r)
-{
-    if (r.partials) (void)hipFree(r.partials);
-    if (r.counter) (void)hipFree(r.counter);
-    r = Reduce{};
-}
-
 int launch_cgs(int n, int m, const double *V, VLayout L, const double *w_in, double *w_out, const double *h_in,
                double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia, int mode, int j, hipStream_t s)
 {
     if (L.tiled)
         return launch_cgs_t<double, true>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
     return launch_cgs_t<double, false>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
-}
-
-int launch_cgs_f32(int n, int m, const float *V, VLayout L, const double *w_in, double *w_out, const double *h_in,
-                   double *h_out, Reduce &red, Ctl *ctl, double *h_sub, double *h_dia, int mode, int j,
-                   hipStream_t s)
-{
-    if (L.tiled)
-        return launch_cgs_t<float, true>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
-    return launch_cgs_t<float, false>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
 }
 
 int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStream_t s)
@@ -729,21 +700,6 @@ int launch_vq(int n, int K, int N, const double *V, VLayout Lin, const double *Q
 #undef HL_VQ_DISPATCH
 }
 
-int launch_vq_f32(int n, int K, int N, const float *V, VLayout Lin, const double *Q, int ldq, float *Out,
-                  VLayout Lout, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
-                  hipStream_t s)
-{
-    if (K <= 0 || N <= 0 || K > 128) {
-        set_error("V.Q: K=%d N=%d unsupported (K <= 128)", K, N);
-        return HL_ERR_ARG;
-    }
-    if (Lin.tiled && Lout.tiled)
-        return launch_vq_fma<float, true, true>(n, K, N, V, Lin.ld, Q, ldq, Out, Lout.ld, res_col, r, qlast, beta,
-                                                red, ctl, s);
-    return launch_vq_fma<float, false, false>(n, K, N, V, Lin.ld, Q, ldq, Out, Lout.ld, res_col, r, qlast, beta,
-                                              red, ctl, s);
-}
-
 int launch_start_vector(int n, uint64_t seed, double *r, hipStream_t s)
 {
     const uint64_t base = mix64(seed ^ 0x6A09E667F3BCC909ULL);
@@ -756,13 +712,6 @@ int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn
 {
     scale_copy_kernel<double><<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, r, ctl, v, col.base, col.L.ld, col.j,
                                                                             col.L.tiled ? 1 : 0);
-    HL_TRY_HIP(hipGetLastError());
-    return HL_OK;
-}
-
-int launch_scale_rscale(int n, double *r, double q, const Ctl *ctl, hipStream_t s)
-{
-    scale_rscale_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(n, r, q, ctl);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }
@@ -860,13 +809,6 @@ int launch_seg_readback(const SegMember *M, int nm, int stride, double *out, hip
 int launch_stream_read(const double *a, long long n, double *out, int blocks, hipStream_t s)
 {
     stream_read_kernel<<<blocks, kBlock, 0, s>>>(reinterpret_cast<const dv2 *>(a), n / 2, out);
-    HL_TRY_HIP(hipGetLastError());
-    return HL_OK;
-}
-
-int launch_cast_f32(long long n, const double *in, float *out, hipStream_t s)
-{
-    cast_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(n, in, out);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

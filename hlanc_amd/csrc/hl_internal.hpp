@@ -205,7 +205,6 @@ struct Reduce {
     int max_cols = 0;
 };
 int reduce_alloc(Reduce &r, int max_blocks, int max_cols);
-void reduce_free(Reduce &r);
 int cgs_blocks(int n);
 
 // One fused CGS pass over an FP64 basis (see hl_cgs_pass).  mode selects the
@@ -213,10 +212,6 @@ int cgs_blocks(int n);
 int launch_cgs(int n, int m, const double *V, VLayout L, const double *w_in, double *w_out,
                const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub,
                double *h_dia, int mode, int j, hipStream_t s);
-// Same over an FP32 basis (FP64 accumulation, FP64 w).
-int launch_cgs_f32(int n, int m, const float *V, VLayout L, const double *w_in, double *w_out,
-                   const double *h_in, double *h_out, Reduce &red, Ctl *ctl, double *h_sub,
-                   double *h_dia, int mode, int j, hipStream_t s);
 
 // Out(:, 0:N) = V(:, 0:K) Q(0:K, 0:N); optionally fused residual update on
 // output column res_col: r = qlast * rscale * r + beta * Out(:, res_col) and
@@ -224,23 +219,16 @@ int launch_cgs_f32(int n, int m, const float *V, VLayout L, const double *w_in, 
 int launch_vq(int n, int K, int N, const double *V, VLayout Lin, const double *Q, int ldq,
               double *Out, VLayout Lout, int use_mfma, int res_col, double *r, double qlast,
               double beta, Reduce &red, Ctl *ctl, hipStream_t s);
-int launch_vq_f32(int n, int K, int N, const float *V, VLayout Lin, const double *Q, int ldq,
-                  float *Out, VLayout Lout, int res_col, double *r, double qlast, double beta,
-                  Reduce &red, Ctl *ctl, hipStream_t s);
 
 // r = start vector (counter-based splitmix64, seed) or keep r; then ctl->rnorm.
 int launch_start_vector(int n, uint64_t seed, double *r, hipStream_t s);
 // v = r * ctl->xscale into a contiguous vector v (may be null) and the basis
 // column col (host-operator path normalisation).
 int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn col, hipStream_t s);
-// r *= q * ctl->rscale (dsapps residual when beta_k = 0, HLanc.hpp:1605-1607).
-int launch_scale_rscale(int n, double *r, double q, const Ctl *ctl, hipStream_t s);
 // ||r||^2 -> epilogue (mode EPI_RNORM sets rnorm/xscale/rscale).
 int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStream_t s);
 // HBM read probe: sums n doubles (n even, 16-byte aligned) into out[blocks*4].
 int launch_stream_read(const double *a, long long n, double *out, int blocks, hipStream_t s);
-// out = float(in)
-int launch_cast_f32(long long n, const double *in, float *out, hipStream_t s);
 
 // ------------------------------------------------------------ profiling
 // Profile classes: SpMV; CGS partials kernels of the two passes every step
