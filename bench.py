@@ -18,7 +18,7 @@ default run):
   shift), nev = 8, ncv = 24.  One step solves all 512: once one solve at a time
   (the plain ds_solver_gpu path) and once as one batch (hl_ds_solve_batch); the
   value is the batched eigensolves/s.
-* ``--workload serve``: Poisson arrivals at ``--rate`` requests/s of solves
+* ``--workload serve``: Poisson arrivals at each of ``--rates`` requests/s of solves
   drawn from a pool of matrices with n log-uniform in [4e3, 2e6], nev uniform
   in {4, 8, 16, 32}, ncv = 3 nev; continuous batching under an HBM budget,
   against first-come-first-served one at a time on the same arrival stream;
@@ -68,7 +68,8 @@ def residuals(op, values, vectors, n):
     lam = torch.tensor(values, device="cuda")[:, None]
     r = torch.linalg.norm(AV - lam * V, dim=1).cpu().numpy()
     nrm = float(np.max(np.abs(values)))
-    ortho = float((V @ V.T - torch.eye(len(values), device="cuda", dtype=V.dtype)).abs().max())
+    Vh = np.ascontiguousarray(vectors[:, : len(values)])
+    ortho = float(np.abs(Vh.T @ Vh - np.eye(len(values))).max())  # on the host: a skinny product
     return float(r.max() / nrm), ortho
 
 
@@ -122,6 +123,32 @@ def check_pairs(ops, results, nevs):
     return worst
 
 
+def batch_cpu_baseline(host, sample, results, ns, nnzs, nev, ncv, tol, maxiter, budget_s=20.0):
+    """The CPU oracle's full solves of a sample of the mix (smallest first, until
+    about budget_s seconds are spent), extrapolated to the whole mix with a per-solve
+    cost model: Lanczos steps x (nnz + 3 ncv n / 2), the SpMV and CGS work of a
+    step, using each matrix's step count from the GPU run."""
+    import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    cost = [r.info.spmv * (nnzs[i] + 1.5 * ncv * ns[i]) for i, r in enumerate(results)]
+    t_cpu, c_done, done = 0.0, 0.0, 0
+    for i in sorted(sample, key=lambda k: ns[k]):
+        rp, col, val = host[i]
+        t = time.perf_counter()
+        O.ds_solver(O.Csr(rp, col, val), nev, ncv, tol=tol, maxiter=maxiter, nthreads=threads)
+        t_cpu += time.perf_counter() - t
+        c_done += cost[i]
+        done += 1
+        if t_cpu > budget_s:
+            break
+    total = t_cpu / c_done * sum(cost)
+    return {"value": len(results) / total, "unit": "eigensolves/s", "cores": threads, "kind": "port",
+            "sample": f"oracle full solves of {done} matrices of the mix at evenly spaced size ranks, "
+                      f"smallest first ({t_cpu:.1f} s on {threads} threads), scaled to all "
+                      f"{len(results)} by Lanczos steps x (nnz + 1.5 ncv n)"}
+
+
 def run_batch(args, H, torch):
     from hlanc_amd import workloads as W
 
@@ -130,7 +157,11 @@ def run_batch(args, H, torch):
     mix = W.batch_mix(count, seed=0)
     ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=m[1][0], col=m[1][1], val=m[1][2])) for m in mix]
     ns = [op.nrows for op in ops]
-    nnz = sum(len(m[1][1]) for m in mix)
+    nnzs = [len(m[1][1]) for m in mix]
+    nnz = sum(nnzs)
+    # host copies of 16 matrices at evenly spaced size ranks, for the CPU oracle's sample
+    sample = sorted(range(count), key=lambda i: ns[i])[:: max(1, count // 16)]
+    host = {i: mix[i][1] for i in sample}
     del mix
     log(f"workload batch: {count} matrices, sum n {sum(ns)}, sum nnz {nnz}, built in "
         f"{time.perf_counter() - t0:.1f} s")
@@ -155,10 +186,13 @@ def run_batch(args, H, torch):
         rb = batched()
         walls_b.append(time.perf_counter() - t)
         log(f"batched step {i}: {walls_b[-1]:.3f} s")
-    t = time.perf_counter()
-    rp = plain()
-    walls_p.append(time.perf_counter() - t)
-    log(f"plain one-at-a-time: {walls_p[-1]:.3f} s")
+    if args.no_plain:
+        rp, walls_p = None, [float("nan")]
+    else:
+        t = time.perf_counter()
+        rp = plain()
+        walls_p.append(time.perf_counter() - t)
+        log(f"plain one-at-a-time: {walls_p[-1]:.3f} s")
     classes = None
     if args.profile:
         import ctypes
@@ -171,7 +205,7 @@ def run_batch(args, H, torch):
                    for k in range(len(H.PROFILE_CLASSES)) if pr.launches[k]}
         log("profiled batch: " + ", ".join(f"{k} {v['launches']}x {v['seconds']:.3f}s" for k, v in classes.items()))
     fails = sum(r.status != 0 for r in rb)
-    same = all(np.array_equal(a.values, b.values) and a.status == b.ret for a, b in zip(rb, rp))
+    same = rp is not None and all(np.array_equal(a.values, b.values) and a.status == b.ret for a, b in zip(rb, rp))
     worst = check_pairs(ops, rb, [nev] * count)
     lat = [r.t_done for r in rb]
     value = count / float(np.mean(walls_b))
@@ -179,6 +213,10 @@ def run_batch(args, H, torch):
     log(f"batched {value:.1f} solves/s vs plain {plain_value:.1f} solves/s ({value / plain_value:.2f}x); "
         f"failures {fails}; bitwise same as plain: {same}; max rel residual {worst:.2e}")
     restarts = [r.info.restarts for r in rb]
+    cpu = None
+    if not args.no_cpu:
+        cpu = batch_cpu_baseline(host, sample, rb, ns, nnzs, nev, ncv, args.tol, args.maxiter)
+        log(f"cpu baseline: {cpu}")
     return {
         "metric": "eigensolves/s at rel.resid<=1e-10 (+ p50/p99 latency)",
         "value": value, "unit": "eigensolves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -192,6 +230,7 @@ def run_batch(args, H, torch):
         "bitwise_same_as_plain": bool(same), "failures": int(fails), "max_rel_resid": worst,
         "restarts": {"min": int(min(restarts)), "median": float(np.median(restarts)), "max": int(max(restarts))},
         "kernel_classes": classes,
+        "cpu_baseline": cpu,
     }
 
 
@@ -208,42 +247,53 @@ def run_serve(args, H, torch):
     torch.cuda.init()
     rng = np.random.default_rng(args.seed)
     R = args.requests
-    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, R))
+    gaps = rng.exponential(1.0, R)  # one request sequence, scaled to each offered rate
     pick = rng.integers(0, len(ops), R)
     nevs = rng.choice([4, 8, 16, 32], R)
     req_ops = [ops[k] for k in pick]
-    kw = dict(tol=args.tol, maxiter=args.maxiter, arrivals=arrivals, vectors=True)
     H.ds_solve_batch(req_ops[:2], list(nevs[:2]), list(3 * nevs[:2]), tol=args.tol, maxiter=args.maxiter)
 
-    def serve(max_active, budget):
+    def serve(arrivals, max_active, budget):
         t = time.perf_counter()
-        res = H.ds_solve_batch(req_ops, list(nevs), list(3 * nevs), max_active=max_active, hbm_budget=budget, **kw)
+        res = H.ds_solve_batch(req_ops, list(nevs), list(3 * nevs), max_active=max_active, hbm_budget=budget,
+                               tol=args.tol, maxiter=args.maxiter, arrivals=arrivals, vectors=True)
         return res, time.perf_counter() - t
 
-    rs, wall_s = serve(args.max_active, args.hbm_budget)
-    worst = check_pairs(req_ops, rs, list(nevs))
-    for r in rs:
-        r.vectors = None  # host memory: the stream's vectors are checked, then dropped
-    rf, wall_f = serve(1, 0.0)  # first come, first served, one at a time
-    lat_s = [r.latency for r in rs]
-    lat_f = [r.latency for r in rf]
-    same = all(np.array_equal(a.values, b.values) and a.status == b.status for a, b in zip(rs, rf))
-    fails = sum(r.status != 0 for r in rs)
-    value = R / wall_s
-    log(f"served {value:.2f} solves/s p50 {percentile(lat_s, 50):.3f} s p99 {percentile(lat_s, 99):.3f} s; "
-        f"FCFS {R / wall_f:.2f} solves/s p50 {percentile(lat_f, 50):.3f} s p99 {percentile(lat_f, 99):.3f} s; "
-        f"same results: {same}; failures {fails}; max rel residual {worst:.2e}")
+    rates = [float(x) for x in str(args.rates).split(",")]
+    per_rate, worst, same, fails = [], 0.0, True, 0
+    for rate in rates:
+        arrivals = np.cumsum(gaps) / rate
+        rs, wall_s = serve(arrivals, args.max_active, args.hbm_budget)
+        worst = max(worst, check_pairs(req_ops, rs, list(nevs)))
+        for r in rs:
+            r.vectors = None  # host memory: the stream's vectors are checked, then dropped
+        rf, wall_f = serve(arrivals, 1, 0.0)  # first come, first served, one at a time
+        for r in rf:
+            r.vectors = None
+        lat_s = [r.latency for r in rs]
+        lat_f = [r.latency for r in rf]
+        same &= all(np.array_equal(a.values, b.values) and a.status == b.status for a, b in zip(rs, rf))
+        fails += sum(r.status != 0 for r in rs)
+        rec = {"offered_req_per_s": rate, "value": R / wall_s,
+               "latency_s": {"p50": percentile(lat_s, 50), "p99": percentile(lat_s, 99)},
+               "fcfs": {"value": R / wall_f, "latency_s": {"p50": percentile(lat_f, 50),
+                                                          "p99": percentile(lat_f, 99)}}}
+        per_rate.append(rec)
+        log(f"rate {rate}: served {rec['value']:.2f} solves/s p50 {rec['latency_s']['p50']:.3f} s p99 "
+            f"{rec['latency_s']['p99']:.3f} s; FCFS {rec['fcfs']['value']:.2f} solves/s p50 "
+            f"{rec['fcfs']['latency_s']['p50']:.3f} s p99 {rec['fcfs']['latency_s']['p99']:.3f} s")
+    log(f"same results served and FCFS: {same}; failures {fails}; max rel residual {worst:.2e}")
+    top = per_rate[-1]
     return {
         "metric": "eigensolves/s at rel.resid<=1e-10 (+ p50/p99 latency)",
-        "value": value, "unit": "eigensolves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": 1e3 * wall_s, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (seeded generator)",
-        "config": {"workload": f"Poisson stream, {R} requests at {args.rate} req/s offered, pool of {len(ops)} "
-                               "matrices n log-uniform [4e3, 2e6], nev in {4,8,16,32}, ncv = 3 nev",
-                   "tol": args.tol, "max_active": args.max_active, "hbm_budget_bytes": args.hbm_budget,
-                   "basis": "fp64"},
-        "latency_s": {"p50": percentile(lat_s, 50), "p99": percentile(lat_s, 99)},
-        "fcfs": {"value": R / wall_f, "latency_s": {"p50": percentile(lat_f, 50), "p99": percentile(lat_f, 99)}},
+        "value": top["value"], "unit": "eigensolves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": 1e3 * R / top["value"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
+        "config": {"workload": f"Poisson stream, {R} requests, pool of {len(ops)} matrices n log-uniform "
+                               "[4e3, 2e6], nev in {4,8,16,32}, ncv = 3 nev",
+                   "offered_req_per_s": rates, "tol": args.tol, "max_active": args.max_active,
+                   "hbm_budget_bytes": args.hbm_budget, "basis": "fp64"},
+        "latency_s": top["latency_s"], "fcfs": top["fcfs"], "rates": per_rate,
         "same_results_as_fcfs": bool(same), "failures": int(fails), "max_rel_resid": worst,
     }
 
@@ -257,12 +307,13 @@ def main():
     ap.add_argument("--count", type=int, default=512, help="batch: number of matrices")
     ap.add_argument("--max-active", type=int, default=0, help="batch/serve: solves in flight (0: no cap)")
     ap.add_argument("--hbm-budget", type=float, default=64e9, help="serve: workspace budget in bytes")
-    ap.add_argument("--rate", type=float, default=200.0, help="serve: offered requests/s")
+    ap.add_argument("--rates", default="2,200", help="serve: offered requests/s, comma-separated")
     ap.add_argument("--requests", type=int, default=100, help="serve: requests in the stream")
     ap.add_argument("--pool", type=int, default=32, help="serve: distinct matrices")
     ap.add_argument("--seed", type=int, default=0, help="serve: arrival stream seed")
     ap.add_argument("--profile", action="store_true", help="batch: one more profiled batched pass")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV work-item shape")
+    ap.add_argument("--no-plain", action="store_true", help="batch: skip the one-at-a-time comparison")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")

@@ -113,9 +113,15 @@ def test_laplacian_256_config0():
     rp, col, val = W.laplacian_2d(256)
     assert len(val) == 326656
     A = sp.csr_matrix((val, col, rp))
-    r = H.ds_solver_gpu(65536, 10, 30, op_for(rp, col, val), tol=1e-10, maxiter=1000)
+    op = op_for(rp, col, val)
+    r = H.ds_solver_gpu(65536, 10, 30, op, tol=1e-10, maxiter=1000)
     assert r.ret == 0, r.info.restarts
     check_pairs(A, r.values, r.vectors, W.laplacian_2d_eigs(256)[-10:], 8.0, 1e-10)
+    # the drop-in defaults (tol = eps, 200 restarts) run out of restarts here:
+    # return 1 with the outputs untouched, after exactly 200 restarts
+    d = H.ds_solver_gpu(65536, 10, 30, op)
+    assert d.ret == 1 and d.info.restarts == 200
+    assert not d.values.any()
 
 
 def test_random_spd_dense_and_oracle():
