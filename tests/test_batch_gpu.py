@@ -175,3 +175,16 @@ def test_batch_with_and_without_graphs(mixed):
     assert_same(a, alone)
     assert_same(b, alone)
     assert all(r.info.graph_sweeps == 0 for r in b)
+
+
+def test_many_similar_solves():
+    # 24 solves of similar size in every launch; each result must equal its
+    # standalone solve
+    mats = [W.erdos_renyi(3000 + 37 * i, 12 + i % 5, seed=200 + i) for i in range(24)]
+    ops = [op_for(*m) for m in mats]
+    res = H.ds_solve_batch(ops, 6, 18, tol=1e-10, maxiter=2000, seeds=list(range(24)))
+    for i, (op, r) in enumerate(zip(ops, res)):
+        a = H.ds_solver_gpu(op.nrows, 6, 18, op, tol=1e-10, maxiter=2000, seed=i)
+        assert r.status == a.ret == 0
+        np.testing.assert_array_equal(r.values, a.values)
+        np.testing.assert_array_equal(r.vectors, a.vectors)
