@@ -168,13 +168,15 @@ def run_batch(args, H, torch):
     torch.cuda.init()
     kw = dict(tol=args.tol, maxiter=args.maxiter)
 
+    vec_out = [H.pinned_empty(op.nrows, ncv) for op in ops]  # page-locked outputs, allocated once
+
     def batched():
-        return H.ds_solve_batch(ops, nev, ncv, max_active=args.max_active, **kw)
+        return H.ds_solve_batch(ops, nev, ncv, max_active=args.max_active, vectors_out=vec_out, **kw)
 
     def plain():
         out = []
-        for op in ops:
-            out.append(H.ds_solver_gpu(op.nrows, nev, ncv, op, **kw))
+        for op, v in zip(ops, vec_out):
+            out.append(H.ds_solver_gpu(op.nrows, nev, ncv, op, vectors_out=v, **kw))
         return out
 
     for i in range(args.warmup):
@@ -358,8 +360,9 @@ def main():
         items = {"nnz_cap": nc, "rows_cap": rc_}
         log(f"SpMV autotune: items of <= {nc} nnz / <= {rc_} rows, {t_best * 1e6:.1f} us per launch")
 
+    vec_out = H.pinned_empty(n, ncv)  # page-locked output: the final copy runs at link rate
     for i in range(args.warmup):
-        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter)
+        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_out)
         log(f"warmup {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} {r.info.seconds:.3f} s")
 
     lat = []
@@ -367,7 +370,7 @@ def main():
     t_start = time.perf_counter()
     for i in range(args.steps):
         t = time.perf_counter()
-        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter)
+        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_out)
         lat.append(time.perf_counter() - t)
         log(f"solve {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} {lat[-1]:.3f} s")
         if r.ret != 0:
@@ -376,7 +379,7 @@ def main():
     wall = time.perf_counter() - t_start
     value = args.steps / wall
 
-    resid, ortho = residuals(op, r.values, r.vectors, n)
+    resid, ortho = residuals(op, r.values, np.array(r.vectors), n)
     log(f"max rel residual {resid:.2e}, orthogonality {ortho:.2e}")
 
     # per-kernel-class profile of one more solve (HIP events around each launch)
@@ -385,9 +388,8 @@ def main():
     o = H.options(args.tol, args.maxiter)
     o.flags = H.HL_FLAG_PROFILE
     values = np.zeros(ncv)
-    vecs = np.zeros((ncv, n)).T
     info = H.Info()
-    rc = H.lib().hl_ds_solve_csr(op.h, nev, ncv, values.ctypes.data_as(H._dp), vecs.ctypes.data_as(H._dp), n,
+    rc = H.lib().hl_ds_solve_csr(op.h, nev, ncv, values.ctypes.data_as(H._dp), vec_out.ctypes.data_as(H._dp), n,
                                  ctypes.byref(o), ctypes.byref(info))
     pr = H.Profile()
     H.lib().hl_profile_last(ctypes.byref(pr))

@@ -81,6 +81,8 @@ _SIGS = [
     ("hl_last_error", ctypes.c_char_p, []),
     ("hl_version", ctypes.c_char_p, []),
     ("hl_device_count", ctypes.c_int, [_ip]),
+    ("hl_host_alloc", ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    ("hl_host_free", ctypes.c_int, [ctypes.c_void_p]),
     ("hl_matrix_read", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_Matrix)]),
     ("hl_matrix_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_Matrix)]),
     ("hl_matrix_free", None, [ctypes.POINTER(_Matrix)]),
@@ -307,6 +309,31 @@ class dcsrmv_operator_mgpu(dcsrmv_operator_gpu):
 
 # ------------------------------------------------------------------- solver
 
+class _PinnedBuffer:
+    """Page-locked host memory from hl_host_alloc, freed with the last array
+    that views it."""
+
+    def __init__(self, count: int):
+        p = ctypes.c_void_p()
+        _check(lib().hl_host_alloc(8 * max(count, 1), ctypes.byref(p)))
+        self.ptr = p.value
+        self.__array_interface__ = {"shape": (max(count, 1),), "typestr": "<f8", "data": (self.ptr, False),
+                                    "version": 3}
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.hl_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(rows: int, cols: int) -> np.ndarray:
+    """A column-major float64 rows x cols array in page-locked host memory, for
+    the ``vectors`` output: the final copy of the basis then runs at full link
+    rate."""
+    buf = np.asarray(_PinnedBuffer(rows * cols))
+    return buf[: rows * cols].reshape(cols, rows).T
+
+
 @dataclass
 class SolveResult:
     ret: int
@@ -332,17 +359,28 @@ def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mf
     return o
 
 
+def _vectors_buffer(out, ldv: int, ncv: int) -> np.ndarray:
+    if out is None:
+        return np.zeros((ncv, ldv)).T  # column-major ldv x ncv
+    if out.dtype != np.float64 or out.shape[0] < ldv or out.shape[1] < ncv or not out.flags.f_contiguous:
+        raise HLancError(-1, f"vectors_out must be a column-major float64 array of at least {ldv} x {ncv}")
+    return out
+
+
 def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, v0=None,
                   seed: int = 0, use_mfma: Optional[bool] = None, ldv: Optional[int] = None,
-                  flags: int = 0) -> SolveResult:
+                  flags: int = 0, vectors_out: Optional[np.ndarray] = None) -> SolveResult:
     """ds_solver_gpu (HLanc.hpp:1868-1872).  ``op`` is a dcsrmv_operator_gpu
     (device-resident path) or any callable ``op(x, y)`` on numpy arrays (host
     path, as the reference calls arbitrary operators with host pointers).
     ``ret`` is 0 on success and 1 on the reference's numerical failures; values
-    and vectors are written only on success."""
+    and vectors are written only on success.  ``vectors_out`` (column-major
+    float64, at least ldv x ncv, e.g. from pinned_empty) receives the vectors
+    instead of a fresh array."""
     ldv = n if ldv is None else ldv
     values = np.zeros(ncv)
-    vectors = np.zeros((ncv, ldv)).T  # column-major ldv x ncv
+    vectors = _vectors_buffer(vectors_out, ldv, ncv)
+    ldv = vectors.shape[0] if vectors_out is not None else ldv
     o = options(tol, maxiter, v0, seed, use_mfma, flags=flags)
     info = Info()
     if isinstance(op, dcsrmv_operator_gpu):
@@ -381,14 +419,15 @@ class BatchResult:
 
 def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, seeds=None,
                    v0s=None, use_mfma: Optional[bool] = None, vectors: bool = True, arrivals=None,
-                   max_active: int = 0, hbm_budget: float = 0.0, flags: int = 0):
+                   max_active: int = 0, hbm_budget: float = 0.0, flags: int = 0, vectors_out=None):
     """Many solves on one device with continuous batching (hl_ds_solve_batch).
 
     ``ops`` are dcsrmv_operator_gpu on ``dev``; nev, ncv, tol and maxiter are
     one value for all solves or one per solve.  ``arrivals`` (seconds after the
     call, non-decreasing) delays admission, as a serving loop sees requests
     arrive.  Each result is bitwise what ds_solver_gpu returns for the same
-    operator and options."""
+    operator and options.  ``vectors_out`` optionally lists one vectors buffer
+    per solve (as ds_solver_gpu's)."""
     k = len(ops)
     per = lambda x, i: x[i] if isinstance(x, (list, tuple, np.ndarray)) else x  # noqa: E731
     items = (SolveItem * k)()
@@ -399,7 +438,9 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
         n = op.nrows
         nev_i, ncv_i = int(per(nev, i)), int(per(ncv, i))
         vals = np.zeros(max(ncv_i, 1))
-        vecs = np.zeros((max(ncv_i, 1), n)).T if vectors else None
+        vecs = None
+        if vectors:
+            vecs = _vectors_buffer(None if vectors_out is None else vectors_out[i], n, max(ncv_i, 1))
         seed = 0 if seeds is None else int(seeds[i])
         v0 = None if v0s is None else v0s[i]
         o = options(float(per(tol, i)), int(per(maxiter, i)), v0, seed, use_mfma, flags=flags)
@@ -409,7 +450,7 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
         it.ncv = ncv_i
         it.values = _d(vals)
         it.vectors = vecs.ctypes.data_as(_dp) if vecs is not None else None
-        it.ldv = n
+        it.ldv = vecs.shape[0] if vecs is not None else n
         it.opts = o
         it.arrival = 0.0 if arrivals is None else float(arrivals[i])
         keep.append((vals, vecs, o))
@@ -420,7 +461,8 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
         it = items[i]
         vals, vecs, _ = keep[i]
         info = Info.from_buffer_copy(it.info)
-        out.append(BatchResult(it.status, vals[:it.nev].copy(), None if vecs is None else np.asarray(vecs),
+        out.append(BatchResult(it.status, vals[:it.nev].copy(),
+                               None if vecs is None else np.asarray(vecs[: ops[i].nrows]),
                                info, it.t_admit, it.t_done, it.arrival))
     return out
 

@@ -125,6 +125,20 @@ const char *hl_last_error(void) { return g_err; }
 
 const char *hl_version(void) { return "1.0"; }
 
+int hl_host_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr) return HL_ERR_ARG;
+    *ptr = nullptr;
+    HL_TRY_HIP(hipHostMalloc(ptr, bytes > 0 ? bytes : 1, hipHostMallocDefault));
+    return HL_OK;
+}
+
+int hl_host_free(void *ptr)
+{
+    if (ptr) HL_TRY_HIP(hipHostFree(ptr));
+    return HL_OK;
+}
+
 int hl_device_count(int *count)
 {
     int c = 0;

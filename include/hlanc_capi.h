@@ -50,6 +50,12 @@ const char *hl_last_error(void);
 const char *hl_version(void);
 /* Number of visible HIP devices (0 when none). */
 int hl_device_count(int *count);
+/* Page-locked host memory, e.g. for the `vectors` output of the solvers: the
+ * final device-to-host copy of the basis then runs at full link rate instead of
+ * through the runtime's staging buffers.  No counterpart in the reference
+ * (its outputs are caller-allocated host memory, as these may also be). */
+int hl_host_alloc(size_t bytes, void **ptr);
+int hl_host_free(void *ptr);
 
 /* ---------------------------------------------------------------- matrices
  * Host CSR in dcsr_matrix_t's layout (HLanc.hpp:74-100): zero-based int32

@@ -255,3 +255,18 @@ def test_graph_replay_matches_launch_by_launch():
     np.testing.assert_array_equal(a.values, b.values)
     np.testing.assert_array_equal(a.vectors, b.vectors)
     np.testing.assert_allclose(a.values, W.laplacian_2d_eigs(48)[-8:], rtol=1e-10)
+
+
+def test_pinned_vectors_output_is_the_same_solve():
+    rp, col, val = W.stencil27(11, seed=21)
+    n = len(rp) - 1
+    op = op_for(rp, col, val)
+    a = H.ds_solver_gpu(n, 6, 18, op, tol=1e-10, maxiter=1000)
+    buf = H.pinned_empty(n + 5, 20)  # larger than needed: ldv = n + 5
+    b = H.ds_solver_gpu(n, 6, 18, op, tol=1e-10, maxiter=1000, vectors_out=buf)
+    assert a.ret == b.ret == 0
+    np.testing.assert_array_equal(a.values, b.values)
+    np.testing.assert_array_equal(a.vectors[:, :18], b.vectors[:, :18])
+    assert np.shares_memory(b.vectors, buf)
+    with pytest.raises(H.HLancError):
+        H.ds_solver_gpu(n, 6, 18, op, vectors_out=np.zeros((n, 18)))  # row-major: refused
