@@ -353,17 +353,20 @@ void Engine::drop_graphs()
 void Engine::release()
 {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (Pending &p : pend) (void)hipEventDestroy(p.ev);
+    pend.clear();
+    bury();
     drop_graphs();
     void *d[] = {dM, dS, dV, dRB, dQ};
     for (void *p : d)
         if (p) (void)hipFree(p);
-    void *h[] = {hM, hS, hV, hRB, hQ};
+    void *h[] = {hMs[0], hMs[1], hSs[0], hSs[1], hVs[0], hVs[1], hRB, hQs[0], hQs[1]};
     for (void *p : h)
         if (p) (void)hipHostFree(p);
-    dM = hM = nullptr;
-    dS = hS = nullptr;
-    dV = hV = nullptr;
-    dRB = hRB = dQ = hQ = nullptr;
+    dM = hM = hMs[0] = hMs[1] = nullptr;
+    dS = hS = hSs[0] = hSs[1] = nullptr;
+    dV = hV = hVs[0] = hVs[1] = nullptr;
+    dRB = hRB = dQ = hQ = hQs[0] = hQs[1] = nullptr;
     capM = capS = capV = capRB = capQ = 0;
     prof.release();
     if (stream) (void)hipStreamDestroy(stream);
@@ -373,31 +376,75 @@ void Engine::release()
 }
 
 namespace {
+// A device table and its `nh` pinned staging copies, grown to `need` entries.
 template <typename T>
-int grow_pair(T *&d, T *&h, size_t &cap, size_t need)
+int grow_set(T *&d, T **h, int nh, size_t &cap, size_t need)
 {
     if (need <= cap) return HL_OK;
     if (d) (void)hipFree(d);
-    if (h) (void)hipHostFree(h);
     d = nullptr;
-    h = nullptr;
+    for (int i = 0; i < nh; ++i) {
+        if (h[i]) (void)hipHostFree(h[i]);
+        h[i] = nullptr;
+    }
     cap = std::max(need, 2 * cap);
     HL_TRY_HIP(hipMalloc(&d, sizeof(T) * cap));
-    HL_TRY_HIP(hipHostMalloc(&h, sizeof(T) * cap, hipHostMallocDefault));
+    for (int i = 0; i < nh; ++i) HL_TRY_HIP(hipHostMalloc(&h[i], sizeof(T) * cap, hipHostMallocDefault));
     return HL_OK;
 }
 } // namespace
 
 int Engine::grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd)
 {
-    HL_TRY_HIP(hipStreamSynchronize(stream)); // the pinned tables are rewritten every sweep
-    if (nm > capM || ns > capS) drop_graphs(); // captured launches name the old tables
+    if (nm > capM || ns > capS || nv > capV || rb > capRB || qd > capQ) {
+        HL_TRY_HIP(hipStreamSynchronize(stream)); // uploads from the old buffers may be in flight
+        if (nm > capM || ns > capS) drop_graphs(); // captured launches name the old tables
+    }
     int rc;
-    if ((rc = grow_pair(dM, hM, capM, nm))) return rc;
-    if ((rc = grow_pair(dS, hS, capS, ns))) return rc;
-    if ((rc = grow_pair(dV, hV, capV, nv))) return rc;
-    if ((rc = grow_pair(dRB, hRB, capRB, rb))) return rc;
-    return grow_pair(dQ, hQ, capQ, qd);
+    if ((rc = grow_set(dM, hMs, 2, capM, nm))) return rc;
+    if ((rc = grow_set(dS, hSs, 2, capS, ns))) return rc;
+    if ((rc = grow_set(dV, hVs, 2, capV, nv))) return rc;
+    if ((rc = grow_set(dRB, &hRB, 1, capRB, rb))) return rc;
+    if ((rc = grow_set(dQ, hQs, 2, capQ, qd))) return rc;
+    // the other staging set: its last uploads were ordered before the previous
+    // sweep's read-back, which the host has waited for
+    par ^= 1;
+    hM = hMs[par];
+    hS = hSs[par];
+    hV = hVs[par];
+    hQ = hQs[par];
+    return HL_OK;
+}
+
+void Engine::bury()
+{
+    for (auto &p : graveyard) {
+        if (p.first) (void)hipFree(p.first);
+        if (p.second) (void)hipHostFree(p.second);
+    }
+    graveyard.clear();
+}
+
+int Engine::poll_finished(bool wait)
+{
+    size_t keep = 0;
+    for (size_t k = 0; k < pend.size(); ++k) {
+        Pending &p = pend[k];
+        hipError_t e = wait ? hipEventSynchronize(p.ev) : hipEventQuery(p.ev);
+        if (e == hipErrorNotReady) {
+            pend[keep++] = p;
+            continue;
+        }
+        (void)hipEventDestroy(p.ev);
+        if (e != hipSuccess) {
+            for (size_t r = k + 1; r < pend.size(); ++r) pend[keep++] = pend[r];
+            pend.resize(keep);
+            return hip_status(e, "Ritz-vector copy");
+        }
+        for (Member *m : p.ms) m->complete();
+    }
+    pend.resize(keep);
+    return HL_OK;
 }
 
 int Engine::sweep(std::vector<Member *> &ms)
@@ -568,6 +615,7 @@ int Engine::sweep(std::vector<Member *> &ms)
     HL_TRY_HIP(hipMemcpyAsync(hRB, dRB, sizeof(double) * nm * stride, hipMemcpyDeviceToHost, stream));
     HL_TRY_HIP(hipStreamSynchronize(stream));
     const double th1 = now_s();
+    bury(); // the device is idle: frees here cost no stall
     if (prof.on && (rc = prof.flush())) return rc;
     for (size_t i = 0; i < nm; ++i) {
         act[i]->absorb(hRB + i * stride);
@@ -631,9 +679,16 @@ int Engine::after_sweep(std::vector<Member *> &act)
             if (m->act == Member::ACT_RESTART) m->ws.cur = 1 - m->ws.cur;
     }
     if (nfinish == 0) return HL_OK;
+    Pending fin{nullptr, {}};
     for (size_t i = 0; i < nm; ++i) {
         Member &m = *act[i];
-        if (m.act != Member::ACT_FINISH || !m.with_vectors) continue;
+        if (m.act != Member::ACT_FINISH) continue;
+        if (!m.with_vectors) { // nothing on the device: the outputs are ready
+            m.complete();
+            continue;
+        }
+        m.phase = Member::FINISHING;
+        fin.ms.push_back(&m);
         Workspace &W = m.ws;
         const int nxt = 1 - W.cur;
         if ((rc = prof.begin(stream))) return rc;
@@ -646,9 +701,11 @@ int Engine::after_sweep(std::vector<Member *> &act)
                                     sizeof(double) * (size_t)W.ldv, sizeof(double) * (size_t)m.n, m.ncv,
                                     hipMemcpyDeviceToHost, stream));
     }
-    HL_TRY_HIP(hipStreamSynchronize(stream));
-    for (Member *m : act)
-        if (m->act == Member::ACT_FINISH) m->complete();
+    if (fin.ms.empty()) return HL_OK;
+    // completed when this event has passed: the next sweep is enqueued meanwhile
+    HL_TRY_HIP(hipEventCreateWithFlags(&fin.ev, hipEventDisableTiming));
+    HL_TRY_HIP(hipEventRecord(fin.ev, stream));
+    pend.push_back(std::move(fin));
     return HL_OK;
 }
 
@@ -662,7 +719,29 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
     auto need = [](const Member *m) {
         return Workspace::bytes_needed(m->n, m->ncv, m->A == nullptr, m->A ? m->A->nslots : 0);
     };
+    // members done since the last call leave the active set; their workspaces
+    // are freed at the next point where the device is idle
+    auto retire = [&]() {
+        size_t keep = 0;
+        for (Member *m : active) {
+            if (m->phase == Member::DONE) {
+                m->t_done = now_s() - t_start;
+                graveyard.push_back(m->ws.detach());
+                in_use -= need(m);
+            } else {
+                active[keep++] = m;
+            }
+        }
+        active.resize(keep);
+    };
+    auto sweeping = [&]() {
+        for (Member *m : active)
+            if (m->phase != Member::FINISHING) return true;
+        return false;
+    };
     while (next < queue.size() || !active.empty()) {
+        if ((rc = poll_finished(false))) break; // Ritz vectors that have arrived
+        retire();
         // admission at the sweep boundary, in queue order
         double t = now_s() - t_start;
         if (active.empty() && next < queue.size() && queue[next]->arrival > t) {
@@ -684,33 +763,32 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
             in_use += need(m);
             active.push_back(m);
         }
-        if (active.empty()) continue;
-        if ((rc = sweep(active))) break;
-        // retire finished members and free their workspaces
-        size_t keep = 0;
-        for (Member *m : active) {
-            if (m->phase == Member::DONE) {
-                m->t_done = now_s() - t_start;
-                m->ws.release();
-                in_use -= need(m);
-            } else {
-                active[keep++] = m;
-            }
+        if (sweeping()) {
+            if ((rc = sweep(active))) break;
+            retire();
+        } else if (!pend.empty()) { // only finishing members are left: wait for their vectors
+            if ((rc = poll_finished(true))) break;
+            retire();
         }
-        active.resize(keep);
     }
+    if (!rc) rc = poll_finished(true);
+    if (!rc) retire();
     if (!rc && prof.on) { // the V.Q launches after the last sweep
         HL_TRY_HIP(hipStreamSynchronize(stream));
         rc = prof.flush();
     }
     if (rc) { // a HIP failure ends every member in flight
         (void)hipStreamSynchronize(stream);
+        for (Pending &p : pend) (void)hipEventDestroy(p.ev);
+        pend.clear();
         for (Member *m : active) {
             m->err = rc;
             m->ws.release();
             m->phase = Member::DONE;
         }
     }
+    HL_TRY_HIP(hipStreamSynchronize(stream));
+    bury();
     return rc;
 }
 

@@ -199,6 +199,14 @@ hl_profile &thread_profile()
 
 Workspace::~Workspace() { release(); }
 
+std::pair<void *, void *> Workspace::detach()
+{
+    std::pair<void *, void *> p{blob, hblob};
+    blob = hblob = nullptr;
+    release();
+    return p;
+}
+
 void Workspace::release()
 {
     if (blob) (void)hipFree(blob);

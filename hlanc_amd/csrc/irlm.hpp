@@ -1,6 +1,7 @@
 // irlm.hpp -- solve state and the multi-solve engine of libhlanc_hip.so.
 #pragma once
 
+#include <utility>
 #include <vector>
 
 #include "hl_internal.hpp"
@@ -30,6 +31,9 @@ struct Workspace {
     VLayout vl() const { return VLayout{ncv, true}; }
     ~Workspace();
     int init(int dev, int n, int ncv, bool host_op, int nslots, hipStream_t s); // zeroing on s
+    // Hands the two allocations to the caller (to free once the device is idle)
+    // and clears the workspace.
+    std::pair<void *, void *> detach();
     static double bytes_needed(int n, int ncv, bool host_op, int nslots); // device bytes of init
     void release();
 };
@@ -59,7 +63,7 @@ struct Member {
     // device state
     Workspace ws;
     // host state
-    enum Phase { NEW, ACTIVE, DONE } phase = NEW;
+    enum Phase { NEW, ACTIVE, FINISHING, DONE } phase = NEW; // FINISHING: Ritz vectors in flight
     enum Act { ACT_NONE, ACT_SWEEP, ACT_RESTART, ACT_FINISH } act = ACT_NONE;
     int stage = 0;                       // 0: initial sweep pending, 1: main loop
     int sweep_k = 0, sweep_np = 0;
@@ -98,14 +102,25 @@ struct Engine {
     int dev = 0;
     hipStream_t stream = nullptr;
     Profiler prof;
-    // launch tables (device) and their pinned staging
-    SegMember *dM = nullptr, *hM = nullptr;
-    SegStep *dS = nullptr, *hS = nullptr;
-    SegVQ *dV = nullptr, *hV = nullptr;
+    // launch tables (device) and their pinned staging, two sets used in turn
+    // so that a sweep's uploads may still be in flight while the next sweep's
+    // tables are written
+    SegMember *dM = nullptr, *hM = nullptr, *hMs[2] = {nullptr, nullptr};
+    SegStep *dS = nullptr, *hS = nullptr, *hSs[2] = {nullptr, nullptr};
+    SegVQ *dV = nullptr, *hV = nullptr, *hVs[2] = {nullptr, nullptr};
     size_t capM = 0, capS = 0, capV = 0;
-    // read-back and Q arenas (device + pinned)
-    double *dRB = nullptr, *hRB = nullptr, *dQ = nullptr, *hQ = nullptr;
+    // read-back (device + pinned) and Q arenas (device + two pinned)
+    double *dRB = nullptr, *hRB = nullptr, *dQ = nullptr, *hQ = nullptr, *hQs[2] = {nullptr, nullptr};
     size_t capRB = 0, capQ = 0;
+    int par = 0;
+    // finishing members whose Ritz vectors are in flight, completed once
+    // their event has passed; workspaces freed at the next idle point
+    struct Pending {
+        hipEvent_t ev;
+        std::vector<Member *> ms;
+    };
+    std::vector<Pending> pend;
+    std::vector<std::pair<void *, void *>> graveyard;
     struct Pool;
     Pool *pool = nullptr;                // host threads for decide()
     // Sweep launch sequences replayed as HIP graphs: a sweep's launches depend
@@ -141,6 +156,8 @@ struct Engine {
     int grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd);
     int after_sweep(std::vector<Member *> &act);
     void drop_graphs();
+    int poll_finished(bool wait);         // complete members whose vectors have arrived
+    void bury();                          // free detached workspaces (device idle)
 };
 
 }  // namespace hl
