@@ -106,6 +106,8 @@ _SIGS = [
     ("hl_tune_get", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
     ("hl_bench_cgs", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_double)]),
+    ("hl_bench_gather", ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int,
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     ("hl_bench_stream", ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_double)]),
     ("hl_options_default", None, [ctypes.POINTER(Options)]),
@@ -502,4 +504,11 @@ def bench_stream(nbytes: int = 4 << 30, reps: int = 10, dev: int = 0) -> float:
     """Seconds per read-only streaming pass over nbytes of HBM."""
     t = ctypes.c_double()
     _check(lib().hl_bench_stream(dev, int(nbytes), reps, ctypes.byref(t)))
+    return t.value
+
+
+def bench_gather(nx: int, m: int, random: bool = True, reps: int = 10, dev: int = 0) -> float:
+    """Seconds per pass of m gathers from a vector of nx doubles (hl_bench_gather)."""
+    t = ctypes.c_double()
+    _check(lib().hl_bench_gather(dev, int(nx), int(m), int(random), reps, ctypes.byref(t)))
     return t.value

@@ -269,6 +269,30 @@ __global__ __launch_bounds__(kBlock) void stream_read_kernel(const dv2 *__restri
     if ((threadIdx.x & (kWave - 1)) == 0) out[blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave] = s;
 }
 
+// Gather probe: sums x[idx[i]] over m indices, eight index loads in flight
+// per thread and then their eight gathers (the SpMV's access pattern without
+// the matrix values).
+__global__ __launch_bounds__(kBlock) void gather_kernel(const double *__restrict__ x, const int *__restrict__ idx,
+                                                        long long m, double *__restrict__ out)
+{
+    constexpr int kPer = 8;
+    double s = 0.0;
+    const long long stride = (long long)gridDim.x * kBlock * kPer;
+    for (long long base = (long long)blockIdx.x * kBlock * kPer + threadIdx.x; base < m; base += stride) {
+        int ci[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const long long i = base + (long long)k * kBlock;
+            ci[k] = i < m ? __builtin_nontemporal_load(&idx[i]) : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (ci[k] >= 0) s += x[ci[k]];
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) out[blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave] = s;
+}
+
 // Out(:, c0:c0+CB) = V(:, 0:K) Q(0:K, c0:c0+CB); thread per row, FP64 FMA.
 template <int CB, typename VT, bool TIN, bool TOUT>
 __global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, const VT *__restrict__ V,
@@ -802,6 +826,13 @@ int launch_seg_readback(const SegMember *M, int nm, int stride, double *out, hip
 {
     if (nm <= 0) return HL_OK;
     seg_readback_kernel<<<nm, kBlock, 0, s>>>(M, stride, out);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_gather(const double *x, const int *idx, long long m, double *out, int blocks, hipStream_t s)
+{
+    gather_kernel<<<blocks, kBlock, 0, s>>>(x, idx, m, out);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

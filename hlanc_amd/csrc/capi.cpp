@@ -4,6 +4,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <vector>
 
 #include "hl_internal.hpp"
 #include "irlm.hpp"
@@ -435,6 +436,53 @@ int hl_bench_stream(int dev, long long bytes, int reps, double *seconds)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(a);
+    (void)hipFree(out);
+    return rc;
+}
+
+int hl_bench_gather(int dev, long long nx, long long m, int random, int reps, double *seconds)
+{
+    if (nx <= 0 || nx > 0x7fffffffLL || m < 1024 || reps <= 0 || !seconds) return HL_ERR_ARG;
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    std::vector<int> idx(static_cast<size_t>(m));
+    uint64_t z = 0x9E3779B97F4A7C15ULL;
+    for (long long i = 0; i < m; ++i) {
+        if (random) { // splitmix64 stream, reduced to [0, nx)
+            z += 0x9E3779B97F4A7C15ULL;
+            uint64_t v = z;
+            v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            v = (v ^ (v >> 27)) * 0x94D049BB133111EBULL;
+            v ^= v >> 31;
+            idx[i] = static_cast<int>(v % static_cast<uint64_t>(nx));
+        } else {
+            idx[i] = static_cast<int>(i % nx);
+        }
+    }
+    const int blocks = 2048;
+    double *x = nullptr, *out = nullptr;
+    int *di = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HL_TRY_HIP(hipMalloc(&x, sizeof(double) * nx));
+    HL_TRY_HIP(hipMalloc(&di, sizeof(int) * m));
+    HL_TRY_HIP(hipMalloc(&out, sizeof(double) * blocks * (kBlock / kWave)));
+    HL_TRY_HIP(hipMemset(x, 0, sizeof(double) * nx));
+    HL_TRY_HIP(hipMemcpy(di, idx.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+    HL_TRY_HIP(hipEventCreate(&e0));
+    HL_TRY_HIP(hipEventCreate(&e1));
+    rc = launch_gather(x, di, m, out, blocks, nullptr);
+    if (!rc) rc = hipEventRecord(e0, nullptr) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    for (int i = 0; i < reps && !rc; ++i) rc = launch_gather(x, di, m, out, blocks, nullptr);
+    float ms = 0.0f;
+    if (!rc) rc = hipEventRecord(e1, nullptr) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    if (!rc) rc = hipEventElapsedTime(&ms, e0, e1) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    *seconds = 1e-3 * ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(x);
+    (void)hipFree(di);
     (void)hipFree(out);
     return rc;
 }

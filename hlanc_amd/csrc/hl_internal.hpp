@@ -229,6 +229,8 @@ int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn
 int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStream_t s);
 // HBM read probe: sums n doubles (n even, 16-byte aligned) into out[blocks*4].
 int launch_stream_read(const double *a, long long n, double *out, int blocks, hipStream_t s);
+// Gather probe: sums x[idx[i]] for i < m into out[blocks*4].
+int launch_gather(const double *x, const int *idx, long long m, double *out, int blocks, hipStream_t s);
 
 // ------------------------------------------------------------ profiling
 // Profile classes: SpMV; CGS partials kernels of the two passes every step

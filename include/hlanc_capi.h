@@ -154,6 +154,11 @@ int hl_bench_cgs(int dev, int n, int m, int tiled, int reps, double *seconds);
 /* Average seconds per pass of a read-only 16-byte-load stream over `bytes`
  * bytes of HBM (own buffer): the read rate this device reaches. */
 int hl_bench_stream(int dev, long long bytes, int reps, double *seconds);
+/* Average seconds per pass of a gather probe: m 4-byte indices streamed, each
+ * followed by an 8-byte read of x[index] from a vector of nx doubles; indices
+ * uniformly random (random = 1) or i mod nx (0).  The SpMV's x-gather pattern
+ * without the matrix values. */
+int hl_bench_gather(int dev, long long nx, long long m, int random, int reps, double *seconds);
 
 /* ------------------------------------------------------------------ solver */
 
