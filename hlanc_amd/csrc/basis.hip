@@ -2,18 +2,22 @@
 //
 // Replaces mat_v_gpu_t's cuBLAS calls (HLanc.hpp:973-1119) and the host BLAS-1
 // of dsaitr (HLanc.hpp:1411-1465):
-//   * cgs_kernel: one fused classical Gram-Schmidt pass
+//   * cgs_block: one fused classical Gram-Schmidt pass
 //       w_out = w_in - V h_in ;  h_out = V^T w_out ;  ||w_out||^2
 //     A wave covers a 16-row x m tile as 4 lane groups of 16 rows x ceil(m/4)
 //     columns, so each load instruction touches four 128-byte column segments.
-//     Partial sums meet in a two-stage reduction whose last block also runs the
+//     Blocks write their partial sums; a second kernel with one block per
+//     column adds them in block order, and the ||w||^2 block runs the
 //     Lanczos/DGKS bookkeeping (Ctl), so a whole dsaitr sweep needs no host
 //     round trip.  Grid shape depends only on n: results are bitwise
-//     repeatable, the same for a solve alone or beside others.
-//   * vq kernels: Out = V Q (dsapps' basis update and the Ritz vectors), as a
-//     v_mfma_f64_16x16x4f64 kernel computing Out^T = Q^T V^T (so stores are
-//     column-coalesced) or as a plain FMA kernel; optional fused residual
-//     update r = qlast r + beta Out(:, k) with ||r||.
+//     repeatable, the same for a solve alone or beside others.  The segmented
+//     kernels (cgs_seg_kernel, cgs_finalize_seg_kernel) run one pass of many
+//     solves in one launch; cgs_kernel is the standalone form (hl_cgs_pass).
+//   * vq kernels: Out = V Q.  vq_seg_kernel restarts every listed solve in one
+//     launch with the dsapps residual update r = qlast r + beta Out(:, k) and
+//     ||r|| partials fused; vq_mfma_kernel (v_mfma_f64_16x16x4f64, Out^T =
+//     Q^T V^T so stores are column-coalesced) and vq_fma_kernel form the Ritz
+//     vectors and serve hl_vq.
 #include <cmath>
 #include <type_traits>
 
@@ -205,8 +209,9 @@ __device__ __forceinline__ int seg_find(const int *first, int stride, int ns, in
     return lo;
 }
 
-// Pointers of CGS pass `mode` of a Lanczos step (the same sequence Solver::step
-// launches for one solve).
+// Operands of CGS pass `mode` of a Lanczos step: the first pass projects w
+// (h0 = V^T w), the second forms r = w - V h0 (h1 = V^T r), the DGKS
+// corrections update r in place with h1, then h0.
 struct PassPtrs {
     const double *w_in;
     double *w_out;

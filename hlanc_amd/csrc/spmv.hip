@@ -1,15 +1,19 @@
 // spmv.hip -- adaptive CSR SpMV for CDNA4 (gfx950).
 //
 // Replaces cusparseDcsrmv inside dcsrmv_operator_gpu::operator() (HLanc.hpp:553-567).
-// The matrix is cut on the host into work blocks by an nnz walk over row_ptr:
-//   * stream blocks: up to 256 whole short rows holding <= 2048 nnz.  The block
-//     streams val/col coalesced, multiplies by the gathered x into LDS, then
-//     reduces each row from LDS with tpr lanes per row (tpr = 1 is "a lane per
-//     short row", tpr = 64 "a wavefront per medium row");
-//   * chunk blocks: a row longer than 2048 nnz gets blocks of <= 8192 nnz each;
+// The matrix is cut on the host into work items by an nnz walk over row_ptr
+// (plan_items; the shape is a plan parameter, default 256 rows / 2048 nnz):
+//   * stream items: whole short rows up to the item's row and nnz caps.  The
+//     block streams val/col coalesced (non-temporal, all of a thread's loads
+//     in flight before its x gathers), multiplies by the gathered x into LDS,
+//     then reduces each row from LDS with tpr lanes per row (tpr = 1 is "a
+//     lane per short row", tpr = 64 "a wavefront per medium row");
+//   * chunk items: a row longer than the nnz cap gets blocks of <= 8192 nnz;
 //     the partial sums of a row are added in chunk order by whichever block
 //     finishes last (a deterministic split of heavy rows).
-// Every row sum is formed in a fixed order, so results are bitwise repeatable.
+// spmv_seg_kernel runs the SpMV of many solves in one launch (one segment of
+// the grid per solve).  Every row sum is formed in a fixed order, so results
+// are bitwise repeatable.
 #include <algorithm>
 #include <vector>
 
