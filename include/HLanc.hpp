@@ -278,6 +278,55 @@ int ds_solver_gpu(int n, int nev, int ncv, double *values, double *vectors, int 
                                         ldv, dspmv_operator, dev, &opts, info));
 }
 
+// ---------------------------------------------------------- batched solves
+// Extension without a counterpart in the reference: many solves of operators on
+// one device in one call, continuously batched (hl_ds_solve_batch).  Each
+// solve's values and vectors equal what ds_solver_gpu returns for the same
+// operator and options, bit for bit.
+struct batch_solve {
+    dcsrmv_operator_gpu const *op = nullptr;
+    int nev = 0, ncv = 0;
+    double *values = nullptr;  // host, >= nev
+    double *vectors = nullptr; // host, column-major ldv x ncv, or nullptr: values only
+    int ldv = 0;
+    hl_options opts;           // the reference defaults unless changed
+    double arrival = 0.0;      // seconds after the call; admitted no earlier
+    hl_info info;              // out
+    int status = HL_ERR_ARG;   // out: HL_OK, HL_NOT_CONVERGED or an error code
+    batch_solve() { hl_options_default(&opts); std::memset(&info, 0, sizeof(info)); }
+};
+
+// Returns HL_OK once every solve has run (see each status); hard errors exit
+// like SAFE_CALL.  max_active <= 0 and hbm_budget <= 0 mean no cap.
+inline int ds_solver_gpu_batch(std::vector<batch_solve> &solves, int dev = 0, int max_active = 0,
+                               double hbm_budget = 0.0)
+{
+    std::vector<hl_solve_item> items(solves.size());
+    for (size_t i = 0; i < solves.size(); ++i) {
+        hl_solve_item &it = items[i];
+        std::memset(&it, 0, sizeof(it));
+        const batch_solve &b = solves[i];
+        it.A = b.op ? b.op->handle() : nullptr;
+        it.nev = b.nev;
+        it.ncv = b.ncv;
+        it.values = b.values;
+        it.vectors = b.vectors;
+        it.ldv = b.ldv;
+        it.opts = b.opts;
+        it.arrival = b.arrival;
+    }
+    const int rc = hl_ds_solve_batch(dev, items.data(), static_cast<int>(items.size()), max_active, hbm_budget);
+    if (rc) {
+        std::fprintf(stderr, "ds_solver_gpu_batch failed: %d (%s)\n", rc, hl_last_error());
+        std::exit(rc);
+    }
+    for (size_t i = 0; i < solves.size(); ++i) {
+        solves[i].info = items[i].info;
+        solves[i].status = items[i].status;
+    }
+    return HL_OK;
+}
+
 // HLanc.hpp:1868-1872: tol = machine epsilon, 200 restarts, largest algebraic.
 template <typename dspmv_operator_t>
 int ds_solver_gpu(int n, int nev, int ncv, double *values, double *vectors, int ldv,

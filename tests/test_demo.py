@@ -60,3 +60,22 @@ def test_solver_cli_and_verifier(tmp_path):
                        capture_output=True, text=True, timeout=300)
     assert v.returncode == 0, v.stderr
     assert v.stderr.count("max r-diff") == 4
+
+
+@pytest.mark.gpu
+def test_batch_solver_cli_is_bitwise_the_single_solves():
+    j = os.path.join(R.GOLDEN, "Journals.mtx")
+    s = os.path.join(R.GOLDEN, "simple.pp")
+    r = subprocess.run([prog("hlanc_batch_solver"), "4", "12", "mm", j, "pp", s, "mm", j],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "0 mismatches" in r.stderr
+    lines = r.stdout.split("\n")
+    assert lines[0] == "4" and lines[1] == "124"
+    np.testing.assert_allclose([float(x) for x in lines[2:6]],
+                               [19795.37857462678, 23432.29299233234, 25008.48027847430, 68542.71718825979],
+                               rtol=1e-13)
+    assert lines[6] == "4" and lines[7] == "16"
+    np.testing.assert_allclose([float(x) for x in lines[8:12]],
+                               [2.337028429548383, 4.497836710984441, 12.34353751967706, 110.4221400959146],
+                               rtol=1e-13)
