@@ -99,7 +99,10 @@ int hl_csr_info(hl_csr_t h, int *dev, int *nrows, int *ncols, long long *nnz_gen
  * means the result is complete on return, device y means the call is ordered
  * on the default stream.  FP64 values. */
 int hl_csr_apply(hl_csr_t h, const double *x, double *y);
-/* Device-pointer form on an explicit stream (no pointer probing, no sync). */
+/* Device-pointer form on an explicit stream (no pointer probing, no sync).
+ * Rows longer than the work-item capacity are summed through the operator's
+ * own partial-sum slots, so calls on one operator must not overlap on
+ * different streams; solves keep their own slots and may share an operator. */
 int hl_csr_spmv(hl_csr_t h, const double *x_dev, double *y_dev, void *stream);
 /* The FP32-value kernel (requires val_prec = HL_PREC_F32 at creation). */
 int hl_csr_spmv_f32val(hl_csr_t h, const double *x_dev, double *y_dev, void *stream);
