@@ -813,13 +813,19 @@ int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N
         set_error("V.Q: K=%d unsupported (K <= 128)", max_K);
         return HL_ERR_ARG;
     }
+    // one chunk of output columns when Q's K x CB slice fits 48 KB of LDS: V is
+    // then read once (the accumulation order over k does not depend on CB)
 #define HL_VQS(CB) vq_seg_kernel<CB><<<blocks_total, kBlock, sizeof(double) * (size_t)max_K * (CB), s>>>(T, nt)
     if (max_N <= 8)
         HL_VQS(8);
     else if (max_N <= 16)
         HL_VQS(16);
-    else
+    else if (max_N <= 32 || max_K * 48 > 6144)
         HL_VQS(32);
+    else if (max_N <= 48 || max_K * 64 > 6144)
+        HL_VQS(48);
+    else
+        HL_VQS(64);
 #undef HL_VQS
     HL_TRY_HIP(hipGetLastError());
     vq_seg_finalize_kernel<<<nt, kBlock, 0, s>>>(T);
