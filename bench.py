@@ -166,7 +166,7 @@ def run_batch(args, H, torch):
     log(f"workload batch: {count} matrices, sum n {sum(ns)}, sum nnz {nnz}, built in "
         f"{time.perf_counter() - t0:.1f} s")
     torch.cuda.init()
-    kw = dict(tol=args.tol, maxiter=args.maxiter)
+    kw = dict(tol=args.tol, maxiter=args.maxiter, use_mfma=args.mfma)
 
     vec_out = [H.pinned_empty(op.nrows, ncv) for op in ops]  # page-locked outputs, allocated once
 
@@ -316,6 +316,8 @@ def main():
     ap.add_argument("--profile", action="store_true", help="batch: one more profiled batched pass")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV work-item shape")
     ap.add_argument("--no-plain", action="store_true", help="batch: skip the one-at-a-time comparison")
+    ap.add_argument("--mfma", type=int, choices=(0, 1), default=None,
+                    help="restart/Ritz V.Q on the MFMA (1) or FMA (0) kernel; default: the library's")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -362,7 +364,8 @@ def main():
 
     vec_out = H.pinned_empty(n, ncv)  # page-locked output: the final copy runs at link rate
     for i in range(args.warmup):
-        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_out)
+        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_out,
+                            use_mfma=args.mfma)
         log(f"warmup {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} {r.info.seconds:.3f} s")
 
     lat = []
@@ -370,7 +373,8 @@ def main():
     t_start = time.perf_counter()
     for i in range(args.steps):
         t = time.perf_counter()
-        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_out)
+        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_out,
+                            use_mfma=args.mfma)
         lat.append(time.perf_counter() - t)
         log(f"solve {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} {lat[-1]:.3f} s")
         if r.ret != 0:
@@ -385,7 +389,7 @@ def main():
     # per-kernel-class profile of one more solve (HIP events around each launch)
     import ctypes
 
-    o = H.options(args.tol, args.maxiter)
+    o = H.options(args.tol, args.maxiter, use_mfma=args.mfma)
     o.flags = H.HL_FLAG_PROFILE
     values = np.zeros(ncv)
     info = H.Info()

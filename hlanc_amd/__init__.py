@@ -424,8 +424,8 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
                    max_active: int = 0, hbm_budget: float = 0.0, flags: int = 0, vectors_out=None):
     """Many solves on one device with continuous batching (hl_ds_solve_batch).
 
-    ``ops`` are dcsrmv_operator_gpu on ``dev``; nev, ncv, tol and maxiter are
-    one value for all solves or one per solve.  ``arrivals`` (seconds after the
+    ``ops`` are dcsrmv_operator_gpu on ``dev``; nev, ncv, tol, maxiter and
+    use_mfma are one value for all solves or one per solve.  ``arrivals`` (seconds after the
     call, non-decreasing) delays admission, as a serving loop sees requests
     arrive.  Each result is bitwise what ds_solver_gpu returns for the same
     operator and options.  ``vectors_out`` optionally lists one vectors buffer
@@ -445,7 +445,7 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
             vecs = _vectors_buffer(None if vectors_out is None else vectors_out[i], n, max(ncv_i, 1))
         seed = 0 if seeds is None else int(seeds[i])
         v0 = None if v0s is None else v0s[i]
-        o = options(float(per(tol, i)), int(per(maxiter, i)), v0, seed, use_mfma, flags=flags)
+        o = options(float(per(tol, i)), int(per(maxiter, i)), v0, seed, per(use_mfma, i), flags=flags)
         it = items[i]
         it.A = op.h
         it.nev = nev_i

@@ -13,11 +13,11 @@
 //     repeatable, the same for a solve alone or beside others.  The segmented
 //     kernels (cgs_seg_kernel, cgs_finalize_seg_kernel) run one pass of many
 //     solves in one launch; cgs_kernel is the standalone form (hl_cgs_pass).
-//   * vq kernels: Out = V Q.  vq_seg_kernel restarts every listed solve in one
-//     launch with the dsapps residual update r = qlast r + beta Out(:, k) and
-//     ||r|| partials fused; vq_mfma_kernel (v_mfma_f64_16x16x4f64, Out^T =
-//     Q^T V^T so stores are column-coalesced) and vq_fma_kernel form the Ritz
-//     vectors and serve hl_vq.
+//   * vq kernels: Out = V Q.  vq_seg_mfma_kernel (v_mfma_f64_16x16x4f64,
+//     Out^T = Q^T V^T so stores are column-coalesced) and vq_seg_kernel (FMA)
+//     restart every listed solve in one pass with the dsapps residual update
+//     r = qlast r + beta Out(:, k) and ||r|| partials fused; vq_mfma_kernel and
+//     vq_fma_kernel form the Ritz vectors and serve hl_vq.
 #include <cmath>
 #include <type_traits>
 
@@ -354,6 +354,8 @@ __global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, con
     if (threadIdx.x == 0) partials[blockIdx.x] = vals[0];
 }
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
 // The restart V.Q of every listed solve in one launch: Out = V Q(:, 0:N) on
 // tiled bases (ld K), then the dsapps residual update
 // r = Q(ncv-1, nev-1) rscale r + beta Out(:, res_col) (res_col < 0: beta term
@@ -409,6 +411,86 @@ __global__ __launch_bounds__(kBlock) void vq_seg_kernel(const SegVQ *__restrict_
     }
 }
 
+// The same restart on v_mfma_f64_16x16x4f64 (layout as vq_mfma_kernel): each
+// wave takes 16-row tiles, reads V once per tile and accumulates all of the
+// solve's 16-column output tiles (at most NT); Q^T sits in LDS.  Used for the
+// solves whose K x N fits (vq_seg_mfma_fits).
+template <int NT>
+__global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__restrict__ T, int nt)
+{
+    extern __shared__ double qs[]; // Kp x NC, row k
+    __shared__ double red[kBlock / kWave];
+    const int i = seg_find(&T[0].blk0, sizeof(SegVQ) / sizeof(int), nt, blockIdx.x);
+    const SegVQ t = T[i];
+    const int bid = blockIdx.x - t.blk0;
+    const int nct = (t.N + 15) / 16; // this solve's column tiles, <= NT
+    const int NC = nct * 16;
+    const int Kp = (t.K + 3) & ~3;
+    for (int x = threadIdx.x; x < Kp * NC; x += kBlock) {
+        const int k = x / NC, c = x % NC;
+        qs[x] = (k < t.K && c < t.N) ? t.Q[(long long)c * t.K + k] : 0.0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int lr = lane & 15, lk = lane >> 4;
+    const double qrs = t.qlast * t.ctl->rscale;
+    double nrm = 0.0;
+    const long long ntiles = (t.n + 15) / 16;
+    for (long long tile = (long long)bid * (kBlock / kWave) + wv; tile < ntiles;
+         tile += (long long)t.nb * (kBlock / kWave)) {
+        const long long row = tile * 16 + lr;
+        const bool ok = row < t.n;
+        dbl4 acc[NT];
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+        // 8 loads of V in flight per wave before their 8 x nct MFMAs
+        for (int k0 = 0; k0 < Kp; k0 += 32) {
+            double b[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = k0 + 4 * j + lk;
+                b[j] = (ok && k < t.K) ? t.V[vpos<true>(row, k, t.K)] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (k0 + 4 * j >= Kp) break;
+                const int k = k0 + 4 * j + lk;
+#pragma unroll
+                for (int ct = 0; ct < NT; ++ct)
+                    if (ct < nct)
+                        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(qs[k * NC + ct * 16 + lr], b[j], acc[ct], 0, 0, 0);
+            }
+        }
+        double rk = 0.0;
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+            if (ct >= nct) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int cl = ct * 16 + lk + 4 * e;
+                if (ok && cl < t.N) t.Out[vpos<true>(row, cl, t.K)] = acc[ct][e];
+                if (cl == t.res_col) rk = acc[ct][e];
+            }
+        }
+        // the lanes holding column res_col of this tile hand it to lanes 0..15
+        const double rkv = t.res_col >= 0 ? __shfl(rk, ((t.res_col % 16) % 4) * 16 + lr) : 0.0;
+        if (lk == 0 && ok) { // dsapps residual (HLanc.hpp:1605-1614)
+            const double rr = qrs * t.r[row] + t.beta * rkv;
+            t.r[row] = rr;
+            nrm += rr * rr;
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
+    if (lane == 0) red[wv] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) sum += red[w];
+        t.partials[bid] = sum;
+    }
+}
+
 // One block per listed solve: rnorm = sqrt(sum of its V.Q block partials).
 __global__ __launch_bounds__(kBlock) void vq_seg_finalize_kernel(const SegVQ *__restrict__ T)
 {
@@ -431,7 +513,6 @@ __global__ __launch_bounds__(kBlock) void seg_readback_kernel(const SegMember *_
     for (int x = threadIdx.x; x < kCtlWords; x += kBlock) o[2 * mb.ncv + x] = c[x];
 }
 
-typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // Out^T(c, row) = sum_k Q(k, c) V(row, k) on v_mfma_f64_16x16x4f64.
 // A (16 x 4): lane l holds Q^T(ct*16 + l%16, k0 + l/16); B (4 x 16): lane l holds
@@ -469,13 +550,20 @@ __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, co
         dbl4 acc[NT];
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
-        for (int k0 = 0; k0 < Kp; k0 += 4) {
-            const int k = k0 + lk;
-            const double b = (ok && k < K) ? V[vpos<TIN>(row, k, ldv)] : 0.0;
+        for (int k0 = 0; k0 < Kp; k0 += 32) { // 8 loads of V in flight per wave
+            double b[8];
 #pragma unroll
-            for (int ct = 0; ct < NT; ++ct) {
-                const double a = qs[k * NC + ct * 16 + lr];
-                acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[ct], 0, 0, 0);
+            for (int j = 0; j < 8; ++j) {
+                const int k = k0 + 4 * j + lk;
+                b[j] = (ok && k < K) ? V[vpos<TIN>(row, k, ldv)] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (k0 + 4 * j >= Kp) break;
+                const int k = k0 + 4 * j + lk;
+#pragma unroll
+                for (int ct = 0; ct < NT; ++ct)
+                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(qs[k * NC + ct * 16 + lr], b[j], acc[ct], 0, 0, 0);
             }
         }
         double rk = 0.0;
@@ -806,12 +894,44 @@ int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fi
 
 int vq_seg_blocks(int n) { return grid_for(n, kBlock * 4, 1024); }
 
-int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, hipStream_t s)
+bool vq_seg_mfma_fits(int K, int N)
 {
-    if (nt <= 0) return HL_OK;
-    if (max_K > 128 || max_K <= 0) {
+    const int Kp = (K + 3) & ~3, nct = (N + 15) / 16;
+    return K > 0 && N > 0 && nct <= 8 && (size_t)Kp * nct * 16 * sizeof(double) <= kVqMfmaLds;
+}
+
+int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, int nt_mfma, int blocks_mfma,
+                  int mfma_nct, size_t mfma_lds, hipStream_t s)
+{
+    if (nt + nt_mfma <= 0) return HL_OK;
+    if (max_K > 128) {
         set_error("V.Q: K=%d unsupported (K <= 128)", max_K);
         return HL_ERR_ARG;
+    }
+    if (nt_mfma > 0) { // T[nt, nt + nt_mfma): their own block numbering from 0
+        if (mfma_nct < 1 || mfma_nct > 8 || mfma_lds > kVqMfmaLds) {
+            set_error("V.Q: %d column tiles / %zu B of Q unsupported on MFMA", mfma_nct, mfma_lds);
+            return HL_ERR_ARG;
+        }
+        const SegVQ *Tm = T + nt;
+#define HL_VQM2(NT) vq_seg_mfma_kernel<NT><<<blocks_mfma, kBlock, mfma_lds, s>>>(Tm, nt_mfma)
+        switch (mfma_nct) {
+        case 1: HL_VQM2(1); break;
+        case 2: HL_VQM2(2); break;
+        case 3: HL_VQM2(3); break;
+        case 4: HL_VQM2(4); break;
+        case 5: HL_VQM2(5); break;
+        case 6: HL_VQM2(6); break;
+        case 7: HL_VQM2(7); break;
+        default: HL_VQM2(8); break;
+        }
+#undef HL_VQM2
+        HL_TRY_HIP(hipGetLastError());
+    }
+    if (nt <= 0) {
+        vq_seg_finalize_kernel<<<nt_mfma, kBlock, 0, s>>>(T);
+        HL_TRY_HIP(hipGetLastError());
+        return HL_OK;
     }
     // one chunk of output columns when Q's K x CB slice fits 48 KB of LDS: V is
     // then read once (the accumulation order over k does not depend on CB)
@@ -828,7 +948,7 @@ int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N
         HL_VQS(64);
 #undef HL_VQS
     HL_TRY_HIP(hipGetLastError());
-    vq_seg_finalize_kernel<<<nt, kBlock, 0, s>>>(T);
+    vq_seg_finalize_kernel<<<nt + nt_mfma, kBlock, 0, s>>>(T);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

@@ -156,7 +156,7 @@ void hl_options_default(hl_options *o)
     o->v0 = nullptr;
     o->seed = 0;
     o->basis_prec = HL_PREC_F64;
-    o->use_mfma = 0;
+    o->use_mfma = 1; // measured faster than the FMA kernel on every benchmarked workload
 }
 
 int hl_csr_create(int dev, int nrows, int ncols, int nnz, const int *row_ptr, const int *col,

@@ -640,14 +640,23 @@ int Engine::after_sweep(std::vector<Member *> &act)
     const size_t qstride = (size_t)ncvmax * ncvmax;
     pool->run(static_cast<int>(nm), [&](int i) { act[i]->decide(hQ + (size_t)i * qstride); });
 
-    int nv = 0, vblocks = 0, maxK = 0, maxN = 0, nfinish = 0;
+    // FMA-kernel restarts first, then MFMA-kernel ones (each member's kernel
+    // follows from its own options and shape alone, so a solve gives the same
+    // bits in any batch)
+    int nfinish = 0, nres = 0;
+    for (size_t i = 0; i < nm; ++i) {
+        nfinish += act[i]->act == Member::ACT_FINISH;
+        nres += act[i]->act == Member::ACT_RESTART;
+    }
+    int nv = 0, nmf = 0, vblocks = 0, mblocks = 0, maxK = 0, maxN = 0, mnct = 0;
+    size_t mlds = 0;
     double bytes = 0.0;
     for (size_t i = 0; i < nm; ++i) {
         Member &m = *act[i];
-        if (m.act == Member::ACT_FINISH) ++nfinish;
         if (m.act != Member::ACT_RESTART) continue;
+        const bool mf = m.opts.use_mfma && vq_seg_mfma_fits(m.ncv, m.vq_N);
         Workspace &W = m.ws;
-        SegVQ &t = hV[nv++];
+        SegVQ &t = mf ? hV[nres - 1 - nmf++] : hV[nv++];
         t.V = W.V[W.cur];
         t.Out = W.V[1 - W.cur];
         t.Q = dQ + i * qstride;
@@ -661,19 +670,29 @@ int Engine::after_sweep(std::vector<Member *> &act)
         t.N = m.vq_N;
         t.res_col = m.vq_res;
         t.nb = vq_seg_blocks(m.n);
-        t.blk0 = vblocks;
-        vblocks += t.nb;
-        maxK = std::max(maxK, t.K);
-        maxN = std::max(maxN, t.N);
+        if (mf) {
+            t.blk0 = mblocks;
+            mblocks += t.nb;
+            const int nct = (t.N + 15) / 16;
+            mnct = std::max(mnct, nct);
+            mlds = std::max(mlds, sizeof(double) * (size_t)((t.K + 3) & ~3) * nct * 16);
+        } else {
+            t.blk0 = vblocks;
+            vblocks += t.nb;
+            maxK = std::max(maxK, t.K);
+            maxN = std::max(maxN, t.N);
+        }
         bytes += 8.0 * m.n * (t.K + t.N + 2);
     }
+    // the MFMA entries were filled from the back: restore their member order
+    std::reverse(hV + nv, hV + nres);
     int rc;
-    if (nv > 0 || nfinish > 0)
+    if (nres > 0 || nfinish > 0)
         HL_TRY_HIP(hipMemcpyAsync(dQ, hQ, sizeof(double) * nm * qstride, hipMemcpyHostToDevice, stream));
-    if (nv > 0) {
-        HL_TRY_HIP(hipMemcpyAsync(dV, hV, sizeof(SegVQ) * nv, hipMemcpyHostToDevice, stream));
+    if (nres > 0) {
+        HL_TRY_HIP(hipMemcpyAsync(dV, hV, sizeof(SegVQ) * nres, hipMemcpyHostToDevice, stream));
         if ((rc = prof.begin(stream))) return rc;
-        if ((rc = launch_vq_seg(dV, nv, vblocks, maxK, maxN, stream))) return rc;
+        if ((rc = launch_vq_seg(dV, nv, vblocks, maxK, maxN, nmf, mblocks, mnct, mlds, stream))) return rc;
         if ((rc = prof.end(stream, K_VQ, bytes))) return rc;
         for (Member *m : act)
             if (m->act == Member::ACT_RESTART) m->ws.cur = 1 - m->ws.cur;

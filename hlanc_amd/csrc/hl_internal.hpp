@@ -192,7 +192,16 @@ struct SegVQ {
     int nb, blk0;                // blocks (vq_seg_blocks(n)); first block in the launch
 };
 int vq_seg_blocks(int n);
-int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, hipStream_t s);
+// LDS of the MFMA restart kernel: Q^T of one solve, K rounded up to 4 by N
+// rounded up to 16.
+constexpr size_t kVqMfmaLds = 48 * 1024;
+bool vq_seg_mfma_fits(int K, int N);
+// T[0, nt) run on the FMA kernel (blocks numbered 0..blocks_total), then
+// T[nt, nt + nt_mfma) on the MFMA kernel (numbered 0..blocks_mfma, at most
+// mfma_nct 16-column tiles and mfma_lds bytes of Q^T each), then one
+// finalize over all entries.
+int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, int nt_mfma, int blocks_mfma,
+                  int mfma_nct, size_t mfma_lds, hipStream_t s);
 // Packs hsub, hdia (ncv each) and the Ctl words of every member at
 // out + member * stride (stride >= 2 ncv + kCtlWords).
 int launch_seg_readback(const SegMember *M, int nm, int stride, double *out, hipStream_t s);

@@ -175,7 +175,9 @@ typedef struct {
     const double *v0;  /* host start vector of length n, or NULL: seeded generator */
     uint64_t seed;     /* generator seed when v0 == NULL */
     int basis_prec;    /* HL_PREC_F64 (reference semantics) or HL_PREC_F32 (mixed) */
-    int use_mfma;      /* 1: V.Q on the MFMA kernel, 0: FMA kernel (hl_options_default says which) */
+    int use_mfma;      /* 1 (default): restart and Ritz-vector V.Q on v_mfma_f64_16x16x4f64
+                          (a restart whose K x N block of Q exceeds 48 KB runs on the FMA
+                          kernel); 0: FMA kernel.  The results differ in rounding only. */
     int flags;         /* bit 0: do not use hipGraph replay for the Lanczos steps */
     int reserved[5];
 } hl_options;

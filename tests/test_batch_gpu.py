@@ -74,6 +74,22 @@ def test_batch_is_bitwise_the_standalone_solve(mixed):
     assert_same(run_batch(ops), alone)
 
 
+def test_mfma_restarts_in_a_mixed_batch(mixed):
+    """Solves whose restart V.Q runs on MFMA next to solves on the FMA kernel:
+    each is bitwise its own standalone solve and matches the oracle."""
+    mats, ops, _ = mixed
+    mf = [i % 2 == 0 for i in range(len(ops))]
+    alone = [H.ds_solver_gpu(op.nrows, NEV[i], NCV[i], op, tol=1e-10, maxiter=3000, seed=100 + i,
+                             use_mfma=mf[i]) for i, op in enumerate(ops)]
+    assert_same(run_batch(ops, use_mfma=mf), alone)
+    for i, (rp, col, val) in enumerate(mats):
+        if not mf[i]:
+            continue
+        o = O.ds_solver(O.Csr(rp, col, val), NEV[i], NCV[i], tol=1e-10, maxiter=3000)
+        assert o.ret == 0 and alone[i].ret == 0
+        np.testing.assert_allclose(alone[i].values, o.values, rtol=1e-10)
+
+
 def test_batch_matches_oracle(mixed):
     mats, ops, alone = mixed
     res = run_batch(ops, vectors=False)
