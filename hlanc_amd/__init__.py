@@ -52,7 +52,8 @@ class Info(ctypes.Structure):
                 ("graph_sweeps", ctypes.c_int),
                 ("spmv", ctypes.c_longlong), ("cgs_passes", ctypes.c_longlong), ("rnorm", ctypes.c_double),
                 ("seconds", ctypes.c_double), ("max_rel_resid", ctypes.c_double),
-                ("refine_shift", ctypes.c_double)]
+                ("refine_shift", ctypes.c_double), ("spmv_f32", ctypes.c_longlong),
+                ("restarts_f32", ctypes.c_int), ("reserved_info", ctypes.c_int)]
 
 
 class Profile(ctypes.Structure):
@@ -371,19 +372,22 @@ def _vectors_buffer(out, ldv: int, ncv: int) -> np.ndarray:
 
 def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, v0=None,
                   seed: int = 0, use_mfma: Optional[bool] = None, ldv: Optional[int] = None,
-                  flags: int = 0, vectors_out: Optional[np.ndarray] = None) -> SolveResult:
+                  flags: int = 0, vectors_out: Optional[np.ndarray] = None,
+                  basis_prec: int = HL_PREC_F64) -> SolveResult:
     """ds_solver_gpu (HLanc.hpp:1868-1872).  ``op`` is a dcsrmv_operator_gpu
     (device-resident path) or any callable ``op(x, y)`` on numpy arrays (host
     path, as the reference calls arbitrary operators with host pointers).
     ``ret`` is 0 on success and 1 on the reference's numerical failures; values
     and vectors are written only on success.  ``vectors_out`` (column-major
     float64, at least ldv x ncv, e.g. from pinned_empty) receives the vectors
-    instead of a fresh array."""
+    instead of a fresh array.  ``basis_prec=HL_PREC_F32`` runs the
+    mixed-precision solve (device operators only): an FP32-basis phase, then
+    an FP64 phase that returns FP64 results to ``tol``."""
     ldv = n if ldv is None else ldv
     values = np.zeros(ncv)
     vectors = _vectors_buffer(vectors_out, ldv, ncv)
     ldv = vectors.shape[0] if vectors_out is not None else ldv
-    o = options(tol, maxiter, v0, seed, use_mfma, flags=flags)
+    o = options(tol, maxiter, v0, seed, use_mfma, basis_prec=basis_prec, flags=flags)
     info = Info()
     if isinstance(op, dcsrmv_operator_gpu):
         if op.nrows != n:
@@ -421,11 +425,12 @@ class BatchResult:
 
 def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, seeds=None,
                    v0s=None, use_mfma: Optional[bool] = None, vectors: bool = True, arrivals=None,
-                   max_active: int = 0, hbm_budget: float = 0.0, flags: int = 0, vectors_out=None):
+                   max_active: int = 0, hbm_budget: float = 0.0, flags: int = 0, vectors_out=None,
+                   basis_prec=HL_PREC_F64):
     """Many solves on one device with continuous batching (hl_ds_solve_batch).
 
-    ``ops`` are dcsrmv_operator_gpu on ``dev``; nev, ncv, tol, maxiter and
-    use_mfma are one value for all solves or one per solve.  ``arrivals`` (seconds after the
+    ``ops`` are dcsrmv_operator_gpu on ``dev``; nev, ncv, tol, maxiter,
+    use_mfma and basis_prec are one value for all solves or one per solve.  ``arrivals`` (seconds after the
     call, non-decreasing) delays admission, as a serving loop sees requests
     arrive.  Each result is bitwise what ds_solver_gpu returns for the same
     operator and options.  ``vectors_out`` optionally lists one vectors buffer
@@ -445,7 +450,8 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
             vecs = _vectors_buffer(None if vectors_out is None else vectors_out[i], n, max(ncv_i, 1))
         seed = 0 if seeds is None else int(seeds[i])
         v0 = None if v0s is None else v0s[i]
-        o = options(float(per(tol, i)), int(per(maxiter, i)), v0, seed, per(use_mfma, i), flags=flags)
+        o = options(float(per(tol, i)), int(per(maxiter, i)), v0, seed, per(use_mfma, i),
+                    basis_prec=int(per(basis_prec, i)), flags=flags)
         it = items[i]
         it.A = op.h
         it.nev = nev_i

@@ -235,7 +235,7 @@ __device__ __forceinline__ PassPtrs pass_ptrs(const SegMember &mb, int mode)
 // One CGS pass of every listed solve in a single launch (split form).  The
 // pass (MODE) is a template argument so that profiles tell the always-run
 // passes (EPI_WNORM, EPI_PASS1) from the DGKS corrections.
-template <int MAXC, int MODE>
+template <int MAXC, int MODE, typename VT>
 __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__restrict__ M,
                                                          const SegStep *__restrict__ S, int ns)
 {
@@ -243,8 +243,9 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
     const SegStep st = S[i];
     const SegMember mb = M[st.mem];
     const PassPtrs p = pass_ptrs(mb, MODE);
-    cgs_block<MAXC, double, true, 9>(blockIdx.x - st.cgs0, mb.nb, mb.n, st.j + 1, mb.V, mb.vld, p.w_in, p.w_out,
-                                     p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub, mb.hdia, MODE, st.j);
+    cgs_block<MAXC, VT, true, 9>(blockIdx.x - st.cgs0, mb.nb, mb.n, st.j + 1, static_cast<const VT *>(mb.V), mb.vld,
+                                 p.w_in, p.w_out, p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub, mb.hdia,
+                                 MODE, st.j);
 }
 
 __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMember *__restrict__ M,
@@ -360,13 +361,15 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // tiled bases (ld K), then the dsapps residual update
 // r = Q(ncv-1, nev-1) rscale r + beta Out(:, res_col) (res_col < 0: beta term
 // absent) and each block's partial ||r||^2 in partials[block].
-template <int CB>
+template <int CB, typename VT>
 __global__ __launch_bounds__(kBlock) void vq_seg_kernel(const SegVQ *__restrict__ T, int nt)
 {
     extern __shared__ double qs[]; // K x CB
     __shared__ double red[kBlock / kWave];
     const int i = seg_find(&T[0].blk0, sizeof(SegVQ) / sizeof(int), nt, blockIdx.x);
     const SegVQ t = T[i];
+    const VT *__restrict__ V = static_cast<const VT *>(t.V);
+    VT *__restrict__ Out = static_cast<VT *>(t.Out);
     const int bid = blockIdx.x - t.blk0;
     const double qrs = t.qlast * t.ctl->rscale;
     double nrm = 0.0;
@@ -384,14 +387,14 @@ __global__ __launch_bounds__(kBlock) void vq_seg_kernel(const SegVQ *__restrict_
 #pragma unroll
             for (int c = 0; c < CB; ++c) acc[c] = 0.0;
             for (int k = 0; k < t.K; ++k) {
-                const double v = t.V[vpos<true>(row, k, t.K)];
+                const double v = static_cast<double>(V[vpos<true>(row, k, t.K)]);
 #pragma unroll
                 for (int c = 0; c < CB; ++c) acc[c] += v * qs[k * CB + c];
             }
             double rk = 0.0;
 #pragma unroll
             for (int c = 0; c < CB; ++c) {
-                if (c0 + c < t.N) t.Out[vpos<true>(row, c0 + c, t.K)] = acc[c];
+                if (c0 + c < t.N) Out[vpos<true>(row, c0 + c, t.K)] = static_cast<VT>(acc[c]);
                 if (c == rl) rk = acc[c];
             }
             if (upd) { // dsapps residual (HLanc.hpp:1605-1614)
@@ -415,13 +418,15 @@ __global__ __launch_bounds__(kBlock) void vq_seg_kernel(const SegVQ *__restrict_
 // wave takes 16-row tiles, reads V once per tile and accumulates all of the
 // solve's 16-column output tiles (at most NT); Q^T sits in LDS.  Used for the
 // solves whose K x N fits (vq_seg_mfma_fits).
-template <int NT>
+template <int NT, typename VT>
 __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__restrict__ T, int nt)
 {
     extern __shared__ double qs[]; // Kp x NC, row k
     __shared__ double red[kBlock / kWave];
     const int i = seg_find(&T[0].blk0, sizeof(SegVQ) / sizeof(int), nt, blockIdx.x);
     const SegVQ t = T[i];
+    const VT *__restrict__ V = static_cast<const VT *>(t.V);
+    VT *__restrict__ Out = static_cast<VT *>(t.Out);
     const int bid = blockIdx.x - t.blk0;
     const int nct = (t.N + 15) / 16; // this solve's column tiles, <= NT
     const int NC = nct * 16;
@@ -450,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int k = k0 + 4 * j + lk;
-                b[j] = (ok && k < t.K) ? t.V[vpos<true>(row, k, t.K)] : 0.0;
+                b[j] = (ok && k < t.K) ? static_cast<double>(V[vpos<true>(row, k, t.K)]) : 0.0;
             }
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -469,7 +474,7 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int cl = ct * 16 + lk + 4 * e;
-                if (ok && cl < t.N) t.Out[vpos<true>(row, cl, t.K)] = acc[ct][e];
+                if (ok && cl < t.N) Out[vpos<true>(row, cl, t.K)] = static_cast<VT>(acc[ct][e]);
                 if (cl == t.res_col) rk = acc[ct][e];
             }
         }
@@ -488,6 +493,22 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
         double sum = 0.0;
         for (int w = 0; w < kBlock / kWave; ++w) sum += red[w];
         t.partials[bid] = sum;
+    }
+}
+
+// out = V(:, 0:K) q on a tiled basis (ld K): one FP64 vector from a basis of
+// either precision (the start vector of a mixed-precision solve's FP64 phase).
+template <typename VT>
+__global__ __launch_bounds__(kBlock) void basis_combine_kernel(int n, int K, const VT *__restrict__ V,
+                                                               const double *__restrict__ q, double *__restrict__ out)
+{
+    __shared__ double qs[128];
+    for (int k = threadIdx.x; k < K; k += kBlock) qs[k] = q[k];
+    __syncthreads();
+    for (long long row = (long long)blockIdx.x * kBlock + threadIdx.x; row < n; row += (long long)gridDim.x * kBlock) {
+        double acc = 0.0;
+        for (int k = 0; k < K; ++k) acc += static_cast<double>(V[vpos<true>(row, k, K)]) * qs[k];
+        out[row] = acc;
     }
 }
 
@@ -834,10 +855,10 @@ int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn
 }
 
 namespace {
-template <int MODE>
+template <int MODE, typename VT>
 void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg, int blocks, hipStream_t s)
 {
-#define HL_SEG(MC) cgs_seg_kernel<MC, MODE><<<blocks, kBlock, 0, s>>>(M, S, ns)
+#define HL_SEG(MC) cgs_seg_kernel<MC, MODE, VT><<<blocks, kBlock, 0, s>>>(M, S, ns)
     if (max_cpg <= 2)
         HL_SEG(2);
     else if (max_cpg <= 4)
@@ -855,30 +876,34 @@ void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg,
 } // namespace
 
 int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total, int mode,
-                   hipStream_t s)
+                   bool v32, hipStream_t s)
 {
     if (ns <= 0) return HL_OK;
     if (max_cpg > 32) {
         set_error("CGS pass: more than 128 columns");
         return HL_ERR_ARG;
     }
+#define HL_CGS_MODE(MODE)                                                                \
+    (v32 ? cgs_seg_dispatch<MODE, float>(M, S, ns, max_cpg, cgs_blocks_total, s)         \
+         : cgs_seg_dispatch<MODE, double>(M, S, ns, max_cpg, cgs_blocks_total, s))
     switch (mode) {
     case EPI_WNORM:
-        cgs_seg_dispatch<EPI_WNORM>(M, S, ns, max_cpg, cgs_blocks_total, s);
+        HL_CGS_MODE(EPI_WNORM);
         break;
     case EPI_PASS1:
-        cgs_seg_dispatch<EPI_PASS1>(M, S, ns, max_cpg, cgs_blocks_total, s);
+        HL_CGS_MODE(EPI_PASS1);
         break;
     case EPI_PASS2:
-        cgs_seg_dispatch<EPI_PASS2>(M, S, ns, max_cpg, cgs_blocks_total, s);
+        HL_CGS_MODE(EPI_PASS2);
         break;
     case EPI_PASS3:
-        cgs_seg_dispatch<EPI_PASS3>(M, S, ns, max_cpg, cgs_blocks_total, s);
+        HL_CGS_MODE(EPI_PASS3);
         break;
     default:
         set_error("CGS pass: bad mode %d", mode);
         return HL_ERR_ARG;
     }
+#undef HL_CGS_MODE
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }
@@ -900,8 +925,10 @@ bool vq_seg_mfma_fits(int K, int N)
     return K > 0 && N > 0 && nct <= 8 && (size_t)Kp * nct * 16 * sizeof(double) <= kVqMfmaLds;
 }
 
-int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, int nt_mfma, int blocks_mfma,
-                  int mfma_nct, size_t mfma_lds, hipStream_t s)
+namespace {
+template <typename VT>
+int launch_vq_seg_t(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, int nt_mfma, int blocks_mfma,
+                    int mfma_nct, size_t mfma_lds, hipStream_t s)
 {
     if (nt + nt_mfma <= 0) return HL_OK;
     if (max_K > 128) {
@@ -914,7 +941,7 @@ int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N
             return HL_ERR_ARG;
         }
         const SegVQ *Tm = T + nt;
-#define HL_VQM2(NT) vq_seg_mfma_kernel<NT><<<blocks_mfma, kBlock, mfma_lds, s>>>(Tm, nt_mfma)
+#define HL_VQM2(NT) vq_seg_mfma_kernel<NT, VT><<<blocks_mfma, kBlock, mfma_lds, s>>>(Tm, nt_mfma)
         switch (mfma_nct) {
         case 1: HL_VQM2(1); break;
         case 2: HL_VQM2(2); break;
@@ -935,7 +962,7 @@ int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N
     }
     // one chunk of output columns when Q's K x CB slice fits 48 KB of LDS: V is
     // then read once (the accumulation order over k does not depend on CB)
-#define HL_VQS(CB) vq_seg_kernel<CB><<<blocks_total, kBlock, sizeof(double) * (size_t)max_K * (CB), s>>>(T, nt)
+#define HL_VQS(CB) vq_seg_kernel<CB, VT><<<blocks_total, kBlock, sizeof(double) * (size_t)max_K * (CB), s>>>(T, nt)
     if (max_N <= 8)
         HL_VQS(8);
     else if (max_N <= 16)
@@ -949,6 +976,31 @@ int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N
 #undef HL_VQS
     HL_TRY_HIP(hipGetLastError());
     vq_seg_finalize_kernel<<<nt + nt_mfma, kBlock, 0, s>>>(T);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+} // namespace
+
+int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, int nt_mfma, int blocks_mfma,
+                  int mfma_nct, size_t mfma_lds, bool v32, hipStream_t s)
+{
+    return v32 ? launch_vq_seg_t<float>(T, nt, blocks_total, max_K, max_N, nt_mfma, blocks_mfma, mfma_nct, mfma_lds, s)
+               : launch_vq_seg_t<double>(T, nt, blocks_total, max_K, max_N, nt_mfma, blocks_mfma, mfma_nct, mfma_lds,
+                                         s);
+}
+
+int launch_basis_combine(int n, int K, const void *V, bool v32, const double *q, double *out, hipStream_t s)
+{
+    if (n <= 0) return HL_OK;
+    if (K <= 0 || K > 128) {
+        set_error("basis combination: K=%d unsupported (1..128)", K);
+        return HL_ERR_ARG;
+    }
+    const int blocks = grid_for(n, kBlock, 2048);
+    if (v32)
+        basis_combine_kernel<float><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const float *>(V), q, out);
+    else
+        basis_combine_kernel<double><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const double *>(V), q, out);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

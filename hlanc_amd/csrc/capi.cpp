@@ -65,7 +65,7 @@ bool is_device_ptr(const void *p, int dev)
 // Argument rules of ds_solver (HLanc.hpp:1649-1660) plus this library's
 // limits (ncv <= 128: the CGS kernels' column capacity).
 bool solve_args_ok(int n, int nev, int ncv, int ldv, const double *values, bool need_vectors,
-                   const double *vectors, const hl_options &o, const char *who)
+                   const double *vectors, const hl_options &o, bool device_op, const char *who)
 {
     if (n <= 0 || nev <= 0 || ncv <= nev || ncv > n || ncv > 128 || !values ||
         (need_vectors && !vectors) || (vectors && ldv < n)) {
@@ -73,8 +73,12 @@ bool solve_args_ok(int n, int nev, int ncv, int ldv, const double *values, bool 
                   ncv, ldv);
         return false;
     }
-    if (o.basis_prec != HL_PREC_F64) {
-        set_error("%s: this entry point takes the FP64 basis (basis_prec = HL_PREC_F64)", who);
+    if (o.basis_prec != HL_PREC_F64 && o.basis_prec != HL_PREC_F32) {
+        set_error("%s: basis_prec must be HL_PREC_F64 or HL_PREC_F32", who);
+        return false;
+    }
+    if (o.basis_prec == HL_PREC_F32 && !device_op) {
+        set_error("%s: the mixed-precision solve (basis_prec = HL_PREC_F32) needs a device CSR operator", who);
         return false;
     }
     return true;
@@ -90,7 +94,7 @@ int solve_common(hl::Member &m, int dev, int n, int nev, int ncv, double *values
     hl_info inf;
     std::memset(&inf, 0, sizeof(inf));
     if (info) *info = inf;
-    if (!solve_args_ok(n, nev, ncv, ldv, values, true, vectors, o, "ds_solver")) return HL_ERR_ARG;
+    if (!solve_args_ok(n, nev, ncv, ldv, values, true, vectors, o, m.A != nullptr, "ds_solver")) return HL_ERR_ARG;
     int rc = check_device(dev);
     if (rc) return rc;
     DeviceGuard g(dev);
@@ -582,7 +586,7 @@ int hl_ds_solve_batch(int dev, hl_solve_item *items, int count, int max_active, 
             set_error("hl_ds_solve_batch: item %d needs a square operator on device %d", i, dev);
             continue;
         }
-        if (!solve_args_ok(p->nrows, it.nev, it.ncv, it.ldv, it.values, false, it.vectors, it.opts,
+        if (!solve_args_ok(p->nrows, it.nev, it.ncv, it.ldv, it.values, false, it.vectors, it.opts, true,
                            "hl_ds_solve_batch") ||
             it.arrival < last_arrival) {
             if (it.arrival < last_arrival) set_error("hl_ds_solve_batch: arrivals must not decrease (item %d)", i);

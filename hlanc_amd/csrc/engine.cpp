@@ -20,6 +20,7 @@
 // results are bitwise the same alone and in any batch.  Return values and the
 // default tol / restart cap are the reference's.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -44,13 +45,13 @@ double now_s()
 
 // ------------------------------------------------------------------ Member
 
-double Member::cgs_bytes_pass(int m) const { return 8.0 * n * (m + 2); }
+double Member::cgs_bytes_pass(int m) const { return vbytes() * n * m + 16.0 * n; }
 
-int Member::start(hipStream_t s)
+int Member::reset(hipStream_t s)
 {
     Workspace &W = ws;
-    t0 = now_s();
     tol = opts.tol > 0.0 ? opts.tol : kEps;
+    if (kind != SEG_F64) tol = std::max(tol, kF32PhaseTol);
     maxiter = opts.maxiter > 0 ? opts.maxiter : 200;
     sub.assign(ncv, 0.0);
     dia.assign(ncv, 0.0);
@@ -63,7 +64,6 @@ int Member::start(hipStream_t s)
     ritz_saved.clear();
     bounds_saved.clear();
     spmv = cgs = p2 = p3 = m2sum = m3sum = d_m23 = 0;
-    graph_sweeps = 0;
     nconv = 0;
     rnorm = 0.0;
     ret = HL_NOT_CONVERGED;
@@ -74,12 +74,13 @@ int Member::start(hipStream_t s)
     HL_TRY_HIP(hipMemcpyAsync(W.ctl, W.ctl_h, sizeof(Ctl), hipMemcpyHostToDevice, s));
     HL_TRY_HIP(hipMemsetAsync(W.hsub, 0, sizeof(double) * ncv, s));
     HL_TRY_HIP(hipMemsetAsync(W.hdia, 0, sizeof(double) * ncv, s));
+    return HL_OK;
+}
+
+int Member::begin(hipStream_t s)
+{
+    Workspace &W = ws;
     int rc;
-    if (opts.v0) {
-        HL_TRY_HIP(hipMemcpyAsync(W.r, opts.v0, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    } else if ((rc = launch_start_vector(n, opts.seed, W.r, s))) { // dgetv0 (HLanc.hpp:1368-1374)
-        return rc;
-    }
     if ((rc = launch_norm(n, W.r, W.red, W.ctl, EPI_RNORM, s))) return rc;
     nev = nev0;
     np = ncv - nev0;
@@ -89,6 +90,38 @@ int Member::start(hipStream_t s)
     sweep_np = nev0; // dsaitr(0, nev0) (HLanc.hpp:1680)
     phase = ACTIVE;
     return HL_OK;
+}
+
+int Member::start(hipStream_t s)
+{
+    Workspace &W = ws;
+    t0 = now_s();
+    mixed = opts.basis_prec == HL_PREC_F32 && A != nullptr;
+    kind = !mixed ? SEG_F64 : (A->val32 ? SEG_V32_A32 : SEG_V32);
+    spmv_f32 = 0;
+    restarts_f32 = 0;
+    vals_f32.clear();
+    graph_sweeps = 0;
+    int rc;
+    if ((rc = reset(s))) return rc;
+    if (opts.v0) {
+        HL_TRY_HIP(hipMemcpyAsync(W.r, opts.v0, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    } else if ((rc = launch_start_vector(n, opts.seed, W.r, s))) { // dgetv0 (HLanc.hpp:1368-1374)
+        return rc;
+    }
+    return begin(s);
+}
+
+// End of the FP32 phase: the engine has formed ws.r = V q (rephase_host's q);
+// the solve starts over in FP64 from it.
+int Member::rephase(hipStream_t s)
+{
+    spmv_f32 = spmv;
+    restarts_f32 = restarts;
+    kind = SEG_F64;
+    int rc;
+    if ((rc = reset(s))) return rc;
+    return begin(s);
 }
 
 // STEP 3 of dsaitr for a host operator: v_j = resid * xscale on the device,
@@ -161,7 +194,10 @@ void Member::decide(double *qslot)
         }
     }
     if (nconv >= nev0 || np == 0) {
-        finish_host(qslot, true);
+        if (kind != SEG_F64)
+            rephase_host(qslot);
+        else
+            finish_host(qslot, true);
         return;
     }
     const int nevbef = nev; // HLanc.hpp:1719-1730
@@ -184,7 +220,10 @@ void Member::decide(double *qslot)
     ++iter;
     restarts = iter;
     if (iter >= maxiter) { // the restart would feed no further sweep: the solve ends unconverged
-        finish_host(qslot, true);
+        if (kind != SEG_F64) // (the FP32 phase hands over what it has instead)
+            rephase_host(qslot);
+        else
+            finish_host(qslot, true);
         return;
     }
     sweep_k = nev;
@@ -251,18 +290,44 @@ void Member::finish_host(double *qslot, bool vectors_ok)
     ret = HL_OK;
 }
 
+// The FP64 phase's start vector: the sum of the FP32 phase's nev0 wanted
+// (largest algebraic) Ritz vectors, V z_i for the eigenvectors z_i of its
+// current T; qslot receives q = sum z_i (ncv entries).
+void Member::rephase_host(double *qslot)
+{
+    act = ACT_REPHASE;
+    std::vector<double> d(dia), b(ncv, 0.0), z((size_t)ncv * ncv);
+    for (int i = 0; i + 1 < ncv; ++i) b[i] = sub[i + 1];
+    std::fill(qslot, qslot + ncv, 0.0);
+    vals_f32.clear();
+    if (tridiag_eig(ncv, d.data(), b.data(), z.data(), ncv)) { // keep the last Lanczos vector's direction
+        qslot[ncv - 1] = 1.0;
+        return;
+    }
+    for (int c = ncv - nev0; c < ncv; ++c) { // ascending eigenvalues: the last nev0 are wanted
+        vals_f32.push_back(d[c]);
+        for (int i = 0; i < ncv; ++i) qslot[i] += z[(size_t)c * ncv + i];
+    }
+}
+
 void Member::complete()
 {
     if (ret == HL_OK) std::copy(vals.begin(), vals.end(), values);
     info.ret = ret;
-    info.restarts = restarts;
+    info.restarts = restarts_f32 + restarts;
     info.nconv = nconv;
-    info.spmv = spmv;
+    info.spmv = spmv_f32 + spmv;
     info.cgs_passes = cgs;
     info.rnorm = rnorm;
     info.seconds = now_s() - t0;
     info.max_rel_resid = -1.0;
     info.refine_shift = 0.0;
+    if (ret == HL_OK && mixed) // how far the FP32 phase's values were from the result
+        for (size_t i = 0; i < vals_f32.size() && i < vals.size(); ++i)
+            info.refine_shift = std::max(info.refine_shift, std::fabs(vals_f32[i] - vals[i]) /
+                                                                std::max(std::fabs(vals[i]), 1e-300));
+    info.spmv_f32 = spmv_f32;
+    info.restarts_f32 = restarts_f32;
     info.graph_sweeps = graph_sweeps;
     phase = DONE;
 }
@@ -476,6 +541,7 @@ int Engine::sweep(std::vector<Member *> &ms)
         M.rp = m.A ? m.A->row_ptr : nullptr;
         M.col = m.A ? m.A->col : nullptr;
         M.val = m.A ? m.A->val : nullptr;
+        M.val32 = m.A ? m.A->val32 : nullptr;
         M.chunk_part = W.chunk_part;
         M.chunk_cnt = W.chunk_cnt;
         M.V = W.V[W.cur];
@@ -492,25 +558,38 @@ int Engine::sweep(std::vector<Member *> &ms)
         M.nb = cgs_blocks(m.n);
         M.ncv = m.ncv;
     }
-    std::vector<int> off(maxsteps + 1, 0), nspmv(maxsteps, 0), ncgs(maxsteps, 0), nfin(maxsteps, 0),
-        maxcpg(maxsteps, 0);
+    // every step index's entries grouped by precision kind: one SpMV launch per
+    // kind, one CGS partials launch per basis type (kind 0; kinds 1-2), one
+    // finalize launch over all
+    std::vector<std::array<int, SEG_NKIND + 1>> koff(maxsteps);
+    std::vector<std::array<int, SEG_NKIND>> nspmv(maxsteps);
+    std::vector<std::array<int, 2>> ncgs(maxsteps), maxcpg(maxsteps);
+    std::vector<int> off(maxsteps + 1, 0), nfin(maxsteps, 0);
     size_t ns = 0;
     for (int st = 0; st < maxsteps; ++st) {
         off[st] = static_cast<int>(ns);
-        for (size_t i = 0; i < nm; ++i) {
-            Member &m = *act[i];
-            if (st >= m.sweep_np) continue;
-            SegStep &S = hS[ns++];
-            S.mem = static_cast<int>(i);
-            S.j = m.sweep_k + st;
-            S.spmv0 = nspmv[st];
-            S.cgs0 = ncgs[st];
-            S.fin0 = nfin[st];
-            if (m.A) nspmv[st] += m.A->nblk;
-            ncgs[st] += hM[i].nb;
-            nfin[st] += S.j + 2; // j + 1 projections and the squared norm
-            maxcpg[st] = std::max(maxcpg[st], (S.j + 1 + dev::kGroups - 1) / dev::kGroups);
+        ncgs[st] = {0, 0};
+        maxcpg[st] = {0, 0};
+        for (int k = 0; k < SEG_NKIND; ++k) {
+            koff[st][k] = static_cast<int>(ns);
+            nspmv[st][k] = 0;
+            const int g = k == SEG_F64 ? 0 : 1;
+            for (size_t i = 0; i < nm; ++i) {
+                Member &m = *act[i];
+                if (st >= m.sweep_np || m.kind != k) continue;
+                SegStep &S = hS[ns++];
+                S.mem = static_cast<int>(i);
+                S.j = m.sweep_k + st;
+                S.spmv0 = nspmv[st][k];
+                S.cgs0 = ncgs[st][g];
+                S.fin0 = nfin[st];
+                if (m.A) nspmv[st][k] += m.A->nblk;
+                ncgs[st][g] += hM[i].nb;
+                nfin[st] += S.j + 2; // j + 1 projections and the squared norm
+                maxcpg[st][g] = std::max(maxcpg[st][g], (S.j + 1 + dev::kGroups - 1) / dev::kGroups);
+            }
         }
+        koff[st][SEG_NKIND] = static_cast<int>(ns);
     }
     off[maxsteps] = static_cast<int>(ns);
     HL_TRY_HIP(hipMemcpyAsync(dM, hM, sizeof(SegMember) * nm, hipMemcpyHostToDevice, stream));
@@ -529,13 +608,19 @@ int Engine::sweep(std::vector<Member *> &ms)
             for (int q = off[st]; q < off[st + 1]; ++q) {
                 Member &m = *act[hS[q].mem];
                 if (!m.A && (r = m.host_apply(hS[q].j, stream))) return r;
-                if (m.A) bytes += 12.0 * m.A->nnz + 4.0 * (m.A->nrows + 1) + 8.0 * m.A->ncols + 16.0 * m.A->nrows;
+                if (m.A)
+                    bytes += (m.kind == SEG_V32_A32 ? 8.0 : 12.0) * m.A->nnz + 4.0 * (m.A->nrows + 1) +
+                             8.0 * m.A->ncols + (8.0 + m.vbytes()) * m.A->nrows;
             }
-            if (nspmv[st] > 0) {
+            if (nspmv[st][0] + nspmv[st][1] + nspmv[st][2] > 0) {
                 if ((r = prof.begin(stream))) return r;
-                if ((r = launch_spmv_seg(dM, S, cnt, nspmv[st], stream))) return r;
+                for (int k = 0; k < SEG_NKIND; ++k)
+                    if (nspmv[st][k] > 0 && (r = launch_spmv_seg(dM, dS + koff[st][k], koff[st][k + 1] - koff[st][k],
+                                                                 nspmv[st][k], k, stream)))
+                        return r;
                 if ((r = prof.end(stream, K_SPMV, bytes))) return r;
             }
+            const int g0[2] = {koff[st][0], koff[st][1]}, g1[2] = {koff[st][1], koff[st][SEG_NKIND]};
             for (int p = 0; p < 4; ++p) {
                 bytes = 0.0;
                 if (p < 2) // executed DGKS corrections are counted from the device counters
@@ -544,7 +629,10 @@ int Engine::sweep(std::vector<Member *> &ms)
                         bytes += m.cgs_bytes_pass(hS[q].j + 1) - (p == 0 ? 8.0 * m.n : 0.0);
                     }
                 if ((r = prof.begin(stream))) return r;
-                if ((r = launch_cgs_seg(dM, S, cnt, maxcpg[st], ncgs[st], modes[p], stream))) return r;
+                for (int g = 0; g < 2; ++g)
+                    if (ncgs[st][g] > 0 && (r = launch_cgs_seg(dM, dS + g0[g], g1[g] - g0[g], maxcpg[st][g],
+                                                               ncgs[st][g], modes[p], g == 1, stream)))
+                        return r;
                 if ((r = prof.end(stream, p < 2 ? K_CGS : K_DGKS, bytes))) return r;
                 if ((r = prof.begin(stream))) return r;
                 if ((r = launch_cgs_seg_finalize(dM, S, cnt, nfin[st], modes[p], stream))) return r;
@@ -558,16 +646,17 @@ int Engine::sweep(std::vector<Member *> &ms)
     } else {
         // the sweep's shape: table addresses and every step's segment and grid sizes
         std::vector<long long> key;
-        key.reserve(2 + 6 * (size_t)maxsteps);
+        key.reserve(2 + 12 * (size_t)maxsteps);
         key.push_back(reinterpret_cast<long long>(dM));
         key.push_back(reinterpret_cast<long long>(dS));
         for (int st = 0; st < maxsteps; ++st) {
-            key.push_back(off[st]);
-            key.push_back(off[st + 1] - off[st]);
-            key.push_back(nspmv[st]);
-            key.push_back(ncgs[st]);
+            for (int k = 0; k <= SEG_NKIND; ++k) key.push_back(koff[st][k]);
+            for (int k = 0; k < SEG_NKIND; ++k) key.push_back(nspmv[st][k]);
+            for (int g = 0; g < 2; ++g) {
+                key.push_back(ncgs[st][g]);
+                key.push_back(maxcpg[st][g]);
+            }
             key.push_back(nfin[st]);
-            key.push_back(maxcpg[st]);
         }
         ++gclock;
         GraphEntry *hit = nullptr;
@@ -619,7 +708,7 @@ int Engine::sweep(std::vector<Member *> &ms)
     if (prof.on && (rc = prof.flush())) return rc;
     for (size_t i = 0; i < nm; ++i) {
         act[i]->absorb(hRB + i * stride);
-        if (prof.on) prof.tot.bytes[K_DGKS] += 8.0 * act[i]->n * (double)act[i]->d_m23;
+        if (prof.on) prof.tot.bytes[K_DGKS] += act[i]->vbytes() * act[i]->n * (double)act[i]->d_m23;
     }
     rc = after_sweep(act);
     if (prof.on) {
@@ -640,62 +729,74 @@ int Engine::after_sweep(std::vector<Member *> &act)
     const size_t qstride = (size_t)ncvmax * ncvmax;
     pool->run(static_cast<int>(nm), [&](int i) { act[i]->decide(hQ + (size_t)i * qstride); });
 
-    // FMA-kernel restarts first, then MFMA-kernel ones (each member's kernel
-    // follows from its own options and shape alone, so a solve gives the same
-    // bits in any batch)
-    int nfinish = 0, nres = 0;
+    // Restart entries in four runs: FP64 bases on the FMA kernel, FP64 on MFMA,
+    // FP32 on FMA, FP32 on MFMA.  Each member's run follows from its own
+    // options, phase and shape alone, so a solve gives the same bits in any
+    // batch; each basis type is one launch_vq_seg call.
+    int nfinish = 0, nrephase = 0;
+    std::vector<int> cat[4];
     for (size_t i = 0; i < nm; ++i) {
-        nfinish += act[i]->act == Member::ACT_FINISH;
-        nres += act[i]->act == Member::ACT_RESTART;
-    }
-    int nv = 0, nmf = 0, vblocks = 0, mblocks = 0, maxK = 0, maxN = 0, mnct = 0;
-    size_t mlds = 0;
-    double bytes = 0.0;
-    for (size_t i = 0; i < nm; ++i) {
-        Member &m = *act[i];
+        const Member &m = *act[i];
+        nfinish += m.act == Member::ACT_FINISH;
+        nrephase += m.act == Member::ACT_REPHASE;
         if (m.act != Member::ACT_RESTART) continue;
         const bool mf = m.opts.use_mfma && vq_seg_mfma_fits(m.ncv, m.vq_N);
-        Workspace &W = m.ws;
-        SegVQ &t = mf ? hV[nres - 1 - nmf++] : hV[nv++];
-        t.V = W.V[W.cur];
-        t.Out = W.V[1 - W.cur];
-        t.Q = dQ + i * qstride;
-        t.r = W.r;
-        t.ctl = W.ctl;
-        t.partials = W.red.partials;
-        t.qlast = m.vq_qlast;
-        t.beta = m.vq_beta;
-        t.n = m.n;
-        t.K = m.ncv;
-        t.N = m.vq_N;
-        t.res_col = m.vq_res;
-        t.nb = vq_seg_blocks(m.n);
-        if (mf) {
-            t.blk0 = mblocks;
-            mblocks += t.nb;
-            const int nct = (t.N + 15) / 16;
-            mnct = std::max(mnct, nct);
-            mlds = std::max(mlds, sizeof(double) * (size_t)((t.K + 3) & ~3) * nct * 16);
-        } else {
-            t.blk0 = vblocks;
-            vblocks += t.nb;
-            maxK = std::max(maxK, t.K);
-            maxN = std::max(maxN, t.N);
-        }
-        bytes += 8.0 * m.n * (t.K + t.N + 2);
+        cat[(m.kind != SEG_F64 ? 2 : 0) + (mf ? 1 : 0)].push_back(static_cast<int>(i));
     }
-    // the MFMA entries were filled from the back: restore their member order
-    std::reverse(hV + nv, hV + nres);
+    int nres = 0, nblk[4] = {0, 0, 0, 0}, maxK[4] = {0, 0, 0, 0}, maxN[4] = {0, 0, 0, 0}, mnct[4] = {0, 0, 0, 0};
+    size_t mlds[4] = {0, 0, 0, 0};
+    double bytes = 0.0;
+    for (int c = 0; c < 4; ++c)
+        for (int i : cat[c]) {
+            Member &m = *act[i];
+            Workspace &W = m.ws;
+            SegVQ &t = hV[nres++];
+            t.V = W.V[W.cur];
+            t.Out = W.V[1 - W.cur];
+            t.Q = dQ + i * qstride;
+            t.r = W.r;
+            t.ctl = W.ctl;
+            t.partials = W.red.partials;
+            t.qlast = m.vq_qlast;
+            t.beta = m.vq_beta;
+            t.n = m.n;
+            t.K = m.ncv;
+            t.N = m.vq_N;
+            t.res_col = m.vq_res;
+            t.nb = vq_seg_blocks(m.n);
+            t.blk0 = nblk[c];
+            nblk[c] += t.nb;
+            maxK[c] = std::max(maxK[c], t.K);
+            maxN[c] = std::max(maxN[c], t.N);
+            const int nct = (t.N + 15) / 16;
+            mnct[c] = std::max(mnct[c], nct);
+            mlds[c] = std::max(mlds[c], sizeof(double) * (size_t)((t.K + 3) & ~3) * nct * 16);
+            bytes += m.vbytes() * m.n * (t.K + t.N) + 16.0 * m.n;
+        }
     int rc;
-    if (nres > 0 || nfinish > 0)
+    if (nres > 0 || nfinish > 0 || nrephase > 0)
         HL_TRY_HIP(hipMemcpyAsync(dQ, hQ, sizeof(double) * nm * qstride, hipMemcpyHostToDevice, stream));
     if (nres > 0) {
         HL_TRY_HIP(hipMemcpyAsync(dV, hV, sizeof(SegVQ) * nres, hipMemcpyHostToDevice, stream));
         if ((rc = prof.begin(stream))) return rc;
-        if ((rc = launch_vq_seg(dV, nv, vblocks, maxK, maxN, nmf, mblocks, mnct, mlds, stream))) return rc;
+        const int n01 = static_cast<int>(cat[0].size() + cat[1].size());
+        if (n01 > 0 && (rc = launch_vq_seg(dV, static_cast<int>(cat[0].size()), nblk[0], maxK[0], maxN[0],
+                                           static_cast<int>(cat[1].size()), nblk[1], mnct[1], mlds[1], false, stream)))
+            return rc;
+        if (nres > n01 && (rc = launch_vq_seg(dV + n01, static_cast<int>(cat[2].size()), nblk[2], maxK[2], maxN[2],
+                                              static_cast<int>(cat[3].size()), nblk[3], mnct[3], mlds[3], true,
+                                              stream)))
+            return rc;
         if ((rc = prof.end(stream, K_VQ, bytes))) return rc;
         for (Member *m : act)
             if (m->act == Member::ACT_RESTART) m->ws.cur = 1 - m->ws.cur;
+    }
+    for (size_t i = 0; i < nm && nrephase > 0; ++i) { // FP32 phase done: FP64 start vector V q, then restart
+        Member &m = *act[i];
+        if (m.act != Member::ACT_REPHASE) continue;
+        Workspace &W = m.ws;
+        if ((rc = launch_basis_combine(m.n, m.ncv, W.V[W.cur], true, dQ + i * qstride, W.r, stream))) return rc;
+        if ((rc = m.rephase(stream))) return rc;
     }
     if (nfinish == 0) return HL_OK;
     Pending fin{nullptr, {}};

@@ -151,13 +151,22 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
 // grid, and every block computes exactly what it computes when the solve runs
 // alone (same block count, same partial-sum order), so results are bitwise the
 // same alone and in a batch.
+// Precision kinds of a segment: each segmented launch serves solves of one
+// kind (the engine groups a step's solves by kind).
+enum SegKind {
+    SEG_F64 = 0,     // FP64 basis and values
+    SEG_V32 = 1,     // FP32 basis, FP64 values
+    SEG_V32_A32 = 2, // FP32 basis and values
+    SEG_NKIND = 3
+};
 struct SegMember {               // per solve, fixed during a sweep
     const SpmvBlk *blks;         // CSR operator (null for host operators)
     const int *rp, *col;
     const double *val;
+    const float *val32;          // FP32 values (SEG_V32_A32 solves)
     double *chunk_part;
     unsigned *chunk_cnt;
-    double *V;                   // tiled basis in use
+    void *V;                     // tiled basis in use (double, or float for SEG_V32*)
     long long vld;
     double *r, *w, *h0, *h1, *hsub, *hdia, *partials;
     Ctl *ctl;
@@ -174,15 +183,15 @@ struct SegStep {                 // per solve and Lanczos step
 // Partials kernel of CGS pass `mode` (EPI_WNORM .. EPI_PASS3) for every listed
 // step, then the per-column finalize kernel (j + 2 blocks per step).
 int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total, int mode,
-                   hipStream_t s);
+                   bool v32, hipStream_t s);
 int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fin_blocks_total, int mode,
                             hipStream_t s);
-int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s);
+int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, hipStream_t s);
 
 // Restart V.Q of one solve within a segmented launch (tiled bases, ld K).
 struct SegVQ {
-    const double *V;             // input basis
-    double *Out;                 // output basis (the other ping-pong buffer)
+    const void *V;               // input basis (double, or float for FP32 bases)
+    void *Out;                   // output basis (the other ping-pong buffer)
     const double *Q;             // K x N, column-major, ld K
     double *r;                   // residual, updated in place
     Ctl *ctl;                    // rnorm / xscale for the next sweep
@@ -201,7 +210,9 @@ bool vq_seg_mfma_fits(int K, int N);
 // mfma_nct 16-column tiles and mfma_lds bytes of Q^T each), then one
 // finalize over all entries.
 int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, int nt_mfma, int blocks_mfma,
-                  int mfma_nct, size_t mfma_lds, hipStream_t s);
+                  int mfma_nct, size_t mfma_lds, bool v32, hipStream_t s);
+// out = V(:, 0:K) q (tiled basis, ld K, FP64 or FP32; q on the device).
+int launch_basis_combine(int n, int K, const void *V, bool v32, const double *q, double *out, hipStream_t s);
 // Packs hsub, hdia (ncv each) and the Ctl words of every member at
 // out + member * stride (stride >= 2 ncv + kCtlWords).
 int launch_seg_readback(const SegMember *M, int nm, int stride, double *out, hipStream_t s);

@@ -64,7 +64,16 @@ struct Member {
     Workspace ws;
     // host state
     enum Phase { NEW, ACTIVE, FINISHING, DONE } phase = NEW; // FINISHING: Ritz vectors in flight
-    enum Act { ACT_NONE, ACT_SWEEP, ACT_RESTART, ACT_FINISH } act = ACT_NONE;
+    enum Act { ACT_NONE, ACT_SWEEP, ACT_RESTART, ACT_FINISH, ACT_REPHASE } act = ACT_NONE;
+    // Mixed precision (opts.basis_prec == HL_PREC_F32): a first phase keeps V
+    // in FP32 (and reads FP32 values when the operator has them) and stops at
+    // kF32PhaseTol; the solve then restarts in FP64 from the sum of that
+    // phase's wanted Ritz vectors (ACT_REPHASE) and ends as an FP64 solve.
+    int kind = SEG_F64;                  // SegKind of the current phase
+    bool mixed = false;
+    long long spmv_f32 = 0;              // FP32-phase counters
+    int restarts_f32 = 0;
+    std::vector<double> vals_f32;        // wanted Ritz values at the end of the FP32 phase
     int stage = 0;                       // 0: initial sweep pending, 1: main loop
     int sweep_k = 0, sweep_np = 0;
     int nev = 0, np = 0, nconv = 0, iter = 0, restarts = 0, maxiter = 200;
@@ -82,6 +91,8 @@ struct Member {
     std::vector<double> sub, dia, ritz, bounds, zlast, q, e, work, ritz_saved, bounds_saved, vals;
 
     int start(hipStream_t s);            // start vector, norm, first sweep [0, nev0)
+    int rephase(hipStream_t s);          // FP64 phase from the start vector already in ws.r
+    double vbytes() const { return kind == SEG_F64 ? 8.0 : 4.0; } // bytes per basis entry
     void absorb(const double *rb);       // hsub, hdia (ncv each) and the Ctl words
     // Host-only (thread-safe across members): restart logic; writes the restart
     // Q (ncv x vq_N) or the Ritz-vector Q (ncv x ncv) to qslot.
@@ -91,8 +102,15 @@ struct Member {
     void complete();                     // outputs and info once the engine synchronised
 
   private:
+    int reset(hipStream_t s);            // host state, Ctl and T for a new phase
+    int begin(hipStream_t s);            // norm of ws.r, first sweep
     void finish_host(double *qslot, bool vectors_ok);
+    void rephase_host(double *qslot);
 };
+
+// Relative Ritz-estimate tolerance of the FP32-basis phase of a mixed solve:
+// above the ~6e-8 relative rounding of an FP32 basis, so that the phase ends.
+constexpr double kF32PhaseTol = 1e-6;
 
 // Runs members to completion, all active members advancing in lockstep: one
 // segmented SpMV launch and four segmented CGS passes per Lanczos step serve

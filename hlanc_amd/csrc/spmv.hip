@@ -40,12 +40,13 @@ __device__ inline long long vcol_pos(long long r, long long ld, int j, bool tile
     return tiled ? vpos<true>(r, j, ld) : vpos<false>(r, j, ld);
 }
 
-// One work item of the SpMV (one block).
-template <typename AT, typename XT>
+// One work item of the SpMV (one block).  AT: matrix values, XT: x, VT: the
+// basis column written alongside (vout).
+template <typename AT, typename XT, typename VT>
 __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict__ rp, const int *__restrict__ col,
                                           const AT *__restrict__ val, const XT *__restrict__ x,
                                           const double *__restrict__ xscale, double *__restrict__ y,
-                                          XT *__restrict__ vout, long long vld, int vj, bool vtiled,
+                                          VT *__restrict__ vout, long long vld, int vj, bool vtiled,
                                           double *__restrict__ chunk_part, unsigned *__restrict__ chunk_cnt)
 {
     __shared__ double prod[kNnzPerBlock];
@@ -80,7 +81,7 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
         }
         if (vout) {
             for (int r = b.row0 + tid; r < b.row1; r += kBlock)
-                vout[vcol_pos(r, vld, vj, vtiled)] = static_cast<XT>(static_cast<double>(x[r]) * s);
+                vout[vcol_pos(r, vld, vj, vtiled)] = static_cast<VT>(static_cast<double>(x[r]) * s);
         }
         __syncthreads();
         const int tpr = b.tpr;
@@ -110,7 +111,7 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
     const double tot = block_sum_to_thread0(acc, wred);
     if (tid != 0) return;
     if (vout && b.chunk == 0)
-        vout[vcol_pos(b.row0, vld, vj, vtiled)] = static_cast<XT>(static_cast<double>(x[b.row0]) * s);
+        vout[vcol_pos(b.row0, vld, vj, vtiled)] = static_cast<VT>(static_cast<double>(x[b.row0]) * s);
     if (b.nchunks == 1) {
         y[b.row0] = tot;
         return;
@@ -134,7 +135,8 @@ __global__ __launch_bounds__(kBlock) void spmv_adaptive(
     double *__restrict__ y, XT *__restrict__ vout, long long vld, int vj, bool vtiled,
     double *__restrict__ chunk_part, unsigned *__restrict__ chunk_cnt)
 {
-    spmv_item<AT, XT>(blks[blockIdx.x], rp, col, val, x, xscale, y, vout, vld, vj, vtiled, chunk_part, chunk_cnt);
+    spmv_item<AT, XT, XT>(blks[blockIdx.x], rp, col, val, x, xscale, y, vout, vld, vj, vtiled, chunk_part,
+                          chunk_cnt);
 }
 
 __device__ __forceinline__ int seg_find_spmv(const SegStep *S, int ns, int b)
@@ -150,15 +152,22 @@ __device__ __forceinline__ int seg_find_spmv(const SegStep *S, int ns, int b)
     return lo;
 }
 
+template <typename AT> __device__ __forceinline__ const AT *seg_values(const SegMember &mb);
+template <> __device__ __forceinline__ const double *seg_values<double>(const SegMember &mb) { return mb.val; }
+template <> __device__ __forceinline__ const float *seg_values<float>(const SegMember &mb) { return mb.val32; }
+
 // The SpMV step of every listed solve in one launch: y = A (r s), V(:, j) = r s.
+// AT: the listed solves' matrix values, VT: their basis.
+template <typename AT, typename VT>
 __global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__restrict__ M,
                                                           const SegStep *__restrict__ S, int ns)
 {
     const int i = seg_find_spmv(S, ns, blockIdx.x);
     const SegStep st = S[i];
     const SegMember mb = M[st.mem];
-    spmv_item<double, double>(mb.blks[blockIdx.x - st.spmv0], mb.rp, mb.col, mb.val, mb.r, &mb.ctl->xscale, mb.w,
-                              mb.V, mb.vld, st.j, true, mb.chunk_part, mb.chunk_cnt);
+    spmv_item<AT, double, VT>(mb.blks[blockIdx.x - st.spmv0], mb.rp, mb.col, seg_values<AT>(mb), mb.r,
+                              &mb.ctl->xscale, mb.w, static_cast<VT *>(mb.V), mb.vld, st.j, true, mb.chunk_part,
+                              mb.chunk_cnt);
 }
 
 template <typename T>
@@ -352,10 +361,23 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
 
 namespace hl {
 
-int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s)
+int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, hipStream_t s)
 {
     if (ns <= 0 || blocks_total <= 0) return HL_OK;
-    spmv_seg_kernel<<<blocks_total, kBlock, 0, s>>>(M, S, ns);
+    switch (kind) {
+    case SEG_F64:
+        spmv_seg_kernel<double, double><<<blocks_total, kBlock, 0, s>>>(M, S, ns);
+        break;
+    case SEG_V32:
+        spmv_seg_kernel<double, float><<<blocks_total, kBlock, 0, s>>>(M, S, ns);
+        break;
+    case SEG_V32_A32:
+        spmv_seg_kernel<float, float><<<blocks_total, kBlock, 0, s>>>(M, S, ns);
+        break;
+    default:
+        set_error("SpMV: bad segment kind %d", kind);
+        return HL_ERR_ARG;
+    }
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

@@ -174,7 +174,12 @@ typedef struct {
     int maxiter;       /* restart cap; <= 0 -> 200 (HLanc.hpp:1682) */
     const double *v0;  /* host start vector of length n, or NULL: seeded generator */
     uint64_t seed;     /* generator seed when v0 == NULL */
-    int basis_prec;    /* HL_PREC_F64 (reference semantics) or HL_PREC_F32 (mixed) */
+    int basis_prec;    /* HL_PREC_F64 (reference semantics), or HL_PREC_F32: mixed precision
+                          for device CSR operators -- a first phase keeps the basis in FP32
+                          (and uses the operator's FP32 values when it was created with them)
+                          to a relative Ritz estimate of 1e-6, then the solve restarts in FP64
+                          from the sum of that phase's wanted Ritz vectors and returns FP64
+                          results to tol */
     int use_mfma;      /* 1 (default): restart and Ritz-vector V.Q on v_mfma_f64_16x16x4f64
                           (a restart whose K x N block of Q exceeds 48 KB runs on the FMA
                           kernel); 0: FMA kernel.  The results differ in rounding only. */
@@ -192,7 +197,13 @@ typedef struct {
     double rnorm;           /* final residual norm */
     double seconds;         /* wall time of the call */
     double max_rel_resid;   /* max ||A v - l v|| / ||A||_est over returned pairs (csr path) */
-    double refine_shift;    /* mixed precision: largest Rayleigh-quotient change */
+    double refine_shift;    /* mixed precision: largest relative change of a returned value
+                               from the FP32-basis phase to the FP64 result */
+    long long spmv_f32;     /* mixed precision: operator applications of the FP32-basis phase
+                               (included in spmv) */
+    int restarts_f32;       /* mixed precision: restarts of the FP32-basis phase (included in
+                               restarts) */
+    int reserved_info;
 } hl_info;
 
 /* hl_options.flags bits. */
@@ -239,7 +250,7 @@ typedef struct {
                             NULL: values only (the Ritz-vector GEMM is skipped) */
     int ldv;
     int pad0;
-    hl_options opts;     /* basis_prec must be HL_PREC_F64 */
+    hl_options opts;     /* any basis_prec */
     double arrival;      /* seconds after the call starts; the solve is admitted
                             no earlier (0: at once); must not decrease along the
                             array */
