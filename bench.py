@@ -47,6 +47,9 @@ WORKLOADS = {
 }
 
 
+BASIS_LABEL = {"fp64": "fp64", "fp32": "mixed: fp32 basis and values to 1e-6, then fp64 to tol"}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -155,7 +158,9 @@ def run_batch(args, H, torch):
     nev, ncv, count = 8, 24, args.count
     t0 = time.perf_counter()
     mix = W.batch_mix(count, seed=0)
-    ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=m[1][0], col=m[1][1], val=m[1][2])) for m in mix]
+    mixed = args.basis == "fp32"
+    ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=m[1][0], col=m[1][1], val=m[1][2]), fp32_values=mixed)
+           for m in mix]
     ns = [op.nrows for op in ops]
     nnzs = [len(m[1][1]) for m in mix]
     nnz = sum(nnzs)
@@ -167,13 +172,15 @@ def run_batch(args, H, torch):
         f"{time.perf_counter() - t0:.1f} s")
     torch.cuda.init()
     kw = dict(tol=args.tol, maxiter=args.maxiter, use_mfma=args.mfma)
+    prec = H.HL_PREC_F32 if mixed else H.HL_PREC_F64
 
     vec_out = [H.pinned_empty(op.nrows, ncv) for op in ops]  # page-locked outputs, allocated once
 
     def batched():
-        return H.ds_solve_batch(ops, nev, ncv, max_active=args.max_active, vectors_out=vec_out, **kw)
+        return H.ds_solve_batch(ops, nev, ncv, max_active=args.max_active, vectors_out=vec_out, basis_prec=prec,
+                                **kw)
 
-    def plain():
+    def plain():  # the plain FP64 ds_solver_gpu path, one solve after another
         out = []
         for op, v in zip(ops, vec_out):
             out.append(H.ds_solver_gpu(op.nrows, nev, ncv, op, vectors_out=v, **kw))
@@ -199,7 +206,7 @@ def run_batch(args, H, torch):
     if args.profile:
         import ctypes
 
-        H.ds_solve_batch(ops, nev, ncv, max_active=args.max_active, flags=H.HL_FLAG_PROFILE, **kw)
+        H.ds_solve_batch(ops, nev, ncv, max_active=args.max_active, flags=H.HL_FLAG_PROFILE, basis_prec=prec, **kw)
         pr = H.Profile()
         H.lib().hl_profile_last(ctypes.byref(pr))
         classes = {H.PROFILE_CLASSES[k]: {"launches": int(pr.launches[k]), "seconds": pr.seconds[k],
@@ -207,13 +214,17 @@ def run_batch(args, H, torch):
                    for k in range(len(H.PROFILE_CLASSES)) if pr.launches[k]}
         log("profiled batch: " + ", ".join(f"{k} {v['launches']}x {v['seconds']:.3f}s" for k, v in classes.items()))
     fails = sum(r.status != 0 for r in rb)
+    worst = check_pairs(ops, rb, [nev] * count)  # before the plain run's vectors overwrite the buffers
     same = rp is not None and all(np.array_equal(a.values, b.values) and a.status == b.ret for a, b in zip(rb, rp))
-    worst = check_pairs(ops, rb, [nev] * count)
+    err64 = None
+    if rp is not None:  # each batched value against the plain FP64 solve's
+        err64 = max(float(np.max(np.abs(a.values - b.values) / np.abs(b.values))) for a, b in zip(rb, rp))
     lat = [r.t_done for r in rb]
     value = count / float(np.mean(walls_b))
     plain_value = count / walls_p[0]
     log(f"batched {value:.1f} solves/s vs plain {plain_value:.1f} solves/s ({value / plain_value:.2f}x); "
-        f"failures {fails}; bitwise same as plain: {same}; max rel residual {worst:.2e}")
+        f"failures {fails}; bitwise same as plain: {same}; max rel value error vs plain FP64 {err64}; "
+        f"max rel residual {worst:.2e}")
     restarts = [r.info.restarts for r in rb]
     cpu = None
     if not args.no_cpu:
@@ -223,11 +234,12 @@ def run_batch(args, H, torch):
         "metric": "eigensolves/s at rel.resid<=1e-10 (+ p50/p99 latency)",
         "value": value, "unit": "eigensolves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * float(np.mean(walls_b)), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
+        "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic (seeded generator)",
         "config": {"workload": f"{count} uneven matrices, n log-uniform [1e3, 2e5], stencils + Erdos-Renyi",
                    "sum_n": int(sum(ns)), "sum_nnz": int(nnz), "nev": nev, "ncv": ncv, "tol": args.tol,
-                   "max_active": args.max_active, "basis": "fp64"},
-        "plain_value": plain_value, "speedup_vs_plain": value / plain_value,
+                   "max_active": args.max_active, "basis": BASIS_LABEL[args.basis]},
+        "plain_value": plain_value, "plain_basis": "fp64", "speedup_vs_plain": value / plain_value,
+        "max_rel_value_err_vs_fp64": err64,
         "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
         "bitwise_same_as_plain": bool(same), "failures": int(fails), "max_rel_resid": worst,
         "restarts": {"min": int(min(restarts)), "median": float(np.median(restarts)), "max": int(max(restarts))},
@@ -241,7 +253,10 @@ def run_serve(args, H, torch):
 
     t0 = time.perf_counter()
     pool = W.serve_mix(args.pool, seed=0)
-    ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=m[1][0], col=m[1][1], val=m[1][2])) for m in pool]
+    mixed = args.basis == "fp32"
+    prec = H.HL_PREC_F32 if mixed else H.HL_PREC_F64
+    ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=m[1][0], col=m[1][1], val=m[1][2]), fp32_values=mixed)
+           for m in pool]
     nnz = sum(len(m[1][1]) for m in pool)
     del pool
     log(f"workload serve: pool of {len(ops)} matrices, sum n {sum(op.nrows for op in ops)}, sum nnz {nnz}, "
@@ -253,12 +268,14 @@ def run_serve(args, H, torch):
     pick = rng.integers(0, len(ops), R)
     nevs = rng.choice([4, 8, 16, 32], R)
     req_ops = [ops[k] for k in pick]
-    H.ds_solve_batch(req_ops[:2], list(nevs[:2]), list(3 * nevs[:2]), tol=args.tol, maxiter=args.maxiter)
+    H.ds_solve_batch(req_ops[:2], list(nevs[:2]), list(3 * nevs[:2]), tol=args.tol, maxiter=args.maxiter,
+                     basis_prec=prec)
 
     def serve(arrivals, max_active, budget):
         t = time.perf_counter()
         res = H.ds_solve_batch(req_ops, list(nevs), list(3 * nevs), max_active=max_active, hbm_budget=budget,
-                               tol=args.tol, maxiter=args.maxiter, arrivals=arrivals, vectors=True)
+                               tol=args.tol, maxiter=args.maxiter, arrivals=arrivals, vectors=True,
+                               basis_prec=prec)
         return res, time.perf_counter() - t
 
     rates = [float(x) for x in str(args.rates).split(",")]
@@ -290,11 +307,11 @@ def run_serve(args, H, torch):
         "metric": "eigensolves/s at rel.resid<=1e-10 (+ p50/p99 latency)",
         "value": top["value"], "unit": "eigensolves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": 1e3 * R / top["value"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
+        "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic (seeded generator)",
         "config": {"workload": f"Poisson stream, {R} requests, pool of {len(ops)} matrices n log-uniform "
                                "[4e3, 2e6], nev in {4,8,16,32}, ncv = 3 nev",
                    "offered_req_per_s": rates, "tol": args.tol, "max_active": args.max_active,
-                   "hbm_budget_bytes": args.hbm_budget, "basis": "fp64"},
+                   "hbm_budget_bytes": args.hbm_budget, "basis": BASIS_LABEL[args.basis]},
         "latency_s": top["latency_s"], "fcfs": top["fcfs"], "rates": per_rate,
         "same_results_as_fcfs": bool(same), "failures": int(fails), "max_rel_resid": worst,
     }
@@ -318,6 +335,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="batch: skip the one-at-a-time comparison")
     ap.add_argument("--mfma", type=int, choices=(0, 1), default=None,
                     help="restart/Ritz V.Q on the MFMA (1) or FMA (0) kernel; default: the library's")
+    ap.add_argument("--basis", default="fp64", choices=("fp64", "fp32"),
+                    help="fp32: the mixed-precision solve (FP32-basis phase, then FP64 to tol)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -354,7 +373,9 @@ def main():
     rp, col, val = getattr(W, gen)(**kw)
     n = len(rp) - 1
     log(f"workload {args.workload}: n={n} nnz={len(val)} generated in {time.perf_counter() - t0:.1f} s")
-    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    mixed = args.basis == "fp32"
+    prec = H.HL_PREC_F32 if mixed else H.HL_PREC_F64
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val), fp32_values=mixed)
     torch.cuda.init()
     items = None
     if not args.no_autotune:
@@ -365,7 +386,7 @@ def main():
     vec_out = H.pinned_empty(n, ncv)  # page-locked output: the final copy runs at link rate
     for i in range(args.warmup):
         r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_out,
-                            use_mfma=args.mfma)
+                            use_mfma=args.mfma, basis_prec=prec)
         log(f"warmup {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} {r.info.seconds:.3f} s")
 
     lat = []
@@ -374,9 +395,10 @@ def main():
     for i in range(args.steps):
         t = time.perf_counter()
         r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_out,
-                            use_mfma=args.mfma)
+                            use_mfma=args.mfma, basis_prec=prec)
         lat.append(time.perf_counter() - t)
-        log(f"solve {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} {lat[-1]:.3f} s")
+        log(f"solve {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} "
+            f"(fp32 phase {r.info.spmv_f32}) {lat[-1]:.3f} s")
         if r.ret != 0:
             raise SystemExit(f"solve {i} did not converge (ret {r.ret})")
     torch.cuda.synchronize()
@@ -385,11 +407,17 @@ def main():
 
     resid, ortho = residuals(op, r.values, np.array(r.vectors), n)
     log(f"max rel residual {resid:.2e}, orthogonality {ortho:.2e}")
+    err64 = None
+    if mixed:  # the mixed result's values against a plain FP64 solve's (untimed)
+        r64 = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, use_mfma=args.mfma)
+        err64 = float(np.max(np.abs(r.values - r64.values) / np.abs(r64.values)))
+        log(f"max rel value error vs FP64 solve {err64:.2e} (FP64 solve {r64.info.seconds:.3f} s, "
+            f"{r64.info.spmv} SpMV)")
 
     # per-kernel-class profile of one more solve (HIP events around each launch)
     import ctypes
 
-    o = H.options(args.tol, args.maxiter, use_mfma=args.mfma)
+    o = H.options(args.tol, args.maxiter, use_mfma=args.mfma, basis_prec=prec)
     o.flags = H.HL_FLAG_PROFILE
     values = np.zeros(ncv)
     info = H.Info()
@@ -408,7 +436,7 @@ def main():
     traffic = None  # measured HBM bytes per dispatch of the dominant kernel (rocprofv3 PMC runs)
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tpath):
-        rec = json.load(open(tpath)).get(args.workload, {}).get(names[dom])
+        rec = json.load(open(tpath)).get(args.workload + ("_mixed" if mixed else ""), {}).get(names[dom])
         if rec:
             traffic = rec["fetch_bytes"] + rec["write_bytes"]
 
@@ -433,15 +461,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (seeded generator)",
         "config": {"workload": label, "n": n, "nnz": int(len(val)), "nev": nev, "ncv": ncv, "tol": args.tol,
-                   "which": "largest algebraic", "basis": "fp64", "spmv_items": items},
+                   "which": "largest algebraic", "basis": BASIS_LABEL[args.basis], "spmv_items": items},
         "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
         "max_rel_resid": resid,
         "ritz_orthogonality": ortho,
         "restarts": int(r.info.restarts),
         "spmv_per_solve": int(r.info.spmv),
+        "spmv_fp32_phase": int(r.info.spmv_f32),
+        "max_rel_value_err_vs_fp64": err64,
         "cgs_passes_per_solve": int(r.info.cgs_passes),
         "spmv": {"GBps": spmv_gbps, "us_per_launch": t_spmv * 1e6, "bytes_per_launch": spmv_bytes,
                  "frac_of_peak": spmv_gbps / HBM_PEAK_GBPS},

@@ -499,6 +499,8 @@ int hl_tune_set(const char *key, int value)
         t.cgs_blocks_max = value;
     else if (!std::strcmp(key, "cgs_flags") && value >= 0 && value <= 15 && !(value & 4))
         t.cgs_flags = value;
+    else if (!std::strcmp(key, "f32_phase_tol_exp") && value >= 3 && value <= 9)
+        t.f32_phase_tol_exp = value;
     else {
         set_error("hl_tune_set: unknown key or value out of range: %s=%d", key, value);
         return HL_ERR_ARG;
@@ -514,6 +516,8 @@ int hl_tune_get(const char *key, int *value)
         *value = t.cgs_blocks_max;
     else if (!std::strcmp(key, "cgs_flags"))
         *value = t.cgs_flags;
+    else if (!std::strcmp(key, "f32_phase_tol_exp"))
+        *value = t.f32_phase_tol_exp;
     else {
         set_error("hl_tune_get: unknown key %s", key);
         return HL_ERR_ARG;

@@ -51,7 +51,7 @@ int Member::reset(hipStream_t s)
 {
     Workspace &W = ws;
     tol = opts.tol > 0.0 ? opts.tol : kEps;
-    if (kind != SEG_F64) tol = std::max(tol, kF32PhaseTol);
+    if (kind != SEG_F64) tol = std::max(tol, std::pow(10.0, -tuning().f32_phase_tol_exp));
     maxiter = opts.maxiter > 0 ? opts.maxiter : 200;
     sub.assign(ncv, 0.0);
     dia.assign(ncv, 0.0);

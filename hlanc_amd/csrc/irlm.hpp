@@ -69,6 +69,9 @@ struct Member {
     // in FP32 (and reads FP32 values when the operator has them) and stops at
     // kF32PhaseTol; the solve then restarts in FP64 from the sum of that
     // phase's wanted Ritz vectors (ACT_REPHASE) and ends as an FP64 solve.
+    // The FP32 phase's relative Ritz-estimate tolerance is 10^-f32_phase_tol_exp
+    // (a tuning key, 1e-6 by default: above the ~6e-8 relative rounding of an
+    // FP32 basis, so that the phase ends).
     int kind = SEG_F64;                  // SegKind of the current phase
     bool mixed = false;
     long long spmv_f32 = 0;              // FP32-phase counters
@@ -108,9 +111,6 @@ struct Member {
     void rephase_host(double *qslot);
 };
 
-// Relative Ritz-estimate tolerance of the FP32-basis phase of a mixed solve:
-// above the ~6e-8 relative rounding of an FP32 basis, so that the phase ends.
-constexpr double kF32PhaseTol = 1e-6;
 
 // Runs members to completion, all active members advancing in lockstep: one
 // segmented SpMV launch and four segmented CGS passes per Lanczos step serve

@@ -144,7 +144,9 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
  * against running solves).  Keys: "cgs_blocks_max" (grid cap of the CGS
  * passes, 1..2048), "cgs_flags" (bit 0 non-temporal basis loads, bit 1
  * prefetch of the next tile, bit 3 finish each pass in a second kernel
- * instead of a last-block reduction; bit 2 is refused).
+ * instead of a last-block reduction; bit 2 is refused), "f32_phase_tol_exp"
+ * (mixed precision: the FP32-basis phase ends at relative Ritz estimates of
+ * 10^-value, 3..9, default 6).
  * The solvers always run the split pass over a 16-row tiled basis; cgs_flags
  * selects the variant of hl_cgs_pass and hl_bench_cgs.  No counterpart in the
  * reference (its kernels are fixed). */
