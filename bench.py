@@ -47,7 +47,12 @@ WORKLOADS = {
 }
 
 
-BASIS_LABEL = {"fp64": "fp64", "fp32": "mixed: fp32 basis and values to 1e-6, then fp64 to tol"}
+BASIS_LABEL = {"fp64": "fp64", "fp32": "mixed: fp32 basis and values to 1e-6, then fp64 to tol",
+               "proj32": "fp64 basis + fp32 copy for the projection passes (DGKS correction every step)"}
+
+
+def basis_prec(H, basis):
+    return {"fp64": H.HL_PREC_F64, "fp32": H.HL_PREC_F32, "proj32": H.HL_PREC_F64_PROJ32}[basis]
 
 
 def log(msg):
@@ -172,7 +177,7 @@ def run_batch(args, H, torch):
         f"{time.perf_counter() - t0:.1f} s")
     torch.cuda.init()
     kw = dict(tol=args.tol, maxiter=args.maxiter, use_mfma=args.mfma)
-    prec = H.HL_PREC_F32 if mixed else H.HL_PREC_F64
+    prec = basis_prec(H, args.basis)
 
     vec_out = [H.pinned_empty(op.nrows, ncv) for op in ops]  # page-locked outputs, allocated once
 
@@ -254,7 +259,7 @@ def run_serve(args, H, torch):
     t0 = time.perf_counter()
     pool = W.serve_mix(args.pool, seed=0)
     mixed = args.basis == "fp32"
-    prec = H.HL_PREC_F32 if mixed else H.HL_PREC_F64
+    prec = basis_prec(H, args.basis)
     ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=m[1][0], col=m[1][1], val=m[1][2]), fp32_values=mixed)
            for m in pool]
     nnz = sum(len(m[1][1]) for m in pool)
@@ -335,8 +340,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="batch: skip the one-at-a-time comparison")
     ap.add_argument("--mfma", type=int, choices=(0, 1), default=None,
                     help="restart/Ritz V.Q on the MFMA (1) or FMA (0) kernel; default: the library's")
-    ap.add_argument("--basis", default="fp64", choices=("fp64", "fp32"),
-                    help="fp32: the mixed-precision solve (FP32-basis phase, then FP64 to tol)")
+    ap.add_argument("--basis", default="fp64", choices=("fp64", "proj32", "fp32"),
+                    help="proj32: FP32 basis copy for the projection passes; fp32: FP32-basis phase, then FP64")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--maxiter", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
@@ -374,7 +379,7 @@ def main():
     n = len(rp) - 1
     log(f"workload {args.workload}: n={n} nnz={len(val)} generated in {time.perf_counter() - t0:.1f} s")
     mixed = args.basis == "fp32"
-    prec = H.HL_PREC_F32 if mixed else H.HL_PREC_F64
+    prec = basis_prec(H, args.basis)
     op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val), fp32_values=mixed)
     torch.cuda.init()
     items = None
@@ -408,7 +413,7 @@ def main():
     resid, ortho = residuals(op, r.values, np.array(r.vectors), n)
     log(f"max rel residual {resid:.2e}, orthogonality {ortho:.2e}")
     err64 = None
-    if mixed:  # the mixed result's values against a plain FP64 solve's (untimed)
+    if args.basis != "fp64":  # the result's values against a plain FP64 solve's (untimed)
         r64 = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, use_mfma=args.mfma)
         err64 = float(np.max(np.abs(r.values - r64.values) / np.abs(r64.values)))
         log(f"max rel value error vs FP64 solve {err64:.2e} (FP64 solve {r64.info.seconds:.3f} s, "
@@ -436,7 +441,8 @@ def main():
     traffic = None  # measured HBM bytes per dispatch of the dominant kernel (rocprofv3 PMC runs)
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if os.path.exists(tpath):
-        rec = json.load(open(tpath)).get(args.workload + ("_mixed" if mixed else ""), {}).get(names[dom])
+        key = args.workload + ("" if args.basis == "fp64" else "_" + args.basis)
+        rec = json.load(open(tpath)).get(key, {}).get(names[dom])
         if rec:
             traffic = rec["fetch_bytes"] + rec["write_bytes"]
 

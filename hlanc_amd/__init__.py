@@ -24,6 +24,7 @@ HL_OK = 0
 HL_NOT_CONVERGED = 1
 HL_PREC_F64 = 0
 HL_PREC_F32 = 1
+HL_PREC_F64_PROJ32 = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -53,7 +54,7 @@ class Info(ctypes.Structure):
                 ("spmv", ctypes.c_longlong), ("cgs_passes", ctypes.c_longlong), ("rnorm", ctypes.c_double),
                 ("seconds", ctypes.c_double), ("max_rel_resid", ctypes.c_double),
                 ("refine_shift", ctypes.c_double), ("spmv_f32", ctypes.c_longlong),
-                ("restarts_f32", ctypes.c_int), ("reserved_info", ctypes.c_int)]
+                ("restarts_f32", ctypes.c_int), ("proj64_steps", ctypes.c_int)]
 
 
 class Profile(ctypes.Structure):

@@ -149,6 +149,130 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
     if (threadIdx.x == 0) run_epilogue(mode, j, fin[m], h_in, ctl, h_sub, h_dia);
 }
 
+typedef float flt2 __attribute__((ext_vector_type(2)));
+
+// The same split pass over an FP32 tiled basis (always tiled and split; FL
+// bit 0: non-temporal loads).  A lane holds two consecutive rows of a column
+// (one 8-byte load); 16 lanes cover a pair of adjacent columns of the 16-row
+// tile (one 128-byte line), so a wave covers the tile as 4 lane groups of
+// ceil(m/8) column pairs and each load instruction moves 4 full lines, as
+// the FP64 pass does.  Arithmetic is FP64.
+template <int MAXC8, int FL>
+__device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, const float *__restrict__ V,
+                                              long long ldv, const double *w_in, double *w_out,
+                                              const double *__restrict__ h_in, double *h_out, double *partials)
+{
+    constexpr int G = 4;                 // lane groups per wave, 2 columns each per step
+    __shared__ double hs[2 * G * MAXC8];
+    __shared__ double red[kBlock / kWave][2 * G * MAXC8 + 1];
+    __shared__ double vals[2 * G * MAXC8 + 1];
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int g = lane / 16;             // column group
+    const int sc = (lane / 8) & 1;       // column of the pair
+    const int q = lane % 8;              // row pair: rows 2q, 2q + 1 of the tile
+    const int cpp = (m + 2 * G - 1) / (2 * G); // column pairs per group
+    const int c0 = g * 2 * cpp + sc;     // this lane's columns: c0 + 2c
+    const bool has_h = h_in != nullptr;
+    for (int c = threadIdx.x; c < 2 * G * MAXC8; c += kBlock) hs[c] = (has_h && c < m) ? h_in[c] : 0.0;
+    __syncthreads();
+
+    double acc[MAXC8];
+#pragma unroll
+    for (int c = 0; c < MAXC8; ++c) acc[c] = 0.0;
+    double nrm = 0.0;
+    const long long ntiles = (n + kTileRows - 1) / kTileRows;
+    const long long W = (long long)nb * (kBlock / kWave);
+    // two tiles per trip: both tiles' loads are in flight before either is used
+    constexpr int TB = 2;
+    for (long long t = (long long)bid * (kBlock / kWave) + wv; t < ntiles; t += TB * W) {
+        flt2 v[TB][MAXC8]; // kept in FP32 registers, widened where used
+        double w0[TB], w1[TB];
+        bool ok0[TB], ok1[TB];
+        long long r0[TB];
+#pragma unroll
+        for (int b = 0; b < TB; ++b) {
+            const long long tt = t + b * W;
+            r0[b] = tt * kTileRows + 2 * q;
+            // the tile's rows exist in the padded basis; rows >= n are masked to 0
+            ok0[b] = tt < ntiles && r0[b] < n;
+            ok1[b] = tt < ntiles && r0[b] + 1 < n;
+#pragma unroll
+            for (int c = 0; c < MAXC8; ++c) {
+                const int colj = c0 + 2 * c;
+                v[b][c] = flt2{0.0f, 0.0f};
+                if (ok0[b] && c < cpp && colj < m)
+                    v[b][c] = ld_stream<flt2, (FL & 1) != 0>(
+                        reinterpret_cast<const flt2 *>(&V[vpos<true>(r0[b], colj, ldv)]));
+            }
+            w0[b] = ok0[b] ? w_in[r0[b]] : 0.0;
+            w1[b] = ok1[b] ? w_in[r0[b] + 1] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < TB; ++b) {
+            if (!ok1[b])
+#pragma unroll
+                for (int c = 0; c < MAXC8; ++c) v[b][c].y = 0.0f;
+            double a0 = w0[b], a1 = w1[b];
+            if (has_h) {
+                double p0 = 0.0, p1 = 0.0;
+#pragma unroll
+                for (int c = 0; c < MAXC8; ++c) {
+                    p0 += static_cast<double>(v[b][c].x) * hs[c0 + 2 * c];
+                    p1 += static_cast<double>(v[b][c].y) * hs[c0 + 2 * c];
+                }
+                p0 += __shfl_xor(p0, 8);
+                p1 += __shfl_xor(p1, 8);
+                p0 += __shfl_xor(p0, 16);
+                p1 += __shfl_xor(p1, 16);
+                p0 += __shfl_xor(p0, 32);
+                p1 += __shfl_xor(p1, 32);
+                a0 -= p0;
+                a1 -= p1;
+            }
+            if (lane < 8) {
+                if (ok0[b]) {
+                    if (w_out) w_out[r0[b]] = a0;
+                    nrm += a0 * a0;
+                }
+                if (ok1[b]) {
+                    if (w_out) w_out[r0[b] + 1] = a1;
+                    nrm += a1 * a1;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < MAXC8; ++c)
+                acc[c] += static_cast<double>(v[b][c].x) * a0 + static_cast<double>(v[b][c].y) * a1;
+        }
+    }
+
+#pragma unroll
+    for (int c = 0; c < MAXC8; ++c) {
+        double sm = acc[c];
+        sm += __shfl_xor(sm, 1);
+        sm += __shfl_xor(sm, 2);
+        sm += __shfl_xor(sm, 4);
+        acc[c] = sm;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
+    if (q == 0) {
+#pragma unroll
+        for (int c = 0; c < MAXC8; ++c)
+            if (c < cpp && c0 + 2 * c < m) red[wv][c0 + 2 * c] = acc[c];
+    }
+    if (lane == 0) red[wv][m] = nrm;
+    __syncthreads();
+    const int M1 = m + 1;
+    for (int c = threadIdx.x; c < M1; c += kBlock) {
+        double sm = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) sm += red[w][c];
+        vals[c] = sm;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < M1; c += kBlock) partials[(long long)c * nb + bid] = vals[c];
+}
+
 template <int MAXC, typename VT, bool TILED, int FL>
 __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__restrict__ V, long long ldv,
                                                      const double *w_in, double *w_out,
@@ -235,17 +359,31 @@ __device__ __forceinline__ PassPtrs pass_ptrs(const SegMember &mb, int mode)
 // One CGS pass of every listed solve in a single launch (split form).  The
 // pass (MODE) is a template argument so that profiles tell the always-run
 // passes (EPI_WNORM, EPI_PASS1) from the DGKS corrections.
+// One CGS pass of the listed steps.  FP64 launches read mb.V, FP32 launches
+// mb.Vproj (the FP32 basis, or the FP32 copy of an FP64 basis).  `sub`
+// selects the block numbering of the launch (SegStep::cgs0s, else cgs0).
+// With an FP32 copy, a step's first correction runs on the copy unless the
+// step cancelled strongly (ctl->exact): then an FP64 launch of its own runs it.
 template <int MAXC, int MODE, typename VT>
 __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__restrict__ M,
-                                                         const SegStep *__restrict__ S, int ns)
+                                                         const SegStep *__restrict__ S, int ns, int sub)
 {
-    const int i = seg_find(&S[0].cgs0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
+    constexpr bool F32 = std::is_same<VT, float>::value;
+    const int i = seg_find(sub ? &S[0].cgs0s : &S[0].cgs0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
     const SegStep st = S[i];
     const SegMember mb = M[st.mem];
+    if (MODE == EPI_PASS2 && mb.V32 && (mb.ctl->exact != 0) == F32) return; // the other launch's step
     const PassPtrs p = pass_ptrs(mb, MODE);
-    cgs_block<MAXC, VT, true, 9>(blockIdx.x - st.cgs0, mb.nb, mb.n, st.j + 1, static_cast<const VT *>(mb.V), mb.vld,
-                                 p.w_in, p.w_out, p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub, mb.hdia,
-                                 MODE, st.j);
+    const int bid = blockIdx.x - (sub ? st.cgs0s : st.cgs0);
+    if constexpr (F32) {
+        if ((MODE == EPI_PASS2 || MODE == EPI_PASS3) && !mb.ctl->need) return; // DGKS decided on the device
+        cgs_block_f32<(MAXC + 1) / 2, 1>(bid, mb.nb, mb.n, st.j + 1, static_cast<const float *>(mb.Vproj), mb.vld,
+                                         p.w_in, p.w_out, p.h_in, p.h_out, mb.partials);
+    } else {
+        cgs_block<MAXC, double, true, 9>(bid, mb.nb, mb.n, st.j + 1, static_cast<const double *>(mb.V), mb.vld,
+                                         p.w_in, p.w_out, p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub,
+                                         mb.hdia, MODE, st.j);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMember *__restrict__ M,
@@ -394,7 +532,10 @@ __global__ __launch_bounds__(kBlock) void vq_seg_kernel(const SegVQ *__restrict_
             double rk = 0.0;
 #pragma unroll
             for (int c = 0; c < CB; ++c) {
-                if (c0 + c < t.N) Out[vpos<true>(row, c0 + c, t.K)] = static_cast<VT>(acc[c]);
+                if (c0 + c < t.N) {
+                    Out[vpos<true>(row, c0 + c, t.K)] = static_cast<VT>(acc[c]);
+                    if (t.Out32) t.Out32[vpos<true>(row, c0 + c, t.K)] = static_cast<float>(acc[c]);
+                }
                 if (c == rl) rk = acc[c];
             }
             if (upd) { // dsapps residual (HLanc.hpp:1605-1614)
@@ -474,7 +615,10 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int cl = ct * 16 + lk + 4 * e;
-                if (ok && cl < t.N) Out[vpos<true>(row, cl, t.K)] = static_cast<VT>(acc[ct][e]);
+                if (ok && cl < t.N) {
+                    Out[vpos<true>(row, cl, t.K)] = static_cast<VT>(acc[ct][e]);
+                    if (t.Out32) t.Out32[vpos<true>(row, cl, t.K)] = static_cast<float>(acc[ct][e]);
+                }
                 if (cl == t.res_col) rk = acc[ct][e];
             }
         }
@@ -856,9 +1000,9 @@ int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn
 
 namespace {
 template <int MODE, typename VT>
-void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg, int blocks, hipStream_t s)
+void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg, int blocks, int sub, hipStream_t s)
 {
-#define HL_SEG(MC) cgs_seg_kernel<MC, MODE, VT><<<blocks, kBlock, 0, s>>>(M, S, ns)
+#define HL_SEG(MC) cgs_seg_kernel<MC, MODE, VT><<<blocks, kBlock, 0, s>>>(M, S, ns, sub)
     if (max_cpg <= 2)
         HL_SEG(2);
     else if (max_cpg <= 4)
@@ -876,7 +1020,7 @@ void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg,
 } // namespace
 
 int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total, int mode,
-                   bool v32, hipStream_t s)
+                   bool v32, bool sub, hipStream_t s)
 {
     if (ns <= 0) return HL_OK;
     if (max_cpg > 32) {
@@ -884,8 +1028,8 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
         return HL_ERR_ARG;
     }
 #define HL_CGS_MODE(MODE)                                                                \
-    (v32 ? cgs_seg_dispatch<MODE, float>(M, S, ns, max_cpg, cgs_blocks_total, s)         \
-         : cgs_seg_dispatch<MODE, double>(M, S, ns, max_cpg, cgs_blocks_total, s))
+    (v32 ? cgs_seg_dispatch<MODE, float>(M, S, ns, max_cpg, cgs_blocks_total, sub, s)    \
+         : cgs_seg_dispatch<MODE, double>(M, S, ns, max_cpg, cgs_blocks_total, sub, s))
     switch (mode) {
     case EPI_WNORM:
         HL_CGS_MODE(EPI_WNORM);

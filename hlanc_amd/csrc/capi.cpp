@@ -73,8 +73,8 @@ bool solve_args_ok(int n, int nev, int ncv, int ldv, const double *values, bool 
                   ncv, ldv);
         return false;
     }
-    if (o.basis_prec != HL_PREC_F64 && o.basis_prec != HL_PREC_F32) {
-        set_error("%s: basis_prec must be HL_PREC_F64 or HL_PREC_F32", who);
+    if (o.basis_prec != HL_PREC_F64 && o.basis_prec != HL_PREC_F32 && o.basis_prec != HL_PREC_F64_PROJ32) {
+        set_error("%s: basis_prec must be HL_PREC_F64, HL_PREC_F64_PROJ32 or HL_PREC_F32", who);
         return false;
     }
     if (o.basis_prec == HL_PREC_F32 && !device_op) {

@@ -45,13 +45,17 @@ double now_s()
 
 // ------------------------------------------------------------------ Member
 
-double Member::cgs_bytes_pass(int m) const { return vbytes() * n * m + 16.0 * n; }
+double Member::cgs_bytes_pass(int m, bool sub) const
+{
+    const double b = sub ? (f32_phase() ? 4.0 : 8.0) : vbytes();
+    return b * n * m + 16.0 * n;
+}
 
 int Member::reset(hipStream_t s)
 {
     Workspace &W = ws;
     tol = opts.tol > 0.0 ? opts.tol : kEps;
-    if (kind != SEG_F64) tol = std::max(tol, std::pow(10.0, -tuning().f32_phase_tol_exp));
+    if (f32_phase()) tol = std::max(tol, std::pow(10.0, -tuning().f32_phase_tol_exp));
     maxiter = opts.maxiter > 0 ? opts.maxiter : 200;
     sub.assign(ncv, 0.0);
     dia.assign(ncv, 0.0);
@@ -63,7 +67,8 @@ int Member::reset(hipStream_t s)
     work.assign(ncv, 0.0);
     ritz_saved.clear();
     bounds_saved.clear();
-    spmv = cgs = p2 = p3 = m2sum = m3sum = d_m23 = 0;
+    spmv = cgs = p2 = p3 = m2sum = m3sum = nexact = mxsum = 0;
+    dgks_bytes = 0.0;
     nconv = 0;
     rnorm = 0.0;
     ret = HL_NOT_CONVERGED;
@@ -71,6 +76,7 @@ int Member::reset(hipStream_t s)
     std::memset(W.ctl_h, 0, sizeof(Ctl));
     W.ctl_h->rscale = 1.0;
     W.ctl_h->xscale = 1.0;
+    W.ctl_h->dgks_always = kind == SEG_PROJ32; // the FP32-copy projections leave ~6e-8 of V in w
     HL_TRY_HIP(hipMemcpyAsync(W.ctl, W.ctl_h, sizeof(Ctl), hipMemcpyHostToDevice, s));
     HL_TRY_HIP(hipMemsetAsync(W.hsub, 0, sizeof(double) * ncv, s));
     HL_TRY_HIP(hipMemsetAsync(W.hdia, 0, sizeof(double) * ncv, s));
@@ -97,7 +103,7 @@ int Member::start(hipStream_t s)
     Workspace &W = ws;
     t0 = now_s();
     mixed = opts.basis_prec == HL_PREC_F32 && A != nullptr;
-    kind = !mixed ? SEG_F64 : (A->val32 ? SEG_V32_A32 : SEG_V32);
+    kind = mixed ? (A->val32 ? SEG_V32_A32 : SEG_V32) : copy32() ? SEG_PROJ32 : SEG_F64;
     spmv_f32 = 0;
     restarts_f32 = 0;
     vals_f32.clear();
@@ -150,11 +156,22 @@ void Member::absorb(const double *rb)
     std::memcpy(&c, rb + 2 * ncv, sizeof(Ctl));
     rnorm = c.rnorm;
     cgs = c.cgs;
-    d_m23 = (c.m2sum - m2sum) + 2 * (c.p2 - p2) + (c.m3sum - m3sum) + 2 * (c.p3 - p3);
+    // basis columns read by the corrections (plus w read and written): first
+    // corrections on the projected basis (FP64 for exact steps), second ones
+    // on the FP64 basis for FP32-copy projections
+    const double c2 = (double)(c.m2sum - m2sum) + 2.0 * (double)(c.p2 - p2);
+    const double c3 = (double)(c.m3sum - m3sum) + 2.0 * (double)(c.p3 - p3);
+    const double cx = (double)(c.mxsum - mxsum) + 2.0 * (double)(c.nexact - nexact);
+    if (kind == SEG_PROJ32)
+        dgks_bytes = n * (4.0 * c2 + 4.0 * cx + 8.0 * c3);
+    else
+        dgks_bytes = vbytes() * n * (c2 + c3);
     p2 = c.p2;
     p3 = c.p3;
     m2sum = c.m2sum;
     m3sum = c.m3sum;
+    nexact = c.nexact;
+    mxsum = c.mxsum;
     spmv += sweep_np;
 }
 
@@ -194,7 +211,7 @@ void Member::decide(double *qslot)
         }
     }
     if (nconv >= nev0 || np == 0) {
-        if (kind != SEG_F64)
+        if (f32_phase())
             rephase_host(qslot);
         else
             finish_host(qslot, true);
@@ -220,7 +237,7 @@ void Member::decide(double *qslot)
     ++iter;
     restarts = iter;
     if (iter >= maxiter) { // the restart would feed no further sweep: the solve ends unconverged
-        if (kind != SEG_F64) // (the FP32 phase hands over what it has instead)
+        if (f32_phase()) // (the FP32 phase hands over what it has instead)
             rephase_host(qslot);
         else
             finish_host(qslot, true);
@@ -328,6 +345,7 @@ void Member::complete()
                                                                 std::max(std::fabs(vals[i]), 1e-300));
     info.spmv_f32 = spmv_f32;
     info.restarts_f32 = restarts_f32;
+    info.proj64_steps = nexact;
     info.graph_sweeps = graph_sweeps;
     phase = DONE;
 }
@@ -545,6 +563,8 @@ int Engine::sweep(std::vector<Member *> &ms)
         M.chunk_part = W.chunk_part;
         M.chunk_cnt = W.chunk_cnt;
         M.V = W.V[W.cur];
+        M.V32 = m.kind == SEG_PROJ32 ? W.V32[W.cur] : nullptr;
+        M.Vproj = M.V32 ? static_cast<const void *>(M.V32) : M.V;
         M.vld = W.vl().ld;
         M.r = W.r;
         M.w = W.w;
@@ -558,38 +578,57 @@ int Engine::sweep(std::vector<Member *> &ms)
         M.nb = cgs_blocks(m.n);
         M.ncv = m.ncv;
     }
-    // every step index's entries grouped by precision kind: one SpMV launch per
-    // kind, one CGS partials launch per basis type (kind 0; kinds 1-2), one
-    // finalize launch over all
-    std::vector<std::array<int, SEG_NKIND + 1>> koff(maxsteps);
-    std::vector<std::array<int, SEG_NKIND>> nspmv(maxsteps);
-    std::vector<std::array<int, 2>> ncgs(maxsteps), maxcpg(maxsteps);
-    std::vector<int> off(maxsteps + 1, 0), nfin(maxsteps, 0);
+    // every step index's entries grouped by precision kind (kSegOrder): one SpMV
+    // launch per value/basis type, one CGS partials launch per basis type read,
+    // one finalize launch over all
+    auto spmv_group = [](int k) { return k == SEG_V32 ? 0 : k == SEG_V32_A32 ? 1 : 2; };
+    auto proj_group = [](int k) { return k == SEG_F64 ? 1 : 0; };            // 0: FP32 reads
+    auto sub_group = [](int k) { return k == SEG_F64 || k == SEG_PROJ32 ? 1 : 0; };
+    static const int kSpmvKind[3] = {SEG_V32, SEG_V32_A32, SEG_F64};
+    std::vector<std::array<int, 4>> goff_spmv(maxsteps), goff_proj(maxsteps), goff_sub(maxsteps);
+    std::vector<std::array<int, 3>> nspmv(maxsteps);
+    std::vector<std::array<int, 2>> nproj(maxsteps), nsub(maxsteps), cpg_proj(maxsteps), cpg_sub(maxsteps);
+    std::vector<int> off(maxsteps + 1, 0), nfin(maxsteps, 0), n32(maxsteps, 0), nb32(maxsteps, 0), cpg32(maxsteps, 0);
     size_t ns = 0;
     for (int st = 0; st < maxsteps; ++st) {
         off[st] = static_cast<int>(ns);
-        ncgs[st] = {0, 0};
-        maxcpg[st] = {0, 0};
-        for (int k = 0; k < SEG_NKIND; ++k) {
-            koff[st][k] = static_cast<int>(ns);
-            nspmv[st][k] = 0;
-            const int g = k == SEG_F64 ? 0 : 1;
+        nspmv[st] = {0, 0, 0};
+        nproj[st] = nsub[st] = cpg_proj[st] = cpg_sub[st] = {0, 0};
+        int cur_sp = -1, cur_pr = -1, cur_sb = -1;
+        for (int ko = 0; ko < SEG_NKIND; ++ko) {
+            const int k = kSegOrder[ko];
             for (size_t i = 0; i < nm; ++i) {
                 Member &m = *act[i];
                 if (st >= m.sweep_np || m.kind != k) continue;
+                // group boundaries: every group is a contiguous run of entries
+                const int gs = spmv_group(k), gp = proj_group(k), gb = sub_group(k);
+                for (; cur_sp < gs; ++cur_sp) goff_spmv[st][cur_sp + 1] = static_cast<int>(ns);
+                for (; cur_pr < gp; ++cur_pr) goff_proj[st][cur_pr + 1] = static_cast<int>(ns);
+                for (; cur_sb < gb; ++cur_sb) goff_sub[st][cur_sb + 1] = static_cast<int>(ns);
                 SegStep &S = hS[ns++];
                 S.mem = static_cast<int>(i);
                 S.j = m.sweep_k + st;
-                S.spmv0 = nspmv[st][k];
-                S.cgs0 = ncgs[st][g];
+                S.spmv0 = nspmv[st][gs];
+                S.cgs0 = nproj[st][gp];
+                S.cgs0s = nsub[st][gb];
                 S.fin0 = nfin[st];
-                if (m.A) nspmv[st][k] += m.A->nblk;
-                ncgs[st][g] += hM[i].nb;
+                if (m.A) nspmv[st][gs] += m.A->nblk;
+                nproj[st][gp] += hM[i].nb;
+                nsub[st][gb] += hM[i].nb;
                 nfin[st] += S.j + 2; // j + 1 projections and the squared norm
-                maxcpg[st][g] = std::max(maxcpg[st][g], (S.j + 1 + dev::kGroups - 1) / dev::kGroups);
+                const int cpg = (S.j + 1 + dev::kGroups - 1) / dev::kGroups; // columns per lane group
+                cpg_proj[st][gp] = std::max(cpg_proj[st][gp], cpg);
+                cpg_sub[st][gb] = std::max(cpg_sub[st][gb], cpg);
+                if (k == SEG_PROJ32) { // leads its cgs0s group: the exact corrections launch over these alone
+                    ++n32[st];
+                    nb32[st] += hM[i].nb;
+                    cpg32[st] = std::max(cpg32[st], cpg);
+                }
             }
         }
-        koff[st][SEG_NKIND] = static_cast<int>(ns);
+        for (; cur_sp < 3; ++cur_sp) goff_spmv[st][cur_sp + 1] = static_cast<int>(ns);
+        for (; cur_pr < 2; ++cur_pr) goff_proj[st][cur_pr + 1] = static_cast<int>(ns);
+        for (; cur_sb < 2; ++cur_sb) goff_sub[st][cur_sb + 1] = static_cast<int>(ns);
     }
     off[maxsteps] = static_cast<int>(ns);
     HL_TRY_HIP(hipMemcpyAsync(dM, hM, sizeof(SegMember) * nm, hipMemcpyHostToDevice, stream));
@@ -608,31 +647,41 @@ int Engine::sweep(std::vector<Member *> &ms)
             for (int q = off[st]; q < off[st + 1]; ++q) {
                 Member &m = *act[hS[q].mem];
                 if (!m.A && (r = m.host_apply(hS[q].j, stream))) return r;
-                if (m.A)
+                if (m.A) // values + indices, row pointers, x, y, the basis column (and its FP32 copy)
                     bytes += (m.kind == SEG_V32_A32 ? 8.0 : 12.0) * m.A->nnz + 4.0 * (m.A->nrows + 1) +
-                             8.0 * m.A->ncols + (8.0 + m.vbytes()) * m.A->nrows;
+                             8.0 * m.A->ncols + 8.0 * m.A->nrows +
+                             (m.kind == SEG_PROJ32 ? 12.0 : m.vbytes()) * m.A->nrows;
             }
             if (nspmv[st][0] + nspmv[st][1] + nspmv[st][2] > 0) {
                 if ((r = prof.begin(stream))) return r;
-                for (int k = 0; k < SEG_NKIND; ++k)
-                    if (nspmv[st][k] > 0 && (r = launch_spmv_seg(dM, dS + koff[st][k], koff[st][k + 1] - koff[st][k],
-                                                                 nspmv[st][k], k, stream)))
+                for (int g = 0; g < 3; ++g)
+                    if (nspmv[st][g] > 0 &&
+                        (r = launch_spmv_seg(dM, dS + goff_spmv[st][g], goff_spmv[st][g + 1] - goff_spmv[st][g],
+                                             nspmv[st][g], kSpmvKind[g], stream)))
                         return r;
                 if ((r = prof.end(stream, K_SPMV, bytes))) return r;
             }
-            const int g0[2] = {koff[st][0], koff[st][1]}, g1[2] = {koff[st][1], koff[st][SEG_NKIND]};
             for (int p = 0; p < 4; ++p) {
+                // EPI_PASS1 and EPI_PASS3 read the FP64 basis of PROJ32 solves
+                const bool sub = modes[p] == EPI_PASS1 || modes[p] == EPI_PASS3;
                 bytes = 0.0;
                 if (p < 2) // executed DGKS corrections are counted from the device counters
                     for (int q = off[st]; q < off[st + 1]; ++q) {
                         const Member &m = *act[hS[q].mem];
-                        bytes += m.cgs_bytes_pass(hS[q].j + 1) - (p == 0 ? 8.0 * m.n : 0.0);
+                        bytes += m.cgs_bytes_pass(hS[q].j + 1, sub) - (p == 0 ? 8.0 * m.n : 0.0);
                     }
+                const auto &go = sub ? goff_sub[st] : goff_proj[st];
+                const auto &nb = sub ? nsub[st] : nproj[st];
+                const auto &cp = sub ? cpg_sub[st] : cpg_proj[st];
                 if ((r = prof.begin(stream))) return r;
-                for (int g = 0; g < 2; ++g)
-                    if (ncgs[st][g] > 0 && (r = launch_cgs_seg(dM, dS + g0[g], g1[g] - g0[g], maxcpg[st][g],
-                                                               ncgs[st][g], modes[p], g == 1, stream)))
+                for (int g = 0; g < 2; ++g) // group 0 reads FP32 bases, group 1 FP64
+                    if (nb[g] > 0 && (r = launch_cgs_seg(dM, dS + go[g], go[g + 1] - go[g], cp[g], nb[g], modes[p],
+                                                         g == 0, sub, stream)))
                         return r;
+                if (modes[p] == EPI_PASS2 && n32[st] > 0 && // strongly cancelling PROJ32 steps, on FP64
+                    (r = launch_cgs_seg(dM, dS + goff_sub[st][1], n32[st], cpg32[st], nb32[st], modes[p], false, true,
+                                        stream)))
+                    return r;
                 if ((r = prof.end(stream, p < 2 ? K_CGS : K_DGKS, bytes))) return r;
                 if ((r = prof.begin(stream))) return r;
                 if ((r = launch_cgs_seg_finalize(dM, S, cnt, nfin[st], modes[p], stream))) return r;
@@ -646,17 +695,27 @@ int Engine::sweep(std::vector<Member *> &ms)
     } else {
         // the sweep's shape: table addresses and every step's segment and grid sizes
         std::vector<long long> key;
-        key.reserve(2 + 12 * (size_t)maxsteps);
+        key.reserve(2 + 27 * (size_t)maxsteps);
         key.push_back(reinterpret_cast<long long>(dM));
         key.push_back(reinterpret_cast<long long>(dS));
         for (int st = 0; st < maxsteps; ++st) {
-            for (int k = 0; k <= SEG_NKIND; ++k) key.push_back(koff[st][k]);
-            for (int k = 0; k < SEG_NKIND; ++k) key.push_back(nspmv[st][k]);
+            key.push_back(off[st]);
+            for (int g = 0; g < 4; ++g) {
+                key.push_back(goff_spmv[st][g]);
+                if (g < 3) key.push_back(goff_proj[st][g]);
+                if (g < 3) key.push_back(goff_sub[st][g]);
+            }
+            for (int g = 0; g < 3; ++g) key.push_back(nspmv[st][g]);
             for (int g = 0; g < 2; ++g) {
-                key.push_back(ncgs[st][g]);
-                key.push_back(maxcpg[st][g]);
+                key.push_back(nproj[st][g]);
+                key.push_back(nsub[st][g]);
+                key.push_back(cpg_proj[st][g]);
+                key.push_back(cpg_sub[st][g]);
             }
             key.push_back(nfin[st]);
+            key.push_back(n32[st]);
+            key.push_back(nb32[st]);
+            key.push_back(cpg32[st]);
         }
         ++gclock;
         GraphEntry *hit = nullptr;
@@ -708,7 +767,7 @@ int Engine::sweep(std::vector<Member *> &ms)
     if (prof.on && (rc = prof.flush())) return rc;
     for (size_t i = 0; i < nm; ++i) {
         act[i]->absorb(hRB + i * stride);
-        if (prof.on) prof.tot.bytes[K_DGKS] += act[i]->vbytes() * act[i]->n * (double)act[i]->d_m23;
+        if (prof.on) prof.tot.bytes[K_DGKS] += act[i]->dgks_bytes;
     }
     rc = after_sweep(act);
     if (prof.on) {
@@ -741,7 +800,8 @@ int Engine::after_sweep(std::vector<Member *> &act)
         nrephase += m.act == Member::ACT_REPHASE;
         if (m.act != Member::ACT_RESTART) continue;
         const bool mf = m.opts.use_mfma && vq_seg_mfma_fits(m.ncv, m.vq_N);
-        cat[(m.kind != SEG_F64 ? 2 : 0) + (mf ? 1 : 0)].push_back(static_cast<int>(i));
+        const bool v32 = m.kind == SEG_V32 || m.kind == SEG_V32_A32;
+        cat[(v32 ? 2 : 0) + (mf ? 1 : 0)].push_back(static_cast<int>(i));
     }
     int nres = 0, nblk[4] = {0, 0, 0, 0}, maxK[4] = {0, 0, 0, 0}, maxN[4] = {0, 0, 0, 0}, mnct[4] = {0, 0, 0, 0};
     size_t mlds[4] = {0, 0, 0, 0};
@@ -753,6 +813,7 @@ int Engine::after_sweep(std::vector<Member *> &act)
             SegVQ &t = hV[nres++];
             t.V = W.V[W.cur];
             t.Out = W.V[1 - W.cur];
+            t.Out32 = m.kind == SEG_PROJ32 ? W.V32[1 - W.cur] : nullptr;
             t.Q = dQ + i * qstride;
             t.r = W.r;
             t.ctl = W.ctl;
@@ -771,7 +832,8 @@ int Engine::after_sweep(std::vector<Member *> &act)
             const int nct = (t.N + 15) / 16;
             mnct[c] = std::max(mnct[c], nct);
             mlds[c] = std::max(mlds[c], sizeof(double) * (size_t)((t.K + 3) & ~3) * nct * 16);
-            bytes += m.vbytes() * m.n * (t.K + t.N) + 16.0 * m.n;
+            const double vb = c >= 2 ? 4.0 : 8.0;
+            bytes += vb * m.n * (t.K + t.N) + 16.0 * m.n + (t.Out32 ? 4.0 * m.n * t.N : 0.0);
         }
     int rc;
     if (nres > 0 || nfinish > 0 || nrephase > 0)
@@ -837,7 +899,7 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
     double in_use = 0.0;
     int rc = HL_OK;
     auto need = [](const Member *m) {
-        return Workspace::bytes_needed(m->n, m->ncv, m->A == nullptr, m->A ? m->A->nslots : 0);
+        return Workspace::bytes_needed(m->n, m->ncv, m->A == nullptr, m->A ? m->A->nslots : 0, m->copy32());
     };
     // members done since the last call leave the active set; their workspaces
     // are freed at the next point where the device is idle
@@ -874,7 +936,8 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
             Member *m = queue[next++];
             m->t_admit = t;
             m->phase = Member::NEW;
-            if ((m->err = m->ws.init(dev, m->n, m->ncv, m->A == nullptr, m->A ? m->A->nslots : 0, stream))) {
+            if ((m->err = m->ws.init(dev, m->n, m->ncv, m->A == nullptr, m->A ? m->A->nslots : 0, m->copy32(),
+                                     stream))) {
                 m->ws.release();
                 m->phase = Member::DONE;
                 m->t_done = now_s() - t_start;

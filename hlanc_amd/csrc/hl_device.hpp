@@ -25,7 +25,16 @@ __device__ __forceinline__ void run_epilogue(int mode, int j, double nrm2, const
         h_dia[j] = h_in[j];
         h_sub[j] = (j == 0) ? 0.0 : ctl->rnorm;
         ctl->rnorm = sqrt(nrm2);
-        ctl->need = ctl->rnorm < kDgks * ctl->wnorm;
+        if (ctl->dgks_always) {
+            ctl->need = 1;
+            ctl->exact = ctl->rnorm < kProjExact * ctl->wnorm;
+            if (ctl->exact) {
+                ctl->nexact += 1;
+                ctl->mxsum += j + 1;
+            }
+        } else {
+            ctl->need = ctl->rnorm < kDgks * ctl->wnorm;
+        }
         ctl->iter = 0;
         ctl->cgs += 1;
         break;

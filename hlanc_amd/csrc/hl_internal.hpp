@@ -56,6 +56,9 @@ struct Ctl {
     double nrm2;     // last reduced squared norm (standalone passes)
     long long p2, p3;       // DGKS corrections executed (first, second)
     long long m2sum, m3sum; // sum of their basis widths (for byte counts)
+    int dgks_always;        // run the first correction every step (FP32-copy projections)
+    int exact;              // this step's corrections read the FP64 basis (FP32-copy projections)
+    long long nexact, mxsum; // steps with exact corrections and the sum of their basis widths
 };
 
 static_assert(sizeof(Ctl) % sizeof(double) == 0, "Ctl is read back as doubles");
@@ -158,8 +161,22 @@ enum SegKind {
     SEG_F64 = 0,     // FP64 basis and values
     SEG_V32 = 1,     // FP32 basis, FP64 values
     SEG_V32_A32 = 2, // FP32 basis and values
-    SEG_NKIND = 3
+    SEG_PROJ32 = 3,  // FP64 basis and values; projection passes read an FP32 copy
+    SEG_NKIND = 4
 };
+// Order of the kinds within a step's entries, so that every launch group is
+// one contiguous run: SpMV {V32} {V32_A32} {PROJ32, F64}; CGS passes that read
+// the FP64 basis of PROJ32 solves (EPI_PASS1, EPI_PASS3; numbered by cgs0s)
+// FP32 {V32, V32_A32}, FP64 {PROJ32, F64}; passes that read its FP32 copy
+// (EPI_WNORM, EPI_PASS2; numbered by cgs0) FP32 {V32, V32_A32, PROJ32}, FP64
+// {F64}; and the exact first corrections of PROJ32 solves, FP64 {PROJ32}
+// (numbered by cgs0s, as PROJ32 leads its cgs0s group).
+constexpr int kSegOrder[SEG_NKIND] = {SEG_V32, SEG_V32_A32, SEG_PROJ32, SEG_F64};
+// FP32-copy projections: a step whose first pass leaves ||r|| < kProjExact
+// ||w|| (strong cancellation) runs its DGKS correction on the FP64 basis,
+// since the FP32 copy's rounding (~6e-8 ||w||) is then no longer small
+// against r; the second correction always reads the FP64 basis.
+constexpr double kProjExact = 1e-2;
 struct SegMember {               // per solve, fixed during a sweep
     const SpmvBlk *blks;         // CSR operator (null for host operators)
     const int *rp, *col;
@@ -168,6 +185,8 @@ struct SegMember {               // per solve, fixed during a sweep
     double *chunk_part;
     unsigned *chunk_cnt;
     void *V;                     // tiled basis in use (double, or float for SEG_V32*)
+    const void *Vproj;           // basis read by the projection-only CGS passes (V, or the FP32 copy)
+    float *V32;                  // FP32 copy written beside V (SEG_PROJ32), else null
     long long vld;
     double *r, *w, *h0, *h1, *hsub, *hdia, *partials;
     Ctl *ctl;
@@ -178,13 +197,16 @@ struct SegStep {                 // per solve and Lanczos step
     int mem;                     // index into the SegMember table
     int j;                       // step index (projects onto j + 1 columns)
     int spmv0;                   // first block in the SpMV launch
-    int cgs0;                    // first block in the CGS launch
+    int cgs0;                    // first block in the launches of EPI_WNORM and EPI_PASS2
+    int cgs0s;                   // first block in the launches of EPI_PASS1 and EPI_PASS3
     int fin0;                    // first block in the finalize launch
 };
 // Partials kernel of CGS pass `mode` (EPI_WNORM .. EPI_PASS3) for every listed
 // step, then the per-column finalize kernel (j + 2 blocks per step).
+// v32: the launch reads FP32 bases (SegMember::Vproj); sub: its blocks are
+// numbered by SegStep::cgs0s (else cgs0).
 int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total, int mode,
-                   bool v32, hipStream_t s);
+                   bool v32, bool sub, hipStream_t s);
 int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fin_blocks_total, int mode,
                             hipStream_t s);
 int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, hipStream_t s);
@@ -193,6 +215,7 @@ int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_tot
 struct SegVQ {
     const void *V;               // input basis (double, or float for FP32 bases)
     void *Out;                   // output basis (the other ping-pong buffer)
+    float *Out32;                // FP32 copy of Out (FP64 bases with an FP32 copy), else null
     const double *Q;             // K x N, column-major, ld K
     double *r;                   // residual, updated in place
     Ctl *ctl;                    // rnorm / xscale for the next sweep

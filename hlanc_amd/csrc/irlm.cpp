@@ -213,6 +213,7 @@ void Workspace::release()
     if (hblob) (void)hipHostFree(hblob);
     blob = hblob = nullptr;
     V[0] = V[1] = nullptr;
+    V32[0] = V32[1] = nullptr;
     r = w = hsub = hdia = hbuf = xv = chunk_part = nullptr;
     chunk_cnt = nullptr;
     ctl = nullptr;
@@ -236,13 +237,15 @@ struct Carve {
 };
 
 struct WsLayout {
-    size_t V0, V1, r, w, hsub, hdia, hbuf, ctl, xv, cpart, ccnt, part, cnt, total;
-    WsLayout(int n, int ncv, bool host_op, int nslots)
+    size_t V0, V1, S0, S1, r, w, hsub, hdia, hbuf, ctl, xv, cpart, ccnt, part, cnt, total;
+    WsLayout(int n, int ncv, bool host_op, int nslots, bool copy32)
     {
         const size_t ldv = static_cast<size_t>(padded_rows(n));
         Carve c;
         V0 = c.take(8 * ldv * ncv);
         V1 = c.take(8 * ldv * ncv);
+        S0 = copy32 ? c.take(4 * ldv * ncv) : 0;
+        S1 = copy32 ? c.take(4 * ldv * ncv) : 0;
         r = c.take(8 * ldv);
         w = c.take(8 * ldv);
         hsub = c.take(8 * (size_t)ncv);
@@ -259,23 +262,25 @@ struct WsLayout {
 };
 } // namespace
 
-double Workspace::bytes_needed(int n, int ncv, bool host_op, int nslots)
+double Workspace::bytes_needed(int n, int ncv, bool host_op, int nslots, bool copy32)
 {
-    return static_cast<double>(WsLayout(n, ncv, host_op, nslots).total);
+    return static_cast<double>(WsLayout(n, ncv, host_op, nslots, copy32).total);
 }
 
-int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, hipStream_t s)
+int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, bool copy32, hipStream_t s)
 {
     dev = dev_;
     n = n_;
     ncv = ncv_;
     ldv = padded_rows(n);
     cur = 0;
-    const WsLayout L(n, ncv, host_op, nslots);
+    const WsLayout L(n, ncv, host_op, nslots, copy32);
     HL_TRY_HIP(hipMalloc(&blob, L.total));
     char *b = static_cast<char *>(blob);
     V[0] = reinterpret_cast<double *>(b + L.V0);
     V[1] = reinterpret_cast<double *>(b + L.V1);
+    V32[0] = copy32 ? reinterpret_cast<float *>(b + L.S0) : nullptr;
+    V32[1] = copy32 ? reinterpret_cast<float *>(b + L.S1) : nullptr;
     r = reinterpret_cast<double *>(b + L.r);
     w = reinterpret_cast<double *>(b + L.w);
     hsub = reinterpret_cast<double *>(b + L.hsub);

@@ -14,6 +14,7 @@ struct Workspace {
     int dev = 0, n = 0, ncv = 0;
     long long ldv = 0;                  // column-major leading dimension, multiple of 64
     double *V[2] = {nullptr, nullptr};  // FP64 tiled basis, ping-pong for V Q
+    float *V32[2] = {nullptr, nullptr}; // FP32 copies of V (projection passes), if requested
     int cur = 0;
     double *r = nullptr, *w = nullptr;  // residual and A v_j
     double *hsub = nullptr, *hdia = nullptr, *hbuf = nullptr;
@@ -30,11 +31,11 @@ struct Workspace {
     // The ping-pong bases are tiled: 16-row blocks of ncv columns.
     VLayout vl() const { return VLayout{ncv, true}; }
     ~Workspace();
-    int init(int dev, int n, int ncv, bool host_op, int nslots, hipStream_t s); // zeroing on s
+    int init(int dev, int n, int ncv, bool host_op, int nslots, bool copy32, hipStream_t s); // zeroing on s
     // Hands the two allocations to the caller (to free once the device is idle)
     // and clears the workspace.
     std::pair<void *, void *> detach();
-    static double bytes_needed(int n, int ncv, bool host_op, int nslots); // device bytes of init
+    static double bytes_needed(int n, int ncv, bool host_op, int nslots, bool copy32); // device bytes of init
     void release();
 };
 
@@ -81,8 +82,8 @@ struct Member {
     int sweep_k = 0, sweep_np = 0;
     int nev = 0, np = 0, nconv = 0, iter = 0, restarts = 0, maxiter = 200;
     double tol = 0.0, rnorm = 0.0, t0 = 0.0;
-    long long spmv = 0, cgs = 0, p2 = 0, p3 = 0, m2sum = 0, m3sum = 0;
-    long long d_m23 = 0;                 // DGKS basis columns read in the last sweep (profile bytes)
+    long long spmv = 0, cgs = 0, p2 = 0, p3 = 0, m2sum = 0, m3sum = 0, nexact = 0, mxsum = 0;
+    double dgks_bytes = 0.0;             // algorithmic bytes of the DGKS corrections of the last sweep
     int graph_sweeps = 0;                // sweeps replayed from a captured graph
     int ret = HL_NOT_CONVERGED;
     int err = HL_OK;                     // a HIP or allocation failure of this member
@@ -95,13 +96,15 @@ struct Member {
 
     int start(hipStream_t s);            // start vector, norm, first sweep [0, nev0)
     int rephase(hipStream_t s);          // FP64 phase from the start vector already in ws.r
-    double vbytes() const { return kind == SEG_F64 ? 8.0 : 4.0; } // bytes per basis entry
+    bool f32_phase() const { return kind == SEG_V32 || kind == SEG_V32_A32; } // FP32 phase of a two-phase solve
+    double vbytes() const { return kind == SEG_F64 ? 8.0 : 4.0; } // bytes per entry of the projected basis
+    bool copy32() const { return A != nullptr && opts.basis_prec == HL_PREC_F64_PROJ32; } // FP32 basis copy
     void absorb(const double *rb);       // hsub, hdia (ncv each) and the Ctl words
     // Host-only (thread-safe across members): restart logic; writes the restart
     // Q (ncv x vq_N) or the Ritz-vector Q (ncv x ncv) to qslot.
     void decide(double *qslot);
     int host_apply(int j, hipStream_t s); // SpMV of a host operator for step j
-    double cgs_bytes_pass(int m) const;  // algorithmic bytes of one CGS pass
+    double cgs_bytes_pass(int m, bool sub) const; // algorithmic bytes of one CGS pass (sub: EPI_PASS1)
     void complete();                     // outputs and info once the engine synchronised
 
   private:

@@ -46,8 +46,9 @@ template <typename AT, typename XT, typename VT>
 __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict__ rp, const int *__restrict__ col,
                                           const AT *__restrict__ val, const XT *__restrict__ x,
                                           const double *__restrict__ xscale, double *__restrict__ y,
-                                          VT *__restrict__ vout, long long vld, int vj, bool vtiled,
-                                          double *__restrict__ chunk_part, unsigned *__restrict__ chunk_cnt)
+                                          VT *__restrict__ vout, float *__restrict__ vout32, long long vld,
+                                          int vj, bool vtiled, double *__restrict__ chunk_part,
+                                          unsigned *__restrict__ chunk_cnt)
 {
     __shared__ double prod[kNnzPerBlock];
     __shared__ double wred[kBlock / kWave];
@@ -80,8 +81,11 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
             if (i < nnz) prod[i] = static_cast<double>(av[k]) * (static_cast<double>(xv[k]) * s);
         }
         if (vout) {
-            for (int r = b.row0 + tid; r < b.row1; r += kBlock)
-                vout[vcol_pos(r, vld, vj, vtiled)] = static_cast<VT>(static_cast<double>(x[r]) * s);
+            for (int r = b.row0 + tid; r < b.row1; r += kBlock) {
+                const double xs = static_cast<double>(x[r]) * s;
+                vout[vcol_pos(r, vld, vj, vtiled)] = static_cast<VT>(xs);
+                if (vout32) vout32[vcol_pos(r, vld, vj, vtiled)] = static_cast<float>(xs);
+            }
         }
         __syncthreads();
         const int tpr = b.tpr;
@@ -110,8 +114,11 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
                (static_cast<double>(x[__builtin_nontemporal_load(&col[p])]) * s);
     const double tot = block_sum_to_thread0(acc, wred);
     if (tid != 0) return;
-    if (vout && b.chunk == 0)
-        vout[vcol_pos(b.row0, vld, vj, vtiled)] = static_cast<VT>(static_cast<double>(x[b.row0]) * s);
+    if (vout && b.chunk == 0) {
+        const double xs = static_cast<double>(x[b.row0]) * s;
+        vout[vcol_pos(b.row0, vld, vj, vtiled)] = static_cast<VT>(xs);
+        if (vout32) vout32[vcol_pos(b.row0, vld, vj, vtiled)] = static_cast<float>(xs);
+    }
     if (b.nchunks == 1) {
         y[b.row0] = tot;
         return;
@@ -135,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void spmv_adaptive(
     double *__restrict__ y, XT *__restrict__ vout, long long vld, int vj, bool vtiled,
     double *__restrict__ chunk_part, unsigned *__restrict__ chunk_cnt)
 {
-    spmv_item<AT, XT, XT>(blks[blockIdx.x], rp, col, val, x, xscale, y, vout, vld, vj, vtiled, chunk_part,
+    spmv_item<AT, XT, XT>(blks[blockIdx.x], rp, col, val, x, xscale, y, vout, nullptr, vld, vj, vtiled, chunk_part,
                           chunk_cnt);
 }
 
@@ -166,8 +173,8 @@ __global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__res
     const SegStep st = S[i];
     const SegMember mb = M[st.mem];
     spmv_item<AT, double, VT>(mb.blks[blockIdx.x - st.spmv0], mb.rp, mb.col, seg_values<AT>(mb), mb.r,
-                              &mb.ctl->xscale, mb.w, static_cast<VT *>(mb.V), mb.vld, st.j, true, mb.chunk_part,
-                              mb.chunk_cnt);
+                              &mb.ctl->xscale, mb.w, static_cast<VT *>(mb.V), mb.V32, mb.vld, st.j, true,
+                              mb.chunk_part, mb.chunk_cnt);
 }
 
 template <typename T>
@@ -366,6 +373,7 @@ int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_tot
     if (ns <= 0 || blocks_total <= 0) return HL_OK;
     switch (kind) {
     case SEG_F64:
+    case SEG_PROJ32: // (writes the FP32 copy of the basis column too)
         spmv_seg_kernel<double, double><<<blocks_total, kBlock, 0, s>>>(M, S, ns);
         break;
     case SEG_V32:

@@ -41,6 +41,9 @@ extern "C" {
 /* Storage precision of matrix values or of the Krylov basis. */
 #define HL_PREC_F64 0
 #define HL_PREC_F32 1
+/* Basis only: the FP64 basis plus an FP32 copy of it that the projection-only
+ * Gram-Schmidt passes read (see hl_options.basis_prec). */
+#define HL_PREC_F64_PROJ32 2
 
 typedef struct hl_csr_s *hl_csr_t;
 
@@ -176,12 +179,17 @@ typedef struct {
     int maxiter;       /* restart cap; <= 0 -> 200 (HLanc.hpp:1682) */
     const double *v0;  /* host start vector of length n, or NULL: seeded generator */
     uint64_t seed;     /* generator seed when v0 == NULL */
-    int basis_prec;    /* HL_PREC_F64 (reference semantics), or HL_PREC_F32: mixed precision
-                          for device CSR operators -- a first phase keeps the basis in FP32
-                          (and uses the operator's FP32 values when it was created with them)
-                          to a relative Ritz estimate of 1e-6, then the solve restarts in FP64
-                          from the sum of that phase's wanted Ritz vectors and returns FP64
-                          results to tol */
+    int basis_prec;    /* HL_PREC_F64 (reference semantics);
+                          HL_PREC_F64_PROJ32: the FP64 basis plus an FP32 copy -- each
+                          Lanczos step projects onto the FP32 copy, subtracts with the FP64
+                          basis, and always runs the DGKS correction on the FP32 copy, so
+                          the basis stays orthogonal to FP64 rounding while the projection
+                          passes read half the bytes (host operators run plain FP64);
+                          HL_PREC_F32: two phases for device CSR operators -- the basis in
+                          FP32 (and the operator's FP32 values when it was created with
+                          them) to a relative Ritz estimate of 1e-6, then an FP64 restart
+                          from the sum of that phase's wanted Ritz vectors.  Every mode
+                          returns FP64 results to tol. */
     int use_mfma;      /* 1 (default): restart and Ritz-vector V.Q on v_mfma_f64_16x16x4f64
                           (a restart whose K x N block of Q exceeds 48 KB runs on the FMA
                           kernel); 0: FMA kernel.  The results differ in rounding only. */
@@ -205,7 +213,8 @@ typedef struct {
                                (included in spmv) */
     int restarts_f32;       /* mixed precision: restarts of the FP32-basis phase (included in
                                restarts) */
-    int reserved_info;
+    int proj64_steps;       /* FP32-copy projections: Lanczos steps whose DGKS correction read
+                               the FP64 basis (strong cancellation) */
 } hl_info;
 
 /* hl_options.flags bits. */

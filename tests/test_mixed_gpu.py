@@ -1,10 +1,13 @@
-"""Mixed precision (basis_prec = HL_PREC_F32): a first phase keeps the Krylov
-basis in FP32 (and the values in FP32 when the operator holds them) to a 1e-6
-Ritz estimate; the solve then restarts in FP64 from the sum of that phase's
-wanted Ritz vectors.  What it returns must be FP64 results: eigenvalues within
-1e-10 of the plain FP64 solve and of the CPU oracle, every pair with
-||A v - l v|| <= 1e-10 ||A||, orthonormal vectors.  A mixed solve is bitwise
-the same alone and inside a batch next to FP64 solves."""
+"""Mixed precision.  Two modes, both returning FP64 results (eigenvalues
+within 1e-10 of the plain FP64 solve and of the CPU oracle, every pair with
+||A v - l v|| <= 1e-10 ||A||, orthonormal vectors), and both bitwise the same
+alone and inside a batch next to solves of the other modes:
+
+* HL_PREC_F64_PROJ32: the FP64 basis plus an FP32 copy that the projection-only
+  Gram-Schmidt passes read; the DGKS correction runs every step.
+* HL_PREC_F32: a first phase keeps the Krylov basis in FP32 (and the values in
+  FP32 when the operator holds them) to a 1e-6 Ritz estimate; the solve then
+  restarts in FP64 from the sum of that phase's wanted Ritz vectors."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -56,6 +59,46 @@ def test_mixed_solve_returns_fp64_results(case, fp32_values):
     assert ref.info.spmv_f32 == 0 and ref.info.refine_shift == 0.0
 
 
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_fp32_copy_projections_return_fp64_results(case):
+    gen, nev, ncv = CASES[case]
+    rp, col, val = gen()
+    A = sp.csr_matrix((val, col, rp))
+    n = A.shape[0]
+    op = op_for(rp, col, val)
+    ref = H.ds_solver_gpu(n, nev, ncv, op, tol=TOL, maxiter=3000)
+    pj = H.ds_solver_gpu(n, nev, ncv, op, tol=TOL, maxiter=3000, basis_prec=H.HL_PREC_F64_PROJ32)
+    assert ref.ret == 0 and pj.ret == 0
+    check_fp64_quality(A, pj, nev, ref.values)
+    # the first DGKS correction runs every step: two counted passes per Lanczos step
+    assert pj.info.cgs_passes >= 2 * pj.info.spmv
+    assert pj.info.spmv_f32 == 0
+
+
+def test_fp32_copy_projections_match_oracle():
+    rp, col, val = W.stencil27(14, seed=3)
+    A = sp.csr_matrix((val, col, rp))
+    n = A.shape[0]
+    pj = H.ds_solver_gpu(n, 16, 48, op_for(rp, col, val), tol=TOL, maxiter=2000, basis_prec=H.HL_PREC_F64_PROJ32)
+    o = O.ds_solver(O.Csr(rp, col, val), 16, 48, tol=TOL, maxiter=2000)
+    assert pj.ret == 0 and o.ret == 0
+    check_fp64_quality(A, pj, 16, o.values)
+
+
+def test_fp32_copy_projections_with_a_host_operator_run_fp64():
+    rp, col, val = W.laplacian_2d(16)
+    A = sp.csr_matrix((val, col, rp))
+
+    def apply(x, y):
+        y[:] = A @ x
+
+    a = H.ds_solver_gpu(256, 4, 12, apply, tol=TOL, maxiter=500, basis_prec=H.HL_PREC_F64_PROJ32)
+    b = H.ds_solver_gpu(256, 4, 12, apply, tol=TOL, maxiter=500)
+    assert a.ret == 0
+    np.testing.assert_array_equal(a.values, b.values)
+    np.testing.assert_array_equal(a.vectors, b.vectors)
+
+
 def test_mixed_solve_matches_oracle():
     rp, col, val = W.stencil27(10, seed=5)
     A = sp.csr_matrix((val, col, rp))
@@ -66,15 +109,18 @@ def test_mixed_solve_matches_oracle():
     check_fp64_quality(A, mix, 10, o.values)
 
 
-def test_mixed_and_fp64_in_one_batch_are_bitwise_their_standalone_solves():
+def test_all_precision_modes_in_one_batch_are_bitwise_their_standalone_solves():
     mats = [W.stencil27(9, seed=1), W.chung_lu_laplacian(1 << 13, max_degree=1500, seed=2),
             W.laplacian_2d(30), W.stencil27(7, seed=4)]
+    mats += [W.chung_lu_laplacian(1 << 12, max_degree=800, seed=8), W.stencil27(8, seed=9)]
     ops = [op_for(*m, fp32_values=(i % 2 == 0)) for i, m in enumerate(mats)]
-    precs = [H.HL_PREC_F32, H.HL_PREC_F64, H.HL_PREC_F32, H.HL_PREC_F32]
-    nev, ncv = [6, 6, 5, 4], [18, 18, 15, 12]
+    precs = [H.HL_PREC_F32, H.HL_PREC_F64, H.HL_PREC_F32, H.HL_PREC_F64_PROJ32, H.HL_PREC_F64_PROJ32,
+             H.HL_PREC_F64]
+    nev, ncv = [6, 6, 5, 4, 5, 6], [18, 18, 15, 12, 16, 18]
+    k = len(ops)
     alone = [H.ds_solver_gpu(op.nrows, nev[i], ncv[i], op, tol=TOL, maxiter=3000, seed=50 + i, basis_prec=precs[i])
              for i, op in enumerate(ops)]
-    batch = H.ds_solve_batch(ops, nev, ncv, tol=TOL, maxiter=3000, seeds=[50 + i for i in range(4)],
+    batch = H.ds_solve_batch(ops, nev, ncv, tol=TOL, maxiter=3000, seeds=[50 + i for i in range(k)],
                              basis_prec=precs)
     for b, a in zip(batch, alone):
         assert a.ret == 0 and b.status == 0
