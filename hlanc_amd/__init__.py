@@ -347,7 +347,7 @@ class SolveResult:
 
 
 def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mfma: Optional[bool] = None,
-            basis_prec: int = HL_PREC_F64, flags: int = 0) -> Options:
+            basis_prec: Optional[int] = None, flags: int = 0) -> Options:
     o = Options()
     lib().hl_options_default(ctypes.byref(o))
     o.tol = tol
@@ -355,7 +355,8 @@ def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mf
     o.seed = seed
     if use_mfma is not None:  # None keeps the library default (hl_options_default)
         o.use_mfma = int(use_mfma)
-    o.basis_prec = basis_prec
+    if basis_prec is not None:  # None keeps the library default (hl_options_default)
+        o.basis_prec = basis_prec
     o.flags = flags
     if v0 is not None:
         o._v0 = np.ascontiguousarray(v0, dtype=np.float64)
@@ -374,16 +375,16 @@ def _vectors_buffer(out, ldv: int, ncv: int) -> np.ndarray:
 def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, v0=None,
                   seed: int = 0, use_mfma: Optional[bool] = None, ldv: Optional[int] = None,
                   flags: int = 0, vectors_out: Optional[np.ndarray] = None,
-                  basis_prec: int = HL_PREC_F64) -> SolveResult:
+                  basis_prec: Optional[int] = None) -> SolveResult:
     """ds_solver_gpu (HLanc.hpp:1868-1872).  ``op`` is a dcsrmv_operator_gpu
     (device-resident path) or any callable ``op(x, y)`` on numpy arrays (host
     path, as the reference calls arbitrary operators with host pointers).
     ``ret`` is 0 on success and 1 on the reference's numerical failures; values
     and vectors are written only on success.  ``vectors_out`` (column-major
     float64, at least ldv x ncv, e.g. from pinned_empty) receives the vectors
-    instead of a fresh array.  ``basis_prec=HL_PREC_F32`` runs the
-    mixed-precision solve (device operators only): an FP32-basis phase, then
-    an FP64 phase that returns FP64 results to ``tol``."""
+    instead of a fresh array.  ``basis_prec`` (None: the library default,
+    HL_PREC_F64_PROJ32) selects the basis storage; every mode returns FP64
+    results to ``tol`` (hlanc_capi.h, hl_options.basis_prec)."""
     ldv = n if ldv is None else ldv
     values = np.zeros(ncv)
     vectors = _vectors_buffer(vectors_out, ldv, ncv)
@@ -427,7 +428,7 @@ class BatchResult:
 def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, seeds=None,
                    v0s=None, use_mfma: Optional[bool] = None, vectors: bool = True, arrivals=None,
                    max_active: int = 0, hbm_budget: float = 0.0, flags: int = 0, vectors_out=None,
-                   basis_prec=HL_PREC_F64):
+                   basis_prec=None):
     """Many solves on one device with continuous batching (hl_ds_solve_batch).
 
     ``ops`` are dcsrmv_operator_gpu on ``dev``; nev, ncv, tol, maxiter,
@@ -452,7 +453,7 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
         seed = 0 if seeds is None else int(seeds[i])
         v0 = None if v0s is None else v0s[i]
         o = options(float(per(tol, i)), int(per(maxiter, i)), v0, seed, per(use_mfma, i),
-                    basis_prec=int(per(basis_prec, i)), flags=flags)
+                    basis_prec=per(basis_prec, i), flags=flags)
         it = items[i]
         it.A = op.h
         it.nev = nev_i

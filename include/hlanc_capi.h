@@ -179,8 +179,9 @@ typedef struct {
     int maxiter;       /* restart cap; <= 0 -> 200 (HLanc.hpp:1682) */
     const double *v0;  /* host start vector of length n, or NULL: seeded generator */
     uint64_t seed;     /* generator seed when v0 == NULL */
-    int basis_prec;    /* HL_PREC_F64 (reference semantics);
-                          HL_PREC_F64_PROJ32: the FP64 basis plus an FP32 copy -- each
+    int basis_prec;    /* HL_PREC_F64 (the reference's DGKS semantics exactly);
+                          HL_PREC_F64_PROJ32 (hl_options_default): the FP64 basis plus an
+                          FP32 copy -- each
                           Lanczos step projects onto the FP32 copy, subtracts with the FP64
                           basis, and always runs the DGKS correction on the FP32 copy, so
                           the basis stays orthogonal to FP64 rounding while the projection
