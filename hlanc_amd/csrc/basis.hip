@@ -157,7 +157,7 @@ typedef float flt2 __attribute__((ext_vector_type(2)));
 // tile (one 128-byte line), so a wave covers the tile as 4 lane groups of
 // ceil(m/8) column pairs and each load instruction moves 4 full lines, as
 // the FP64 pass does.  Arithmetic is FP64.
-template <int MAXC8, int FL>
+template <int MAXC8, int FL, int TB>
 __device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, const float *__restrict__ V,
                                               long long ldv, const double *w_in, double *w_out,
                                               const double *__restrict__ h_in, double *h_out, double *partials)
@@ -184,8 +184,7 @@ __device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, con
     double nrm = 0.0;
     const long long ntiles = (n + kTileRows - 1) / kTileRows;
     const long long W = (long long)nb * (kBlock / kWave);
-    // two tiles per trip: both tiles' loads are in flight before either is used
-    constexpr int TB = 2;
+    // TB tiles per trip: all their loads are in flight before any is used
     for (long long t = (long long)bid * (kBlock / kWave) + wv; t < ntiles; t += TB * W) {
         flt2 v[TB][MAXC8]; // kept in FP32 registers, widened where used
         double w0[TB], w1[TB];
@@ -364,7 +363,7 @@ __device__ __forceinline__ PassPtrs pass_ptrs(const SegMember &mb, int mode)
 // selects the block numbering of the launch (SegStep::cgs0s, else cgs0).
 // With an FP32 copy, a step's first correction runs on the copy unless the
 // step cancelled strongly (ctl->exact): then an FP64 launch of its own runs it.
-template <int MAXC, int MODE, typename VT>
+template <int MAXC, int MODE, typename VT, int TB = 2>
 __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__restrict__ M,
                                                          const SegStep *__restrict__ S, int ns, int sub)
 {
@@ -377,7 +376,8 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
     const int bid = blockIdx.x - (sub ? st.cgs0s : st.cgs0);
     if constexpr (F32) {
         if ((MODE == EPI_PASS2 || MODE == EPI_PASS3) && !mb.ctl->need) return; // DGKS decided on the device
-        cgs_block_f32<(MAXC + 1) / 2, 1>(bid, mb.nb, mb.n, st.j + 1, static_cast<const float *>(mb.Vproj), mb.vld,
+        cgs_block_f32<(MAXC + 1) / 2, 1, TB>(bid, mb.nb, mb.n, st.j + 1, static_cast<const float *>(mb.Vproj),
+                                             mb.vld,
                                          p.w_in, p.w_out, p.h_in, p.h_out, mb.partials);
     } else {
         cgs_block<MAXC, double, true, 9>(bid, mb.nb, mb.n, st.j + 1, static_cast<const double *>(mb.V), mb.vld,
@@ -999,10 +999,10 @@ int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn
 }
 
 namespace {
-template <int MODE, typename VT>
+template <int MODE, typename VT, int TB = 2>
 void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg, int blocks, int sub, hipStream_t s)
 {
-#define HL_SEG(MC) cgs_seg_kernel<MC, MODE, VT><<<blocks, kBlock, 0, s>>>(M, S, ns, sub)
+#define HL_SEG(MC) cgs_seg_kernel<MC, MODE, VT, TB><<<blocks, kBlock, 0, s>>>(M, S, ns, sub)
     if (max_cpg <= 2)
         HL_SEG(2);
     else if (max_cpg <= 4)
@@ -1027,9 +1027,10 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
         set_error("CGS pass: more than 128 columns");
         return HL_ERR_ARG;
     }
-#define HL_CGS_MODE(MODE)                                                                \
-    (v32 ? cgs_seg_dispatch<MODE, float>(M, S, ns, max_cpg, cgs_blocks_total, sub, s)    \
-         : cgs_seg_dispatch<MODE, double>(M, S, ns, max_cpg, cgs_blocks_total, sub, s))
+#define HL_CGS_MODE(MODE)                                                                              \
+    (!v32 ? cgs_seg_dispatch<MODE, double>(M, S, ns, max_cpg, cgs_blocks_total, sub, s)                \
+     : tuning().cgs_f32_tiles == 1 ? cgs_seg_dispatch<MODE, float, 1>(M, S, ns, max_cpg, cgs_blocks_total, sub, s) \
+                                   : cgs_seg_dispatch<MODE, float, 2>(M, S, ns, max_cpg, cgs_blocks_total, sub, s))
     switch (mode) {
     case EPI_WNORM:
         HL_CGS_MODE(EPI_WNORM);

@@ -501,6 +501,8 @@ int hl_tune_set(const char *key, int value)
         t.cgs_flags = value;
     else if (!std::strcmp(key, "f32_phase_tol_exp") && value >= 3 && value <= 9)
         t.f32_phase_tol_exp = value;
+    else if (!std::strcmp(key, "cgs_f32_tiles") && (value == 1 || value == 2))
+        t.cgs_f32_tiles = value;
     else {
         set_error("hl_tune_set: unknown key or value out of range: %s=%d", key, value);
         return HL_ERR_ARG;
@@ -518,6 +520,8 @@ int hl_tune_get(const char *key, int *value)
         *value = t.cgs_flags;
     else if (!std::strcmp(key, "f32_phase_tol_exp"))
         *value = t.f32_phase_tol_exp;
+    else if (!std::strcmp(key, "cgs_f32_tiles"))
+        *value = t.cgs_f32_tiles;
     else {
         set_error("hl_tune_get: unknown key %s", key);
         return HL_ERR_ARG;

@@ -80,6 +80,7 @@ enum Epi : int {
 // variant of the standalone pass (hl_cgs_pass, hl_bench_cgs) only.
 struct Tuning {
     int f32_phase_tol_exp = 6;  // mixed precision: the FP32 phase stops at a 10^-exp Ritz estimate
+    int cgs_f32_tiles = 1;      // tiles per trip of the FP32-basis CGS passes (1 or 2; equal speed, 1 fewer VGPRs)
     int cgs_blocks_max = 2048; // grid cap of the CGS passes (<= kMaxReduceBlocks)
     int cgs_flags = 9;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile,
                                // bit 3: split (partials kernel + finalize kernel); bit 2 unused

@@ -149,7 +149,8 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
  * prefetch of the next tile, bit 3 finish each pass in a second kernel
  * instead of a last-block reduction; bit 2 is refused), "f32_phase_tol_exp"
  * (mixed precision: the FP32-basis phase ends at relative Ritz estimates of
- * 10^-value, 3..9, default 6).
+ * 10^-value, 3..9, default 6), "cgs_f32_tiles" (16-row tiles per trip of the
+ * FP32-basis CGS passes, 1 or 2, default 1).
  * The solvers always run the split pass over a 16-row tiled basis; cgs_flags
  * selects the variant of hl_cgs_pass and hl_bench_cgs.  No counterpart in the
  * reference (its kernels are fixed). */
