@@ -387,8 +387,10 @@ def main():
     items = None
     if not args.no_autotune:
         nc, rc_, t_best = op.autotune(reps=10)
-        items = {"nnz_cap": nc, "rows_cap": rc_}
-        log(f"SpMV autotune: items of <= {nc} nnz / <= {rc_} rows, {t_best * 1e6:.1f} us per launch")
+        sw, npairs = op.slices()
+        items = {"nnz_cap": nc, "rows_cap": rc_, "slice_columns": sw, "virtual_rows": npairs}
+        log(f"SpMV autotune: items of <= {nc} nnz / <= {rc_} rows, column slices of {sw} "
+            f"({npairs} virtual rows), {t_best * 1e6:.1f} us per launch")
 
     vec_out = H.pinned_empty(n, ncv)  # page-locked output: the final copy runs at link rate
     for i in range(args.warmup):

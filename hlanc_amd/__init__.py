@@ -98,6 +98,8 @@ _SIGS = [
     ("hl_csr_spmv_bytes", ctypes.c_double, [ctypes.c_void_p, ctypes.c_int]),
     ("hl_csr_set_items", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("hl_csr_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip, ctypes.POINTER(ctypes.c_double)]),
+    ("hl_csr_set_slices", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hl_csr_get_slices", ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
     ("hl_cgs_pass", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p]),
@@ -289,11 +291,21 @@ class dcsrmv_operator_gpu:
         _check(lib().hl_csr_set_items(self.h, int(nnz_cap), int(rows_cap)))
 
     def autotune(self, reps: int = 20):
-        """Pick the fastest work-item shape (hl_csr_autotune); returns
-        (nnz_cap, rows_cap, seconds per SpMV)."""
+        """Pick the fastest work-item shape and column slicing (hl_csr_autotune);
+        returns (nnz_cap, rows_cap, seconds per SpMV); slices() tells the slicing."""
         nc, rc_, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _check(lib().hl_csr_autotune(self.h, int(reps), ctypes.byref(nc), ctypes.byref(rc_), ctypes.byref(t)))
         return nc.value, rc_.value, t.value
+
+    def set_slices(self, width: int) -> None:
+        """Column-sliced SpMV with slices of `width` columns; 0 drops it (hl_csr_set_slices)."""
+        _check(lib().hl_csr_set_slices(self.h, int(width)))
+
+    def slices(self):
+        """(slice width, virtual rows) of the sliced form; (0, 0) when unsliced."""
+        w, npr = ctypes.c_int(), ctypes.c_int()
+        _check(lib().hl_csr_get_slices(self.h, ctypes.byref(w), ctypes.byref(npr)))
+        return w.value, npr.value
 
 
 class dcsrmv_operator_mgpu(dcsrmv_operator_gpu):

@@ -258,9 +258,14 @@ double hl_csr_spmv_bytes(hl_csr_t h, int val_prec)
 namespace {
 int csr_items(hl_csr_t h, int nnz_cap, int rows_cap)
 {
-    std::vector<int> R(static_cast<size_t>(h->plan.nrows) + 1);
-    HL_TRY_HIP(hipMemcpy(R.data(), h->plan.row_ptr, sizeof(int) * R.size(), hipMemcpyDeviceToHost));
-    return plan_items(h->plan, R.data(), nnz_cap, rows_cap);
+    CsrPlan &p = h->plan;
+    std::vector<int> R(static_cast<size_t>(p.nrows) + 1);
+    HL_TRY_HIP(hipMemcpy(R.data(), p.row_ptr, sizeof(int) * R.size(), hipMemcpyDeviceToHost));
+    int rc = plan_items(p.items, R.data(), p.nrows, nnz_cap, rows_cap);
+    if (rc || p.slice_w == 0) return rc;
+    R.resize(static_cast<size_t>(p.npairs) + 1); // the sliced form's items take the same shape
+    HL_TRY_HIP(hipMemcpy(R.data(), p.s_rp, sizeof(int) * R.size(), hipMemcpyDeviceToHost));
+    return plan_items(p.sitems, R.data(), p.npairs, nnz_cap, rows_cap);
 }
 } // namespace
 
@@ -270,6 +275,22 @@ int hl_csr_set_items(hl_csr_t h, int nnz_cap, int rows_cap)
     DeviceGuard g(h->plan.dev);
     std::lock_guard<std::mutex> lk(h->mu);
     return csr_items(h, nnz_cap, rows_cap);
+}
+
+int hl_csr_set_slices(hl_csr_t h, int width)
+{
+    if (!h) return HL_ERR_ARG;
+    DeviceGuard g(h->plan.dev);
+    std::lock_guard<std::mutex> lk(h->mu);
+    return plan_slices(h->plan, width);
+}
+
+int hl_csr_get_slices(hl_csr_t h, int *width, int *npairs)
+{
+    if (!h) return HL_ERR_ARG;
+    if (width) *width = h->plan.slice_w;
+    if (npairs) *npairs = h->plan.npairs;
+    return HL_OK;
 }
 
 int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *seconds)
@@ -298,6 +319,13 @@ int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *s
     }
     const int ncap[] = {512, 1024, 2048};
     const int rcap[] = {64, 128, 256, 512, 1024, 2048};
+    // column slices only where x (8 ncols bytes) exceeds an XCD's 4 MB L2 twice over
+    std::vector<int> widths{0};
+    if (p.ncols >= (1 << 20))
+        for (int w : {1 << 17, 1 << 18, 1 << 19}) widths.push_back(w);
+    int best_w = 0;
+    for (int wdt : widths) {
+        if ((rc = plan_slices(p, wdt))) break;
     for (int nc : ncap)
         for (int rcp : rcap) {
             if ((rc = csr_items(h, nc, rcp))) break;
@@ -314,11 +342,15 @@ int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *s
                 best_t = t;
                 best_n = nc;
                 best_r = rcp;
+                best_w = wdt;
             }
         }
+        if (rc || e != hipSuccess) break;
+    }
     if (!rc && e != hipSuccess) rc = hip_status(e, "hl_csr_autotune");
     cleanup();
     if (rc) return rc;
+    if ((rc = plan_slices(p, best_w))) return rc;
     if ((rc = csr_items(h, best_n, best_r))) return rc;
     if (nnz_cap) *nnz_cap = best_n;
     if (rows_cap) *rows_cap = best_r;

@@ -555,11 +555,15 @@ int Engine::sweep(std::vector<Member *> &ms)
         Member &m = *act[i];
         Workspace &W = m.ws;
         SegMember &M = hM[i];
-        M.blks = m.A ? m.A->blks : nullptr;
-        M.rp = m.A ? m.A->row_ptr : nullptr;
-        M.col = m.A ? m.A->col : nullptr;
-        M.val = m.A ? m.A->val : nullptr;
+        const bool sl = m.sliced();
+        M.blks = !m.A ? nullptr : sl ? m.A->sitems.blks : m.A->items.blks;
+        M.rp = !m.A ? nullptr : sl ? m.A->s_rp : m.A->row_ptr;
+        M.col = !m.A ? nullptr : sl ? m.A->s_col : m.A->col;
+        M.val = !m.A ? nullptr : sl ? m.A->s_val : m.A->val;
         M.val32 = m.A ? m.A->val32 : nullptr;
+        M.part = sl ? W.part : nullptr;
+        M.r_pp = sl ? m.A->r_pp : nullptr;
+        M.r_pairs = sl ? m.A->r_pairs : nullptr;
         M.chunk_part = W.chunk_part;
         M.chunk_cnt = W.chunk_cnt;
         M.V = W.V[W.cur];
@@ -588,7 +592,8 @@ int Engine::sweep(std::vector<Member *> &ms)
     std::vector<std::array<int, 4>> goff_spmv(maxsteps), goff_proj(maxsteps), goff_sub(maxsteps);
     std::vector<std::array<int, 3>> nspmv(maxsteps);
     std::vector<std::array<int, 2>> nproj(maxsteps), nsub(maxsteps), cpg_proj(maxsteps), cpg_sub(maxsteps);
-    std::vector<int> off(maxsteps + 1, 0), nfin(maxsteps, 0), n32(maxsteps, 0), nb32(maxsteps, 0), cpg32(maxsteps, 0);
+    std::vector<int> off(maxsteps + 1, 0), nfin(maxsteps, 0), n32(maxsteps, 0), nb32(maxsteps, 0), cpg32(maxsteps, 0),
+        nred(maxsteps, 0);
     size_t ns = 0;
     for (int st = 0; st < maxsteps; ++st) {
         off[st] = static_cast<int>(ns);
@@ -612,7 +617,9 @@ int Engine::sweep(std::vector<Member *> &ms)
                 S.cgs0 = nproj[st][gp];
                 S.cgs0s = nsub[st][gb];
                 S.fin0 = nfin[st];
-                if (m.A) nspmv[st][gs] += m.A->nblk;
+                S.red0 = nred[st];
+                if (m.A) nspmv[st][gs] += m.sliced() ? m.A->sitems.nblk : m.A->items.nblk;
+                if (m.sliced()) nred[st] += (m.n + kRowsPerSumBlock - 1) / kRowsPerSumBlock;
                 nproj[st][gp] += hM[i].nb;
                 nsub[st][gb] += hM[i].nb;
                 nfin[st] += S.j + 2; // j + 1 projections and the squared norm
@@ -659,6 +666,10 @@ int Engine::sweep(std::vector<Member *> &ms)
                         (r = launch_spmv_seg(dM, dS + goff_spmv[st][g], goff_spmv[st][g + 1] - goff_spmv[st][g],
                                              nspmv[st][g], kSpmvKind[g], stream)))
                         return r;
+                if (nred[st] > 0 && // row sums of the sliced SpMVs (FP64 bases: the last group)
+                    (r = launch_spmv_rows_seg(dM, dS + goff_spmv[st][2], goff_spmv[st][3] - goff_spmv[st][2],
+                                              nred[st], stream)))
+                    return r;
                 if ((r = prof.end(stream, K_SPMV, bytes))) return r;
             }
             for (int p = 0; p < 4; ++p) {
@@ -695,7 +706,7 @@ int Engine::sweep(std::vector<Member *> &ms)
     } else {
         // the sweep's shape: table addresses and every step's segment and grid sizes
         std::vector<long long> key;
-        key.reserve(2 + 27 * (size_t)maxsteps);
+        key.reserve(2 + 28 * (size_t)maxsteps);
         key.push_back(reinterpret_cast<long long>(dM));
         key.push_back(reinterpret_cast<long long>(dS));
         for (int st = 0; st < maxsteps; ++st) {
@@ -713,6 +724,7 @@ int Engine::sweep(std::vector<Member *> &ms)
                 key.push_back(cpg_sub[st][g]);
             }
             key.push_back(nfin[st]);
+            key.push_back(nred[st]);
             key.push_back(n32[st]);
             key.push_back(nb32[st]);
             key.push_back(cpg32[st]);
@@ -899,7 +911,7 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
     double in_use = 0.0;
     int rc = HL_OK;
     auto need = [](const Member *m) {
-        return Workspace::bytes_needed(m->n, m->ncv, m->A == nullptr, m->A ? m->A->nslots : 0, m->copy32());
+        return Workspace::bytes_needed(m->n, m->ncv, m->A == nullptr, m->nslots(), m->npairs(), m->copy32());
     };
     // members done since the last call leave the active set; their workspaces
     // are freed at the next point where the device is idle
@@ -936,7 +948,7 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
             Member *m = queue[next++];
             m->t_admit = t;
             m->phase = Member::NEW;
-            if ((m->err = m->ws.init(dev, m->n, m->ncv, m->A == nullptr, m->A ? m->A->nslots : 0, m->copy32(),
+            if ((m->err = m->ws.init(dev, m->n, m->ncv, m->A == nullptr, m->nslots(), m->npairs(), m->copy32(),
                                      stream))) {
                 m->ws.release();
                 m->phase = Member::DONE;

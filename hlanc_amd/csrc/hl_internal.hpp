@@ -123,6 +123,17 @@ constexpr int kRowsPerBlock = 256;  // default rows per stream item
 constexpr int kMaxRowsPerBlock = 2048;
 constexpr int kLongChunk = 8192;    // nnz per chunk of a long row
 
+// The work items of one CSR (its rows, or the virtual rows of the sliced form)
+// and the long-row slots of standalone calls.
+struct ItemSet {
+    SpmvBlk *blks = nullptr;  // device
+    int nblk = 0;
+    double *chunk_part = nullptr;
+    unsigned *chunk_cnt = nullptr;
+    int nslots = 0;
+    int nnz_cap = kNnzPerBlock, rows_cap = kRowsPerBlock; // work-item shape in use
+};
+
 struct CsrPlan {
     int dev = 0;
     int nrows = 0, ncols = 0;
@@ -131,20 +142,33 @@ struct CsrPlan {
     int *col = nullptr;       // device
     double *val = nullptr;    // device FP64 values
     float *val32 = nullptr;   // device FP32 values (optional)
-    SpmvBlk *blks = nullptr;  // device
-    int nblk = 0;
-    double *chunk_part = nullptr;
-    unsigned *chunk_cnt = nullptr;
-    int nslots = 0;
+    ItemSet items;            // over the rows
     double *x_stage = nullptr, *y_stage = nullptr;  // for host-pointer calls
-    int nnz_cap = kNnzPerBlock, rows_cap = kRowsPerBlock; // work-item shape in use
+    // Column-sliced form of the FP64 matrix (slice_w > 0): the nonzeros regrouped
+    // into virtual rows (row, slice) of slice_w columns each, slice-major and
+    // rows ascending within a slice, so that the x gathers of one stretch of
+    // work items stay within one slice (L2-sized).  Each virtual row's sum is a
+    // partial; a second kernel adds a row's partials in slice order.
+    int slice_w = 0;
+    int npairs = 0;           // virtual rows
+    int *s_rp = nullptr;      // npairs + 1
+    int *s_col = nullptr;     // nnz, regrouped
+    double *s_val = nullptr;  // nnz, regrouped
+    int *r_pp = nullptr;      // nrows + 1: each row's partials in r_pairs
+    int *r_pairs = nullptr;   // npairs: virtual row of each partial, slice order
+    double *s_part = nullptr; // npairs: partials of standalone calls
+    ItemSet sitems;           // over the virtual rows
 };
 
 int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *col,
                    const double *val, int sym_lower, int val_prec);
 void csr_plan_free(CsrPlan &p);
-// Re-cuts the work items (see hl_csr_set_items); R is a host copy of row_ptr.
-int plan_items(CsrPlan &p, const int *R, int nnz_cap, int rows_cap);
+// Re-cuts the work items of a CSR of `nrows` rows (host row pointers R) into
+// `set` (see hl_csr_set_items).
+int plan_items(ItemSet &set, const int *R, int nrows, int nnz_cap, int rows_cap);
+// Builds (width > 0) or drops (width == 0) the column-sliced form; its items
+// take the plan's current item shape.
+int plan_slices(CsrPlan &p, int width);
 
 // y = A (x * xscale) with the optional fused write vcol(r) = x[r] * xscale.
 // xscale == nullptr means 1.  val_prec selects the FP64 or FP32 value array.
@@ -179,10 +203,12 @@ constexpr int kSegOrder[SEG_NKIND] = {SEG_V32, SEG_V32_A32, SEG_PROJ32, SEG_F64}
 // against r; the second correction always reads the FP64 basis.
 constexpr double kProjExact = 1e-2;
 struct SegMember {               // per solve, fixed during a sweep
-    const SpmvBlk *blks;         // CSR operator (null for host operators)
+    const SpmvBlk *blks;         // CSR operator (null for host operators): items over rp/col/val
     const int *rp, *col;
     const double *val;
     const float *val32;          // FP32 values (SEG_V32_A32 solves)
+    double *part;                // sliced form: partials of the virtual rows (rp/col/val are the
+    const int *r_pp, *r_pairs;   // virtual CSR), added per row by the second kernel; else null
     double *chunk_part;
     unsigned *chunk_cnt;
     void *V;                     // tiled basis in use (double, or float for SEG_V32*)
@@ -201,6 +227,7 @@ struct SegStep {                 // per solve and Lanczos step
     int cgs0;                    // first block in the launches of EPI_WNORM and EPI_PASS2
     int cgs0s;                   // first block in the launches of EPI_PASS1 and EPI_PASS3
     int fin0;                    // first block in the finalize launch
+    int red0;                    // first block in the row-sum launch of sliced SpMVs
 };
 // Partials kernel of CGS pass `mode` (EPI_WNORM .. EPI_PASS3) for every listed
 // step, then the per-column finalize kernel (j + 2 blocks per step).
@@ -211,6 +238,10 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
 int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fin_blocks_total, int mode,
                             hipStream_t s);
 int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, hipStream_t s);
+// Second kernel of the sliced SpMVs among S[0, ns): w = row sums of the
+// partials, V(:, j) = r xscale (and its FP32 copy).
+int launch_spmv_rows_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s);
+constexpr int kRowsPerSumBlock = 256; // rows per block of the row-sum kernels
 
 // Restart V.Q of one solve within a segmented launch (tiled bases, ld K).
 struct SegVQ {

@@ -214,6 +214,7 @@ void Workspace::release()
     blob = hblob = nullptr;
     V[0] = V[1] = nullptr;
     V32[0] = V32[1] = nullptr;
+    part = nullptr;
     r = w = hsub = hdia = hbuf = xv = chunk_part = nullptr;
     chunk_cnt = nullptr;
     ctl = nullptr;
@@ -237,8 +238,8 @@ struct Carve {
 };
 
 struct WsLayout {
-    size_t V0, V1, S0, S1, r, w, hsub, hdia, hbuf, ctl, xv, cpart, ccnt, part, cnt, total;
-    WsLayout(int n, int ncv, bool host_op, int nslots, bool copy32)
+    size_t V0, V1, S0, S1, r, w, hsub, hdia, hbuf, ctl, xv, spart, cpart, ccnt, part, cnt, total;
+    WsLayout(int n, int ncv, bool host_op, int nslots, int npairs, bool copy32)
     {
         const size_t ldv = static_cast<size_t>(padded_rows(n));
         Carve c;
@@ -253,6 +254,7 @@ struct WsLayout {
         hbuf = c.take(16 * (size_t)ncv);
         ctl = c.take(sizeof(Ctl));
         xv = host_op ? c.take(8 * ldv) : 0;
+        spart = npairs > 0 ? c.take(8 * (size_t)npairs) : 0;
         cpart = nslots > 0 ? c.take(8 * (size_t)nslots) : 0;
         ccnt = nslots > 0 ? c.take(4 * (size_t)nslots) : 0;
         part = c.take(8 * (size_t)kMaxReduceBlocks * (ncv + 1));
@@ -262,19 +264,19 @@ struct WsLayout {
 };
 } // namespace
 
-double Workspace::bytes_needed(int n, int ncv, bool host_op, int nslots, bool copy32)
+double Workspace::bytes_needed(int n, int ncv, bool host_op, int nslots, int npairs, bool copy32)
 {
-    return static_cast<double>(WsLayout(n, ncv, host_op, nslots, copy32).total);
+    return static_cast<double>(WsLayout(n, ncv, host_op, nslots, npairs, copy32).total);
 }
 
-int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, bool copy32, hipStream_t s)
+int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, int npairs, bool copy32, hipStream_t s)
 {
     dev = dev_;
     n = n_;
     ncv = ncv_;
     ldv = padded_rows(n);
     cur = 0;
-    const WsLayout L(n, ncv, host_op, nslots, copy32);
+    const WsLayout L(n, ncv, host_op, nslots, npairs, copy32);
     HL_TRY_HIP(hipMalloc(&blob, L.total));
     char *b = static_cast<char *>(blob);
     V[0] = reinterpret_cast<double *>(b + L.V0);
@@ -288,6 +290,7 @@ int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, bool c
     hbuf = reinterpret_cast<double *>(b + L.hbuf);
     ctl = reinterpret_cast<Ctl *>(b + L.ctl);
     xv = host_op ? reinterpret_cast<double *>(b + L.xv) : nullptr;
+    part = npairs > 0 ? reinterpret_cast<double *>(b + L.spart) : nullptr;
     chunk_part = nslots > 0 ? reinterpret_cast<double *>(b + L.cpart) : nullptr;
     chunk_cnt = nslots > 0 ? reinterpret_cast<unsigned *>(b + L.ccnt) : nullptr;
     red.partials = reinterpret_cast<double *>(b + L.part);

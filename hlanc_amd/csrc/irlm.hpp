@@ -1,6 +1,7 @@
 // irlm.hpp -- solve state and the multi-solve engine of libhlanc_hip.so.
 #pragma once
 
+#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -19,6 +20,7 @@ struct Workspace {
     double *r = nullptr, *w = nullptr;  // residual and A v_j
     double *hsub = nullptr, *hdia = nullptr, *hbuf = nullptr;
     double *xv = nullptr;               // contiguous v_j for host operators
+    double *part = nullptr;             // partials of a sliced SpMV (npairs)
     double *chunk_part = nullptr;       // this solve's long-row partial sums (nslots)
     unsigned *chunk_cnt = nullptr;      // and arrival counters, so that solves of
                                         // one operator can share an SpMV launch
@@ -31,11 +33,13 @@ struct Workspace {
     // The ping-pong bases are tiled: 16-row blocks of ncv columns.
     VLayout vl() const { return VLayout{ncv, true}; }
     ~Workspace();
-    int init(int dev, int n, int ncv, bool host_op, int nslots, bool copy32, hipStream_t s); // zeroing on s
+    int init(int dev, int n, int ncv, bool host_op, int nslots, int npairs, bool copy32,
+             hipStream_t s); // zeroing on s
     // Hands the two allocations to the caller (to free once the device is idle)
     // and clears the workspace.
     std::pair<void *, void *> detach();
-    static double bytes_needed(int n, int ncv, bool host_op, int nslots, bool copy32); // device bytes of init
+    static double bytes_needed(int n, int ncv, bool host_op, int nslots, int npairs,
+                               bool copy32); // device bytes of init
     void release();
 };
 
@@ -99,6 +103,10 @@ struct Member {
     bool f32_phase() const { return kind == SEG_V32 || kind == SEG_V32_A32; } // FP32 phase of a two-phase solve
     double vbytes() const { return kind == SEG_F64 ? 8.0 : 4.0; } // bytes per entry of the projected basis
     bool copy32() const { return A != nullptr && opts.basis_prec == HL_PREC_F64_PROJ32; } // FP32 basis copy
+    // the operator's column-sliced form serves the SpMV (FP64 values and basis)
+    bool sliced() const { return A != nullptr && A->slice_w > 0 && (kind == SEG_F64 || kind == SEG_PROJ32); }
+    int nslots() const { return A ? std::max(A->items.nslots, A->sitems.nslots) : 0; }
+    int npairs() const { return A ? A->npairs : 0; }
     void absorb(const double *rb);       // hsub, hdia (ncv each) and the Ctl words
     // Host-only (thread-safe across members): restart logic; writes the restart
     // Q (ncv x vq_N) or the Ritz-vector Q (ncv x ncv) to qslot.

@@ -172,9 +172,59 @@ __global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__res
     const int i = seg_find_spmv(S, ns, blockIdx.x);
     const SegStep st = S[i];
     const SegMember mb = M[st.mem];
+    const bool sliced = mb.part != nullptr; // partials now; rows and the basis column in the row-sum kernel
     spmv_item<AT, double, VT>(mb.blks[blockIdx.x - st.spmv0], mb.rp, mb.col, seg_values<AT>(mb), mb.r,
-                              &mb.ctl->xscale, mb.w, static_cast<VT *>(mb.V), mb.V32, mb.vld, st.j, true,
-                              mb.chunk_part, mb.chunk_cnt);
+                              &mb.ctl->xscale, sliced ? mb.part : mb.w, sliced ? nullptr : static_cast<VT *>(mb.V),
+                              sliced ? nullptr : mb.V32, mb.vld, st.j, true, mb.chunk_part, mb.chunk_cnt);
+}
+
+// Rows [r0, r1) of a sliced SpMV: y(r) = its partials added in slice order,
+// and the basis column v(r) = x(r) s (and its FP32 copy) written beside.
+__device__ __forceinline__ void spmv_rows(int r0, int r1, const int *__restrict__ r_pp,
+                                          const int *__restrict__ r_pairs, const double *__restrict__ part,
+                                          const double *__restrict__ x, double s, double *__restrict__ y,
+                                          double *__restrict__ vout, float *__restrict__ vout32, long long vld,
+                                          int vj, bool vtiled)
+{
+    for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        double acc = 0.0;
+        for (int k = r_pp[r]; k < r_pp[r + 1]; ++k) acc += part[r_pairs[k]];
+        y[r] = acc;
+        if (vout) {
+            const double xs = x[r] * s;
+            vout[vcol_pos(r, vld, vj, vtiled)] = xs;
+            if (vout32) vout32[vcol_pos(r, vld, vj, vtiled)] = static_cast<float>(xs);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void spmv_rows_seg_kernel(const SegMember *__restrict__ M,
+                                                               const SegStep *__restrict__ S, int ns)
+{
+    int lo = 0, hi = ns - 1; // the step owning this block (by red0)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (S[mid].red0 <= (int)blockIdx.x)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const SegStep st = S[lo];
+    const SegMember mb = M[st.mem];
+    const int r0 = (blockIdx.x - st.red0) * kRowsPerSumBlock;
+    spmv_rows(r0, min(r0 + kRowsPerSumBlock, mb.n), mb.r_pp, mb.r_pairs, mb.part, mb.r, mb.ctl->xscale, mb.w,
+              static_cast<double *>(mb.V), mb.V32, mb.vld, st.j, true);
+}
+
+__global__ __launch_bounds__(kBlock) void spmv_rows_kernel(int n, const int *__restrict__ r_pp,
+                                                           const int *__restrict__ r_pairs,
+                                                           const double *__restrict__ part, const double *x,
+                                                           const double *xscale, double *y, double *vout,
+                                                           long long vld, int vj, bool vtiled)
+{
+    const int r0 = blockIdx.x * kRowsPerSumBlock;
+    spmv_rows(r0, min(r0 + kRowsPerSumBlock, n), r_pp, r_pairs, part, x, xscale ? *xscale : 1.0, y, vout, nullptr,
+              vld, vj, vtiled);
 }
 
 template <typename T>
@@ -260,21 +310,20 @@ int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *c
     }
     if ((rc = dev_upload<double>(&p.x_stage, nullptr, static_cast<size_t>(ncols)))) return rc;
     if ((rc = dev_upload<double>(&p.y_stage, nullptr, static_cast<size_t>(nrows)))) return rc;
-    return plan_items(p, R, kNnzPerBlock, kRowsPerBlock);
+    return plan_items(p.items, R, nrows, kNnzPerBlock, kRowsPerBlock);
 }
 
 // Cuts the rows into work items by an nnz walk over row_ptr (host copy R):
 // stream items of <= rows_cap whole rows holding <= nnz_cap nnz, and chunk items
 // of <= kLongChunk nnz for every row longer than nnz_cap.  Replaces the plan's
 // items and long-row slots.
-int plan_items(CsrPlan &p, const int *R, int nnz_cap, int rows_cap)
+int plan_items(ItemSet &p, const int *R, int nrows, int nnz_cap, int rows_cap)
 {
     if (nnz_cap < kBlock || nnz_cap > kNnzPerBlock || rows_cap < 1 || rows_cap > kMaxRowsPerBlock) {
         set_error("SpMV items: nnz cap %d (256..%d) or rows cap %d (1..%d) out of range", nnz_cap, kNnzPerBlock,
                   rows_cap, kMaxRowsPerBlock);
         return HL_ERR_ARG;
     }
-    const int nrows = p.nrows;
     std::vector<SpmvBlk> blks;
     int slots = 0;
     for (int r = 0; r < nrows;) {
@@ -335,9 +384,127 @@ int plan_items(CsrPlan &p, const int *R, int nnz_cap, int rows_cap)
     return HL_OK;
 }
 
+namespace {
+void items_free(ItemSet &t)
+{
+    void *ptrs[] = {t.blks, t.chunk_part, t.chunk_cnt};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    t = ItemSet{};
+}
+
+void slices_free(CsrPlan &p)
+{
+    void *ptrs[] = {p.s_rp, p.s_col, p.s_val, p.r_pp, p.r_pairs, p.s_part};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    p.s_rp = p.s_col = p.r_pp = p.r_pairs = nullptr;
+    p.s_val = p.s_part = nullptr;
+    p.slice_w = 0;
+    p.npairs = 0;
+    items_free(p.sitems);
+}
+} // namespace
+
+// Virtual rows (row, slice): slice-major, rows ascending within a slice, each
+// virtual row's entries in the row's own order.  r_pp/r_pairs list every row's
+// virtual rows in slice order (the order its partials are added in).
+int plan_slices(CsrPlan &p, int width)
+{
+    if (width < 0) {
+        set_error("SpMV slices: width %d < 0", width);
+        return HL_ERR_ARG;
+    }
+    slices_free(p);
+    if (width == 0 || p.nrows == 0) return HL_OK;
+    const int n = p.nrows;
+    const long long nnz = p.nnz;
+    std::vector<int> R(static_cast<size_t>(n) + 1), C(static_cast<size_t>(nnz));
+    std::vector<double> A(static_cast<size_t>(nnz));
+    HL_TRY_HIP(hipMemcpy(R.data(), p.row_ptr, sizeof(int) * R.size(), hipMemcpyDeviceToHost));
+    if (nnz > 0) {
+        HL_TRY_HIP(hipMemcpy(C.data(), p.col, sizeof(int) * C.size(), hipMemcpyDeviceToHost));
+        HL_TRY_HIP(hipMemcpy(A.data(), p.val, sizeof(double) * A.size(), hipMemcpyDeviceToHost));
+    }
+    const int ns = (p.ncols + width - 1) / width;
+    // pass 1: virtual rows per slice and entries per virtual row
+    std::vector<long long> srows(static_cast<size_t>(ns) + 1, 0);
+    std::vector<int> last(static_cast<size_t>(ns), -1);
+    for (int r = 0; r < n; ++r)
+        for (int q = R[r]; q < R[r + 1]; ++q) {
+            const int sl = C[q] / width;
+            if (last[sl] != r) {
+                last[sl] = r;
+                ++srows[sl + 1];
+            }
+        }
+    for (int sl = 0; sl < ns; ++sl) srows[sl + 1] += srows[sl];
+    const long long np = srows[ns];
+    if (np > 0x7fffffffLL) {
+        set_error("SpMV slices: %lld virtual rows exceed int32", np);
+        return HL_ERR_ARG;
+    }
+    // pass 2: each row's virtual rows (slice order) and their sizes
+    std::vector<int> vrow_len(static_cast<size_t>(np), 0), rpp(static_cast<size_t>(n) + 1, 0),
+        rpairs(static_cast<size_t>(np));
+    std::vector<long long> fill(srows.begin(), srows.end() - 1);
+    std::vector<int> slot(static_cast<size_t>(ns), -1);
+    std::fill(last.begin(), last.end(), -1);
+    std::vector<int> touched;
+    for (int r = 0; r < n; ++r) {
+        touched.clear();
+        for (int q = R[r]; q < R[r + 1]; ++q) {
+            const int sl = C[q] / width;
+            if (last[sl] != r) {
+                last[sl] = r;
+                slot[sl] = static_cast<int>(fill[sl]++);
+                touched.push_back(sl);
+            }
+            ++vrow_len[slot[sl]];
+        }
+        std::sort(touched.begin(), touched.end());
+        rpp[r + 1] = rpp[r] + static_cast<int>(touched.size());
+        for (size_t t = 0; t < touched.size(); ++t) rpairs[rpp[r] + t] = slot[touched[t]];
+    }
+    std::vector<int> srp(static_cast<size_t>(np) + 1, 0);
+    for (long long v = 0; v < np; ++v) srp[v + 1] = srp[v] + vrow_len[v];
+    // pass 3: the entries, each row's entries of a slice in their own order
+    std::vector<int> sc(static_cast<size_t>(nnz));
+    std::vector<double> sv(static_cast<size_t>(nnz));
+    std::vector<int> put(srp.begin(), srp.end() - 1);
+    std::fill(last.begin(), last.end(), -1);
+    std::fill(fill.begin(), fill.end(), 0);
+    for (int sl = 0; sl < ns; ++sl) fill[sl] = srows[sl];
+    for (int r = 0; r < n; ++r)
+        for (int q = R[r]; q < R[r + 1]; ++q) {
+            const int sl = C[q] / width;
+            if (last[sl] != r) {
+                last[sl] = r;
+                slot[sl] = static_cast<int>(fill[sl]++);
+            }
+            const int at = put[slot[sl]]++;
+            sc[at] = C[q];
+            sv[at] = A[q];
+        }
+    int rc;
+    p.slice_w = width;
+    p.npairs = static_cast<int>(np);
+    if ((rc = dev_upload(&p.s_rp, srp.data(), srp.size())) || (rc = dev_upload(&p.s_col, sc.data(), sc.size())) ||
+        (rc = dev_upload(&p.s_val, sv.data(), sv.size())) || (rc = dev_upload(&p.r_pp, rpp.data(), rpp.size())) ||
+        (rc = dev_upload(&p.r_pairs, rpairs.data(), rpairs.size())) ||
+        (rc = dev_upload<double>(&p.s_part, nullptr, static_cast<size_t>(np))) ||
+        (rc = plan_items(p.sitems, srp.data(), p.npairs, p.items.nnz_cap, p.items.rows_cap))) {
+        slices_free(p);
+        return rc;
+    }
+    return HL_OK;
+}
+
 void csr_plan_free(CsrPlan &p)
 {
-    void *ptrs[] = {p.row_ptr, p.col, p.val, p.val32, p.blks, p.chunk_part, p.chunk_cnt, p.x_stage, p.y_stage};
+    slices_free(p);
+    items_free(p.items);
+    void *ptrs[] = {p.row_ptr, p.col, p.val, p.val32, p.x_stage, p.y_stage};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p = CsrPlan{};
@@ -346,19 +513,29 @@ void csr_plan_free(CsrPlan &p)
 int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double *y, VColumn vcol,
                 int val_prec, hipStream_t s)
 {
-    if (p.nblk == 0) return HL_OK;
     if (val_prec == HL_PREC_F32) {
         if (!p.val32) {
             set_error("SpMV: FP32 values were not requested at creation");
             return HL_ERR_STATE;
         }
-        spmv_adaptive<float, double><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val32, x, xscale, y,
-                                                                vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled,
-                                                                p.chunk_part, p.chunk_cnt);
+        if (p.items.nblk == 0) return HL_OK;
+        spmv_adaptive<float, double><<<p.items.nblk, kBlock, 0, s>>>(
+            p.items.blks, p.row_ptr, p.col, p.val32, x, xscale, y, vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled,
+            p.items.chunk_part, p.items.chunk_cnt);
+    } else if (p.slice_w > 0) { // partials of the virtual rows, then the row sums
+        if (p.sitems.nblk > 0)
+            spmv_adaptive<double, double><<<p.sitems.nblk, kBlock, 0, s>>>(
+                p.sitems.blks, p.s_rp, p.s_col, p.s_val, x, xscale, p.s_part, nullptr, 0, 0, false,
+                p.sitems.chunk_part, p.sitems.chunk_cnt);
+        HL_TRY_HIP(hipGetLastError());
+        if (p.nrows > 0)
+            spmv_rows_kernel<<<(p.nrows + kRowsPerSumBlock - 1) / kRowsPerSumBlock, kBlock, 0, s>>>(
+                p.nrows, p.r_pp, p.r_pairs, p.s_part, x, xscale, y, vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled);
     } else {
-        spmv_adaptive<double, double><<<p.nblk, kBlock, 0, s>>>(p.blks, p.row_ptr, p.col, p.val, x, xscale, y,
-                                                                 vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled,
-                                                                 p.chunk_part, p.chunk_cnt);
+        if (p.items.nblk == 0) return HL_OK;
+        spmv_adaptive<double, double><<<p.items.nblk, kBlock, 0, s>>>(
+            p.items.blks, p.row_ptr, p.col, p.val, x, xscale, y, vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled,
+            p.items.chunk_part, p.items.chunk_cnt);
     }
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
@@ -367,6 +544,14 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
 }  // namespace hl
 
 namespace hl {
+
+int launch_spmv_rows_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s)
+{
+    if (ns <= 0 || blocks_total <= 0) return HL_OK;
+    spmv_rows_seg_kernel<<<blocks_total, kBlock, 0, s>>>(M, S, ns);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
 
 int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, hipStream_t s)
 {

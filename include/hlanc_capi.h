@@ -118,10 +118,22 @@ double hl_csr_spmv_bytes(hl_csr_t h, int val_prec);
  * that use the operator.  No counterpart in the reference (cuSPARSE picks its
  * own schedule). */
 int hl_csr_set_items(hl_csr_t h, int nnz_cap, int rows_cap);
+/* Column-sliced SpMV: width > 0 regroups the FP64 matrix into virtual rows
+ * (row, slice of `width` columns), slice-major, so that the x gathers of a
+ * stretch of work items stay inside an L2-sized slice of x; a second kernel
+ * adds each row's partials in slice order.  width = 0 drops the form.  Used by
+ * every FP64-value SpMV of the operator (standalone calls and FP64-basis
+ * solves); costs 12 nnz + 8 virtual rows bytes on the device and 8 bytes per
+ * virtual row per solve.  Same rules as hl_csr_set_items.  No counterpart in
+ * the reference. */
+int hl_csr_set_slices(hl_csr_t h, int width);
+int hl_csr_get_slices(hl_csr_t h, int *width, int *npairs); /* outputs may be NULL */
 /* Autotuner: times `reps` SpMV launches for every shape of a small grid
- * (nnz_cap in {512, 1024, 2048} x rows_cap in {64 .. 2048}) with HIP events on
- * the null stream, keeps the fastest and reports it (outputs may be NULL).
- * Same rules as hl_csr_set_items. */
+ * (nnz_cap in {512, 1024, 2048} x rows_cap in {64 .. 2048}), and, for
+ * operators of >= 2^20 columns, every shape again under column slices of 2^17,
+ * 2^18 and 2^19 columns, with HIP events on the null stream; keeps the fastest
+ * and reports its shape (outputs may be NULL; hl_csr_get_slices tells the
+ * slicing).  Same rules as hl_csr_set_items. */
 int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *seconds);
 
 /* ------------------------------------------------------ basis operations

@@ -177,3 +177,88 @@ def test_autotune_then_solve_matches_oracle():
     o = O.ds_solver(O.Csr(rp, col, val), 8, 24, tol=1e-10, maxiter=1000)
     assert r.ret == 0 and o.ret == 0
     np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
+
+
+def ragged_matrix(seed=5, n=777, ncols=50000):
+    """Empty rows, a row of 2049 nnz, rows of 8192 / 8193 / 40000 (chunked),
+    unsorted columns with duplicates, n % 64 != 0."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 40, n)
+    lens[::9] = 0
+    lens[10], lens[20], lens[30], lens[40] = 2049, 8192, 8193, 40000
+    lens[-1] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = rng.integers(0, ncols, rp[-1]).astype(np.int32)
+    val = rng.uniform(-1, 1, rp[-1])
+    return rp, col, val, ncols
+
+
+@pytest.mark.parametrize("width", [1, 64, 4096, 1 << 17])
+@pytest.mark.parametrize("case", ["ragged", "chunglu", "stencil"])
+def test_column_sliced_spmv_matches_oracle(case, width):
+    """hl_csr_set_slices: virtual rows (row, slice), partials added per row in
+    slice order; every item shape, chunked virtual rows and empty rows."""
+    if case == "ragged":
+        rp, col, val, ncols = ragged_matrix()
+    elif case == "chunglu":
+        rp, col, val = W.chung_lu_laplacian(1 << 15, max_degree=9000, seed=3)
+        ncols = len(rp) - 1
+    else:
+        rp, col, val = W.stencil27(16, seed=2)
+        ncols = len(rp) - 1
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, ncols=ncols))
+    op.set_slices(width)
+    w, npairs = op.slices()
+    # every (row, slice) pair holding entries is one virtual row
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    assert w == width and npairs == len(np.unique(rows.astype(np.int64) * (ncols // width + 1) + col // width))
+    x = np.random.default_rng(7).uniform(-1, 1, ncols)
+    y_ref = O.spmv(O.Csr(rp, col, val, ncols, False), x)
+    scale = np.linalg.norm(y_ref)
+    for nc, rc in ((2048, 256), (256, 1), (512, 2048)):
+        op.set_items(nc, rc)
+        y = op(x)
+        assert np.linalg.norm(y - y_ref) <= TOL * scale
+        np.testing.assert_array_equal(op(x), y)  # repeatable
+    op.set_slices(0)
+    assert op.slices() == (0, 0)
+    assert np.linalg.norm(op(x) - y_ref) <= TOL * scale
+
+
+def test_sliced_operator_solves_match_oracle_alone_and_batched():
+    rp, col, val = W.chung_lu_laplacian(1 << 14, max_degree=2000, seed=11)
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    op.set_slices(2048)
+    o = O.ds_solver(O.Csr(rp, col, val), 8, 24, tol=1e-10, maxiter=1000)
+    A = sp.csr_matrix((val, col, rp))
+    alone = []
+    for prec in (H.HL_PREC_F64, H.HL_PREC_F64_PROJ32, H.HL_PREC_F32):
+        r = H.ds_solver_gpu(n, 8, 24, op, tol=1e-10, maxiter=1000, basis_prec=prec)
+        assert r.ret == 0 and o.ret == 0
+        np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
+        for i in range(8):
+            v = r.vectors[:, i]
+            assert np.linalg.norm(A @ v - r.values[i] * v) <= 1e-10 * np.abs(r.values).max()
+        alone.append(r)
+    plain = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    precs = [H.HL_PREC_F64, H.HL_PREC_F64_PROJ32, H.HL_PREC_F32, H.HL_PREC_F64]
+    b = H.ds_solve_batch([op, op, op, plain], 8, 24, tol=1e-10, maxiter=1000, basis_prec=precs)
+    for k in range(3):
+        np.testing.assert_array_equal(b[k].values, alone[k].values)
+        np.testing.assert_array_equal(b[k].vectors, alone[k].vectors)
+    r_plain = H.ds_solver_gpu(n, 8, 24, plain, tol=1e-10, maxiter=1000, basis_prec=H.HL_PREC_F64)
+    np.testing.assert_array_equal(b[3].values, r_plain.values)
+
+
+def test_autotune_considers_slices_on_large_operators():
+    rp, col, val = W.chung_lu_laplacian(1 << 20, seed=1)
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    nc, rc, t = op.autotune(reps=3)
+    w, npairs = op.slices()
+    assert w in (0, 1 << 17, 1 << 18, 1 << 19) and (w == 0) == (npairs == 0) and t > 0
+    x = np.random.default_rng(8).uniform(-1, 1, n)
+    y_ref = O.spmv(O.Csr(rp, col, val, n, False), x)
+    assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
