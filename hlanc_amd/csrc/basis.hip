@@ -376,9 +376,14 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
     const int bid = blockIdx.x - (sub ? st.cgs0s : st.cgs0);
     if constexpr (F32) {
         if ((MODE == EPI_PASS2 || MODE == EPI_PASS3) && !mb.ctl->need) return; // DGKS decided on the device
-        cgs_block_f32<(MAXC + 1) / 2, 1, TB>(bid, mb.nb, mb.n, st.j + 1, static_cast<const float *>(mb.Vproj),
-                                             mb.vld,
-                                         p.w_in, p.w_out, p.h_in, p.h_out, mb.partials);
+        const float *V = static_cast<const float *>(mb.Vproj);
+        if (MODE == EPI_WNORM && mb.V32) { // FP32 copy: only alpha_j = v_j^T w (three-term first projection)
+            cgs_block_f32<(MAXC + 1) / 2, 1, TB>(bid, mb.nb, mb.n, 1, V + 16 * st.j, mb.vld, p.w_in, p.w_out,
+                                                 p.h_in, p.h_out, mb.partials);
+            return;
+        }
+        cgs_block_f32<(MAXC + 1) / 2, 1, TB>(bid, mb.nb, mb.n, st.j + 1, V, mb.vld, p.w_in, p.w_out, p.h_in,
+                                             p.h_out, mb.partials);
     } else {
         cgs_block<MAXC, double, true, 9>(bid, mb.nb, mb.n, st.j + 1, static_cast<const double *>(mb.V), mb.vld,
                                          p.w_in, p.w_out, p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub,
@@ -393,8 +398,21 @@ __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMembe
     const SegStep st = S[i];
     const SegMember mb = M[st.mem];
     const PassPtrs p = pass_ptrs(mb, mode);
-    cgs_finalize_block(blockIdx.x - st.fin0, mb.nb, st.j + 1, mb.partials, p.h_in, p.h_out, mb.ctl, mb.hsub, mb.hdia,
-                       mode, st.j);
+    const int c = blockIdx.x - st.fin0;
+    if (mode == EPI_WNORM && mb.V32) {
+        // FP32 copy: the first projection is the Lanczos three-term one,
+        // h0 = (0, .., 0, beta_j, alpha_j) with beta_j = ||r|| of the last
+        // step (HLanc.hpp:1431-1438 keeps it as h_sub[j]); the subtracting
+        // pass's exact FP64 projection catches whatever else w holds along V
+        if (c > 1) return;
+        if (c == 1) {
+            for (int x = threadIdx.x; x + 1 < st.j; x += kBlock) p.h_out[x] = 0.0;
+            if (threadIdx.x == 0 && st.j > 0) p.h_out[st.j - 1] = mb.ctl->rnorm;
+        }
+        cgs_finalize_block(c, mb.nb, 1, mb.partials, p.h_in, p.h_out + st.j, mb.ctl, mb.hsub, mb.hdia, mode, st.j);
+        return;
+    }
+    cgs_finalize_block(c, mb.nb, st.j + 1, mb.partials, p.h_in, p.h_out, mb.ctl, mb.hsub, mb.hdia, mode, st.j);
 }
 
 typedef double dv2 __attribute__((ext_vector_type(2)));

@@ -48,6 +48,7 @@ double now_s()
 double Member::cgs_bytes_pass(int m, bool sub) const
 {
     const double b = sub ? (f32_phase() ? 4.0 : 8.0) : vbytes();
+    if (!sub && kind == SEG_PROJ32) m = 1; // the first pass reads column j only (three-term projection)
     return b * n * m + 16.0 * n;
 }
 
