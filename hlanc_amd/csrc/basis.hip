@@ -272,6 +272,36 @@ __device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, con
     for (int c = threadIdx.x; c < M1; c += kBlock) partials[(long long)c * nb + bid] = vals[c];
 }
 
+// Block partials of alpha = v^T w and ||w||^2 for one column v of an FP32
+// tiled basis (v points at its first tile entry; ld columns per tile row):
+// every lane active, row-coalesced.  Written as partials[c * nb + bid],
+// c = 0 (alpha) and 1 (norm), the layout the finalize reads for m = 1.
+__device__ __forceinline__ void coldot_block_f32(int bid, int nb, int n, const float *__restrict__ v, long long ldv,
+                                                 const double *__restrict__ w, double *partials)
+{
+    __shared__ double red[2][kBlock / kWave];
+    double a = 0.0, s2 = 0.0;
+    for (long long r = (long long)bid * kBlock + threadIdx.x; r < n; r += (long long)nb * kBlock) {
+        const double wr = w[r];
+        a += static_cast<double>(__builtin_nontemporal_load(&v[vpos<true>(r, 0, ldv)])) * wr;
+        s2 += wr * wr;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        s2 += __shfl_xor(s2, off);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[0][threadIdx.x / kWave] = a;
+        red[1][threadIdx.x / kWave] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) t += red[threadIdx.x][k];
+        partials[(long long)threadIdx.x * nb + bid] = t;
+    }
+}
+
 template <int MAXC, typename VT, bool TILED, int FL>
 __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__restrict__ V, long long ldv,
                                                      const double *w_in, double *w_out,
@@ -378,8 +408,7 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
         if ((MODE == EPI_PASS2 || MODE == EPI_PASS3) && !mb.ctl->need) return; // DGKS decided on the device
         const float *V = static_cast<const float *>(mb.Vproj);
         if (MODE == EPI_WNORM && mb.V32) { // FP32 copy: only alpha_j = v_j^T w (three-term first projection)
-            cgs_block_f32<(MAXC + 1) / 2, 1, TB>(bid, mb.nb, mb.n, 1, V + 16 * st.j, mb.vld, p.w_in, p.w_out,
-                                                 p.h_in, p.h_out, mb.partials);
+            coldot_block_f32(bid, mb.nb, mb.n, V + 16 * st.j, mb.vld, p.w_in, mb.partials);
             return;
         }
         cgs_block_f32<(MAXC + 1) / 2, 1, TB>(bid, mb.nb, mb.n, st.j + 1, V, mb.vld, p.w_in, p.w_out, p.h_in,
