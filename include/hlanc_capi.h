@@ -194,11 +194,12 @@ typedef struct {
     uint64_t seed;     /* generator seed when v0 == NULL */
     int basis_prec;    /* HL_PREC_F64 (the reference's DGKS semantics exactly);
                           HL_PREC_F64_PROJ32 (hl_options_default): the FP64 basis plus an
-                          FP32 copy -- each
-                          Lanczos step projects onto the FP32 copy, subtracts with the FP64
-                          basis, and always runs the DGKS correction on the FP32 copy, so
-                          the basis stays orthogonal to FP64 rounding while the projection
-                          passes read half the bytes (host operators run plain FP64);
+                          FP32 copy -- each Lanczos step takes the three-term first
+                          projection (alpha_j from column j of the copy, beta_j known),
+                          subtracts with the FP64 basis while projecting exactly, and
+                          always runs the DGKS correction on the FP32 copy, so the basis
+                          stays orthogonal to FP64 rounding while reading 12 instead of
+                          24 bytes per entry (host operators run plain FP64);
                           HL_PREC_F32: two phases for device CSR operators -- the basis in
                           FP32 (and the operator's FP32 values when it was created with
                           them) to a relative Ritz estimate of 1e-6, then an FP64 restart
