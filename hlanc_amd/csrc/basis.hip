@@ -272,12 +272,13 @@ __device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, con
     for (int c = threadIdx.x; c < M1; c += kBlock) partials[(long long)c * nb + bid] = vals[c];
 }
 
-// Block partials of alpha = v^T w and ||w||^2 for one column v of an FP32
-// tiled basis (v points at its first tile entry; ld columns per tile row):
+// Block partials of alpha = v^T w and ||w||^2 for one column v of a tiled
+// basis (v points at its first tile entry; ld columns per tile row):
 // every lane active, row-coalesced.  Written as partials[c * nb + bid],
 // c = 0 (alpha) and 1 (norm), the layout the finalize reads for m = 1.
-__device__ __forceinline__ void coldot_block_f32(int bid, int nb, int n, const float *__restrict__ v, long long ldv,
-                                                 const double *__restrict__ w, double *partials)
+template <typename VT>
+__device__ __forceinline__ void coldot_block(int bid, int nb, int n, const VT *__restrict__ v, long long ldv,
+                                             const double *__restrict__ w, double *partials)
 {
     __shared__ double red[2][kBlock / kWave];
     double a = 0.0, s2 = 0.0;
@@ -407,8 +408,9 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
     if constexpr (F32) {
         if ((MODE == EPI_PASS2 || MODE == EPI_PASS3) && !mb.ctl->need) return; // DGKS decided on the device
         const float *V = static_cast<const float *>(mb.Vproj);
-        if (MODE == EPI_WNORM && mb.V32) { // FP32 copy: only alpha_j = v_j^T w (three-term first projection)
-            coldot_block_f32(bid, mb.nb, mb.n, V + 16 * st.j, mb.vld, p.w_in, mb.partials);
+        if (MODE == EPI_WNORM && mb.V32) { // FP32 copy: only alpha_j = v_j^T w, on the FP64 column
+            coldot_block(bid, mb.nb, mb.n, static_cast<const double *>(mb.V) + 16 * st.j, mb.vld, p.w_in,
+                         mb.partials);
             return;
         }
         cgs_block_f32<(MAXC + 1) / 2, 1, TB>(bid, mb.nb, mb.n, st.j + 1, V, mb.vld, p.w_in, p.w_out, p.h_in,
@@ -442,6 +444,34 @@ __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMembe
         return;
     }
     cgs_finalize_block(c, mb.nb, st.j + 1, mb.partials, p.h_in, p.h_out, mb.ctl, mb.hsub, mb.hdia, mode, st.j);
+}
+
+// After the subtracting pass of solves that run the correction every step
+// (FP32-copy projections): the pass's exact FP64 projection h1 = V^T r is
+// what the correction would remove.  When ||h1|| <= tau ||r|| the correction
+// would move r by at most tau ||r||, so it is skipped (ctl->need = 0).  One
+// block per listed step.
+__global__ __launch_bounds__(kBlock) void dgks_decide_seg_kernel(const SegMember *__restrict__ M,
+                                                                 const SegStep *__restrict__ S, double tau2)
+{
+    const SegStep st = S[blockIdx.x];
+    const SegMember mb = M[st.mem];
+    if (!mb.V32 || tau2 <= 0.0) return;
+    __shared__ double red[kBlock / kWave];
+    double s2 = 0.0;
+    for (int c = threadIdx.x; c <= st.j; c += kBlock) s2 += mb.h1[c] * mb.h1[c];
+    for (int off = kWave / 2; off > 0; off >>= 1) s2 += __shfl_xor(s2, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
+        Ctl *ctl = mb.ctl;
+        if (ctl->need && t <= tau2 * ctl->rnorm * ctl->rnorm) {
+            ctl->need = 0;
+            ctl->nskip += 1;
+        }
+    }
 }
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
@@ -1096,6 +1126,14 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
         return HL_ERR_ARG;
     }
 #undef HL_CGS_MODE
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_dgks_decide_seg(const SegMember *M, const SegStep *S, int ns, double tau, hipStream_t s)
+{
+    if (ns <= 0 || tau <= 0.0) return HL_OK;
+    dgks_decide_seg_kernel<<<ns, kBlock, 0, s>>>(M, S, tau * tau);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

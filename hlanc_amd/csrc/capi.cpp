@@ -535,6 +535,8 @@ int hl_tune_set(const char *key, int value)
         t.f32_phase_tol_exp = value;
     else if (!std::strcmp(key, "cgs_f32_tiles") && (value == 1 || value == 2))
         t.cgs_f32_tiles = value;
+    else if (!std::strcmp(key, "dgks_skip_exp") && value >= 0 && value <= 16)
+        t.dgks_skip_exp = value;
     else {
         set_error("hl_tune_set: unknown key or value out of range: %s=%d", key, value);
         return HL_ERR_ARG;
@@ -554,6 +556,8 @@ int hl_tune_get(const char *key, int *value)
         *value = t.f32_phase_tol_exp;
     else if (!std::strcmp(key, "cgs_f32_tiles"))
         *value = t.cgs_f32_tiles;
+    else if (!std::strcmp(key, "dgks_skip_exp"))
+        *value = t.dgks_skip_exp;
     else {
         set_error("hl_tune_get: unknown key %s", key);
         return HL_ERR_ARG;

@@ -47,8 +47,8 @@ double now_s()
 
 double Member::cgs_bytes_pass(int m, bool sub) const
 {
+    if (!sub && kind == SEG_PROJ32) return 8.0 * n + 16.0 * n; // three-term first pass: FP64 column j
     const double b = sub ? (f32_phase() ? 4.0 : 8.0) : vbytes();
-    if (!sub && kind == SEG_PROJ32) m = 1; // the first pass reads column j only (three-term projection)
     return b * n * m + 16.0 * n;
 }
 
@@ -644,8 +644,11 @@ int Engine::sweep(std::vector<Member *> &ms)
     double host_s = now_s() - th0; // host time of this sweep (profiled runs report it)
 
     const int modes[4] = {EPI_WNORM, EPI_PASS1, EPI_PASS2, EPI_PASS3};
-    bool any_host = false;
-    for (Member *m : act) any_host |= (m->A == nullptr);
+    bool any_host = false, any_proj32 = false;
+    for (Member *m : act) {
+        any_host |= (m->A == nullptr);
+        any_proj32 |= m->kind == SEG_PROJ32;
+    }
     auto launch_steps = [&]() -> int {
         int r;
         for (int st = 0; st < maxsteps; ++st) {
@@ -697,6 +700,9 @@ int Engine::sweep(std::vector<Member *> &ms)
                 if ((r = prof.end(stream, p < 2 ? K_CGS : K_DGKS, bytes))) return r;
                 if ((r = prof.begin(stream))) return r;
                 if ((r = launch_cgs_seg_finalize(dM, S, cnt, nfin[st], modes[p], stream))) return r;
+                if (modes[p] == EPI_PASS1 && any_proj32 && tuning().dgks_skip_exp > 0 &&
+                    (r = launch_dgks_decide_seg(dM, S, cnt, std::pow(10.0, -tuning().dgks_skip_exp), stream)))
+                    return r;
                 if ((r = prof.end(stream, K_MISC, 0.0))) return r;
             }
         }

@@ -56,9 +56,11 @@ struct Ctl {
     double nrm2;     // last reduced squared norm (standalone passes)
     long long p2, p3;       // DGKS corrections executed (first, second)
     long long m2sum, m3sum; // sum of their basis widths (for byte counts)
-    int dgks_always;        // run the first correction every step (FP32-copy projections)
-    int exact;              // this step's corrections read the FP64 basis (FP32-copy projections)
+    int dgks_always;        // run the first correction every step (FP32-copy projections) unless
+    int exact;              // its exact coefficients are negligible; this step's corrections
+                            // read the FP64 basis (FP32-copy projections)
     long long nexact, mxsum; // steps with exact corrections and the sum of their basis widths
+    long long nskip;        // corrections skipped as negligible (FP32-copy projections)
 };
 
 static_assert(sizeof(Ctl) % sizeof(double) == 0, "Ctl is read back as doubles");
@@ -81,6 +83,7 @@ enum Epi : int {
 struct Tuning {
     int f32_phase_tol_exp = 6;  // mixed precision: the FP32 phase stops at a 10^-exp Ritz estimate
     int cgs_f32_tiles = 1;      // tiles per trip of the FP32-basis CGS passes (1 or 2; equal speed, 1 fewer VGPRs)
+    int dgks_skip_exp = 13;     // FP32-copy projections: skip the correction when ||h1|| <= 10^-exp ||r|| (0: never)
     int cgs_blocks_max = 2048; // grid cap of the CGS passes (<= kMaxReduceBlocks)
     int cgs_flags = 9;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile,
                                // bit 3: split (partials kernel + finalize kernel); bit 2 unused
@@ -237,6 +240,9 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
                    bool v32, bool sub, hipStream_t s);
 int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fin_blocks_total, int mode,
                             hipStream_t s);
+// Skips the pending first correction of FP32-copy solves whose exact projection
+// h1 satisfies ||h1|| <= tau ||r|| (one block per listed step; tau <= 0: none).
+int launch_dgks_decide_seg(const SegMember *M, const SegStep *S, int ns, double tau, hipStream_t s);
 int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, hipStream_t s);
 // Second kernel of the sliced SpMVs among S[0, ns): w = row sums of the
 // partials, V(:, j) = r xscale (and its FP32 copy).

@@ -162,7 +162,9 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
  * instead of a last-block reduction; bit 2 is refused), "f32_phase_tol_exp"
  * (mixed precision: the FP32-basis phase ends at relative Ritz estimates of
  * 10^-value, 3..9, default 6), "cgs_f32_tiles" (16-row tiles per trip of the
- * FP32-basis CGS passes, 1 or 2, default 1).
+ * FP32-basis CGS passes, 1 or 2, default 1), "dgks_skip_exp" (HL_PREC_F64_PROJ32
+ * solves skip a step's DGKS correction when its exact coefficients h satisfy
+ * ||h|| <= 10^-value ||r||; 0..16, default 13, 0 = never skip).
  * The solvers always run the split pass over a 16-row tiled basis; cgs_flags
  * selects the variant of hl_cgs_pass and hl_bench_cgs.  No counterpart in the
  * reference (its kernels are fixed). */
@@ -197,9 +199,10 @@ typedef struct {
                           FP32 copy -- each Lanczos step takes the three-term first
                           projection (alpha_j from column j of the copy, beta_j known),
                           subtracts with the FP64 basis while projecting exactly, and
-                          always runs the DGKS correction on the FP32 copy, so the basis
-                          stays orthogonal to FP64 rounding while reading 12 instead of
-                          24 bytes per entry (host operators run plain FP64);
+                          runs the DGKS correction on the FP32 copy unless those exact
+                          coefficients are negligible (||h|| <= 1e-13 ||r||), so the basis
+                          stays orthogonal to FP64 rounding while reading at most 12
+                          instead of 24 bytes per entry (host operators run plain FP64);
                           HL_PREC_F32: two phases for device CSR operators -- the basis in
                           FP32 (and the operator's FP32 values when it was created with
                           them) to a relative Ritz estimate of 1e-6, then an FP64 restart

@@ -66,13 +66,22 @@ def test_fp32_copy_projections_return_fp64_results(case):
     A = sp.csr_matrix((val, col, rp))
     n = A.shape[0]
     op = op_for(rp, col, val)
-    ref = H.ds_solver_gpu(n, nev, ncv, op, tol=TOL, maxiter=3000)
+    ref = H.ds_solver_gpu(n, nev, ncv, op, tol=TOL, maxiter=3000, basis_prec=H.HL_PREC_F64)
     pj = H.ds_solver_gpu(n, nev, ncv, op, tol=TOL, maxiter=3000, basis_prec=H.HL_PREC_F64_PROJ32)
     assert ref.ret == 0 and pj.ret == 0
     check_fp64_quality(A, pj, nev, ref.values)
-    # the first DGKS correction runs every step: two counted passes per Lanczos step
-    assert pj.info.cgs_passes >= 2 * pj.info.spmv
     assert pj.info.spmv_f32 == 0
+    # the correction is skipped where its exact coefficients are negligible
+    # (dgks_skip_exp, 1e-13 by default); with skipping off it runs every step
+    assert pj.info.spmv <= pj.info.cgs_passes < 2 * pj.info.spmv
+    old = H.tune_get("dgks_skip_exp")
+    H.tune_set("dgks_skip_exp", 0)
+    try:
+        every = H.ds_solver_gpu(n, nev, ncv, op, tol=TOL, maxiter=3000, basis_prec=H.HL_PREC_F64_PROJ32)
+    finally:
+        H.tune_set("dgks_skip_exp", old)
+    assert every.ret == 0 and every.info.cgs_passes >= 2 * every.info.spmv
+    check_fp64_quality(A, every, nev, ref.values)
 
 
 def test_fp32_copy_projections_match_oracle():
