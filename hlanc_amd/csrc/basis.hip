@@ -13,6 +13,12 @@
 //     repeatable, the same for a solve alone or beside others.  The segmented
 //     kernels (cgs_seg_kernel, cgs_finalize_seg_kernel) run one pass of many
 //     solves in one launch; cgs_kernel is the standalone form (hl_cgs_pass).
+//   * cgs_block_f32 / coldot_block / dgks_decide_seg_kernel: the default basis
+//     mode (an FP32 copy of the FP64 basis for the projection-only passes):
+//     the FP32 pass loads row pairs of adjacent column pairs (full 128-byte
+//     lines per 16 lanes), the first pass is the Lanczos three-term projection
+//     (one FP64 column), and a correction whose exact coefficients are
+//     negligible is skipped.
 //   * vq kernels: Out = V Q.  vq_seg_mfma_kernel (v_mfma_f64_16x16x4f64,
 //     Out^T = Q^T V^T so stores are column-coalesced) and vq_seg_kernel (FMA)
 //     restart every listed solve in one pass with the dsapps residual update
@@ -446,8 +452,8 @@ __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMembe
     cgs_finalize_block(c, mb.nb, st.j + 1, mb.partials, p.h_in, p.h_out, mb.ctl, mb.hsub, mb.hdia, mode, st.j);
 }
 
-// After the subtracting pass of solves that run the correction every step
-// (FP32-copy projections): the pass's exact FP64 projection h1 = V^T r is
+// After the subtracting pass of FP32-copy solves (their first correction is
+// pending after every such pass): the pass's exact FP64 projection h1 = V^T r is
 // what the correction would remove.  When ||h1|| <= tau ||r|| the correction
 // would move r by at most tau ||r||, so it is skipped (ctl->need = 0).  One
 // block per listed step.

@@ -9,6 +9,9 @@
 //   * the basis V, the residual and the alpha/beta recurrence stay in HBM; the
 //     DGKS decisions are taken on the device (a correction pass that is not
 //     needed exits at once);
+//   * a step's solves are grouped by precision kind (FP64; FP64 with an FP32
+//     copy for the projection passes, the default; FP32 basis), each kind's
+//     launches reading its own basis type;
 //   * after a sweep one kernel packs every member's alpha/beta and Ctl and one
 //     copy brings them home; the members' ncv x ncv work runs on host threads
 //     in FP64;
@@ -77,7 +80,9 @@ int Member::reset(hipStream_t s)
     std::memset(W.ctl_h, 0, sizeof(Ctl));
     W.ctl_h->rscale = 1.0;
     W.ctl_h->xscale = 1.0;
-    W.ctl_h->dgks_always = kind == SEG_PROJ32; // the FP32-copy projections leave ~6e-8 of V in w
+    // FP32-copy projections: the first correction is decided from its exact
+    // coefficients (dgks_decide_seg_kernel), not by the 0.717 cancellation test
+    W.ctl_h->dgks_always = kind == SEG_PROJ32;
     HL_TRY_HIP(hipMemcpyAsync(W.ctl, W.ctl_h, sizeof(Ctl), hipMemcpyHostToDevice, s));
     HL_TRY_HIP(hipMemsetAsync(W.hsub, 0, sizeof(double) * ncv, s));
     HL_TRY_HIP(hipMemsetAsync(W.hdia, 0, sizeof(double) * ncv, s));

@@ -56,9 +56,9 @@ struct Ctl {
     double nrm2;     // last reduced squared norm (standalone passes)
     long long p2, p3;       // DGKS corrections executed (first, second)
     long long m2sum, m3sum; // sum of their basis widths (for byte counts)
-    int dgks_always;        // run the first correction every step (FP32-copy projections) unless
-    int exact;              // its exact coefficients are negligible; this step's corrections
-                            // read the FP64 basis (FP32-copy projections)
+    int dgks_always;        // FP32-copy projections: the first correction is pending after every
+                            // subtracting pass (dgks_decide_seg_kernel may cancel it)
+    int exact;              // this step's corrections read the FP64 basis (FP32-copy projections)
     long long nexact, mxsum; // steps with exact corrections and the sum of their basis widths
     long long nskip;        // corrections skipped as negligible (FP32-copy projections)
 };
