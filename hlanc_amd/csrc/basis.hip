@@ -43,9 +43,16 @@ __device__ __forceinline__ T ld_stream(const T *p)
         return *p;
 }
 
+typedef const double __attribute__((address_space(1))) gdouble; // a double in global memory
+
 // Block `bid` of an `nb`-block CGS pass.  FL bit 0: non-temporal basis loads;
 // bit 1: software prefetch of the next tile; bit 3: split (write this block's
-// partial sums and leave the rest to cgs_finalize_kernel).
+// partial sums and leave the rest to cgs_finalize_kernel); bit 4: lean loads.
+// Lean loads serve the solver's own bases (tiled FP64, a tile's leading
+// dimension a multiple of 4, every column and padding row finite): a lane
+// reads its ceil(m/4) columns of a tile as global loads at constant offsets
+// from one address, with no per-lane masks.  Columns m .. 4 ceil(m/4) - 1 of
+// the last lane group are read and multiplied by h = 0, and rows past n by a = 0.
 template <int MAXC, typename VT, bool TILED, int FL>
 __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const VT *__restrict__ V, long long ldv,
                                           const double *w_in, double *w_out, const double *__restrict__ h_in,
@@ -93,9 +100,24 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
         }                                                                                        \
     }
     constexpr bool PF = (FL & 2) != 0;
+    constexpr bool LEAN = (FL & 16) != 0;
+    static_assert(!LEAN || (TILED && !PF && std::is_same<VT, double>::value), "lean loads: tiled FP64, no prefetch");
+#define HL_CGS_FETCH_LEAN(TT)                                                                    \
+    {                                                                                            \
+        const long long rr = (TT) * kTileRows + rl;                                              \
+        wn = ((gdouble *)(w_in))[rr < n ? rr : n - 1];                            \
+        gdouble *pv = ((gdouble *)(V)) + ((TT) * ldv + c0) * kTileRows + rl;        \
+        _Pragma("unroll") for (int c = 0; c < MAXC; ++c)                                         \
+            vn[c] = c < cpg ? ((FL & 1) ? __builtin_nontemporal_load(pv + c * kTileRows)         \
+                                        : pv[c * kTileRows])                                     \
+                            : 0.0;                                                               \
+    }
     if (PF) HL_CGS_FETCH(t);
     for (; t < ntiles; t += W) {
-        if (!PF) HL_CGS_FETCH(t);
+        if constexpr (LEAN)
+            HL_CGS_FETCH_LEAN(t)
+        else if (!PF)
+            HL_CGS_FETCH(t);
         double v[MAXC];
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) v[c] = vn[c];
@@ -112,6 +134,7 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
             p += __shfl_xor(p, 32);
             a = win - p;
         }
+        if (LEAN && !ok) a = 0.0;
         if (g == 0 && ok) {
             if (w_out) w_out[r] = a;
             nrm += a * a;
@@ -120,6 +143,7 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
         for (int c = 0; c < MAXC; ++c) acc[c] += v[c] * a;
     }
 #undef HL_CGS_FETCH
+#undef HL_CGS_FETCH_LEAN
 
 #pragma unroll
     for (int c = 0; c < MAXC; ++c) {
@@ -422,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
         cgs_block_f32<(MAXC + 1) / 2, 1, TB>(bid, mb.nb, mb.n, st.j + 1, V, mb.vld, p.w_in, p.w_out, p.h_in,
                                              p.h_out, mb.partials);
     } else {
-        cgs_block<MAXC, double, true, 9>(bid, mb.nb, mb.n, st.j + 1, static_cast<const double *>(mb.V), mb.vld,
+        cgs_block<MAXC, double, true, 25>(bid, mb.nb, mb.n, st.j + 1, static_cast<const double *>(mb.V), mb.vld,
                                          p.w_in, p.w_out, p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub,
                                          mb.hdia, MODE, st.j);
     }
@@ -608,7 +632,7 @@ __global__ __launch_bounds__(kBlock) void vq_seg_kernel(const SegVQ *__restrict_
 #pragma unroll
             for (int c = 0; c < CB; ++c) acc[c] = 0.0;
             for (int k = 0; k < t.K; ++k) {
-                const double v = static_cast<double>(V[vpos<true>(row, k, t.K)]);
+                const double v = static_cast<double>(V[vpos<true>(row, k, t.ldv)]);
 #pragma unroll
                 for (int c = 0; c < CB; ++c) acc[c] += v * qs[k * CB + c];
             }
@@ -616,8 +640,8 @@ __global__ __launch_bounds__(kBlock) void vq_seg_kernel(const SegVQ *__restrict_
 #pragma unroll
             for (int c = 0; c < CB; ++c) {
                 if (c0 + c < t.N) {
-                    Out[vpos<true>(row, c0 + c, t.K)] = static_cast<VT>(acc[c]);
-                    if (t.Out32) t.Out32[vpos<true>(row, c0 + c, t.K)] = static_cast<float>(acc[c]);
+                    Out[vpos<true>(row, c0 + c, t.ldv)] = static_cast<VT>(acc[c]);
+                    if (t.Out32) t.Out32[vpos<true>(row, c0 + c, t.ldv)] = static_cast<float>(acc[c]);
                 }
                 if (c == rl) rk = acc[c];
             }
@@ -679,7 +703,7 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int k = k0 + 4 * j + lk;
-                b[j] = (ok && k < t.K) ? static_cast<double>(V[vpos<true>(row, k, t.K)]) : 0.0;
+                b[j] = (ok && k < t.K) ? static_cast<double>(V[vpos<true>(row, k, t.ldv)]) : 0.0;
             }
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
@@ -699,8 +723,8 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
             for (int e = 0; e < 4; ++e) {
                 const int cl = ct * 16 + lk + 4 * e;
                 if (ok && cl < t.N) {
-                    Out[vpos<true>(row, cl, t.K)] = static_cast<VT>(acc[ct][e]);
-                    if (t.Out32) t.Out32[vpos<true>(row, cl, t.K)] = static_cast<float>(acc[ct][e]);
+                    Out[vpos<true>(row, cl, t.ldv)] = static_cast<VT>(acc[ct][e]);
+                    if (t.Out32) t.Out32[vpos<true>(row, cl, t.ldv)] = static_cast<float>(acc[ct][e]);
                 }
                 if (cl == t.res_col) rk = acc[ct][e];
             }
@@ -723,10 +747,11 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
     }
 }
 
-// out = V(:, 0:K) q on a tiled basis (ld K): one FP64 vector from a basis of
-// either precision (the start vector of a mixed-precision solve's FP64 phase).
+// out = V(:, 0:K) q on a tiled basis (ld columns per tile): one FP64 vector
+// from a basis of either precision (the start vector of a mixed-precision
+// solve's FP64 phase).
 template <typename VT>
-__global__ __launch_bounds__(kBlock) void basis_combine_kernel(int n, int K, const VT *__restrict__ V,
+__global__ __launch_bounds__(kBlock) void basis_combine_kernel(int n, int K, const VT *__restrict__ V, long long ld,
                                                                const double *__restrict__ q, double *__restrict__ out)
 {
     __shared__ double qs[128];
@@ -734,7 +759,7 @@ __global__ __launch_bounds__(kBlock) void basis_combine_kernel(int n, int K, con
     __syncthreads();
     for (long long row = (long long)blockIdx.x * kBlock + threadIdx.x; row < n; row += (long long)gridDim.x * kBlock) {
         double acc = 0.0;
-        for (int k = 0; k < K; ++k) acc += static_cast<double>(V[vpos<true>(row, k, K)]) * qs[k];
+        for (int k = 0; k < K; ++k) acc += static_cast<double>(V[vpos<true>(row, k, ld)]) * qs[k];
         out[row] = acc;
     }
 }
@@ -1225,7 +1250,8 @@ int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N
                                          s);
 }
 
-int launch_basis_combine(int n, int K, const void *V, bool v32, const double *q, double *out, hipStream_t s)
+int launch_basis_combine(int n, int K, const void *V, long long ld, bool v32, const double *q, double *out,
+                         hipStream_t s)
 {
     if (n <= 0) return HL_OK;
     if (K <= 0 || K > 128) {
@@ -1234,9 +1260,9 @@ int launch_basis_combine(int n, int K, const void *V, bool v32, const double *q,
     }
     const int blocks = grid_for(n, kBlock, 2048);
     if (v32)
-        basis_combine_kernel<float><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const float *>(V), q, out);
+        basis_combine_kernel<float><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const float *>(V), ld, q, out);
     else
-        basis_combine_kernel<double><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const double *>(V), q, out);
+        basis_combine_kernel<double><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const double *>(V), ld, q, out);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

@@ -131,6 +131,10 @@ int Member::rephase(hipStream_t s)
     spmv_f32 = spmv;
     restarts_f32 = restarts;
     kind = SEG_F64;
+    // the FP64 passes read whole tiles: clear what the FP32 phase left in the basis
+    const size_t vbytes = 8 * static_cast<size_t>(ws.ldv) * static_cast<size_t>(ws.vl().ld);
+    HL_TRY_HIP(hipMemsetAsync(ws.V[0], 0, vbytes, s));
+    HL_TRY_HIP(hipMemsetAsync(ws.V[1], 0, vbytes, s));
     int rc;
     if ((rc = reset(s))) return rc;
     return begin(s);
@@ -846,6 +850,7 @@ int Engine::after_sweep(std::vector<Member *> &act)
             t.beta = m.vq_beta;
             t.n = m.n;
             t.K = m.ncv;
+            t.ldv = W.vl().ld;
             t.N = m.vq_N;
             t.res_col = m.vq_res;
             t.nb = vq_seg_blocks(m.n);
@@ -881,7 +886,8 @@ int Engine::after_sweep(std::vector<Member *> &act)
         Member &m = *act[i];
         if (m.act != Member::ACT_REPHASE) continue;
         Workspace &W = m.ws;
-        if ((rc = launch_basis_combine(m.n, m.ncv, W.V[W.cur], true, dQ + i * qstride, W.r, stream))) return rc;
+        if ((rc = launch_basis_combine(m.n, m.ncv, W.V[W.cur], W.vl().ld, true, dQ + i * qstride, W.r, stream)))
+            return rc;
         if ((rc = m.rephase(stream))) return rc;
     }
     if (nfinish == 0) return HL_OK;

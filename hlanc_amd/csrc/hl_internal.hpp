@@ -104,6 +104,10 @@ __host__ __device__ inline long long vpos(long long r, long long c, long long ld
 {
     return TILED ? (((r >> 4) * ld + c) << 4) + (r & 15) : c * ld + r;
 }
+// Columns per tile of a solver-owned tiled basis: ncv rounded up to a multiple
+// of 4, so that each of a wave's 4 lane groups reads its ceil(m/4) columns of
+// a tile inside that tile.
+inline long long tiled_ld(int ncv) { return (static_cast<long long>(ncv) + 3) / 4 * 4; }
 // One column j of a basis, as a write target (base == nullptr: none).
 struct VColumn {
     double *base = nullptr;
@@ -260,6 +264,7 @@ struct SegVQ {
     double *partials;            // nb block partials of ||r||^2
     double qlast, beta;          // dsapps residual coefficients
     int n, K, N, res_col;        // res_col < 0: no beta term
+    long long ldv;               // columns per tile of V, Out and Out32 (tiled)
     int nb, blk0;                // blocks (vq_seg_blocks(n)); first block in the launch
 };
 int vq_seg_blocks(int n);
@@ -274,7 +279,8 @@ bool vq_seg_mfma_fits(int K, int N);
 int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, int nt_mfma, int blocks_mfma,
                   int mfma_nct, size_t mfma_lds, bool v32, hipStream_t s);
 // out = V(:, 0:K) q (tiled basis, ld K, FP64 or FP32; q on the device).
-int launch_basis_combine(int n, int K, const void *V, bool v32, const double *q, double *out, hipStream_t s);
+int launch_basis_combine(int n, int K, const void *V, long long ld, bool v32, const double *q, double *out,
+                         hipStream_t s);
 // Packs hsub, hdia (ncv each) and the Ctl words of every member at
 // out + member * stride (stride >= 2 ncv + kCtlWords).
 int launch_seg_readback(const SegMember *M, int nm, int stride, double *out, hipStream_t s);

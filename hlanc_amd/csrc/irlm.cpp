@@ -242,11 +242,12 @@ struct WsLayout {
     WsLayout(int n, int ncv, bool host_op, int nslots, int npairs, bool copy32)
     {
         const size_t ldv = static_cast<size_t>(padded_rows(n));
+        const size_t ldt = static_cast<size_t>(tiled_ld(ncv));
         Carve c;
-        V0 = c.take(8 * ldv * ncv);
-        V1 = c.take(8 * ldv * ncv);
-        S0 = copy32 ? c.take(4 * ldv * ncv) : 0;
-        S1 = copy32 ? c.take(4 * ldv * ncv) : 0;
+        V0 = c.take(8 * ldv * ldt);
+        V1 = c.take(8 * ldv * ldt);
+        S0 = copy32 ? c.take(4 * ldv * ldt) : 0;
+        S1 = copy32 ? c.take(4 * ldv * ldt) : 0;
         r = c.take(8 * ldv);
         w = c.take(8 * ldv);
         hsub = c.take(8 * (size_t)ncv);
@@ -297,6 +298,9 @@ int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, int np
     red.counter = reinterpret_cast<unsigned *>(b + L.cnt);
     red.max_blocks = kMaxReduceBlocks;
     red.max_cols = ncv;
+    // the CGS passes read whole tiles: unwritten columns and padding rows must be finite
+    HL_TRY_HIP(hipMemsetAsync(V[0], 0, L.V1 - L.V0, s));
+    HL_TRY_HIP(hipMemsetAsync(V[1], 0, 8 * static_cast<size_t>(ldv) * static_cast<size_t>(tiled_ld(ncv)), s));
     if (chunk_cnt) HL_TRY_HIP(hipMemsetAsync(chunk_cnt, 0, sizeof(unsigned) * (size_t)nslots, s));
     HL_TRY_HIP(hipMemsetAsync(red.counter, 0, 16, s));
     const size_t hbytes = sizeof(Ctl) + (host_op ? 16 * (size_t)n : 0);

@@ -31,7 +31,7 @@ struct Workspace {
     void *blob = nullptr, *hblob = nullptr;
 
     // The ping-pong bases are tiled: 16-row blocks of ncv columns.
-    VLayout vl() const { return VLayout{ncv, true}; }
+    VLayout vl() const { return VLayout{tiled_ld(ncv), true}; }
     ~Workspace();
     int init(int dev, int n, int ncv, bool host_op, int nslots, int npairs, bool copy32,
              hipStream_t s); // zeroing on s
