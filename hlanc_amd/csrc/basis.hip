@@ -34,16 +34,14 @@ namespace hl {
 namespace {
 using namespace dev;
 
-template <typename T, bool NT>
-__device__ __forceinline__ T ld_stream(const T *p)
+template <typename T, bool NT, typename P>
+__device__ __forceinline__ T ld_stream(P p)
 {
     if constexpr (NT)
         return __builtin_nontemporal_load(p);
     else
         return *p;
 }
-
-typedef const double __attribute__((address_space(1))) gdouble; // a double in global memory
 
 // Block `bid` of an `nb`-block CGS pass.  FL bit 0: non-temporal basis loads;
 // bit 1: software prefetch of the next tile; bit 3: split (write this block's
@@ -54,12 +52,16 @@ typedef const double __attribute__((address_space(1))) gdouble; // a double in g
 // from one address, with no per-lane masks.  Columns m .. 4 ceil(m/4) - 1 of
 // the last lane group are read and multiplied by h = 0, and rows past n by a = 0.
 template <int MAXC, typename VT, bool TILED, int FL>
-__device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const VT *__restrict__ V, long long ldv,
-                                          const double *w_in, double *w_out, const double *__restrict__ h_in,
-                                          double *h_out, double *partials, unsigned *counter, Ctl *ctl,
+__device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const VT *__restrict__ V_, long long ldv,
+                                          const double *w_in_, double *w_out_, const double *__restrict__ h_in,
+                                          double *h_out, double *partials_, unsigned *counter, Ctl *ctl,
                                           double *h_sub, double *h_dia, int mode, int j)
 {
     if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return; // DGKS decided on the device
+    const auto V = gptr(V_);
+    const auto w_in = gptr(w_in_);
+    const auto w_out = gptr(w_out_);
+    const auto partials = gptr(partials_);
     __shared__ double hs[kGroups * MAXC];
     __shared__ double red[kBlock / kWave][kGroups * MAXC + 1];
     __shared__ double vals[kGroups * MAXC + 1];
@@ -105,8 +107,8 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
 #define HL_CGS_FETCH_LEAN(TT)                                                                    \
     {                                                                                            \
         const long long rr = (TT) * kTileRows + rl;                                              \
-        wn = ((gdouble *)(w_in))[rr < n ? rr : n - 1];                            \
-        gdouble *pv = ((gdouble *)(V)) + ((TT) * ldv + c0) * kTileRows + rl;        \
+        wn = w_in[rr < n ? rr : n - 1];                                                          \
+        const auto pv = V + ((TT) * ldv + c0) * kTileRows + rl;                                  \
         _Pragma("unroll") for (int c = 0; c < MAXC; ++c)                                         \
             vn[c] = c < cpg ? ((FL & 1) ? __builtin_nontemporal_load(pv + c * kTileRows)         \
                                         : pv[c * kTileRows])                                     \
@@ -173,7 +175,7 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
         for (int c = threadIdx.x; c < M1; c += kBlock) partials[(long long)c * nb + bid] = vals[c];
         return;
     }
-    if (!reduce_blocks(vals, M1, partials, counter, scratch, fin)) return;
+    if (!reduce_blocks(vals, M1, partials_, counter, scratch, fin)) return;
     if (h_out)
         for (int c = threadIdx.x; c < m; c += kBlock) h_out[c] = fin[c];
     if (threadIdx.x == 0) run_epilogue(mode, j, fin[m], h_in, ctl, h_sub, h_dia);
@@ -188,10 +190,14 @@ typedef float flt2 __attribute__((ext_vector_type(2)));
 // ceil(m/8) column pairs and each load instruction moves 4 full lines, as
 // the FP64 pass does.  Arithmetic is FP64.
 template <int MAXC8, int FL, int TB>
-__device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, const float *__restrict__ V,
-                                              long long ldv, const double *w_in, double *w_out,
-                                              const double *__restrict__ h_in, double *h_out, double *partials)
+__device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, const float *__restrict__ V_,
+                                              long long ldv, const double *w_in_, double *w_out_,
+                                              const double *__restrict__ h_in, double *h_out, double *partials_)
 {
+    const auto V = gptr(V_);
+    const auto w_in = gptr(w_in_);
+    const auto w_out = gptr(w_out_);
+    const auto partials = gptr(partials_);
     constexpr int G = 4;                 // lane groups per wave, 2 columns each per step
     __shared__ double hs[2 * G * MAXC8];
     __shared__ double red[kBlock / kWave][2 * G * MAXC8 + 1];
@@ -233,7 +239,7 @@ __device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, con
                 v[b][c] = flt2{0.0f, 0.0f};
                 if (ok0[b] && c < cpp && colj < m)
                     v[b][c] = ld_stream<flt2, (FL & 1) != 0>(
-                        reinterpret_cast<const flt2 *>(&V[vpos<true>(r0[b], colj, ldv)]));
+                        reinterpret_cast<gmem<const flt2> *>(&V[vpos<true>(r0[b], colj, ldv)]));
             }
             w0[b] = ok0[b] ? w_in[r0[b]] : 0.0;
             w1[b] = ok1[b] ? w_in[r0[b] + 1] : 0.0;
@@ -307,9 +313,12 @@ __device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, con
 // every lane active, row-coalesced.  Written as partials[c * nb + bid],
 // c = 0 (alpha) and 1 (norm), the layout the finalize reads for m = 1.
 template <typename VT>
-__device__ __forceinline__ void coldot_block(int bid, int nb, int n, const VT *__restrict__ v, long long ldv,
-                                             const double *__restrict__ w, double *partials)
+__device__ __forceinline__ void coldot_block(int bid, int nb, int n, const VT *__restrict__ v_, long long ldv,
+                                             const double *__restrict__ w_, double *partials_)
 {
+    const auto v = gptr(v_);
+    const auto w = gptr(w_);
+    const auto partials = gptr(partials_);
     __shared__ double red[2][kBlock / kWave];
     double a = 0.0, s2 = 0.0;
     for (long long r = (long long)bid * kBlock + threadIdx.x; r < n; r += (long long)nb * kBlock) {
@@ -348,11 +357,12 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
 // block partials in block order; block m (the squared norm) runs the epilogue.
 // Columns whose block starts after the epilogue cleared ctl->need are not
 // written, and no later kernel reads them.
-__device__ __forceinline__ void cgs_finalize_block(int c, int nb, int m, const double *__restrict__ partials,
+__device__ __forceinline__ void cgs_finalize_block(int c, int nb, int m, const double *__restrict__ partials_,
                                                    const double *h_in, double *h_out, Ctl *ctl, double *h_sub,
                                                    double *h_dia, int mode, int j)
 {
     if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return;
+    const auto partials = gptr(partials_);
     __shared__ double sred[kBlock];
     const int tid = threadIdx.x;
     double t = 0.0;
@@ -673,8 +683,10 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
     __shared__ double red[kBlock / kWave];
     const int i = seg_find(&T[0].blk0, sizeof(SegVQ) / sizeof(int), nt, blockIdx.x);
     const SegVQ t = T[i];
-    const VT *__restrict__ V = static_cast<const VT *>(t.V);
-    VT *__restrict__ Out = static_cast<VT *>(t.Out);
+    const auto V = gptr(static_cast<const VT *>(t.V));
+    const auto Out = gptr(static_cast<VT *>(t.Out));
+    const auto Out32 = gptr(t.Out32);
+    const auto r = gptr(t.r);
     const int bid = blockIdx.x - t.blk0;
     const int nct = (t.N + 15) / 16; // this solve's column tiles, <= NT
     const int NC = nct * 16;
@@ -724,7 +736,7 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
                 const int cl = ct * 16 + lk + 4 * e;
                 if (ok && cl < t.N) {
                     Out[vpos<true>(row, cl, t.ldv)] = static_cast<VT>(acc[ct][e]);
-                    if (t.Out32) t.Out32[vpos<true>(row, cl, t.ldv)] = static_cast<float>(acc[ct][e]);
+                    if (Out32) Out32[vpos<true>(row, cl, t.ldv)] = static_cast<float>(acc[ct][e]);
                 }
                 if (cl == t.res_col) rk = acc[ct][e];
             }
@@ -732,8 +744,8 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
         // the lanes holding column res_col of this tile hand it to lanes 0..15
         const double rkv = t.res_col >= 0 ? __shfl(rk, ((t.res_col % 16) % 4) * 16 + lr) : 0.0;
         if (lk == 0 && ok) { // dsapps residual (HLanc.hpp:1605-1614)
-            const double rr = qrs * t.r[row] + t.beta * rkv;
-            t.r[row] = rr;
+            const double rr = qrs * r[row] + t.beta * rkv;
+            r[row] = rr;
             nrm += rr * rr;
         }
     }

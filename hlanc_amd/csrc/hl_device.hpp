@@ -8,6 +8,13 @@
 namespace hl {
 namespace dev {
 
+// A pointer into global memory, typed so: loads and stores through it compile
+// to global (not flat) instructions.  Pointers that a kernel reads from a
+// struct in memory are generic otherwise.
+template <typename T> using gmem = __attribute__((address_space(1))) T;
+template <typename T> __device__ __forceinline__ gmem<const T> *gptr(const T *p) { return (gmem<const T> *)p; }
+template <typename T> __device__ __forceinline__ gmem<T> *gptr(T *p) { return (gmem<T> *)p; }
+
 constexpr int kGroups = 4;                 // column groups per wave
 constexpr int kTileRows = kWave / kGroups; // 16 rows per wave tile
 

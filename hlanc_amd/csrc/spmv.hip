@@ -43,13 +43,21 @@ __device__ inline long long vcol_pos(long long r, long long ld, int j, bool tile
 // One work item of the SpMV (one block).  AT: matrix values, XT: x, VT: the
 // basis column written alongside (vout).
 template <typename AT, typename XT, typename VT>
-__device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict__ rp, const int *__restrict__ col,
-                                          const AT *__restrict__ val, const XT *__restrict__ x,
-                                          const double *__restrict__ xscale, double *__restrict__ y,
-                                          VT *__restrict__ vout, float *__restrict__ vout32, long long vld,
+__device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict__ rp_, const int *__restrict__ col_,
+                                          const AT *__restrict__ val_, const XT *__restrict__ x_,
+                                          const double *__restrict__ xscale, double *__restrict__ y_,
+                                          VT *__restrict__ vout_, float *__restrict__ vout32_, long long vld,
                                           int vj, bool vtiled, double *__restrict__ chunk_part,
                                           unsigned *__restrict__ chunk_cnt)
 {
+    // global (not flat) memory instructions for the streams and the gathers
+    const auto rp = dev::gptr(rp_);
+    const auto col = dev::gptr(col_);
+    const auto val = dev::gptr(val_);
+    const auto x = dev::gptr(x_);
+    const auto y = dev::gptr(y_);
+    const auto vout = dev::gptr(vout_);
+    const auto vout32 = dev::gptr(vout32_);
     __shared__ double prod[kNnzPerBlock];
     __shared__ double wred[kBlock / kWave];
     const double s = xscale ? *xscale : 1.0;
@@ -180,12 +188,19 @@ __global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__res
 
 // Rows [r0, r1) of a sliced SpMV: y(r) = its partials added in slice order,
 // and the basis column v(r) = x(r) s (and its FP32 copy) written beside.
-__device__ __forceinline__ void spmv_rows(int r0, int r1, const int *__restrict__ r_pp,
-                                          const int *__restrict__ r_pairs, const double *__restrict__ part,
-                                          const double *__restrict__ x, double s, double *__restrict__ y,
-                                          double *__restrict__ vout, float *__restrict__ vout32, long long vld,
+__device__ __forceinline__ void spmv_rows(int r0, int r1, const int *__restrict__ r_pp_,
+                                          const int *__restrict__ r_pairs_, const double *__restrict__ part_,
+                                          const double *__restrict__ x_, double s, double *__restrict__ y_,
+                                          double *__restrict__ vout_, float *__restrict__ vout32_, long long vld,
                                           int vj, bool vtiled)
 {
+    const auto r_pp = dev::gptr(r_pp_);
+    const auto r_pairs = dev::gptr(r_pairs_);
+    const auto part = dev::gptr(part_);
+    const auto x = dev::gptr(x_);
+    const auto y = dev::gptr(y_);
+    const auto vout = dev::gptr(vout_);
+    const auto vout32 = dev::gptr(vout32_);
     for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
         double acc = 0.0;
         for (int k = r_pp[r]; k < r_pp[r + 1]; ++k) acc += part[r_pairs[k]];
