@@ -49,8 +49,9 @@ __device__ __forceinline__ T ld_stream(P p)
 // Lean loads serve the solver's own bases (tiled FP64, a tile's leading
 // dimension a multiple of 4, every column and padding row finite): a lane
 // reads its ceil(m/4) columns of a tile as global loads at constant offsets
-// from one address, with no per-lane masks.  Columns m .. 4 ceil(m/4) - 1 of
-// the last lane group are read and multiplied by h = 0, and rows past n by a = 0.
+// from one address.  Only the last 3 columns of a lane group can lie past m
+// (in the last group), so only their loads carry a per-lane mask.  Padding
+// rows past n are read and multiplied by a = 0.
 template <int MAXC, typename VT, bool TILED, int FL>
 __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const VT *__restrict__ V_, long long ldv,
                                           const double *w_in_, double *w_out_, const double *__restrict__ h_in,
@@ -110,9 +111,9 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
         wn = w_in[rr < n ? rr : n - 1];                                                          \
         const auto pv = V + ((TT) * ldv + c0) * kTileRows + rl;                                  \
         _Pragma("unroll") for (int c = 0; c < MAXC; ++c)                                         \
-            vn[c] = c < cpg ? ((FL & 1) ? __builtin_nontemporal_load(pv + c * kTileRows)         \
-                                        : pv[c * kTileRows])                                     \
-                            : 0.0;                                                               \
+            vn[c] = c < cpg && (c + 3 < cpg || c0 + c < m)                                       \
+                        ? ((FL & 1) ? __builtin_nontemporal_load(pv + c * kTileRows) : pv[c * kTileRows]) \
+                        : 0.0;                                                                   \
     }
     if (PF) HL_CGS_FETCH(t);
     for (; t < ntiles; t += W) {

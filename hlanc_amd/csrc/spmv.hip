@@ -65,6 +65,19 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
 
     if (b.tpr > 0) {
         const int nnz = b.p1 - b.p0;
+        const int tpr = b.tpr;
+        const int nrows = b.row1 - b.row0;
+        const int lr = tid / tpr, ll = tid & (tpr - 1);
+        const int rstep = kBlock / tpr;
+        // loads that do not wait on the gathers go out first: the bounds of
+        // this thread's first row of the reduction and, for the basis column,
+        // x of row row0 + tid
+        int e0 = 0, e1 = 0;
+        if (lr < nrows) {
+            e0 = rp[b.row0 + lr];
+            e1 = rp[b.row0 + lr + 1];
+        }
+        const double xr = (vout && tid < nrows) ? static_cast<double>(x[b.row0 + tid]) : 0.0;
         // all of this thread's (<= 8) column indices and values in flight at
         // once, then all of its x gathers: the matrix is streamed once
         // (non-temporal), so L2 keeps x
@@ -90,22 +103,18 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
         }
         if (vout) {
             for (int r = b.row0 + tid; r < b.row1; r += kBlock) {
-                const double xs = static_cast<double>(x[r]) * s;
+                const double xs = (r == b.row0 + tid ? xr : static_cast<double>(x[r])) * s;
                 vout[vcol_pos(r, vld, vj, vtiled)] = static_cast<VT>(xs);
                 if (vout32) vout32[vcol_pos(r, vld, vj, vtiled)] = static_cast<float>(xs);
             }
         }
         __syncthreads();
-        const int tpr = b.tpr;
-        const int nrows = b.row1 - b.row0;
-        const int lr = tid / tpr, ll = tid & (tpr - 1);
-        const int rstep = kBlock / tpr;
         for (int base = 0; base < nrows; base += rstep) {
             const int rr = base + lr;
             double acc = 0.0;
             if (rr < nrows) {
                 const int r = b.row0 + rr;
-                const int a0 = rp[r] - b.p0, a1 = rp[r + 1] - b.p0;
+                const int a0 = (base == 0 ? e0 : rp[r]) - b.p0, a1 = (base == 0 ? e1 : rp[r + 1]) - b.p0;
                 for (int k = a0 + ll; k < a1; k += tpr) acc += prod[k];
             }
             for (int off = tpr >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
