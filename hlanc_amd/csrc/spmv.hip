@@ -211,11 +211,23 @@ __device__ __forceinline__ void spmv_rows(int r0, int r1, const int *__restrict_
     const auto vout = dev::gptr(vout_);
     const auto vout32 = dev::gptr(vout32_);
     for (int r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        const double xr = vout ? x[r] : 0.0; // independent of the sum: issued first
+        const int k0 = r_pp[r], k1 = r_pp[r + 1];
         double acc = 0.0;
-        for (int k = r_pp[r]; k < r_pp[r + 1]; ++k) acc += part[r_pairs[k]];
+        for (int k = k0; k < k1; k += 4) { // four partials in flight, added in slice order
+            int id[4];
+            double pv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) id[u] = k + u < k1 ? r_pairs[k + u] : -1;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pv[u] = id[u] >= 0 ? part[id[u]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k + u < k1) acc += pv[u];
+        }
         y[r] = acc;
         if (vout) {
-            const double xs = x[r] * s;
+            const double xs = xr * s;
             vout[vcol_pos(r, vld, vj, vtiled)] = xs;
             if (vout32) vout32[vcol_pos(r, vld, vj, vtiled)] = static_cast<float>(xs);
         }
