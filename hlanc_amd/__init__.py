@@ -376,12 +376,29 @@ def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mf
     return o
 
 
-def _vectors_buffer(out, ldv: int, ncv: int) -> np.ndarray:
+def _is_device_array(out) -> bool:
+    return hasattr(out, "data_ptr") and bool(getattr(out, "is_cuda", False))
+
+
+def _vectors_buffer(out, ldv: int, ncv: int):
     if out is None:
         return np.zeros((ncv, ldv)).T  # column-major ldv x ncv
+    if _is_device_array(out):  # a device tensor: the vectors stay in HBM
+        if (str(out.dtype) != "torch.float64" or out.dim() != 2 or out.shape[0] < ldv or out.shape[1] < ncv
+                or out.stride(0) != 1 or out.stride(1) < out.shape[0]):
+            raise HLancError(-1, f"device vectors_out must be a column-major float64 tensor of at least "
+                                 f"{ldv} x {ncv} (e.g. torch.empty(ncv, ldv).T)")
+        return out
     if out.dtype != np.float64 or out.shape[0] < ldv or out.shape[1] < ncv or not out.flags.f_contiguous:
         raise HLancError(-1, f"vectors_out must be a column-major float64 array of at least {ldv} x {ncv}")
     return out
+
+
+def _vectors_args(vectors):
+    """(pointer, leading dimension) of a host array or device tensor from _vectors_buffer."""
+    if _is_device_array(vectors):
+        return ctypes.cast(vectors.data_ptr(), _dp), vectors.stride(1)
+    return vectors.ctypes.data_as(_dp), vectors.shape[0]
 
 
 def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, v0=None,
@@ -393,21 +410,22 @@ def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 
     path, as the reference calls arbitrary operators with host pointers).
     ``ret`` is 0 on success and 1 on the reference's numerical failures; values
     and vectors are written only on success.  ``vectors_out`` (column-major
-    float64, at least ldv x ncv, e.g. from pinned_empty) receives the vectors
+    float64, at least ldv x ncv: a host array, e.g. from pinned_empty, or a
+    device tensor such as ``torch.empty(ncv, ldv, dtype=torch.float64,
+    device="cuda").T``, which keeps the result in HBM) receives the vectors
     instead of a fresh array.  ``basis_prec`` (None: the library default,
     HL_PREC_F64_PROJ32) selects the basis storage; every mode returns FP64
     results to ``tol`` (hlanc_capi.h, hl_options.basis_prec)."""
     ldv = n if ldv is None else ldv
     values = np.zeros(ncv)
     vectors = _vectors_buffer(vectors_out, ldv, ncv)
-    ldv = vectors.shape[0] if vectors_out is not None else ldv
+    vptr, ldv = _vectors_args(vectors)
     o = options(tol, maxiter, v0, seed, use_mfma, basis_prec=basis_prec, flags=flags)
     info = Info()
     if isinstance(op, dcsrmv_operator_gpu):
         if op.nrows != n:
             raise HLancError(-1, f"operator has {op.nrows} rows, n={n}")
-        rc = lib().hl_ds_solve_csr(op.h, nev, ncv, _d(values), vectors.ctypes.data_as(_dp), ldv,
-                                   ctypes.byref(o), ctypes.byref(info))
+        rc = lib().hl_ds_solve_csr(op.h, nev, ncv, _d(values), vptr, ldv, ctypes.byref(o), ctypes.byref(info))
     else:
         def tramp(ctx, x, y, _op=op, _n=n):
             xa = np.ctypeslib.as_array(x, shape=(_n,))
@@ -415,10 +433,11 @@ def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 
             _op(xa, ya)
 
         cb = HOST_OP(tramp)
-        rc = lib().hl_ds_solve_host_op(dev, n, cb, None, nev, ncv, _d(values), vectors.ctypes.data_as(_dp), ldv,
-                                       ctypes.byref(o), ctypes.byref(info))
+        rc = lib().hl_ds_solve_host_op(dev, n, cb, None, nev, ncv, _d(values), vptr, ldv, ctypes.byref(o),
+                                       ctypes.byref(info))
     _check(rc, allow=(HL_OK, HL_NOT_CONVERGED))
-    return SolveResult(rc, values[:nev].copy(), np.asarray(vectors[:n]), info)
+    vecs = vectors[:n] if _is_device_array(vectors) else np.asarray(vectors[:n])
+    return SolveResult(rc, values[:nev].copy(), vecs, info)
 
 
 @dataclass
@@ -471,8 +490,7 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
         it.nev = nev_i
         it.ncv = ncv_i
         it.values = _d(vals)
-        it.vectors = vecs.ctypes.data_as(_dp) if vecs is not None else None
-        it.ldv = vecs.shape[0] if vecs is not None else n
+        it.vectors, it.ldv = _vectors_args(vecs) if vecs is not None else (None, n)
         it.opts = o
         it.arrival = 0.0 if arrivals is None else float(arrivals[i])
         keep.append((vals, vecs, o))
@@ -483,9 +501,9 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
         it = items[i]
         vals, vecs, _ = keep[i]
         info = Info.from_buffer_copy(it.info)
-        out.append(BatchResult(it.status, vals[:it.nev].copy(),
-                               None if vecs is None else np.asarray(vecs[: ops[i].nrows]),
-                               info, it.t_admit, it.t_done, it.arrival))
+        if vecs is not None:
+            vecs = vecs[: ops[i].nrows] if _is_device_array(vecs) else np.asarray(vecs[: ops[i].nrows])
+        out.append(BatchResult(it.status, vals[:it.nev].copy(), vecs, info, it.t_admit, it.t_done, it.arrival))
     return out
 
 

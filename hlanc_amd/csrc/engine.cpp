@@ -908,10 +908,11 @@ int Engine::after_sweep(std::vector<Member *> &act)
                             VLayout{W.ldv, false}, m.opts.use_mfma, -1, nullptr, 0.0, 0.0, W.red, W.ctl, stream)))
             return rc;
         if ((rc = prof.end(stream, K_VQ, 16.0 * m.n * m.ncv))) return rc;
-        // sync_host (HLanc.hpp:1063-1071): all ncv columns to the caller
+        // sync_host (HLanc.hpp:1063-1071): all ncv columns to the caller's
+        // buffer, host (pageable or page-locked) or device memory
         HL_TRY_HIP(hipMemcpy2DAsync(m.vectors, sizeof(double) * (size_t)m.ldv_host, W.V[nxt],
                                     sizeof(double) * (size_t)W.ldv, sizeof(double) * (size_t)m.n, m.ncv,
-                                    hipMemcpyDeviceToHost, stream));
+                                    hipMemcpyDefault, stream));
     }
     if (fin.ms.empty()) return HL_OK;
     // completed when this event has passed: the next sweep is enqueued meanwhile

@@ -263,10 +263,12 @@ void hl_options_default(hl_options *o);
 
 /* ds_solver_gpu<dcsrmv_operator_gpu> (HLanc.hpp:1868-1872 driving 1649-1850):
  * computes the nev largest-algebraic eigenpairs.  values (host, >= nev) gets
- * them ascending; vectors (host, column-major ldv x ncv, ldv >= n) gets all ncv
- * basis columns, the first nev being the unit Ritz vectors.  Both are written
- * only on success.  The operator's SpMV runs on device pointers: no PCIe
- * traffic per Lanczos step.  opts/info may be NULL. */
+ * them ascending; vectors (column-major ldv x ncv, ldv >= n) gets all ncv
+ * basis columns, the first nev being the unit Ritz vectors.  vectors may be
+ * host memory, as the reference's (pageable, or page-locked for a copy at
+ * link rate), or device memory on the operator's device, which keeps the
+ * result in HBM.  Both are written only on success.  The operator's SpMV runs
+ * on device pointers: no PCIe traffic per Lanczos step.  opts/info may be NULL. */
 int hl_ds_solve_csr(hl_csr_t A, int nev, int ncv, double *values, double *vectors, int ldv,
                     const hl_options *opts, hl_info *info);
 
@@ -275,8 +277,9 @@ typedef struct {
     hl_csr_t A;          /* square operator on the batch's device */
     int nev, ncv;        /* as hl_ds_solve_csr */
     double *values;      /* host, >= nev: ascending eigenvalues on success */
-    double *vectors;     /* host, column-major ldv x ncv as hl_ds_solve_csr, or
-                            NULL: values only (the Ritz-vector GEMM is skipped) */
+    double *vectors;     /* column-major ldv x ncv, host or device memory as
+                            hl_ds_solve_csr, or NULL: values only (the
+                            Ritz-vector GEMM is skipped) */
     int ldv;
     int pad0;
     hl_options opts;     /* any basis_prec */

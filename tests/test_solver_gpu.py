@@ -270,3 +270,27 @@ def test_pinned_vectors_output_is_the_same_solve():
     assert np.shares_memory(b.vectors, buf)
     with pytest.raises(H.HLancError):
         H.ds_solver_gpu(n, 6, 18, op, vectors_out=np.zeros((n, 18)))  # row-major: refused
+
+
+def test_device_vectors_output_is_the_same_solve():
+    import torch
+
+    rp, col, val = W.stencil27(11, seed=21)
+    n = len(rp) - 1
+    op = op_for(rp, col, val)
+    a = H.ds_solver_gpu(n, 6, 18, op, tol=1e-10, maxiter=1000)
+    dev = torch.full((20, n + 7), 7.0, dtype=torch.float64, device="cuda").T  # ldv = n + 7, padding kept
+    b = H.ds_solver_gpu(n, 6, 18, op, tol=1e-10, maxiter=1000, vectors_out=dev)
+    assert a.ret == b.ret == 0
+    np.testing.assert_array_equal(a.values, b.values)
+    np.testing.assert_array_equal(a.vectors[:, :18], b.vectors[:, :18].cpu().numpy())
+    assert b.vectors.data_ptr() == dev.data_ptr()
+    full = dev.cpu().numpy()
+    assert (full[n:, :] == 7.0).all() and (full[:, 18:] == 7.0).all()  # nothing written past n x ncv
+    # the batch takes device buffers per item too
+    bb = H.ds_solve_batch([op, op], 6, 18, tol=1e-10, maxiter=1000,
+                          vectors_out=[torch.empty((18, n), dtype=torch.float64, device="cuda").T, None])
+    np.testing.assert_array_equal(bb[0].vectors.cpu().numpy(), a.vectors[:, :18])
+    np.testing.assert_array_equal(bb[1].vectors, a.vectors[:, :18])
+    with pytest.raises(H.HLancError):  # row-major device tensor: refused
+        H.ds_solver_gpu(n, 6, 18, op, vectors_out=torch.empty((n, 18), dtype=torch.float64, device="cuda"))
