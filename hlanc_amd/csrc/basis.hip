@@ -200,6 +200,7 @@ __device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, con
     const auto w_out = gptr(w_out_);
     const auto partials = gptr(partials_);
     constexpr int G = 4;                 // lane groups per wave, 2 columns each per step
+    constexpr bool LEAN = (FL & 16) != 0; // solver-owned basis: finite zero padding rows
     __shared__ double hs[2 * G * MAXC8];
     __shared__ double red[kBlock / kWave][2 * G * MAXC8 + 1];
     __shared__ double vals[2 * G * MAXC8 + 1];
@@ -234,20 +235,23 @@ __device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, con
             // the tile's rows exist in the padded basis; rows >= n are masked to 0
             ok0[b] = tt < ntiles && r0[b] < n;
             ok1[b] = tt < ntiles && r0[b] + 1 < n;
+            // one address per tile, the column pairs at constant offsets; only the
+            // last 4 pairs of a lane group can lie past m.  Lean: padding rows
+            // are read (finite, multiplied by a = 0)
+            const auto pv = reinterpret_cast<gmem<const flt2> *>(
+                V + ((tt < ntiles ? tt : 0) * ldv + c0) * kTileRows + 2 * q);
 #pragma unroll
             for (int c = 0; c < MAXC8; ++c) {
-                const int colj = c0 + 2 * c;
                 v[b][c] = flt2{0.0f, 0.0f};
-                if (ok0[b] && c < cpp && colj < m)
-                    v[b][c] = ld_stream<flt2, (FL & 1) != 0>(
-                        reinterpret_cast<gmem<const flt2> *>(&V[vpos<true>(r0[b], colj, ldv)]));
+                if ((LEAN ? tt < ntiles : ok0[b]) && c < cpp && (c + 4 < cpp || c0 + 2 * c < m))
+                    v[b][c] = ld_stream<flt2, (FL & 1) != 0>(pv + c * kTileRows);
             }
             w0[b] = ok0[b] ? w_in[r0[b]] : 0.0;
             w1[b] = ok1[b] ? w_in[r0[b] + 1] : 0.0;
         }
 #pragma unroll
         for (int b = 0; b < TB; ++b) {
-            if (!ok1[b])
+            if (!LEAN && !ok1[b])
 #pragma unroll
                 for (int c = 0; c < MAXC8; ++c) v[b][c].y = 0.0f;
             double a0 = w0[b], a1 = w1[b];
@@ -266,6 +270,10 @@ __device__ __forceinline__ void cgs_block_f32(int bid, int nb, int n, int m, con
                 p1 += __shfl_xor(p1, 32);
                 a0 -= p0;
                 a1 -= p1;
+            }
+            if (LEAN) {
+                a0 = ok0[b] ? a0 : 0.0;
+                a1 = ok1[b] ? a1 : 0.0;
             }
             if (lane < 8) {
                 if (ok0[b]) {
@@ -454,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
                          mb.partials);
             return;
         }
-        cgs_block_f32<(MAXC + 1) / 2, 1, TB>(bid, mb.nb, mb.n, st.j + 1, V, mb.vld, p.w_in, p.w_out, p.h_in,
+        cgs_block_f32<(MAXC + 1) / 2, 17, TB>(bid, mb.nb, mb.n, st.j + 1, V, mb.vld, p.w_in, p.w_out, p.h_in,
                                              p.h_out, mb.partials);
     } else {
         cgs_block<MAXC, double, true, 25>(bid, mb.nb, mb.n, st.j + 1, static_cast<const double *>(mb.V), mb.vld,

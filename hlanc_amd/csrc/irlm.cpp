@@ -301,6 +301,10 @@ int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, int np
     // the CGS passes read whole tiles: unwritten columns and padding rows must be finite
     HL_TRY_HIP(hipMemsetAsync(V[0], 0, L.V1 - L.V0, s));
     HL_TRY_HIP(hipMemsetAsync(V[1], 0, 8 * static_cast<size_t>(ldv) * static_cast<size_t>(tiled_ld(ncv)), s));
+    if (copy32) {
+        HL_TRY_HIP(hipMemsetAsync(V32[0], 0, 4 * static_cast<size_t>(ldv) * static_cast<size_t>(tiled_ld(ncv)), s));
+        HL_TRY_HIP(hipMemsetAsync(V32[1], 0, 4 * static_cast<size_t>(ldv) * static_cast<size_t>(tiled_ld(ncv)), s));
+    }
     if (chunk_cnt) HL_TRY_HIP(hipMemsetAsync(chunk_cnt, 0, sizeof(unsigned) * (size_t)nslots, s));
     HL_TRY_HIP(hipMemsetAsync(red.counter, 0, 16, s));
     const size_t hbytes = sizeof(Ctl) + (host_op ? 16 * (size_t)n : 0);
