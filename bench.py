@@ -388,9 +388,12 @@ def main():
     if not args.no_autotune:
         nc, rc_, t_best = op.autotune(reps=10)
         sw, npairs = op.slices()
-        items = {"nnz_cap": nc, "rows_cap": rc_, "slice_columns": sw, "virtual_rows": npairs}
-        log(f"SpMV autotune: items of <= {nc} nnz / <= {rc_} rows, column slices of {sw} "
-            f"({npairs} virtual rows), {t_best * 1e6:.1f} us per launch")
+        sell_on, sell_entries = op.sell()
+        items = {"nnz_cap": nc, "rows_cap": rc_, "slice_columns": sw, "virtual_rows": npairs,
+                 "sliced_ell": sell_on, "sliced_ell_entries": sell_entries}
+        form = (f"sliced-ELL ({sell_entries} entries)" if sell_on else
+                f"items of <= {nc} nnz / <= {rc_} rows, column slices of {sw} ({npairs} virtual rows)")
+        log(f"SpMV autotune: {form}, {t_best * 1e6:.1f} us per launch")
 
     # timed solves write their vectors to device memory (the result stays in HBM);
     # one more solve into page-locked host memory gives the rate with the PCIe copy

@@ -100,6 +100,8 @@ _SIGS = [
     ("hl_csr_autotune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, _ip, ctypes.POINTER(ctypes.c_double)]),
     ("hl_csr_set_slices", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hl_csr_get_slices", ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
+    ("hl_csr_set_sell", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hl_csr_get_sell", ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong)]),
     ("hl_cgs_pass", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p]),
@@ -306,6 +308,16 @@ class dcsrmv_operator_gpu:
         w, npr = ctypes.c_int(), ctypes.c_int()
         _check(lib().hl_csr_get_slices(self.h, ctypes.byref(w), ctypes.byref(npr)))
         return w.value, npr.value
+
+    def set_sell(self, on: bool) -> None:
+        """Sliced-ELL SpMV (64-row chunks, a lane per row) on or off (hl_csr_set_sell)."""
+        _check(lib().hl_csr_set_sell(self.h, int(bool(on))))
+
+    def sell(self):
+        """(on, stored entries) of the sliced-ELL form."""
+        on, ne = ctypes.c_int(), ctypes.c_longlong()
+        _check(lib().hl_csr_get_sell(self.h, ctypes.byref(on), ctypes.byref(ne)))
+        return bool(on.value), ne.value
 
 
 class dcsrmv_operator_mgpu(dcsrmv_operator_gpu):

@@ -293,6 +293,25 @@ int hl_csr_get_slices(hl_csr_t h, int *width, int *npairs)
     return HL_OK;
 }
 
+int hl_csr_set_sell(hl_csr_t h, int on)
+{
+    if (!h) return HL_ERR_ARG;
+    DeviceGuard g(h->plan.dev);
+    std::lock_guard<std::mutex> lk(h->mu);
+    return plan_sell(h->plan, on != 0, 2.0);
+}
+
+int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries)
+{
+    if (!h) return HL_ERR_ARG;
+    long long ne = 0;
+    if (h->plan.sell_on)
+        HL_TRY_HIP(hipMemcpy(&ne, h->plan.e_ptr + h->plan.nchunks, sizeof(long long), hipMemcpyDeviceToHost));
+    if (on) *on = h->plan.sell_on ? 1 : 0;
+    if (entries) *entries = ne;
+    return HL_OK;
+}
+
 int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *seconds)
 {
     if (!h || reps <= 0) return HL_ERR_ARG;
@@ -317,6 +336,21 @@ int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *s
         cleanup();
         return hip_status(e, "hl_csr_autotune");
     }
+    // seconds per launch of the operator's current form (3 untimed launches first)
+    auto time_form = [&](double &t) -> int {
+        int r = HL_OK;
+        for (int i = 0; i < 3 && !r; ++i) r = launch_spmv(p, x, nullptr, y, VColumn{}, HL_PREC_F64, nullptr);
+        if (r) return r;
+        float ms = 0.0f;
+        if ((e = hipEventRecord(a, nullptr)) != hipSuccess) return hip_status(e, "hl_csr_autotune");
+        for (int i = 0; i < reps && !r; ++i) r = launch_spmv(p, x, nullptr, y, VColumn{}, HL_PREC_F64, nullptr);
+        if (r) return r;
+        if ((e = hipEventRecord(b, nullptr)) != hipSuccess || (e = hipEventSynchronize(b)) != hipSuccess ||
+            (e = hipEventElapsedTime(&ms, a, b)) != hipSuccess)
+            return hip_status(e, "hl_csr_autotune");
+        t = 1e-3 * ms / reps;
+        return HL_OK;
+    };
     const int ncap[] = {512, 1024, 2048};
     const int rcap[] = {64, 128, 256, 512, 1024, 2048};
     // column slices only where x (8 ncols bytes) exceeds an XCD's 4 MB L2 twice
@@ -326,34 +360,41 @@ int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *s
     if (p.ncols >= (1 << 20))
         for (int w : {1 << 19, 1 << 18, 1 << 17}) widths.push_back(w);
     int best_w = 0;
-    for (int wdt : widths) {
+    bool best_sell = false;
+    rc = plan_sell(p, false, 0.0); // items first
+    for (size_t wi = 0; wi < widths.size() && !rc; ++wi) {
+        const int wdt = widths[wi];
         if ((rc = plan_slices(p, wdt))) break;
-    for (int nc : ncap)
-        for (int rcp : rcap) {
-            if ((rc = csr_items(h, nc, rcp))) break;
-            for (int i = 0; i < 3 && !rc; ++i) rc = launch_spmv(p, x, nullptr, y, VColumn{}, HL_PREC_F64, nullptr);
-            if (rc) break;
-            float ms = 0.0f;
-            if ((e = hipEventRecord(a, nullptr)) != hipSuccess) break;
-            for (int i = 0; i < reps && !rc; ++i) rc = launch_spmv(p, x, nullptr, y, VColumn{}, HL_PREC_F64, nullptr);
-            if (rc || (e = hipEventRecord(b, nullptr)) != hipSuccess || (e = hipEventSynchronize(b)) != hipSuccess ||
-                (e = hipEventElapsedTime(&ms, a, b)) != hipSuccess)
-                break;
-            const double t = 1e-3 * ms / reps;
-            if (t < (wdt != best_w && best_w != 0 ? 0.98 : 1.0) * best_t) {
-                best_t = t;
-                best_n = nc;
-                best_r = rcp;
-                best_w = wdt;
+        for (int nc : ncap)
+            for (int rcp : rcap) {
+                double t = 0.0;
+                if (rc || (rc = csr_items(h, nc, rcp)) || (rc = time_form(t))) break;
+                if (t < (wdt != best_w && best_w != 0 ? 0.98 : 1.0) * best_t) {
+                    best_t = t;
+                    best_n = nc;
+                    best_r = rcp;
+                    best_w = wdt;
+                }
             }
-        }
-        if (rc || e != hipSuccess) break;
     }
-    if (!rc && e != hipSuccess) rc = hip_status(e, "hl_csr_autotune");
+    // then the sliced-ELL form, where its padding stays within 1.5 nnz
+    if (!rc) {
+        const int src = plan_sell(p, true, 1.5);
+        double t = 0.0;
+        if (src == HL_OK) {
+            if (!(rc = time_form(t)) && t < best_t) {
+                best_t = t;
+                best_sell = true;
+            }
+        } else if (src != HL_ERR_ARG) {
+            rc = src;
+        }
+    }
     cleanup();
     if (rc) return rc;
-    if ((rc = plan_slices(p, best_w))) return rc;
-    if ((rc = csr_items(h, best_n, best_r))) return rc;
+    if (!best_sell && (rc = plan_sell(p, false, 0.0))) return rc;
+    if (!best_sell && (rc = plan_slices(p, best_w))) return rc;
+    if ((rc = csr_items(h, best_n, best_r))) return rc; // (FP32-value SpMVs use the items)
     if (nnz_cap) *nnz_cap = best_n;
     if (rows_cap) *rows_cap = best_r;
     if (seconds) *seconds = best_t;

@@ -126,6 +126,13 @@ int Member::start(hipStream_t s)
 
 // End of the FP32 phase: the engine has formed ws.r = V q (rephase_host's q);
 // the solve starts over in FP64 from it.
+int Member::spmv_blocks() const
+{
+    if (!A) return 0;
+    if (sell()) return (A->nchunks + kBlock / kWave - 1) / (kBlock / kWave); // a wave per chunk
+    return sliced() ? A->sitems.nblk : A->items.nblk;
+}
+
 int Member::rephase(hipStream_t s)
 {
     spmv_f32 = spmv;
@@ -576,6 +583,11 @@ int Engine::sweep(std::vector<Member *> &ms)
         M.r_pairs = sl ? m.A->r_pairs : nullptr;
         M.chunk_part = W.chunk_part;
         M.chunk_cnt = W.chunk_cnt;
+        const bool el = m.sell();
+        M.e_ptr = el ? m.A->e_ptr : nullptr;
+        M.e_col = el ? m.A->e_col : nullptr;
+        M.e_val = el ? m.A->e_val : nullptr;
+        M.nchunks = el ? m.A->nchunks : 0;
         M.V = W.V[W.cur];
         M.V32 = m.kind == SEG_PROJ32 ? W.V32[W.cur] : nullptr;
         M.Vproj = M.V32 ? static_cast<const void *>(M.V32) : M.V;
@@ -628,7 +640,7 @@ int Engine::sweep(std::vector<Member *> &ms)
                 S.cgs0s = nsub[st][gb];
                 S.fin0 = nfin[st];
                 S.red0 = nred[st];
-                if (m.A) nspmv[st][gs] += m.sliced() ? m.A->sitems.nblk : m.A->items.nblk;
+                nspmv[st][gs] += m.spmv_blocks();
                 if (m.sliced()) nred[st] += (m.n + kRowsPerSumBlock - 1) / kRowsPerSumBlock;
                 nproj[st][gp] += hM[i].nb;
                 nsub[st][gb] += hM[i].nb;

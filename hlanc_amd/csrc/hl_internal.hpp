@@ -165,6 +165,16 @@ struct CsrPlan {
     int *r_pairs = nullptr;   // npairs: virtual row of each partial, slice order
     double *s_part = nullptr; // npairs: partials of standalone calls
     ItemSet sitems;           // over the virtual rows
+    // Sliced-ELL form of the FP64 matrix (sell_on; exclusive with the column
+    // slices): chunks of 64 consecutive rows, each stored column-major (entry k
+    // of the chunk's row l at e_ptr[c] + 64 k + l) and as wide as its longest
+    // row.  One lane sums one row in column order; padding is never read past
+    // a row's own length.
+    bool sell_on = false;
+    int nchunks = 0;
+    long long *e_ptr = nullptr;  // nchunks + 1
+    int *e_col = nullptr;        // e_ptr[nchunks] entries (padding: column 0)
+    double *e_val = nullptr;     // (padding: 0)
 };
 
 int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *col,
@@ -176,6 +186,9 @@ int plan_items(ItemSet &set, const int *R, int nrows, int nnz_cap, int rows_cap)
 // Builds (width > 0) or drops (width == 0) the column-sliced form; its items
 // take the plan's current item shape.
 int plan_slices(CsrPlan &p, int width);
+// Builds (on) or drops the sliced-ELL form; building it drops the column slices.
+// max_fill > 0 refuses (HL_ERR_ARG) a form of more than max_fill x nnz entries.
+int plan_sell(CsrPlan &p, bool on, double max_fill);
 
 // y = A (x * xscale) with the optional fused write vcol(r) = x[r] * xscale.
 // xscale == nullptr means 1.  val_prec selects the FP64 or FP32 value array.
@@ -218,6 +231,10 @@ struct SegMember {               // per solve, fixed during a sweep
     const int *r_pp, *r_pairs;   // virtual CSR), added per row by the second kernel; else null
     double *chunk_part;
     unsigned *chunk_cnt;
+    const long long *e_ptr;      // sliced-ELL form (FP64 values), else null: then blks/rp/col/val
+    const int *e_col;
+    const double *e_val;
+    int nchunks;
     void *V;                     // tiled basis in use (double, or float for SEG_V32*)
     const void *Vproj;           // basis read by the projection-only CGS passes (V, or the FP32 copy)
     float *V32;                  // FP32 copy written beside V (SEG_PROJ32), else null

@@ -105,6 +105,8 @@ struct Member {
     bool copy32() const { return A != nullptr && opts.basis_prec == HL_PREC_F64_PROJ32; } // FP32 basis copy
     // the operator's column-sliced form serves the SpMV (FP64 values and basis)
     bool sliced() const { return A != nullptr && A->slice_w > 0 && (kind == SEG_F64 || kind == SEG_PROJ32); }
+    bool sell() const { return A != nullptr && A->sell_on && kind != SEG_V32_A32; } // FP64 values only
+    int spmv_blocks() const; // blocks of this solve's SpMV launch segment
     int nslots() const { return A ? std::max(A->items.nslots, A->sitems.nslots) : 0; }
     int npairs() const { return A ? A->npairs : 0; }
     void absorb(const double *rb);       // hsub, hdia (ncv each) and the Ctl words

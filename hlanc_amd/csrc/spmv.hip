@@ -15,6 +15,7 @@
 // the grid per solve).  Every row sum is formed in a fixed order, so results
 // are bitwise repeatable.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "hl_device.hpp"
@@ -152,6 +153,76 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
     }
 }
 
+// Chunk c of the sliced-ELL form, one wave: lane l sums row 64 c + l in its
+// own column order (U entries of the row in flight at once), y(r) = that sum,
+// and writes the basis column v(r) = x(r) s beside.
+template <typename VT>
+__device__ __forceinline__ void sell_chunk(int c, int n, const long long *__restrict__ e_ptr_,
+                                           const int *__restrict__ e_col_, const double *__restrict__ e_val_,
+                                           const int *__restrict__ rp_, const double *__restrict__ x_,
+                                           const double *__restrict__ xscale, double *__restrict__ y_,
+                                           VT *__restrict__ vout_, float *__restrict__ vout32_, long long vld,
+                                           int vj, bool vtiled)
+{
+    const auto e_col = dev::gptr(e_col_);
+    const auto e_val = dev::gptr(e_val_);
+    const auto rp = dev::gptr(rp_);
+    const auto x = dev::gptr(x_);
+    const auto y = dev::gptr(y_);
+    const auto vout = dev::gptr(vout_);
+    const auto vout32 = dev::gptr(vout32_);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int r = c * kWave + lane;
+    const long long b0 = e_ptr_[c] + lane;
+    const int w = static_cast<int>((e_ptr_[c + 1] - e_ptr_[c]) / kWave);
+    const double s = xscale ? *xscale : 1.0;
+    int len = 0;
+    double xr = 0.0;
+    if (r < n) {
+        len = rp[r + 1] - rp[r];
+        if (vout) xr = x[r];
+    }
+    constexpr int U = 8;
+    double acc = 0.0;
+    for (int k0 = 0; k0 < w; k0 += U) {
+        int ci[U];
+        double av[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u < w) { // the whole chunk column: coalesced, padding included
+                ci[u] = __builtin_nontemporal_load(&e_col[b0 + (long long)(k0 + u) * kWave]);
+                av[u] = __builtin_nontemporal_load(&e_val[b0 + (long long)(k0 + u) * kWave]);
+            }
+        double xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u < len) xv[u] = x[ci[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u < len) acc += av[u] * (xv[u] * s);
+    }
+    if (r < n) {
+        y[r] = acc;
+        if (vout) {
+            const double xs = xr * s;
+            vout[vcol_pos(r, vld, vj, vtiled)] = static_cast<VT>(xs);
+            if (vout32) vout32[vcol_pos(r, vld, vj, vtiled)] = static_cast<float>(xs);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void spmv_sell_kernel(int nchunks, int n, const long long *__restrict__ e_ptr,
+                                                           const int *__restrict__ e_col,
+                                                           const double *__restrict__ e_val,
+                                                           const int *__restrict__ rp, const double *__restrict__ x,
+                                                           const double *__restrict__ xscale, double *__restrict__ y,
+                                                           double *__restrict__ vout, long long vld, int vj,
+                                                           bool vtiled)
+{
+    const int c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (c < nchunks) sell_chunk<double>(c, n, e_ptr, e_col, e_val, rp, x, xscale, y, vout, nullptr, vld, vj, vtiled);
+}
+
 template <typename AT, typename XT>
 __global__ __launch_bounds__(kBlock) void spmv_adaptive(
     const SpmvBlk *__restrict__ blks, const int *__restrict__ rp, const int *__restrict__ col,
@@ -189,6 +260,15 @@ __global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__res
     const int i = seg_find_spmv(S, ns, blockIdx.x);
     const SegStep st = S[i];
     const SegMember mb = M[st.mem];
+    if constexpr (std::is_same<AT, double>::value) {
+        if (mb.e_ptr) { // sliced-ELL form: a wave per 64-row chunk
+            const int c = (blockIdx.x - st.spmv0) * (kBlock / kWave) + threadIdx.x / kWave;
+            if (c < mb.nchunks)
+                sell_chunk<VT>(c, mb.n, mb.e_ptr, mb.e_col, mb.e_val, mb.rp, mb.r, &mb.ctl->xscale, mb.w,
+                               static_cast<VT *>(mb.V), mb.V32, mb.vld, st.j, true);
+            return;
+        }
+    }
     const bool sliced = mb.part != nullptr; // partials now; rows and the basis column in the row-sum kernel
     spmv_item<AT, double, VT>(mb.blks[blockIdx.x - st.spmv0], mb.rp, mb.col, seg_values<AT>(mb), mb.r,
                               &mb.ctl->xscale, sliced ? mb.part : mb.w, sliced ? nullptr : static_cast<VT *>(mb.V),
@@ -429,6 +509,18 @@ void items_free(ItemSet &t)
     t = ItemSet{};
 }
 
+void sell_free(CsrPlan &p)
+{
+    void *ptrs[] = {p.e_ptr, p.e_col, p.e_val};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    p.e_ptr = nullptr;
+    p.e_col = nullptr;
+    p.e_val = nullptr;
+    p.nchunks = 0;
+    p.sell_on = false;
+}
+
 void slices_free(CsrPlan &p)
 {
     void *ptrs[] = {p.s_rp, p.s_col, p.s_val, p.r_pp, p.r_pairs, p.s_part};
@@ -453,6 +545,7 @@ int plan_slices(CsrPlan &p, int width)
     }
     slices_free(p);
     if (width == 0 || p.nrows == 0) return HL_OK;
+    sell_free(p);
     const int n = p.nrows;
     const long long nnz = p.nnz;
     std::vector<int> R(static_cast<size_t>(n) + 1), C(static_cast<size_t>(nnz));
@@ -536,8 +629,59 @@ int plan_slices(CsrPlan &p, int width)
     return HL_OK;
 }
 
+// Chunks of kWave rows, entry k of a chunk's row l at e_ptr[c] + kWave k + l;
+// a chunk is as wide as its longest row, padding is column 0 and value 0.
+int plan_sell(CsrPlan &p, bool on, double max_fill)
+{
+    sell_free(p);
+    if (!on || p.nrows == 0) return HL_OK;
+    const int n = p.nrows;
+    const long long nnz = p.nnz;
+    std::vector<int> R(static_cast<size_t>(n) + 1);
+    HL_TRY_HIP(hipMemcpy(R.data(), p.row_ptr, sizeof(int) * R.size(), hipMemcpyDeviceToHost));
+    const int nch = (n + kWave - 1) / kWave;
+    std::vector<long long> ptr(static_cast<size_t>(nch) + 1, 0);
+    for (int c = 0; c < nch; ++c) {
+        int w = 0;
+        for (int r = c * kWave; r < std::min(n, (c + 1) * kWave); ++r) w = std::max(w, R[r + 1] - R[r]);
+        ptr[c + 1] = ptr[c] + static_cast<long long>(kWave) * w;
+    }
+    const long long ne = ptr[nch];
+    if (max_fill > 0.0 && static_cast<double>(ne) > max_fill * static_cast<double>(std::max(nnz, 1LL))) {
+        set_error("SpMV sliced-ELL: %lld entries for %lld nonzeros exceed %.2fx", ne, nnz, max_fill);
+        return HL_ERR_ARG;
+    }
+    std::vector<int> C(static_cast<size_t>(nnz));
+    std::vector<double> A(static_cast<size_t>(nnz));
+    if (nnz > 0) {
+        HL_TRY_HIP(hipMemcpy(C.data(), p.col, sizeof(int) * C.size(), hipMemcpyDeviceToHost));
+        HL_TRY_HIP(hipMemcpy(A.data(), p.val, sizeof(double) * A.size(), hipMemcpyDeviceToHost));
+    }
+    std::vector<int> ec(static_cast<size_t>(ne), 0);
+    std::vector<double> ev(static_cast<size_t>(ne), 0.0);
+    for (int r = 0; r < n; ++r) {
+        const int c = r / kWave, l = r % kWave;
+        for (int q = R[r]; q < R[r + 1]; ++q) {
+            const long long at = ptr[c] + static_cast<long long>(kWave) * (q - R[r]) + l;
+            ec[at] = C[q];
+            ev[at] = A[q];
+        }
+    }
+    int rc;
+    if ((rc = dev_upload(&p.e_ptr, ptr.data(), ptr.size())) || (rc = dev_upload(&p.e_col, ec.data(), ec.size())) ||
+        (rc = dev_upload(&p.e_val, ev.data(), ev.size()))) {
+        sell_free(p);
+        return rc;
+    }
+    slices_free(p);
+    p.nchunks = nch;
+    p.sell_on = true;
+    return HL_OK;
+}
+
 void csr_plan_free(CsrPlan &p)
 {
+    sell_free(p);
     slices_free(p);
     items_free(p.items);
     void *ptrs[] = {p.row_ptr, p.col, p.val, p.val32, p.x_stage, p.y_stage};
@@ -558,6 +702,11 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
         spmv_adaptive<float, double><<<p.items.nblk, kBlock, 0, s>>>(
             p.items.blks, p.row_ptr, p.col, p.val32, x, xscale, y, vcol.base, vcol.L.ld, vcol.j, vcol.L.tiled,
             p.items.chunk_part, p.items.chunk_cnt);
+    } else if (p.sell_on) { // a wave per 64-row chunk
+        if (p.nchunks > 0)
+            spmv_sell_kernel<<<(p.nchunks + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, s>>>(
+                p.nchunks, p.nrows, p.e_ptr, p.e_col, p.e_val, p.row_ptr, x, xscale, y, vcol.base, vcol.L.ld, vcol.j,
+                vcol.L.tiled);
     } else if (p.slice_w > 0) { // partials of the virtual rows, then the row sums
         if (p.sitems.nblk > 0)
             spmv_adaptive<double, double><<<p.sitems.nblk, kBlock, 0, s>>>(

@@ -128,12 +128,24 @@ int hl_csr_set_items(hl_csr_t h, int nnz_cap, int rows_cap);
  * the reference. */
 int hl_csr_set_slices(hl_csr_t h, int width);
 int hl_csr_get_slices(hl_csr_t h, int *width, int *npairs); /* outputs may be NULL */
+/* Sliced-ELL SpMV: on != 0 stores the FP64 matrix again as chunks of 64
+ * consecutive rows, each column-major and as wide as its longest row; one lane
+ * sums one row in its column order, with no partial sums.  Suits matrices of
+ * even row lengths (stencils).  It drops the column slices (and they drop it).
+ * Refused (HL_ERR_ARG) when the chunks would hold more than twice nnz
+ * entries.  Used by every FP64-value SpMV of the operator; costs 12 bytes per
+ * entry on the device.  Same rules as hl_csr_set_items.  No counterpart in the
+ * reference. */
+int hl_csr_set_sell(hl_csr_t h, int on);
+int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries); /* outputs may be NULL */
 /* Autotuner: times `reps` SpMV launches for every shape of a small grid
  * (nnz_cap in {512, 1024, 2048} x rows_cap in {64 .. 2048}), and, for
- * operators of >= 2^20 columns, every shape again under column slices of 2^17,
- * 2^18 and 2^19 columns, with HIP events on the null stream; keeps the fastest
- * and reports its shape (outputs may be NULL; hl_csr_get_slices tells the
- * slicing).  Same rules as hl_csr_set_items. */
+ * operators of >= 2^20 columns, every shape again under column slices of 2^19,
+ * 2^18 and 2^17 columns (widest first; a narrower width must win by 2 %), then
+ * the sliced-ELL form where it holds at most 1.5 nnz entries, with HIP events
+ * on the null stream; keeps the fastest and reports its item shape (outputs
+ * may be NULL; hl_csr_get_slices and hl_csr_get_sell tell the form).  Same
+ * rules as hl_csr_set_items. */
 int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *seconds);
 
 /* ------------------------------------------------------ basis operations

@@ -123,7 +123,7 @@ def test_mgpu_collapses_to_one_device_and_rejects_symmetric():
         H.dcsrmv_operator_mgpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, is_symmetric=True))
 
 
-def ragged_matrix(seed=5):
+def item_cap_matrix(seed=5):
     """Empty rows, rows just past the item capacity, multi-chunk rows."""
     rng = np.random.default_rng(seed)
     n, ncols = 900, 30000
@@ -141,7 +141,7 @@ def ragged_matrix(seed=5):
 def test_every_item_shape_matches_oracle(nnz_cap, rows_cap):
     # work-item shapes the autotuner may pick: rows longer than nnz_cap become
     # chunks, short-row matrices get up to 2048 rows per item
-    rp, col, val, ncols = ragged_matrix()
+    rp, col, val, ncols = item_cap_matrix()
     m = H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, ncols=ncols)
     op = H.dcsrmv_operator_gpu(m)
     op.set_items(nnz_cap, rows_cap)
@@ -260,5 +260,101 @@ def test_autotune_considers_slices_on_large_operators():
     w, npairs = op.slices()
     assert w in (0, 1 << 17, 1 << 18, 1 << 19) and (w == 0) == (npairs == 0) and t > 0
     x = np.random.default_rng(8).uniform(-1, 1, n)
+    y_ref = O.spmv(O.Csr(rp, col, val, n, False), x)
+    assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
+
+
+def even_matrix(seed=4, n=1000, ncols=1200):
+    """Row lengths 0..30 (some empty), n % 64 != 0, unsorted columns with duplicates:
+    chunks of 64 rows stay within the sliced-ELL form's fill limit."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(10, 31, n)
+    lens[::37] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = rng.integers(0, ncols, rp[-1]).astype(np.int32)
+    val = rng.uniform(-1, 1, rp[-1])
+    return rp, col, val, ncols
+
+
+@pytest.mark.parametrize("case", ["even", "stencil", "lap2d"])
+def test_sell_spmv_matches_oracle(case):
+    """hl_csr_set_sell: 64-row chunks, a lane per row summing in column order."""
+    if case == "even":
+        rp, col, val, ncols = even_matrix()
+    elif case == "stencil":
+        rp, col, val = W.stencil27(17, seed=3)
+        ncols = len(rp) - 1
+    else:
+        rp, col, val = W.laplacian_2d(45)
+        ncols = len(rp) - 1
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, ncols=ncols))
+    x = np.random.default_rng(9).uniform(-1, 1, ncols)
+    y_ref = O.spmv(O.Csr(rp, col, val, ncols, False), x)
+    scale = np.linalg.norm(y_ref)
+    op.set_sell(True)
+    on, entries = op.sell()
+    lens = np.diff(rp)
+    widths = [lens[c:c + 64].max() for c in range(0, n, 64)]
+    assert on and entries == 64 * sum(widths)
+    y = op(x)
+    assert np.linalg.norm(y - y_ref) <= TOL * scale
+    np.testing.assert_array_equal(op(x), y)  # repeatable
+    assert (y[lens == 0] == 0.0).all()
+    op.set_sell(False)
+    assert op.sell() == (False, 0)
+    assert np.linalg.norm(op(x) - y_ref) <= TOL * scale
+
+
+def test_sell_refuses_uneven_rows_and_excludes_slices():
+    rp, col, val, ncols = ragged_matrix()  # a 40000-entry row: its chunk would be mostly padding
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, ncols=ncols))
+    with pytest.raises(H.HLancError):
+        op.set_sell(True)
+    assert op.sell() == (False, 0)
+    rp, col, val = W.stencil27(12, seed=1)
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    op.set_slices(256)
+    op.set_sell(True)  # drops the slices
+    assert op.sell()[0] and op.slices() == (0, 0)
+    op.set_slices(256)  # drops the sliced-ELL form
+    assert not op.sell()[0] and op.slices()[0] == 256
+
+
+def test_sell_operator_solves_match_oracle_alone_and_batched():
+    rp, col, val = W.stencil27(14, seed=6)
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val), fp32_values=True)
+    op.set_sell(True)
+    o = O.ds_solver(O.Csr(rp, col, val), 8, 24, tol=1e-10, maxiter=1000)
+    A = sp.csr_matrix((val, col, rp))
+    alone = []
+    for prec in (H.HL_PREC_F64, H.HL_PREC_F64_PROJ32, H.HL_PREC_F32):  # F32: FP32 values, on the items
+        r = H.ds_solver_gpu(n, 8, 24, op, tol=1e-10, maxiter=1000, basis_prec=prec)
+        assert r.ret == 0 and o.ret == 0
+        np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
+        for i in range(8):
+            v = r.vectors[:, i]
+            assert np.linalg.norm(A @ v - r.values[i] * v) <= 1e-10 * np.abs(r.values).max()
+        alone.append(r)
+    plain = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    precs = [H.HL_PREC_F64, H.HL_PREC_F64_PROJ32, H.HL_PREC_F32, H.HL_PREC_F64_PROJ32]
+    b = H.ds_solve_batch([op, op, op, plain], 8, 24, tol=1e-10, maxiter=1000, basis_prec=precs)
+    for k in range(3):
+        np.testing.assert_array_equal(b[k].values, alone[k].values)
+        np.testing.assert_array_equal(b[k].vectors, alone[k].vectors)
+    r_plain = H.ds_solver_gpu(n, 8, 24, plain, tol=1e-10, maxiter=1000, basis_prec=H.HL_PREC_F64_PROJ32)
+    np.testing.assert_array_equal(b[3].values, r_plain.values)
+    np.testing.assert_allclose(r_plain.values, alone[1].values, rtol=1e-12)
+
+
+def test_autotune_considers_sell():
+    rp, col, val = W.stencil27(40, seed=2)
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    nc, rc, t = op.autotune(reps=3)
+    on, entries = op.sell()
+    assert t > 0 and (entries > 0) == on and not (on and op.slices()[0])
+    x = np.random.default_rng(10).uniform(-1, 1, n)
     y_ref = O.spmv(O.Csr(rp, col, val, n, False), x)
     assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
