@@ -157,6 +157,22 @@ def batch_cpu_baseline(host, sample, results, ns, nnzs, nev, ncv, tol, maxiter, 
                       f"{len(results)} by Lanczos steps x (nnz + 1.5 ncv n)"}
 
 
+def sell_where_even(H, ops, nnzs, limit=1.1):
+    """The sliced-ELL SpMV form on every operator whose 64-row chunks hold at
+    most `limit` x nnz entries (even row lengths); returns how many took it."""
+    taken = 0
+    for op, z in zip(ops, nnzs):
+        try:
+            op.set_sell(True)
+        except H.HLancError:  # refused: rows too uneven
+            continue
+        if op.sell()[1] > limit * z:
+            op.set_sell(False)
+        else:
+            taken += 1
+    return taken
+
+
 def run_batch(args, H, torch):
     from hlanc_amd import workloads as W
 
@@ -169,12 +185,14 @@ def run_batch(args, H, torch):
     ns = [op.nrows for op in ops]
     nnzs = [len(m[1][1]) for m in mix]
     nnz = sum(nnzs)
+    # SpMV form per operator (the batched and the plain path share the operators)
+    n_sell = 0 if args.no_autotune or mixed else sell_where_even(H, ops, nnzs)
     # host copies of 16 matrices at evenly spaced size ranks, for the CPU oracle's sample
     sample = sorted(range(count), key=lambda i: ns[i])[:: max(1, count // 16)]
     host = {i: mix[i][1] for i in sample}
     del mix
     log(f"workload batch: {count} matrices, sum n {sum(ns)}, sum nnz {nnz}, built in "
-        f"{time.perf_counter() - t0:.1f} s")
+        f"{time.perf_counter() - t0:.1f} s; {n_sell} operators in the sliced-ELL form")
     torch.cuda.init()
     kw = dict(tol=args.tol, maxiter=args.maxiter, use_mfma=args.mfma)
     prec = basis_prec(H, args.basis)
@@ -244,7 +262,8 @@ def run_batch(args, H, torch):
         "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic (seeded generator)",
         "config": {"workload": f"{count} uneven matrices, n log-uniform [1e3, 2e5], stencils + Erdos-Renyi",
                    "sum_n": int(sum(ns)), "sum_nnz": int(nnz), "nev": nev, "ncv": ncv, "tol": args.tol,
-                   "max_active": args.max_active, "basis": BASIS_LABEL[args.basis]},
+                   "max_active": args.max_active, "basis": BASIS_LABEL[args.basis],
+                   "sliced_ell_operators": n_sell},
         "plain_value": plain_value, "plain_basis": "fp64", "speedup_vs_plain": value / plain_value,
         "max_rel_value_err_vs_fp64": err64,
         "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
@@ -264,10 +283,12 @@ def run_serve(args, H, torch):
     prec = basis_prec(H, args.basis)
     ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=m[1][0], col=m[1][1], val=m[1][2]), fp32_values=mixed)
            for m in pool]
-    nnz = sum(len(m[1][1]) for m in pool)
+    nnzs = [len(m[1][1]) for m in pool]
+    nnz = sum(nnzs)
     del pool
+    n_sell = 0 if args.no_autotune or mixed else sell_where_even(H, ops, nnzs)
     log(f"workload serve: pool of {len(ops)} matrices, sum n {sum(op.nrows for op in ops)}, sum nnz {nnz}, "
-        f"built in {time.perf_counter() - t0:.1f} s")
+        f"built in {time.perf_counter() - t0:.1f} s; {n_sell} operators in the sliced-ELL form")
     torch.cuda.init()
     rng = np.random.default_rng(args.seed)
     R = args.requests
@@ -318,7 +339,8 @@ def run_serve(args, H, torch):
         "config": {"workload": f"Poisson stream, {R} requests, pool of {len(ops)} matrices n log-uniform "
                                "[4e3, 2e6], nev in {4,8,16,32}, ncv = 3 nev",
                    "offered_req_per_s": rates, "tol": args.tol, "max_active": args.max_active,
-                   "hbm_budget_bytes": args.hbm_budget, "basis": BASIS_LABEL[args.basis]},
+                   "hbm_budget_bytes": args.hbm_budget, "basis": BASIS_LABEL[args.basis],
+                   "sliced_ell_operators": n_sell},
         "latency_s": top["latency_s"], "fcfs": top["fcfs"], "rates": per_rate,
         "same_results_as_fcfs": bool(same), "failures": int(fails), "max_rel_resid": worst,
     }
