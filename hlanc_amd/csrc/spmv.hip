@@ -641,12 +641,18 @@ int plan_sell(CsrPlan &p, bool on, double max_fill)
     HL_TRY_HIP(hipMemcpy(R.data(), p.row_ptr, sizeof(int) * R.size(), hipMemcpyDeviceToHost));
     const int nch = (n + kWave - 1) / kWave;
     std::vector<long long> ptr(static_cast<size_t>(nch) + 1, 0);
+    int wmax = 0;
     for (int c = 0; c < nch; ++c) {
         int w = 0;
         for (int r = c * kWave; r < std::min(n, (c + 1) * kWave); ++r) w = std::max(w, R[r + 1] - R[r]);
         ptr[c + 1] = ptr[c] + static_cast<long long>(kWave) * w;
+        wmax = std::max(wmax, w);
     }
     const long long ne = ptr[nch];
+    if (wmax > kNnzPerBlock) { // one wave would walk the whole row: the items split it into chunks
+        set_error("SpMV sliced-ELL: a row of %d nonzeros (more than %d)", wmax, kNnzPerBlock);
+        return HL_ERR_ARG;
+    }
     if (max_fill > 0.0 && static_cast<double>(ne) > max_fill * static_cast<double>(std::max(nnz, 1LL))) {
         set_error("SpMV sliced-ELL: %lld entries for %lld nonzeros exceed %.2fx", ne, nnz, max_fill);
         return HL_ERR_ARG;

@@ -132,8 +132,8 @@ int hl_csr_get_slices(hl_csr_t h, int *width, int *npairs); /* outputs may be NU
  * consecutive rows, each column-major and as wide as its longest row; one lane
  * sums one row in its column order, with no partial sums.  Suits matrices of
  * even row lengths (stencils).  It drops the column slices (and they drop it).
- * Refused (HL_ERR_ARG) when the chunks would hold more than twice nnz
- * entries.  Used by every FP64-value SpMV of the operator; costs 12 bytes per
+ * Refused (HL_ERR_ARG) when the chunks would hold more than twice nnz entries
+ * or a row holds more than 2048 nonzeros (one wave would walk it alone).  Used by every FP64-value SpMV of the operator; costs 12 bytes per
  * entry on the device.  Same rules as hl_csr_set_items.  No counterpart in the
  * reference. */
 int hl_csr_set_sell(hl_csr_t h, int on);

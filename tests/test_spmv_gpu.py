@@ -358,3 +358,24 @@ def test_autotune_considers_sell():
     x = np.random.default_rng(10).uniform(-1, 1, n)
     y_ref = O.spmv(O.Csr(rp, col, val, n, False), x)
     assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
+
+
+def test_sell_refuses_a_long_row():
+    """One row of 2100 nonzeros in a stencil: the fill stays low (1.1x), but a single
+    wave would walk the row; the stream items split it into chunks instead."""
+    rp, col, val = W.stencil27(40, seed=5)
+    n = len(rp) - 1
+    rng = np.random.default_rng(12)
+    long_cols = np.sort(rng.choice(n, 2100, replace=False)).astype(np.int32)
+    lens = np.diff(rp).copy()
+    lens[0] = len(long_cols)
+    rp2 = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col2 = np.concatenate([long_cols, col[rp[1]:]]).astype(np.int32)
+    val2 = np.concatenate([rng.uniform(-1, 1, len(long_cols)), val[rp[1]:]])
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp2, col=col2, val=val2))
+    with pytest.raises(H.HLancError):
+        op.set_sell(True)
+    assert op.sell() == (False, 0)
+    x = np.random.default_rng(13).uniform(-1, 1, n)
+    y_ref = O.spmv(O.Csr(rp2, col2, val2, n, False), x)
+    assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
