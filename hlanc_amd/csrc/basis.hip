@@ -681,44 +681,29 @@ __global__ __launch_bounds__(kBlock) void vq_seg_kernel(const SegVQ *__restrict_
     }
 }
 
-// The same restart on v_mfma_f64_16x16x4f64 (layout as vq_mfma_kernel): each
-// wave takes 16-row tiles, reads V once per tile and accumulates all of the
-// solve's 16-column output tiles (at most NT); Q^T sits in LDS.  Used for the
-// solves whose K x N fits (vq_seg_mfma_fits).
-template <int NT, typename VT>
-__global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__restrict__ T, int nt)
+// The 16-row tiles of this wave for a solve of exactly NCT output column
+// tiles: Out = V Q and the fused dsapps residual update; returns the wave's
+// partial of ||r||^2.  An exact tile count keeps the accumulators in place
+// (a per-tile guard made the compiler copy them between register files).
+template <int NCT, typename VT>
+__device__ __forceinline__ double vq_mfma_rows(const SegVQ &t, const double *qs, int NC, int Kp, int bid, int wv,
+                                               int lane, double qrs)
 {
-    extern __shared__ double qs[]; // Kp x NC, row k
-    __shared__ double red[kBlock / kWave];
-    const int i = seg_find(&T[0].blk0, sizeof(SegVQ) / sizeof(int), nt, blockIdx.x);
-    const SegVQ t = T[i];
     const auto V = gptr(static_cast<const VT *>(t.V));
     const auto Out = gptr(static_cast<VT *>(t.Out));
     const auto Out32 = gptr(t.Out32);
     const auto r = gptr(t.r);
-    const int bid = blockIdx.x - t.blk0;
-    const int nct = (t.N + 15) / 16; // this solve's column tiles, <= NT
-    const int NC = nct * 16;
-    const int Kp = (t.K + 3) & ~3;
-    for (int x = threadIdx.x; x < Kp * NC; x += kBlock) {
-        const int k = x / NC, c = x % NC;
-        qs[x] = (k < t.K && c < t.N) ? t.Q[(long long)c * t.K + k] : 0.0;
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = threadIdx.x / kWave;
     const int lr = lane & 15, lk = lane >> 4;
-    const double qrs = t.qlast * t.ctl->rscale;
     double nrm = 0.0;
     const long long ntiles = (t.n + 15) / 16;
     for (long long tile = (long long)bid * (kBlock / kWave) + wv; tile < ntiles;
          tile += (long long)t.nb * (kBlock / kWave)) {
         const long long row = tile * 16 + lr;
         const bool ok = row < t.n;
-        dbl4 acc[NT];
+        dbl4 acc[NCT];
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
-        // 8 loads of V in flight per wave before their 8 x nct MFMAs
+        for (int ct = 0; ct < NCT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+        // 8 loads of V in flight per wave before their 8 x NCT MFMAs
         for (int k0 = 0; k0 < Kp; k0 += 32) {
             double b[8];
 #pragma unroll
@@ -731,15 +716,13 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
                 if (k0 + 4 * j >= Kp) break;
                 const int k = k0 + 4 * j + lk;
 #pragma unroll
-                for (int ct = 0; ct < NT; ++ct)
-                    if (ct < nct)
-                        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(qs[k * NC + ct * 16 + lr], b[j], acc[ct], 0, 0, 0);
+                for (int ct = 0; ct < NCT; ++ct)
+                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(qs[k * NC + ct * 16 + lr], b[j], acc[ct], 0, 0, 0);
             }
         }
         double rk = 0.0;
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) {
-            if (ct >= nct) continue;
+        for (int ct = 0; ct < NCT; ++ct) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int cl = ct * 16 + lk + 4 * e;
@@ -757,6 +740,49 @@ __global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__rest
             r[row] = rr;
             nrm += rr * rr;
         }
+    }
+    return nrm;
+}
+
+// One block's share of the restart V.Q of one listed solve: up to NT 16-column
+// output tiles; Q^T sits in LDS.  Used for the solves whose K x N fits
+// (vq_seg_mfma_fits).
+template <int NT, typename VT>
+__global__ __launch_bounds__(kBlock) void vq_seg_mfma_kernel(const SegVQ *__restrict__ T, int nt)
+{
+    extern __shared__ double qs[]; // Kp x NC, row k
+    __shared__ double red[kBlock / kWave];
+    const int i = seg_find(&T[0].blk0, sizeof(SegVQ) / sizeof(int), nt, blockIdx.x);
+    const SegVQ t = T[i];
+    const int bid = blockIdx.x - t.blk0;
+    const int nct = (t.N + 15) / 16; // this solve's column tiles, <= NT
+    const int NC = nct * 16;
+    const int Kp = (t.K + 3) & ~3;
+    for (int x = threadIdx.x; x < Kp * NC; x += kBlock) {
+        const int k = x / NC, c = x % NC;
+        qs[x] = (k < t.K && c < t.N) ? t.Q[(long long)c * t.K + k] : 0.0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const double qrs = t.qlast * t.ctl->rscale;
+    double nrm = 0.0;
+    switch (nct) { // the exact tile count (wave-uniform)
+#define HL_VQ_NCT(C)                                                                      \
+    case C:                                                                               \
+        if constexpr (C <= NT) nrm = vq_mfma_rows<C, VT>(t, qs, NC, Kp, bid, wv, lane, qrs); \
+        break;
+        HL_VQ_NCT(1)
+        HL_VQ_NCT(2)
+        HL_VQ_NCT(3)
+        HL_VQ_NCT(4)
+        HL_VQ_NCT(5)
+        HL_VQ_NCT(6)
+        HL_VQ_NCT(7)
+        HL_VQ_NCT(8)
+#undef HL_VQ_NCT
+    default:
+        break;
     }
     for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
     if (lane == 0) red[wv] = nrm;
