@@ -45,7 +45,8 @@ int count_converged(int n, const double *ritz, const double *bounds, double tol)
 
 static inline void rotate_cols(double *q, int ldq, int c0, int rows, double c, double s)
 {
-    double *qa = q + (size_t)c0 * ldq, *qb = qa + ldq;
+    double *__restrict__ qa = q + (size_t)c0 * ldq;
+    double *__restrict__ qb = qa + ldq; // columns c0 and c0 + 1: disjoint
     for (int r = 0; r < rows; ++r) {
         const double a1 = c * qa[r] + s * qb[r];
         qb[r] = -s * qa[r] + c * qb[r];

@@ -17,6 +17,17 @@
 
 namespace hl {
 
+namespace {
+// sqrt(a^2 + b^2): the squares cannot overflow or underflow while the larger
+// magnitude lies in [1e-150, 1e150]; outside that range, std::hypot.
+inline double norm2(double a, double b)
+{
+    const double m = std::fmax(std::fabs(a), std::fabs(b));
+    return (m > 1e-150 && m < 1e150) ? std::sqrt(a * a + b * b) : std::hypot(a, b);
+}
+
+} // namespace
+
 void dlartg(double f, double g, double &c, double &s, double &r)
 {
     if (g == 0.0) {
@@ -31,7 +42,7 @@ void dlartg(double f, double g, double &c, double &s, double &r)
         r = g;
         return;
     }
-    const double h = std::hypot(f, g);
+    const double h = norm2(f, g);
     c = f / h;
     s = g / h;
     r = h;
@@ -71,7 +82,7 @@ int tridiag_ql(int n, double *d, double *e, double *z, int ldz)
             if (++iter > 100) return 1;
             // shift: eigenvalue of the leading 2x2 block nearer to d[l]
             double g = (d[l + 1] - d[l]) / (2.0 * off[l]);
-            double r = std::hypot(g, 1.0);
+            double r = norm2(g, 1.0);
             g = d[m] - d[l] + off[l] / (g + std::copysign(r, g));
             double s = 1.0, c = 1.0, p = 0.0;
             int i = m - 1;
@@ -79,7 +90,7 @@ int tridiag_ql(int n, double *d, double *e, double *z, int ldz)
             for (; i >= l; --i) {
                 const double f = s * off[i];
                 const double b = c * off[i];
-                r = std::hypot(f, g);
+                r = norm2(f, g);
                 off[i + 1] = r;
                 if (r == 0.0) { // split: recover and restart the sweep
                     d[i + 1] -= p;
