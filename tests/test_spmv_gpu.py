@@ -379,3 +379,23 @@ def test_sell_refuses_a_long_row():
     x = np.random.default_rng(13).uniform(-1, 1, n)
     y_ref = O.spmv(O.Csr(rp2, col2, val2, n, False), x)
     assert np.linalg.norm(op(x) - y_ref) <= TOL * np.linalg.norm(y_ref)
+
+
+def test_sell_with_an_fp32_basis_phase():
+    """HL_PREC_F32 on FP64 values: the FP32 phase's SpMV writes an FP32 basis column from
+    the sliced-ELL kernel; the solve still meets the FP64 bar, alone and batched."""
+    rp, col, val = W.stencil27(13, seed=8)
+    n = len(rp) - 1
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    op.set_sell(True)
+    A = sp.csr_matrix((val, col, rp))
+    o = O.ds_solver(O.Csr(rp, col, val), 6, 18, tol=1e-10, maxiter=2000)
+    r = H.ds_solver_gpu(n, 6, 18, op, tol=1e-10, maxiter=2000, basis_prec=H.HL_PREC_F32)
+    assert r.ret == 0 and o.ret == 0 and r.info.spmv_f32 > 0
+    np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
+    for i in range(6):
+        v = r.vectors[:, i]
+        assert np.linalg.norm(A @ v - r.values[i] * v) <= 1e-10 * np.abs(r.values).max()
+    b = H.ds_solve_batch([op, op], 6, 18, tol=1e-10, maxiter=2000, basis_prec=[H.HL_PREC_F32, H.HL_PREC_F64])
+    np.testing.assert_array_equal(b[0].values, r.values)
+    np.testing.assert_array_equal(b[0].vectors, r.vectors)
