@@ -45,7 +45,7 @@ class _Matrix(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("v0", _dp), ("seed", ctypes.c_uint64),
                 ("basis_prec", ctypes.c_int), ("use_mfma", ctypes.c_int), ("flags", ctypes.c_int),
-                ("reserved", ctypes.c_int * 5)]
+                ("refine", ctypes.c_int), ("residuals", _dp), ("reserved", ctypes.c_int * 4)]
 
 
 class Info(ctypes.Structure):
@@ -54,7 +54,9 @@ class Info(ctypes.Structure):
                 ("spmv", ctypes.c_longlong), ("cgs_passes", ctypes.c_longlong), ("rnorm", ctypes.c_double),
                 ("seconds", ctypes.c_double), ("max_rel_resid", ctypes.c_double),
                 ("refine_shift", ctypes.c_double), ("spmv_f32", ctypes.c_longlong),
-                ("restarts_f32", ctypes.c_int), ("proj64_steps", ctypes.c_int)]
+                ("restarts_f32", ctypes.c_int), ("proj64_steps", ctypes.c_int),
+                ("anorm_est", ctypes.c_double), ("rq_shift", ctypes.c_double), ("verified", ctypes.c_int),
+                ("pad", ctypes.c_int)]
 
 
 class Profile(ctypes.Structure):
@@ -67,6 +69,9 @@ class Profile(ctypes.Structure):
 PROFILE_CLASSES = ["spmv", "cgs", "vq", "other", "dgks", "host"]
 HL_FLAG_NO_GRAPH = 1
 HL_FLAG_PROFILE = 2
+HL_FLAG_NO_VERIFY = 4
+HL_REFINE_NONE = 0
+HL_REFINE_RQ = 1
 
 
 class SolveItem(ctypes.Structure):
@@ -368,10 +373,14 @@ class SolveResult:
     values: np.ndarray
     vectors: np.ndarray
     info: Info
+    residuals: Optional[np.ndarray] = None  # per returned pair (the library's residual check)
 
 
 def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mfma: Optional[bool] = None,
-            basis_prec: Optional[int] = None, flags: int = 0) -> Options:
+            basis_prec: Optional[int] = None, flags: int = 0, refine: int = HL_REFINE_NONE,
+            residuals: Optional[np.ndarray] = None) -> Options:
+    """hl_options from the library defaults (hl_options_default).  ``residuals``
+    (float64, >= nev entries) receives each returned pair's relative residual."""
     o = Options()
     lib().hl_options_default(ctypes.byref(o))
     o.tol = tol
@@ -382,9 +391,15 @@ def options(tol: float = 0.0, maxiter: int = 200, v0=None, seed: int = 0, use_mf
     if basis_prec is not None:  # None keeps the library default (hl_options_default)
         o.basis_prec = basis_prec
     o.flags = flags
+    o.refine = int(refine)
     if v0 is not None:
         o._v0 = np.ascontiguousarray(v0, dtype=np.float64)
         o.v0 = _d(o._v0)
+    if residuals is not None:
+        if residuals.dtype != np.float64 or not residuals.flags.c_contiguous:
+            raise HLancError(-1, "residuals must be a contiguous float64 array")
+        o._res = residuals
+        o.residuals = _d(residuals)
     return o
 
 
@@ -416,7 +431,7 @@ def _vectors_args(vectors):
 def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, v0=None,
                   seed: int = 0, use_mfma: Optional[bool] = None, ldv: Optional[int] = None,
                   flags: int = 0, vectors_out: Optional[np.ndarray] = None,
-                  basis_prec: Optional[int] = None) -> SolveResult:
+                  basis_prec: Optional[int] = None, refine: int = HL_REFINE_NONE) -> SolveResult:
     """ds_solver_gpu (HLanc.hpp:1868-1872).  ``op`` is a dcsrmv_operator_gpu
     (device-resident path) or any callable ``op(x, y)`` on numpy arrays (host
     path, as the reference calls arbitrary operators with host pointers).
@@ -426,13 +441,17 @@ def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 
     device tensor such as ``torch.empty(ncv, ldv, dtype=torch.float64,
     device="cuda").T``, which keeps the result in HBM) receives the vectors
     instead of a fresh array.  ``basis_prec`` (None: the library default,
-    HL_PREC_F64_PROJ32) selects the basis storage; every mode returns FP64
-    results to ``tol`` (hlanc_capi.h, hl_options.basis_prec)."""
+    HL_PREC_F64, the reference's Gram-Schmidt) selects the basis storage; every
+    mode returns FP64 results to ``tol`` (hlanc_capi.h, hl_options.basis_prec).
+    ``refine=HL_REFINE_RQ`` returns Rayleigh quotients of the returned vectors.
+    ``residuals`` of the result holds the library's per-pair residual check
+    (NaN where it did not run)."""
     ldv = n if ldv is None else ldv
     values = np.zeros(ncv)
     vectors = _vectors_buffer(vectors_out, ldv, ncv)
     vptr, ldv = _vectors_args(vectors)
-    o = options(tol, maxiter, v0, seed, use_mfma, basis_prec=basis_prec, flags=flags)
+    res = np.full(max(nev, 1), np.nan)
+    o = options(tol, maxiter, v0, seed, use_mfma, basis_prec=basis_prec, flags=flags, refine=refine, residuals=res)
     info = Info()
     if isinstance(op, dcsrmv_operator_gpu):
         if op.nrows != n:
@@ -449,7 +468,7 @@ def ds_solver_gpu(n: int, nev: int, ncv: int, op, dev: int = 0, *, tol: float = 
                                        ctypes.byref(info))
     _check(rc, allow=(HL_OK, HL_NOT_CONVERGED))
     vecs = vectors[:n] if _is_device_array(vectors) else np.asarray(vectors[:n])
-    return SolveResult(rc, values[:nev].copy(), vecs, info)
+    return SolveResult(rc, values[:nev].copy(), vecs, info, res[:nev].copy())
 
 
 @dataclass
@@ -461,6 +480,7 @@ class BatchResult:
     t_admit: float
     t_done: float
     arrival: float
+    residuals: Optional[np.ndarray] = None  # per returned pair (the library's residual check)
 
     @property
     def latency(self) -> float:
@@ -471,7 +491,7 @@ class BatchResult:
 def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, maxiter: int = 200, seeds=None,
                    v0s=None, use_mfma: Optional[bool] = None, vectors: bool = True, arrivals=None,
                    max_active: int = 0, hbm_budget: float = 0.0, flags: int = 0, vectors_out=None,
-                   basis_prec=None):
+                   basis_prec=None, refine=HL_REFINE_NONE):
     """Many solves on one device with continuous batching (hl_ds_solve_batch).
 
     ``ops`` are dcsrmv_operator_gpu on ``dev``; nev, ncv, tol, maxiter,
@@ -495,8 +515,9 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
             vecs = _vectors_buffer(None if vectors_out is None else vectors_out[i], n, max(ncv_i, 1))
         seed = 0 if seeds is None else int(seeds[i])
         v0 = None if v0s is None else v0s[i]
+        res = np.full(max(nev_i, 1), np.nan)
         o = options(float(per(tol, i)), int(per(maxiter, i)), v0, seed, per(use_mfma, i),
-                    basis_prec=per(basis_prec, i), flags=flags)
+                    basis_prec=per(basis_prec, i), flags=flags, refine=per(refine, i), residuals=res)
         it = items[i]
         it.A = op.h
         it.nev = nev_i
@@ -505,17 +526,18 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
         it.vectors, it.ldv = _vectors_args(vecs) if vecs is not None else (None, n)
         it.opts = o
         it.arrival = 0.0 if arrivals is None else float(arrivals[i])
-        keep.append((vals, vecs, o))
+        keep.append((vals, vecs, o, res))
     rc = lib().hl_ds_solve_batch(dev, items, k, int(max_active), float(hbm_budget))
     _check(rc)
     out = []
     for i in range(k):
         it = items[i]
-        vals, vecs, _ = keep[i]
+        vals, vecs, _, res = keep[i]
         info = Info.from_buffer_copy(it.info)
         if vecs is not None:
             vecs = vecs[: ops[i].nrows] if _is_device_array(vecs) else np.asarray(vecs[: ops[i].nrows])
-        out.append(BatchResult(it.status, vals[:it.nev].copy(), vecs, info, it.t_admit, it.t_done, it.arrival))
+        out.append(BatchResult(it.status, vals[:it.nev].copy(), vecs, info, it.t_admit, it.t_done, it.arrival,
+                               res[:it.nev].copy()))
     return out
 
 

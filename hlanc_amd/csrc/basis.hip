@@ -811,6 +811,46 @@ __global__ __launch_bounds__(kBlock) void basis_combine_kernel(int n, int K, con
     }
 }
 
+// Residual check after the Ritz-vector step: block b of pair T[i] sums, over
+// its rows, d = y - lam v into ||d||^2, v^T d and ||v||^2 (FP64; y = A v from
+// the FP64 SpMV) and writes the three partials.  v^T d is formed directly, so
+// the Rayleigh quotient lam + v^T d / v^T v and the residual at it,
+// ||d||^2 - (v^T d)^2 / v^T v, carry no cancellation of the large terms.
+__global__ __launch_bounds__(kBlock) void resid_seg_kernel(const SegRes *__restrict__ T, int nt)
+{
+    __shared__ double red[3][kBlock / kWave];
+    const int i = seg_find(&T[0].blk0, sizeof(SegRes) / sizeof(int), nt, blockIdx.x);
+    const SegRes t = T[i];
+    const int b = blockIdx.x - t.blk0;
+    const auto v = gptr(t.v);
+    const auto y = gptr(t.y);
+    double a = 0.0, p = 0.0, c = 0.0;
+    for (long long r = (long long)b * kBlock + threadIdx.x; r < t.n; r += (long long)t.G * kBlock) {
+        const double vr = v[r];
+        const double d = y[r] - t.lam * vr;
+        a += d * d;
+        p += vr * d;
+        c += vr * vr;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        p += __shfl_xor(p, off);
+        c += __shfl_xor(c, off);
+    }
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        red[0][w] = a;
+        red[1][w] = p;
+        red[2][w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double s = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) s += red[threadIdx.x][k];
+        t.out[3 * b + threadIdx.x] = s;
+    }
+}
+
 // One block per listed solve: rnorm = sqrt(sum of its V.Q block partials).
 __global__ __launch_bounds__(kBlock) void vq_seg_finalize_kernel(const SegVQ *__restrict__ T)
 {
@@ -1310,6 +1350,16 @@ int launch_basis_combine(int n, int K, const void *V, long long ld, bool v32, co
         basis_combine_kernel<float><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const float *>(V), ld, q, out);
     else
         basis_combine_kernel<double><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const double *>(V), ld, q, out);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int res_blocks(int n) { return grid_for(n, kBlock * 16, kResBlocksMax); }
+
+int launch_resid_seg(const SegRes *T, int nt, int blocks_total, hipStream_t s)
+{
+    if (nt <= 0) return HL_OK;
+    resid_seg_kernel<<<blocks_total, kBlock, 0, s>>>(T, nt);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

@@ -77,6 +77,14 @@ bool solve_args_ok(int n, int nev, int ncv, int ldv, const double *values, bool 
         set_error("%s: basis_prec must be HL_PREC_F64, HL_PREC_F64_PROJ32 or HL_PREC_F32", who);
         return false;
     }
+    if (o.refine != HL_REFINE_NONE && o.refine != HL_REFINE_RQ) {
+        set_error("%s: refine must be HL_REFINE_NONE or HL_REFINE_RQ", who);
+        return false;
+    }
+    if (o.refine == HL_REFINE_RQ && (o.flags & HL_FLAG_NO_VERIFY)) {
+        set_error("%s: HL_REFINE_RQ needs the residual check (HL_FLAG_NO_VERIFY is set)", who);
+        return false;
+    }
     if (o.basis_prec == HL_PREC_F32 && !device_op) {
         set_error("%s: the mixed-precision solve (basis_prec = HL_PREC_F32) needs a device CSR operator", who);
         return false;
@@ -159,8 +167,10 @@ void hl_options_default(hl_options *o)
     o->maxiter = 200;
     o->v0 = nullptr;
     o->seed = 0;
-    o->basis_prec = HL_PREC_F64_PROJ32; // FP64 results, projections on an FP32 copy: measured faster
+    o->basis_prec = HL_PREC_F64; // the reference's DGKS rule; HL_PREC_F64_PROJ32 is the faster opt-in
     o->use_mfma = 1; // measured faster than the FMA kernel on every benchmarked workload
+    o->refine = HL_REFINE_NONE;
+    o->residuals = nullptr;
 }
 
 int hl_csr_create(int dev, int nrows, int ncols, int nnz, const int *row_ptr, const int *col,

@@ -60,7 +60,8 @@ int Member::reset(hipStream_t s)
     Workspace &W = ws;
     tol = opts.tol > 0.0 ? opts.tol : kEps;
     if (f32_phase()) tol = std::max(tol, std::pow(10.0, -tuning().f32_phase_tol_exp));
-    maxiter = opts.maxiter > 0 ? opts.maxiter : 200;
+    // the restart cap covers both phases of a two-phase solve
+    maxiter = std::max(1, (opts.maxiter > 0 ? opts.maxiter : 200) - restarts_f32);
     sub.assign(ncv, 0.0);
     dia.assign(ncv, 0.0);
     ritz.assign(ncv, 0.0);
@@ -242,6 +243,12 @@ void Member::decide(double *qslot)
         nev = 2;
     np = ncv - nev;
     if (nevbef < nev) sort_wanted(nev, np, ritz.data(), bounds.data());
+    if (f32_phase() && iter + 1 >= maxiter) { // the FP32 phase's last restart: hand over what
+        ++iter;                                 // T and V hold now (before a chase rotates T)
+        restarts = iter;
+        rephase_host(qslot);
+        return;
+    }
     restart_chase(nev, np, ritz.data(), sub.data(), dia.data(), q.data());
     // device half of dsapps (HLanc.hpp:1591-1614): V := V Q(:, 0:nev) and
     // resid := Q(ncv-1, nev-1) resid + beta V Q(:, nev)
@@ -254,10 +261,7 @@ void Member::decide(double *qslot)
     ++iter;
     restarts = iter;
     if (iter >= maxiter) { // the restart would feed no further sweep: the solve ends unconverged
-        if (f32_phase()) // (the FP32 phase hands over what it has instead)
-            rephase_host(qslot);
-        else
-            finish_host(qslot, true);
+        finish_host(qslot, true);
         return;
     }
     sweep_k = nev;
@@ -271,7 +275,7 @@ void Member::decide(double *qslot)
 void Member::finish_host(double *qslot, bool vectors_ok)
 {
     act = ACT_FINISH;
-    with_vectors = false;
+    with_vectors = need_ritz = verify = false;
     ret = HL_NOT_CONVERGED;
     if (!vectors_ok || nconv < nev0) return; // HLanc.hpp:1762
     const double eps23 = std::pow(kEps, 2.0 / 3.0);
@@ -295,6 +299,8 @@ void Member::finish_host(double *qslot, bool vectors_ok)
     std::vector<double> ihd(dia), ihb(ncv, 0.0), iq((size_t)ncv * ncv), tau(ncv);
     for (int i = 0; i + 1 < ncv; ++i) ihb[i] = sub[i + 1];
     if (tridiag_eig(ncv, ihd.data(), ihb.data(), iq.data(), ncv)) return; // HLanc.hpp:1805-1806
+    anorm = 0.0; // ||A||_est for the residual check: the extreme Ritz values approach ||A||_2 from below
+    for (int i = 0; i < ncv; ++i) anorm = std::max(anorm, std::fabs(ihd[i]));
     if (reord) { // selected columns to the front (HLanc.hpp:1808-1831)
         int left = 0, right = ncv - 1;
         for (;;) {
@@ -321,6 +327,8 @@ void Member::finish_host(double *qslot, bool vectors_ok)
     // dgeqr2 + dorm2r("R","N") (HLanc.hpp:1837-1840) as one V Q_full GEMM
     householder_q(ncv, nconv, qs.data(), ncv, tau.data(), qslot, ncv);
     with_vectors = vectors != nullptr;
+    verify = verifies() && anorm > 0.0;
+    need_ritz = with_vectors || verify;
     ret = HL_OK;
 }
 
@@ -344,8 +352,48 @@ void Member::rephase_host(double *qslot)
     }
 }
 
+// The residual check's sums (block partials added in block order) for pair i:
+// ||A v - theta v||^2, v^T (A v - theta v) and ||v||^2.
+void Member::check_sums(int i, double &a, double &p, double &c) const
+{
+    const int G = res_blocks(n);
+    const double *o = ws.vres_h + (size_t)3 * G * i;
+    a = p = c = 0.0;
+    for (int b = 0; b < G; ++b) {
+        a += o[3 * b];
+        p += o[3 * b + 1];
+        c += o[3 * b + 2];
+    }
+}
+
 void Member::complete()
 {
+    info.max_rel_resid = -1.0;
+    info.anorm_est = 0.0;
+    info.rq_shift = 0.0;
+    info.verified = 0;
+    if (ret == HL_OK && verify) {
+        // true residuals of the returned pairs; with HL_REFINE_RQ the value moves
+        // to the Rayleigh quotient theta + v^T d / v^T v (d = A v - theta v), where
+        // ||A v - rq v||^2 = ||d||^2 - (v^T d)^2 / v^T v
+        const bool rq = opts.refine == HL_REFINE_RQ;
+        double worst = 0.0, shift = 0.0;
+        for (int i = 0; i < nconv; ++i) {
+            double a, p, c;
+            check_sums(i, a, p, c);
+            const double delta = c > 0.0 ? p / c : 0.0;
+            const double r2 = rq ? std::max(a - delta * p, 0.0) : a;
+            const double rel = std::sqrt(r2) / (anorm * std::sqrt(c > 0.0 ? c : 1.0));
+            shift = std::max(shift, std::fabs(delta) / std::max(std::fabs(vals[i]), 1e-300));
+            if (rq) vals[i] += delta;
+            worst = std::max(worst, rel);
+            if (opts.residuals) opts.residuals[i] = rel;
+        }
+        info.max_rel_resid = worst;
+        info.anorm_est = anorm;
+        info.rq_shift = shift;
+        info.verified = nconv;
+    }
     if (ret == HL_OK) std::copy(vals.begin(), vals.end(), values);
     info.ret = ret;
     info.restarts = restarts_f32 + restarts;
@@ -354,7 +402,6 @@ void Member::complete()
     info.cgs_passes = cgs;
     info.rnorm = rnorm;
     info.seconds = now_s() - t0;
-    info.max_rel_resid = -1.0;
     info.refine_shift = 0.0;
     if (ret == HL_OK && mixed) // how far the FP32 phase's values were from the result
         for (size_t i = 0; i < vals_f32.size() && i < vals.size(); ++i)
@@ -438,6 +485,10 @@ int Engine::init(int dev_)
 {
     dev = dev_;
     HL_TRY_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    Ctl one{};
+    one.xscale = one.rscale = 1.0;
+    HL_TRY_HIP(hipMalloc(&dOne, sizeof(Ctl)));
+    HL_TRY_HIP(hipMemcpy(dOne, &one, sizeof(Ctl), hipMemcpyHostToDevice));
     const unsigned hw = std::thread::hardware_concurrency();
     pool = new Pool(static_cast<int>(std::min(15u, hw > 1 ? hw - 1 : 0u)));
     return HL_OK;
@@ -457,12 +508,17 @@ void Engine::release()
     pend.clear();
     bury();
     drop_graphs();
-    void *d[] = {dM, dS, dV, dRB, dQ};
+    void *d[] = {dM, dS, dV, dRB, dQ, dMv, dSv, dRv, dOne};
     for (void *p : d)
         if (p) (void)hipFree(p);
-    void *h[] = {hMs[0], hMs[1], hSs[0], hSs[1], hVs[0], hVs[1], hRB, hQs[0], hQs[1]};
+    void *h[] = {hMs[0], hMs[1], hSs[0], hSs[1], hVs[0], hVs[1], hRB, hQs[0], hQs[1], hMv, hSv, hRv};
     for (void *p : h)
         if (p) (void)hipHostFree(p);
+    dMv = hMv = nullptr;
+    dSv = hSv = nullptr;
+    dRv = hRv = nullptr;
+    dOne = nullptr;
+    capMv = capSv = capRv = 0;
     dM = hM = hMs[0] = hMs[1] = nullptr;
     dS = hS = hSs[0] = hSs[1] = nullptr;
     dV = hV = hVs[0] = hVs[1] = nullptr;
@@ -547,6 +603,31 @@ int Engine::poll_finished(bool wait)
     return HL_OK;
 }
 
+namespace {
+// The operator side of a member's SpMV entry: its current form (items over the
+// rows or the virtual rows, sliced-ELL chunks) and its long-row slots.
+void spmv_fields(SegMember &M, const Member &m)
+{
+    const Workspace &W = m.ws;
+    const bool sl = m.sliced();
+    M.blks = !m.A ? nullptr : sl ? m.A->sitems.blks : m.A->items.blks;
+    M.rp = !m.A ? nullptr : sl ? m.A->s_rp : m.A->row_ptr;
+    M.col = !m.A ? nullptr : sl ? m.A->s_col : m.A->col;
+    M.val = !m.A ? nullptr : sl ? m.A->s_val : m.A->val;
+    M.val32 = m.A ? m.A->val32 : nullptr;
+    M.part = sl ? W.part : nullptr;
+    M.r_pp = sl ? m.A->r_pp : nullptr;
+    M.r_pairs = sl ? m.A->r_pairs : nullptr;
+    M.chunk_part = W.chunk_part;
+    M.chunk_cnt = W.chunk_cnt;
+    const bool el = m.sell();
+    M.e_ptr = el ? m.A->e_ptr : nullptr;
+    M.e_col = el ? m.A->e_col : nullptr;
+    M.e_val = el ? m.A->e_val : nullptr;
+    M.nchunks = el ? m.A->nchunks : 0;
+}
+} // namespace
+
 int Engine::sweep(std::vector<Member *> &ms)
 {
     int rc;
@@ -572,22 +653,7 @@ int Engine::sweep(std::vector<Member *> &ms)
         Member &m = *act[i];
         Workspace &W = m.ws;
         SegMember &M = hM[i];
-        const bool sl = m.sliced();
-        M.blks = !m.A ? nullptr : sl ? m.A->sitems.blks : m.A->items.blks;
-        M.rp = !m.A ? nullptr : sl ? m.A->s_rp : m.A->row_ptr;
-        M.col = !m.A ? nullptr : sl ? m.A->s_col : m.A->col;
-        M.val = !m.A ? nullptr : sl ? m.A->s_val : m.A->val;
-        M.val32 = m.A ? m.A->val32 : nullptr;
-        M.part = sl ? W.part : nullptr;
-        M.r_pp = sl ? m.A->r_pp : nullptr;
-        M.r_pairs = sl ? m.A->r_pairs : nullptr;
-        M.chunk_part = W.chunk_part;
-        M.chunk_cnt = W.chunk_cnt;
-        const bool el = m.sell();
-        M.e_ptr = el ? m.A->e_ptr : nullptr;
-        M.e_col = el ? m.A->e_col : nullptr;
-        M.e_val = el ? m.A->e_val : nullptr;
-        M.nchunks = el ? m.A->nchunks : 0;
+        spmv_fields(M, m);
         M.V = W.V[W.cur];
         M.V32 = m.kind == SEG_PROJ32 ? W.V32[W.cur] : nullptr;
         M.Vproj = M.V32 ? static_cast<const void *>(M.V32) : M.V;
@@ -904,10 +970,11 @@ int Engine::after_sweep(std::vector<Member *> &act)
     }
     if (nfinish == 0) return HL_OK;
     Pending fin{nullptr, {}};
+    std::vector<Member *> ver;
     for (size_t i = 0; i < nm; ++i) {
         Member &m = *act[i];
         if (m.act != Member::ACT_FINISH) continue;
-        if (!m.with_vectors) { // nothing on the device: the outputs are ready
+        if (!m.need_ritz) { // nothing on the device: the outputs are ready
             m.complete();
             continue;
         }
@@ -922,16 +989,121 @@ int Engine::after_sweep(std::vector<Member *> &act)
         if ((rc = prof.end(stream, K_VQ, 16.0 * m.n * m.ncv))) return rc;
         // sync_host (HLanc.hpp:1063-1071): all ncv columns to the caller's
         // buffer, host (pageable or page-locked) or device memory
-        HL_TRY_HIP(hipMemcpy2DAsync(m.vectors, sizeof(double) * (size_t)m.ldv_host, W.V[nxt],
-                                    sizeof(double) * (size_t)W.ldv, sizeof(double) * (size_t)m.n, m.ncv,
-                                    hipMemcpyDefault, stream));
+        if (m.with_vectors)
+            HL_TRY_HIP(hipMemcpy2DAsync(m.vectors, sizeof(double) * (size_t)m.ldv_host, W.V[nxt],
+                                        sizeof(double) * (size_t)W.ldv, sizeof(double) * (size_t)m.n, m.ncv,
+                                        hipMemcpyDefault, stream));
+        if (m.verify) ver.push_back(&m);
     }
+    if (!ver.empty() && (rc = verify_pairs(ver))) return rc;
     if (fin.ms.empty()) return HL_OK;
     // completed when this event has passed: the next sweep is enqueued meanwhile
     HL_TRY_HIP(hipEventCreateWithFlags(&fin.ev, hipEventDisableTiming));
     HL_TRY_HIP(hipEventRecord(fin.ev, stream));
     pend.push_back(std::move(fin));
     return HL_OK;
+}
+
+// The residual check of the finishing members' returned pairs, after their
+// Ritz-vector GEMM (vectors in column-major V[1 - cur], ld ldv): y_i = A v_i into
+// column i of the old basis V[cur] (no longer read), then one launch of the
+// residual sums.  Device operators: one segmented SpMV launch per pair index
+// over every finishing member (each member's own partial and long-row slots,
+// so its pairs run one launch after another); host operators: the callback on
+// host copies.  The partial sums come home with the vectors.
+int Engine::verify_pairs(std::vector<Member *> &ver)
+{
+    int rc;
+    size_t npairs = 0;
+    int maxnev = 0;
+    for (Member *m : ver) {
+        npairs += m->nev0;
+        maxnev = std::max(maxnev, m->nev0);
+    }
+    if (npairs > capMv) {
+        if ((rc = grow_set(dMv, &hMv, 1, capMv, npairs))) return rc;
+        if ((rc = grow_set(dSv, &hSv, 1, capSv, npairs))) return rc;
+        if ((rc = grow_set(dRv, &hRv, 1, capRv, npairs))) return rc;
+    }
+    if ((rc = prof.begin(stream))) return rc;
+    // host operators: y_i = A v_i through the pinned staging
+    for (Member *m : ver) {
+        if (m->A) continue;
+        Workspace &W = m->ws;
+        const double *Vr = W.V[1 - W.cur];
+        double *Y = W.V[W.cur];
+        for (int i = 0; i < m->nev0; ++i) {
+            HL_TRY_HIP(hipMemcpyAsync(W.xh, Vr + (size_t)i * W.ldv, sizeof(double) * m->n, hipMemcpyDeviceToHost,
+                                      stream));
+            HL_TRY_HIP(hipStreamSynchronize(stream));
+            m->host_op(m->host_ctx, W.xh, W.yh);
+            HL_TRY_HIP(hipMemcpyAsync(Y + (size_t)i * W.ldv, W.yh, sizeof(double) * m->n, hipMemcpyHostToDevice,
+                                      stream));
+        }
+        HL_TRY_HIP(hipStreamSynchronize(stream)); // yh is reused by the next member
+    }
+    // device operators: entries ordered by pair index, so launch i is one run
+    std::vector<int> off(maxnev + 1, 0), nblk(maxnev, 0), nred(maxnev, 0);
+    size_t ne = 0;
+    for (int i = 0; i < maxnev; ++i) {
+        off[i] = static_cast<int>(ne);
+        for (Member *m : ver) {
+            if (!m->A || i >= m->nev0) continue;
+            Workspace &W = m->ws;
+            SegMember &M = hMv[ne];
+            std::memset(&M, 0, sizeof(M));
+            spmv_fields(M, *m);
+            M.r = W.V[1 - W.cur] + (size_t)i * W.ldv;
+            M.w = W.V[W.cur] + (size_t)i * W.ldv;
+            M.ctl = dOne;
+            M.n = m->n;
+            M.ncv = m->ncv;
+            SegStep &S = hSv[ne];
+            std::memset(&S, 0, sizeof(S));
+            S.mem = static_cast<int>(ne);
+            S.spmv0 = nblk[i];
+            S.red0 = nred[i];
+            nblk[i] += m->spmv_blocks();
+            if (m->sliced()) nred[i] += (m->n + kRowsPerSumBlock - 1) / kRowsPerSumBlock;
+            ++ne;
+        }
+    }
+    off[maxnev] = static_cast<int>(ne);
+    if (ne > 0) {
+        HL_TRY_HIP(hipMemcpyAsync(dMv, hMv, sizeof(SegMember) * ne, hipMemcpyHostToDevice, stream));
+        HL_TRY_HIP(hipMemcpyAsync(dSv, hSv, sizeof(SegStep) * ne, hipMemcpyHostToDevice, stream));
+        for (int i = 0; i < maxnev; ++i) {
+            const int cnt = off[i + 1] - off[i];
+            if (cnt <= 0) continue;
+            if (nblk[i] > 0 && (rc = launch_spmv_seg(dMv, dSv + off[i], cnt, nblk[i], SEG_F64, stream))) return rc;
+            if (nred[i] > 0 && (rc = launch_spmv_rows_seg(dMv, dSv + off[i], cnt, nred[i], stream))) return rc;
+        }
+    }
+    // the residual sums of every pair in one launch
+    int blocks = 0;
+    size_t nr = 0;
+    for (Member *m : ver) {
+        Workspace &W = m->ws;
+        const int G = res_blocks(m->n);
+        for (int i = 0; i < m->nev0; ++i) {
+            SegRes &t = hRv[nr++];
+            t.v = W.V[1 - W.cur] + (size_t)i * W.ldv;
+            t.y = W.V[W.cur] + (size_t)i * W.ldv;
+            t.out = W.red.partials + (size_t)3 * G * i;
+            t.lam = m->vals[i];
+            t.n = m->n;
+            t.G = G;
+            t.blk0 = blocks;
+            blocks += G;
+        }
+    }
+    HL_TRY_HIP(hipMemcpyAsync(dRv, hRv, sizeof(SegRes) * nr, hipMemcpyHostToDevice, stream));
+    if ((rc = launch_resid_seg(dRv, static_cast<int>(nr), blocks, stream))) return rc;
+    for (Member *m : ver)
+        HL_TRY_HIP(hipMemcpyAsync(m->ws.vres_h, m->ws.red.partials,
+                                  sizeof(double) * 3 * (size_t)res_blocks(m->n) * m->nev0, hipMemcpyDeviceToHost,
+                                  stream));
+    return prof.end(stream, K_MISC, 0.0);
 }
 
 int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
@@ -980,7 +1152,7 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
             m->t_admit = t;
             m->phase = Member::NEW;
             if ((m->err = m->ws.init(dev, m->n, m->ncv, m->A == nullptr, m->nslots(), m->npairs(), m->copy32(),
-                                     stream))) {
+                                     m->verifies() ? m->nev0 : 0, stream))) {
                 m->ws.release();
                 m->phase = Member::DONE;
                 m->t_done = now_s() - t_start;

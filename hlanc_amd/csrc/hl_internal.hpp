@@ -295,6 +295,20 @@ bool vq_seg_mfma_fits(int K, int N);
 // finalize over all entries.
 int launch_vq_seg(const SegVQ *T, int nt, int blocks_total, int max_K, int max_N, int nt_mfma, int blocks_mfma,
                   int mfma_nct, size_t mfma_lds, bool v32, hipStream_t s);
+// Residual check of one returned Ritz pair within a segmented launch: G blocks
+// write their partial sums of ||y - lam v||^2, v^T (y - lam v) and ||v||^2 to
+// out[3 b .. 3 b + 2] (y = A v, both contiguous vectors of n rows); the host
+// adds them in block order.
+struct SegRes {
+    const double *v, *y;
+    double *out;
+    double lam;
+    int n, G, blk0;
+};
+constexpr int kResBlocksMax = 64; // blocks per checked pair
+int res_blocks(int n);
+int launch_resid_seg(const SegRes *T, int nt, int blocks_total, hipStream_t s);
+
 // out = V(:, 0:K) q (tiled basis, ld K, FP64 or FP32; q on the device).
 int launch_basis_combine(int n, int K, const void *V, long long ld, bool v32, const double *q, double *out,
                          hipStream_t s);

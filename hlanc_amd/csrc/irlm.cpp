@@ -220,7 +220,7 @@ void Workspace::release()
     chunk_cnt = nullptr;
     ctl = nullptr;
     ctl_h = nullptr;
-    xh = yh = nullptr;
+    xh = yh = vres_h = nullptr;
     red = Reduce{};
 }
 
@@ -271,7 +271,8 @@ double Workspace::bytes_needed(int n, int ncv, bool host_op, int nslots, int npa
     return static_cast<double>(WsLayout(n, ncv, host_op, nslots, npairs, copy32).total);
 }
 
-int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, int npairs, bool copy32, hipStream_t s)
+int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, int npairs, bool copy32, int nres,
+                    hipStream_t s)
 {
     dev = dev_;
     n = n_;
@@ -308,11 +309,14 @@ int Workspace::init(int dev_, int n_, int ncv_, bool host_op, int nslots, int np
     }
     if (chunk_cnt) HL_TRY_HIP(hipMemsetAsync(chunk_cnt, 0, sizeof(unsigned) * (size_t)nslots, s));
     HL_TRY_HIP(hipMemsetAsync(red.counter, 0, 16, s));
-    const size_t hbytes = sizeof(Ctl) + (host_op ? 16 * (size_t)n : 0);
+    const size_t hres = 3 * (size_t)kResBlocksMax * (size_t)(nres > 0 ? nres : 0);
+    const size_t hbytes = sizeof(Ctl) + 8 * hres + (host_op ? 16 * (size_t)n : 0);
     HL_TRY_HIP(hipHostMalloc(&hblob, hbytes, hipHostMallocDefault));
     ctl_h = static_cast<Ctl *>(hblob);
+    double *hd = reinterpret_cast<double *>(static_cast<char *>(hblob) + sizeof(Ctl));
+    vres_h = nres > 0 ? hd : nullptr;
     if (host_op) {
-        xh = reinterpret_cast<double *>(static_cast<char *>(hblob) + sizeof(Ctl));
+        xh = hd + hres;
         yh = xh + n;
     }
     return HL_OK;

@@ -27,14 +27,15 @@ struct Workspace {
     Ctl *ctl = nullptr;
     Ctl *ctl_h = nullptr;               // pinned
     double *xh = nullptr, *yh = nullptr; // pinned, host operators only
+    double *vres_h = nullptr;           // pinned: residual-check partials (3 kResBlocksMax per pair)
     Reduce red;                         // block partial sums (carved, not owned)
     void *blob = nullptr, *hblob = nullptr;
 
     // The ping-pong bases are tiled: 16-row blocks of ncv columns.
     VLayout vl() const { return VLayout{tiled_ld(ncv), true}; }
     ~Workspace();
-    int init(int dev, int n, int ncv, bool host_op, int nslots, int npairs, bool copy32,
-             hipStream_t s); // zeroing on s
+    int init(int dev, int n, int ncv, bool host_op, int nslots, int npairs, bool copy32, int nres,
+             hipStream_t s); // zeroing on s; nres: pairs the residual check covers
     // Hands the two allocations to the caller (to free once the device is idle)
     // and clears the workspace.
     std::pair<void *, void *> detach();
@@ -96,6 +97,9 @@ struct Member {
     int vq_N = 0, vq_res = -1;           // restart: Q columns and the beta column
     double vq_qlast = 0.0, vq_beta = 0.0;
     bool with_vectors = false;           // finish: Ritz vectors to the caller
+    bool need_ritz = false;              // finish: the Ritz-vector GEMM runs (vectors or the check)
+    bool verify = false;                 // finish: the residual check runs on the returned pairs
+    double anorm = 0.0;                  // largest |eigenvalue| of the final tridiagonal
     std::vector<double> sub, dia, ritz, bounds, zlast, q, e, work, ritz_saved, bounds_saved, vals;
 
     int start(hipStream_t s);            // start vector, norm, first sweep [0, nev0)
@@ -106,6 +110,7 @@ struct Member {
     // the operator's column-sliced form serves the SpMV (FP64 values and basis)
     bool sliced() const { return A != nullptr && A->slice_w > 0 && (kind == SEG_F64 || kind == SEG_PROJ32); }
     bool sell() const { return A != nullptr && A->sell_on && kind != SEG_V32_A32; } // FP64 values only
+    bool verifies() const { return (opts.flags & HL_FLAG_NO_VERIFY) == 0; } // residual check requested
     int spmv_blocks() const; // blocks of this solve's SpMV launch segment
     int nslots() const { return A ? std::max(A->items.nslots, A->sitems.nslots) : 0; }
     int npairs() const { return A ? A->npairs : 0; }
@@ -116,6 +121,7 @@ struct Member {
     int host_apply(int j, hipStream_t s); // SpMV of a host operator for step j
     double cgs_bytes_pass(int m, bool sub) const; // algorithmic bytes of one CGS pass (sub: EPI_PASS1)
     void complete();                     // outputs and info once the engine synchronised
+    void check_sums(int i, double &a, double &p, double &c) const; // residual check of pair i
 
   private:
     int reset(hipStream_t s);            // host state, Ctl and T for a new phase
@@ -143,6 +149,14 @@ struct Engine {
     // read-back (device + pinned) and Q arenas (device + two pinned)
     double *dRB = nullptr, *hRB = nullptr, *dQ = nullptr, *hQ = nullptr, *hQs[2] = {nullptr, nullptr};
     size_t capRB = 0, capQ = 0;
+    // residual check of finishing members: one SpMV entry and one SegRes per
+    // returned pair (device tables and their pinned staging), and a Ctl whose
+    // xscale is 1 (the SpMV kernels scale x by ctl->xscale)
+    SegMember *dMv = nullptr, *hMv = nullptr;
+    SegStep *dSv = nullptr, *hSv = nullptr;
+    SegRes *dRv = nullptr, *hRv = nullptr;
+    size_t capMv = 0, capSv = 0, capRv = 0;
+    Ctl *dOne = nullptr;
     int par = 0;
     // finishing members whose Ritz vectors are in flight, completed once
     // their event has passed; workspaces freed at the next idle point
@@ -188,6 +202,7 @@ struct Engine {
     int after_sweep(std::vector<Member *> &act);
     void drop_graphs();
     int poll_finished(bool wait);         // complete members whose vectors have arrived
+    int verify_pairs(std::vector<Member *> &ver); // residual check, enqueued after the Ritz vectors
     void bury();                          // free detached workspaces (device idle)
 };
 

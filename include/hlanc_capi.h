@@ -203,18 +203,24 @@ typedef void (*hl_host_op_fn)(void *ctx, const double *x, double *y);
 
 typedef struct {
     double tol;        /* convergence tolerance; <= 0 -> machine epsilon (HLanc.hpp:1651) */
-    int maxiter;       /* restart cap; <= 0 -> 200 (HLanc.hpp:1682) */
+    int maxiter;       /* restart cap; <= 0 -> 200 (HLanc.hpp:1682); a two-phase
+                          (HL_PREC_F32) solve counts the restarts of both phases */
     const double *v0;  /* host start vector of length n, or NULL: seeded generator */
     uint64_t seed;     /* generator seed when v0 == NULL */
-    int basis_prec;    /* HL_PREC_F64 (the reference's DGKS semantics exactly);
-                          HL_PREC_F64_PROJ32 (hl_options_default): the FP64 basis plus an
-                          FP32 copy -- each Lanczos step takes the three-term first
-                          projection (alpha_j from column j of the copy, beta_j known),
-                          subtracts with the FP64 basis while projecting exactly, and
-                          runs the DGKS correction on the FP32 copy unless those exact
-                          coefficients are negligible (||h|| <= 1e-13 ||r||), so the basis
+    int basis_prec;    /* HL_PREC_F64 (hl_options_default): the FP64 basis with the
+                          reference's DGKS rule exactly as written (HLanc.hpp:1442-1465:
+                          correct when ||r|| < 0.717 ||w||, at most twice);
+                          HL_PREC_F64_PROJ32 (opt-in, the fastest mode measured): the FP64
+                          basis plus an FP32 copy.  It departs from the reference's
+                          Gram-Schmidt in four ways -- the first projection is the Lanczos
+                          three-term one (alpha_j from FP64 column j, beta_j known); the
+                          subtracting pass reads the FP64 basis and projects exactly; the
+                          first DGKS correction is pending after every step and reads the
+                          FP32 copy (the FP64 basis when the step cancelled strongly,
+                          ||r|| < 1e-2 ||w||); and it is skipped when those exact
+                          coefficients are negligible (||h|| <= 1e-13 ||r||).  The basis
                           stays orthogonal to FP64 rounding while reading at most 12
-                          instead of 24 bytes per entry (host operators run plain FP64);
+                          instead of 24 bytes per entry; host operators run plain FP64;
                           HL_PREC_F32: two phases for device CSR operators -- the basis in
                           FP32 (and the operator's FP32 values when it was created with
                           them) to a relative Ritz estimate of 1e-6, then an FP64 restart
@@ -223,20 +229,37 @@ typedef struct {
     int use_mfma;      /* 1 (default): restart and Ritz-vector V.Q on v_mfma_f64_16x16x4f64
                           (a restart whose K x N block of Q exceeds 48 KB runs on the FMA
                           kernel); 0: FMA kernel.  The results differ in rounding only. */
-    int flags;         /* bit 0: do not use hipGraph replay for the Lanczos steps */
-    int reserved[5];
+    int flags;         /* HL_FLAG_* bits */
+    int refine;        /* HL_REFINE_NONE (default): the returned values are the Ritz
+                          values of T, as the reference returns them (HLanc.hpp:1833);
+                          HL_REFINE_RQ: each returned value is replaced by the Rayleigh
+                          quotient v^T A v / v^T v of its returned FP64 vector, computed
+                          with the FP64 SpMV of the residual check (the order of the
+                          values is kept).  Needs the residual check (no HL_FLAG_NO_VERIFY). */
+    double *residuals; /* host, >= nev, or NULL: on success receives each returned pair's
+                          ||A v_i - l_i v_i||_2 / (||A||_est ||v_i||_2) (see hl_info) */
+    int reserved[4];
 } hl_options;
+
+/* hl_options.refine values. */
+#define HL_REFINE_NONE 0
+#define HL_REFINE_RQ 1
 
 typedef struct {
     int ret;                /* status as returned */
     int restarts;           /* restart iterations run */
     int nconv;              /* converged Ritz values */
     int graph_sweeps;       /* dsaitr sweeps replayed from a captured HIP graph */
-    long long spmv;         /* operator applications */
+    long long spmv;         /* operator applications (the residual check's excluded) */
     long long cgs_passes;   /* CGS passes incl. DGKS corrections */
     double rnorm;           /* final residual norm */
     double seconds;         /* wall time of the call */
-    double max_rel_resid;   /* max ||A v - l v|| / ||A||_est over returned pairs (csr path) */
+    double max_rel_resid;   /* residual check: max over the returned pairs of
+                               ||A v - l v||_2 / (||A||_est ||v||_2), with A v formed by the
+                               FP64 SpMV (or the host operator) after the Ritz-vector step and
+                               l the returned value; -1 when no check ran (failure, or
+                               HL_FLAG_NO_VERIFY).  The convergence test itself is a Ritz
+                               estimate (HLanc.hpp:1376-1384); this is the true residual. */
     double refine_shift;    /* mixed precision: largest relative change of a returned value
                                from the FP32-basis phase to the FP64 result */
     long long spmv_f32;     /* mixed precision: operator applications of the FP32-basis phase
@@ -245,13 +268,23 @@ typedef struct {
                                restarts) */
     int proj64_steps;       /* FP32-copy projections: Lanczos steps whose DGKS correction read
                                the FP64 basis (strong cancellation) */
+    double anorm_est;       /* ||A||_est: the largest |eigenvalue| of the final ncv x ncv
+                               tridiagonal, a lower bound of ||A||_2 (so max_rel_resid errs
+                               high); 0 when no check ran */
+    double rq_shift;        /* residual check: max over the returned pairs of
+                               |v^T A v / v^T v - theta| / |theta|, the Rayleigh quotient's
+                               distance from the Ritz value (applied with HL_REFINE_RQ) */
+    int verified;           /* pairs checked (nev on success, 0 otherwise) */
+    int pad;
 } hl_info;
 
 /* hl_options.flags bits. */
-#define HL_FLAG_NO_GRAPH 1 /* launch every sweep's kernels one by one; by default a sweep
-                              shape met a second time is captured as a HIP graph and
-                              replayed (same kernels, same results) */
-#define HL_FLAG_PROFILE 2  /* time every SpMV, CGS and V.Q launch with HIP events */
+#define HL_FLAG_NO_GRAPH 1  /* launch every sweep's kernels one by one; by default a sweep
+                               shape met a second time is captured as a HIP graph and
+                               replayed (same kernels, same results) */
+#define HL_FLAG_PROFILE 2   /* time every SpMV, CGS and V.Q launch with HIP events */
+#define HL_FLAG_NO_VERIFY 4 /* skip the residual check: no SpMVs after the Ritz vectors;
+                               values-only batch items then skip the Ritz-vector GEMM too */
 
 /* Per-kernel-class totals of the last profiled solve (or batch) on the calling
  * thread: index 0 SpMV, 1 the two CGS passes of every Lanczos step (partials
@@ -270,7 +303,8 @@ typedef struct {
 int hl_profile_last(hl_profile *out);
 
 /* Fills *o with the reference defaults (tol = eps, maxiter = 200, generator
- * seed 0, FP64 basis, MFMA V.Q). */
+ * seed 0, the FP64 basis with the reference's DGKS rule, values not refined)
+ * plus this library's: MFMA V.Q, graphs on, the residual check on. */
 void hl_options_default(hl_options *o);
 
 /* ds_solver_gpu<dcsrmv_operator_gpu> (HLanc.hpp:1868-1872 driving 1649-1850):
