@@ -25,6 +25,7 @@
 //     r = qlast r + beta Out(:, k) and ||r|| partials fused; vq_mfma_kernel and
 //     vq_fma_kernel form the Ritz vectors and serve hl_vq.
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "hl_device.hpp"
@@ -66,7 +67,7 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
     __shared__ double hs[kGroups * MAXC];
     __shared__ double red[kBlock / kWave][kGroups * MAXC + 1];
     __shared__ double vals[kGroups * MAXC + 1];
-    __shared__ double scratch[kBlock];
+    __shared__ double lds_tmp[kBlock];
     __shared__ double fin[kGroups * MAXC + 1];
 
     const int lane = threadIdx.x & (kWave - 1);
@@ -176,7 +177,7 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
         for (int c = threadIdx.x; c < M1; c += kBlock) partials[(long long)c * nb + bid] = vals[c];
         return;
     }
-    if (!reduce_blocks(vals, M1, partials_, counter, scratch, fin)) return;
+    if (!reduce_blocks(vals, M1, partials_, counter, lds_tmp, fin)) return;
     if (h_out)
         for (int c = threadIdx.x; c < m; c += kBlock) h_out[c] = fin[c];
     if (threadIdx.x == 0) run_epilogue(mode, j, fin[m], h_in, ctl, h_sub, h_dia);
@@ -366,11 +367,10 @@ __global__ __launch_bounds__(kBlock) void cgs_kernel(int n, int m, const VT *__r
 // block partials in block order; block m (the squared norm) runs the epilogue.
 // Columns whose block starts after the epilogue cleared ctl->need are not
 // written, and no later kernel reads them.
-__device__ __forceinline__ void cgs_finalize_block(int c, int nb, int m, const double *__restrict__ partials_,
-                                                   const double *h_in, double *h_out, Ctl *ctl, double *h_sub,
-                                                   double *h_dia, int mode, int j)
+// Column c of the column-major block partials added in block order (a fixed
+// tree over the block's lanes); the sum is valid in thread 0.
+__device__ __forceinline__ double column_sum(int c, int nb, const double *__restrict__ partials_)
 {
-    if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return;
     const auto partials = gptr(partials_);
     __shared__ double sred[kBlock];
     const int tid = threadIdx.x;
@@ -382,11 +382,35 @@ __device__ __forceinline__ void cgs_finalize_block(int c, int nb, int m, const d
         if (tid < w) sred[tid] += sred[tid + w];
         __syncthreads();
     }
-    if (tid != 0) return;
+    return sred[0];
+}
+
+__device__ __forceinline__ void cgs_finalize_block(int c, int nb, int m, const double *__restrict__ partials_,
+                                                   const double *h_in, double *h_out, Ctl *ctl, double *h_sub,
+                                                   double *h_dia, int mode, int j)
+{
+    if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return;
+    const double sum = column_sum(c, nb, partials_);
+    if (threadIdx.x != 0) return;
     if (c < m) {
-        if (h_out) h_out[c] = sred[0];
+        if (h_out) h_out[c] = sum;
     } else {
-        run_epilogue(mode, j, sred[0], h_in, ctl, h_sub, h_dia);
+        run_epilogue(mode, j, sum, h_in, ctl, h_sub, h_dia);
+    }
+}
+
+// Finalize of a standalone split pass writing straight to the caller's
+// outputs: h_out[c] for c < m, *nrm2_out for c == m (either may be null).
+__global__ __launch_bounds__(kBlock) void cgs_finalize_out_kernel(int nb, int m, const double *__restrict__ partials,
+                                                                  double *h_out, double *nrm2_out)
+{
+    const int c = blockIdx.x;
+    const double sum = column_sum(c, nb, partials);
+    if (threadIdx.x != 0) return;
+    if (c < m) {
+        if (h_out) h_out[c] = sum;
+    } else if (nrm2_out) {
+        *nrm2_out = sum;
     }
 }
 
@@ -1149,6 +1173,81 @@ int launch_cgs(int n, int m, const double *V, VLayout L, const double *w_in, dou
     if (L.tiled)
         return launch_cgs_t<double, true>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
     return launch_cgs_t<double, false>(n, m, V, L.ld, w_in, w_out, h_in, h_out, red, ctl, h_sub, h_dia, mode, j, s);
+}
+
+// ------------------------------------------------- allocation-free CGS pass
+// The block partials of hl_cgs_pass_noalloc live in this file-scope device
+// array: every GPU the library runs on holds its own copy until the process
+// ends, and every call on one GPU shares it (calls that overlap on one GPU
+// race on it).  Host code never passes the array's name to a kernel -- on the
+// host it is only a stand-in for the symbol; the device address comes from
+// hipGetSymbolAddress, looked up once per device with that device current.
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on device dev (current on entry).
+int scratch_address(int dev, double **out)
+{
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("no-allocation CGS pass: device %d beyond %d", dev, kMaxDevices);
+        return HL_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_status(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<double *>(g_scratch_ptr[dev]);
+    return HL_OK;
+}
+} // namespace
+
+int launch_cgs_noalloc(int dev, int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
+                       const double *h_in, double *h_out, double *nrm2_out, hipStream_t s)
+{
+    if (m < 0 || m > 4 * 32) {
+        set_error("no-allocation CGS pass: m=%d outside [0, 128]", m);
+        return HL_ERR_ARG;
+    }
+    const int nb = cgs_blocks(n);
+    if ((size_t)nb * (size_t)(m + 1) > kNoAllocScratch) {
+        set_error("no-allocation CGS pass: %d blocks x %d columns exceed the %zu-double scratch", nb, m + 1,
+                  kNoAllocScratch);
+        return HL_ERR_ARG;
+    }
+    double *part = nullptr;
+    int rc = scratch_address(dev, &part);
+    if (rc) return rc;
+    // the split pass of hl_cgs_pass (cgs_kernel with FL = 9: non-temporal loads,
+    // partials only), then its per-column sums into the caller's outputs
+    const int cpg = (m + kGroups - 1) / kGroups;
+#define HL_CGSN(MC)                                                                                          \
+    cgs_kernel<MC, double, false, 9><<<nb, kBlock, 0, s>>>(n, m, V, ldv, w_in, w_out, h_in, nullptr, part,   \
+                                                           nullptr, nullptr, nullptr, nullptr, EPI_NONE, 0)
+    if (cpg <= 2)
+        HL_CGSN(2);
+    else if (cpg <= 4)
+        HL_CGSN(4);
+    else if (cpg <= 8)
+        HL_CGSN(8);
+    else if (cpg <= 16)
+        HL_CGSN(16);
+    else if (cpg <= 24)
+        HL_CGSN(24);
+    else
+        HL_CGSN(32);
+#undef HL_CGSN
+    HL_TRY_HIP(hipGetLastError());
+    if (h_out || nrm2_out) {
+        cgs_finalize_out_kernel<<<m + 1, kBlock, 0, s>>>(nb, m, part, h_out, nrm2_out);
+        HL_TRY_HIP(hipGetLastError());
+    }
+    return HL_OK;
 }
 
 int launch_norm(int n, const double *r, Reduce &red, Ctl *ctl, int mode, hipStream_t s)

@@ -459,6 +459,26 @@ int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const dou
     return HL_OK;
 }
 
+int hl_cgs_pass_noalloc(int dev, int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
+                        const double *h_in, double *h_out, double *nrm2_out, void *stream)
+{
+    if (n < 0 || m < 0 || (m > 0 && (!V || ldv < n)) || !w_in) {
+        set_error("hl_cgs_pass_noalloc: bad arguments");
+        return HL_ERR_ARG;
+    }
+    if (m > 128) {
+        set_error("hl_cgs_pass_noalloc: m=%d exceeds the 128 columns its scratch holds", m);
+        return HL_ERR_ARG;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || dev < 0 || dev >= count) {
+        set_error("hl_cgs_pass_noalloc: device %d not visible", dev);
+        return count <= 0 ? HL_ERR_NO_DEVICE : HL_ERR_ARG;
+    }
+    DeviceGuard g(dev);
+    return launch_cgs_noalloc(dev, n, m, V, ldv, w_in, w_out, h_in, h_out, nrm2_out, static_cast<hipStream_t>(stream));
+}
+
 int hl_bench_cgs(int dev, int n, int m, int tiled, int reps, double *seconds)
 {
     if (n <= 0 || m <= 0 || m > 128 || reps <= 0 || !seconds) return HL_ERR_ARG;

@@ -326,6 +326,15 @@ struct Reduce {
 int reduce_alloc(Reduce &r, int max_blocks, int max_cols);
 int cgs_blocks(int n);
 
+// hl_cgs_pass_noalloc: the split pass of hl_cgs_pass (column-major FP64 V)
+// with its block partials in a file-scope device array of kNoAllocScratch
+// doubles (enough for kMaxReduceBlocks blocks of 129 partial sums); writes
+// h_out / *nrm2_out directly; no allocation of any kind.
+constexpr size_t kNoAllocScratch = (size_t)kMaxReduceBlocks * (128 + 1);
+constexpr int kMaxDevices = 64;
+int launch_cgs_noalloc(int dev, int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
+                       const double *h_in, double *h_out, double *nrm2_out, hipStream_t s);
+
 // One fused CGS pass over an FP64 basis (see hl_cgs_pass).  mode selects the
 // epilogue; j is the Lanczos step for the alpha/beta bookkeeping.
 int launch_cgs(int n, int m, const double *V, VLayout L, const double *w_in, double *w_out,

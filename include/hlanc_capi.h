@@ -160,6 +160,19 @@ int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *s
  * different streams of one thread must not overlap. */
 int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const double *w_in,
                 double *w_out, const double *h_in, double *h_out, double *nrm2_out, void *stream);
+/* hl_cgs_pass without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Same operands, kernels
+ * and results as hl_cgs_pass in its default split form (bitwise), but its
+ * block partial sums live in a fixed device array of the library (4 MiB per
+ * GPU, kept until the process ends) instead of a workspace allocated on the
+ * first call, and h_out / *nrm2_out (either may be NULL) are written by the
+ * finalize kernel directly.  m <= 128 (HL_ERR_ARG otherwise).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other host threads) race and must be serialized by the caller.  Calls no
+ * hipMalloc, hipFree or allocating routine. */
+int hl_cgs_pass_noalloc(int dev, int n, int m, const double *V, long long ldv, const double *w_in,
+                        double *w_out, const double *h_in, double *h_out, double *nrm2_out, void *stream);
 /* Out(:, 0:N) = V(:, 0:K) Q(0:K, 0:N): the V.Q product of dsapps and of the
  * Ritz-vector step (HLanc.hpp:1029-1058, 1075-1107).  Out must not alias V.
  * use_mfma = 1 selects the v_mfma_f64_16x16x4 kernel, 0 the FMA kernel. */
