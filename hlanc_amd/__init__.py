@@ -124,6 +124,12 @@ _SIGS = [
                                        ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     ("hl_bench_stream", ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_double)]),
+    ("hl_host_tridiag_eig", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
+    ("hl_host_tridiag_eig_last", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp]),
+    ("hl_host_dlartg", None, [ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
+    ("hl_host_householder_q", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
+                                             ctypes.c_int]),
+    ("hl_host_restart_chase", ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
     ("hl_options_default", None, [ctypes.POINTER(Options)]),
     ("hl_profile_last", ctypes.c_int, [ctypes.POINTER(Profile)]),
     ("hl_ds_solve_csr", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
@@ -554,6 +560,53 @@ def cgs_pass(dev, n, m, V_ptr, ldv, w_in_ptr, w_out_ptr, h_in_ptr, h_out_ptr, nr
 def vq(dev, n, K, N, V_ptr, ldv, Q_ptr, ldq, Out_ptr, ldo, use_mfma=True, stream=0):
     """Out = V Q on device pointers (hl_vq)."""
     _check(lib().hl_vq(dev, n, K, N, V_ptr, ldv, Q_ptr, ldq, Out_ptr, ldo, int(use_mfma), stream))
+
+
+# --------------------------------------------------- host math of the restart
+
+def host_tridiag_eig(d, e, last_row_only: bool = False):
+    """(info, ascending eigenvalues, eigenvectors or their last row) of the
+    symmetric tridiagonal (d, e) by the library's host QR (hl_host_tridiag_eig)."""
+    d = np.array(d, dtype=np.float64)
+    n = len(d)
+    e2 = np.zeros(max(n, 1))
+    e2[: max(n - 1, 0)] = e
+    if last_row_only:
+        z = np.zeros(max(n, 1))
+        info = lib().hl_host_tridiag_eig_last(n, _d(d), _d(e2), _d(z))
+        return info, d, z[:n]
+    z = np.zeros((n, n), order="F")
+    info = lib().hl_host_tridiag_eig(n, _d(d), _d(e2), z.ctypes.data_as(_dp), max(n, 1))
+    return info, d, z
+
+
+def host_dlartg(f: float, g: float):
+    c, s, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    lib().hl_host_dlartg(f, g, ctypes.byref(c), ctypes.byref(s), ctypes.byref(r))
+    return c.value, s.value, r.value
+
+
+def host_householder_q(a):
+    """(factored a, tau, Q = H(0)...H(k-1) as m x m) of the library's Householder QR."""
+    a = np.array(a, dtype=np.float64, order="F")
+    m, k = a.shape
+    tau = np.zeros(max(k, 1))
+    q = np.zeros((m, m), order="F")
+    _check(lib().hl_host_householder_q(m, k, a.ctypes.data_as(_dp), m, _d(tau), q.ctypes.data_as(_dp), m))
+    return a, tau[:k], q
+
+
+def host_restart_chase(nev: int, shifts, sub, dia):
+    """(sub, dia, Q) after the library's dsapps shift sweeps (hl_host_restart_chase);
+    sub[i] = T(i, i-1) with sub[0] unused."""
+    sub = np.array(sub, dtype=np.float64)
+    dia = np.array(dia, dtype=np.float64)
+    sh = np.ascontiguousarray(shifts, dtype=np.float64)
+    ncv = len(dia)
+    q = np.zeros((ncv, ncv), order="F")
+    _check(lib().hl_host_restart_chase(nev, ncv - nev, _d(sh) if len(sh) else None, _d(sub), _d(dia),
+                                       q.ctypes.data_as(_dp)))
+    return sub, dia, q
 
 
 # ------------------------------------------------------------------ tuning

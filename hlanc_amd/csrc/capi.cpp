@@ -653,6 +653,41 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
                      0.0, red, nullptr, static_cast<hipStream_t>(stream));
 }
 
+int hl_host_tridiag_eig(int n, double *d, double *e, double *z, int ldz)
+{
+    if (n < 0 || (n > 0 && (!d || (n > 1 && !e) || !z || ldz < n))) return HL_ERR_ARG;
+    return tridiag_eig(n, d, e, z, ldz);
+}
+
+int hl_host_tridiag_eig_last(int n, double *d, double *e, double *zlast)
+{
+    if (n < 0 || (n > 0 && (!d || (n > 1 && !e) || !zlast))) return HL_ERR_ARG;
+    return tridiag_eig_last(n, d, e, zlast);
+}
+
+void hl_host_dlartg(double f, double g, double *c, double *s, double *r)
+{
+    double cc = 0.0, ss = 0.0, rr = 0.0;
+    dlartg(f, g, cc, ss, rr);
+    if (c) *c = cc;
+    if (s) *s = ss;
+    if (r) *r = rr;
+}
+
+int hl_host_householder_q(int m, int k, double *a, int lda, double *tau, double *q, int ldq)
+{
+    if (m < 0 || k < 0 || k > m || (m > 0 && (!a || !tau || !q || lda < m || ldq < m))) return HL_ERR_ARG;
+    householder_q(m, k, a, lda, tau, q, ldq);
+    return HL_OK;
+}
+
+int hl_host_restart_chase(int nev, int np, const double *shift, double *sub, double *dia, double *q)
+{
+    if (nev <= 0 || np < 0 || (np > 0 && !shift) || !sub || !dia || !q) return HL_ERR_ARG;
+    restart_chase(nev, np, shift, sub, dia, q);
+    return HL_OK;
+}
+
 int hl_profile_last(hl_profile *out)
 {
     if (!out) return HL_ERR_ARG;

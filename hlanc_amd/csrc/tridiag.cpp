@@ -1,11 +1,11 @@
-// tridiag.cpp -- host FP64 dense kernels of the restart (ncv <= 128).
+// tridiag.cpp -- host FP64 dense kernels of the restart.
 //
 // The reference calls LAPACK for these (dsteqr HLanc.hpp:1691,1805; dlartg
-// 1514,1536; dgeqr2 1837; dorm2r 940).  They are O(ncv^3) on tiny matrices, so
+// 1514,1536; dgeqr2 1837; dorm2r 940).  They are O(ncv^3) on small matrices, so
 // they run on the host in FP64, written here from the textbook methods:
-//   * tridiag_eig: implicit QL with Wilkinson shifts on a symmetric tridiagonal
-//     (Bowdler, Martin, Reinsch & Wilkinson, "The QR and QL algorithms for
-//     symmetric matrices", Numer. Math. 11, 1968), accumulating rotations.
+//   * tridiag_eig: the symmetric QR algorithm on a tridiagonal, implicit steps
+//     with the Wilkinson shift (Golub & Van Loan, "Matrix Computations", 4th
+//     ed., Algorithms 8.3.2 and 8.3.3), accumulating the rotations.
 //   * dlartg: Givens rotation with the sign convention of the MKL dlartg the
 //     reference links.
 //   * householder_q: Householder QR with LAPACK's reflector convention, then
@@ -54,11 +54,20 @@ void dlartg(double f, double g, double &c, double &s, double &r)
 }
 
 namespace {
-// Implicit QL.  FULL: z is n x n (ldz), the eigenvectors.  !FULL: z holds only
-// the last row of the eigenvector matrix (z[i] = Z(n-1, i)); the rotations and
-// the eigenvalues are the same, so both forms agree bit for bit there.
+// Symmetric tridiagonal QR (Golub & Van Loan, Alg. 8.3.3).  d[0..n) is the
+// diagonal, e[0..n-1) the off-diagonal (e[i] couples rows i and i+1).  Each
+// pass deflates negligible off-diagonals (|e[i]| <= eps (|d[i]| + |d[i+1]|))
+// from the bottom, then runs one implicit QR step with the Wilkinson shift
+// (Alg. 8.3.2) on the lowest unreduced block [l, m]: a rotation R_k of rows
+// and columns (k, k+1), R_k = [c -s; s c], set by (x, z) = (d[l] - mu, e[l])
+// for k = l and by (e[k-1], bulge) after that, so that T := R_k^T T R_k
+// removes the bulge at (k+1, k-1) and pushes it to (k+2, k).  The rotations
+// accumulate into the eigenvector matrix, Z := Z R_k.
+//   FULL: z is n x n (ldz), the eigenvectors.  !FULL: z holds only the last
+// row of the eigenvector matrix (z[i] = Z(n-1, i)); every row of Z is updated
+// by the same rotations on its own, so both forms agree bit for bit there.
 template <bool FULL>
-int tridiag_ql(int n, double *d, double *e, double *z, int ldz)
+int tridiag_qr(int n, double *d, double *e, double *z, int ldz)
 {
     for (int i = 0; i < n; ++i) {
         if (!std::isfinite(d[i]) || (i + 1 < n && !std::isfinite(e[i]))) return 1;
@@ -67,64 +76,62 @@ int tridiag_ql(int n, double *d, double *e, double *z, int ldz)
         else
             z[i] = (i == n - 1) ? 1.0 : 0.0;
     }
-    if (n == 0) return 0;
-    std::vector<double> off(n, 0.0); // off[i] couples rows i and i+1
-    for (int i = 0; i + 1 < n; ++i) off[i] = e[i];
-    for (int l = 0; l < n; ++l) {
-        int iter = 0;
-        for (;;) {
-            int m = l;
-            for (; m < n - 1; ++m) {
-                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-                if (std::fabs(off[m]) <= kEps * dd) break;
+    if (n <= 1) return 0;
+    std::vector<double> off(e, e + (n - 1));
+    auto small = [&](int i) { return std::fabs(off[i]) <= kEps * (std::fabs(d[i]) + std::fabs(d[i + 1])); };
+    auto rotate = [&](int k, double c, double s) { // Z := Z R_k on columns k, k+1
+        if (FULL) {
+            double *za = z + (size_t)k * ldz, *zb = za + ldz;
+            for (int r = 0; r < n; ++r) {
+                const double a = za[r], b = zb[r];
+                za[r] = c * a + s * b;
+                zb[r] = c * b - s * a;
             }
-            if (m == l) break;
-            if (++iter > 100) return 1;
-            // shift: eigenvalue of the leading 2x2 block nearer to d[l]
-            double g = (d[l + 1] - d[l]) / (2.0 * off[l]);
-            double r = norm2(g, 1.0);
-            g = d[m] - d[l] + off[l] / (g + std::copysign(r, g));
-            double s = 1.0, c = 1.0, p = 0.0;
-            int i = m - 1;
-            bool underflow = false;
-            for (; i >= l; --i) {
-                const double f = s * off[i];
-                const double b = c * off[i];
-                r = norm2(f, g);
-                off[i + 1] = r;
-                if (r == 0.0) { // split: recover and restart the sweep
-                    d[i + 1] -= p;
-                    off[m] = 0.0;
-                    underflow = true;
-                    break;
-                }
-                s = f / r;
-                c = g / r;
-                g = d[i + 1] - p;
-                r = (d[i] - g) * s + 2.0 * c * b;
-                p = s * r;
-                d[i + 1] = g + p;
-                g = c * r - b;
-                if (FULL) {
-                    double *zi = z + (size_t)i * ldz, *zj = z + (size_t)(i + 1) * ldz;
-                    for (int k = 0; k < n; ++k) {
-                        const double t = zj[k];
-                        zj[k] = s * zi[k] + c * t;
-                        zi[k] = c * zi[k] - s * t;
-                    }
-                } else {
-                    const double t = z[i + 1];
-                    z[i + 1] = s * z[i] + c * t;
-                    z[i] = c * z[i] - s * t;
-                }
+        } else {
+            const double a = z[k], b = z[k + 1];
+            z[k] = c * a + s * b;
+            z[k + 1] = c * b - s * a;
+        }
+    };
+    const long long max_steps = 30LL * n; // the sweep budget LAPACK's dsteqr allows
+    long long steps = 0;
+    int m = n - 1;
+    while (m > 0) {
+        if (small(m - 1)) { // d[m] has converged
+            off[m - 1] = 0.0;
+            --m;
+            continue;
+        }
+        int l = m - 1; // [l, m]: the lowest unreduced block
+        while (l > 0 && !small(l - 1)) --l;
+        if (l > 0) off[l - 1] = 0.0;
+        if (++steps > max_steps) return 1;
+        // Wilkinson shift: the eigenvalue of the trailing 2 x 2 block nearer d[m]
+        const double delta = 0.5 * (d[m - 1] - d[m]);
+        const double b = off[m - 1];
+        const double den = delta + std::copysign(norm2(delta, b), delta);
+        const double mu = d[m] - b * (b / den);
+        double x = d[l] - mu, zz = off[l], bulge = 0.0;
+        for (int k = l; k < m; ++k) {
+            const double r = norm2(x, zz);
+            const double c = r > 0.0 ? x / r : 1.0;
+            const double s = r > 0.0 ? zz / r : 0.0;
+            if (k > l) off[k - 1] = r; // R^T (e[k-1], bulge) = (r, 0)
+            const double p = d[k], q = off[k], w = d[k + 1];
+            const double cs = c * s, cc = c * c, ss = s * s;
+            d[k] = cc * p + 2.0 * cs * q + ss * w;
+            d[k + 1] = ss * p - 2.0 * cs * q + cc * w;
+            off[k] = cs * (w - p) + (cc - ss) * q;
+            if (k + 1 < m) { // row k+2 meets the rotated column k: the new bulge
+                bulge = s * off[k + 1];
+                off[k + 1] *= c;
             }
-            if (underflow) continue;
-            d[l] -= p;
-            off[l] = g;
-            off[m] = 0.0;
+            rotate(k, c, s);
+            x = off[k];
+            zz = bulge;
         }
     }
-    // ascending order, eigenvectors alongside (stable selection)
+    // ascending order, eigenvectors alongside (selection, stable for ties)
     for (int i = 0; i < n - 1; ++i) {
         int b = i;
         for (int k = i + 1; k < n; ++k)
@@ -144,9 +151,9 @@ int tridiag_ql(int n, double *d, double *e, double *z, int ldz)
 }
 } // namespace
 
-int tridiag_eig(int n, double *d, double *e, double *z, int ldz) { return tridiag_ql<true>(n, d, e, z, ldz); }
+int tridiag_eig(int n, double *d, double *e, double *z, int ldz) { return tridiag_qr<true>(n, d, e, z, ldz); }
 
-int tridiag_eig_last(int n, double *d, double *e, double *zlast) { return tridiag_ql<false>(n, d, e, zlast, 1); }
+int tridiag_eig_last(int n, double *d, double *e, double *zlast) { return tridiag_qr<false>(n, d, e, zlast, 1); }
 
 void householder_q(int m, int k, double *a, int lda, double *tau, double *q, int ldq)
 {

@@ -179,6 +179,31 @@ int hl_cgs_pass_noalloc(int dev, int n, int m, const double *V, long long ldv, c
 int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq,
           double *Out, long long ldo, int use_mfma, void *stream);
 
+/* ------------------------------------------------ host math of the restart
+ * The FP64 ncv x ncv kernels the solver runs on the host, exported so that
+ * they can be checked on their own (no device work; callable without a GPU).
+ * All matrices are column-major. */
+/* dsteqr("I") (HLanc.hpp:1691,1805): eigenvalues of the symmetric tridiagonal
+ * (d[0..n), e[0..n-1)) ascending in d, eigenvectors in z (n x n, ldz >= n); e
+ * is overwritten.  Returns 0, or 1 on non-finite input or no convergence. */
+int hl_host_tridiag_eig(int n, double *d, double *e, double *z, int ldz);
+/* The same eigenvalues, and only the last row of the eigenvector matrix
+ * (zlast[i] = Z(n-1, i)), bitwise those of hl_host_tridiag_eig: what the
+ * restart's Ritz estimates need (HLanc.hpp:1691-1696). */
+int hl_host_tridiag_eig_last(int n, double *d, double *e, double *zlast);
+/* dlartg (HLanc.hpp:1514,1536) with MKL's sign convention: c f + s g = r,
+ * -s f + c g = 0. */
+void hl_host_dlartg(double f, double g, double *c, double *s, double *r);
+/* dgeqr2 on a (m x k, lda) in LAPACK's storage (HLanc.hpp:1837), then
+ * q = H(0) ... H(k-1) formed explicitly (m x m, ldq): the Q that dorm2r("R",
+ * "N") applies (HLanc.hpp:1840). */
+int hl_host_householder_q(int m, int k, double *a, int lda, double *tau, double *q, int ldq);
+/* The shift sweeps of dsapps (HLanc.hpp:1492-1589) on T = (dia, sub) of order
+ * ncv = nev + np (sub[i] = T(i, i-1), sub[0] unused): np implicit QR steps with
+ * exact shifts shift[0..np) by bulge chasing with the reference's deflation
+ * test; dia/sub become Q^T T Q and q (ncv x ncv, ld ncv) receives Q. */
+int hl_host_restart_chase(int nev, int np, const double *shift, double *sub, double *dia, double *q);
+
 /* ---------------------------------------------------------------- tuning
  * Process-wide kernel parameters, set before solves start (not thread-safe
  * against running solves).  Keys: "cgs_blocks_max" (grid cap of the CGS

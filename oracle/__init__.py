@@ -52,6 +52,7 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         L.hor_csr_spmv.argtypes = [ctypes.POINTER(_Csr), _dp, _dp]
         L.hor_dlartg.argtypes = [ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.hor_dsapps_chase.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.hor_tridiag_eig.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
         L.hor_tridiag_eig.restype = ctypes.c_int
         L.hor_dgeqr2.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp]
@@ -95,6 +96,17 @@ def dlartg(f: float, g: float):
     c, s, r = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
     lib().hor_dlartg(f, g, ctypes.byref(c), ctypes.byref(s), ctypes.byref(r))
     return c.value, s.value, r.value
+
+
+def dsapps_chase(nev: int, shifts, sub, dia):
+    """(sub, dia, Q) after the oracle's dsapps shift sweeps; sub[i] = T(i, i-1)."""
+    sub = np.array(sub, dtype=np.float64)
+    dia = np.array(dia, dtype=np.float64)
+    sh = np.ascontiguousarray(shifts, dtype=np.float64)
+    ncv = len(dia)
+    q = np.zeros((ncv, ncv), order="F")
+    lib().hor_dsapps_chase(nev, ncv - nev, _d(sh) if len(sh) else None, _d(sub), _d(dia), q.ctypes.data_as(_dp))
+    return sub, dia, q
 
 
 def tridiag_eig(d, e):

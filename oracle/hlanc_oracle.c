@@ -447,16 +447,14 @@ static void hor_rot_q(double *q, int ldq, int c0, int rows, double c, double s)
     }
 }
 
-/* dsapps (HLanc.hpp:1484-1618): np implicit exact-shift QR sweeps on T by bulge
- * chasing, then V := V Q(:, 0:nev) and the new residual. */
-static void hor_dsapps(hor_state *st, int nev, int np, const double *shift, double *h, int ldh,
-                       double *resid)
+/* The shift sweeps of dsapps (HLanc.hpp:1492-1589): np implicit exact-shift QR
+ * sweeps on T = (dia, sub) by bulge chasing; q (ncv x ncv, ld ncv) receives Q
+ * and dia/sub become Q^T T Q. */
+void hor_dsapps_chase(int nev, int np, const double *shift, double *sub, double *dia, double *q)
 {
-    const int n = st->n, ncv = nev + np, ldq = ncv;
-    double *q = (double *)calloc((size_t)ncv * ncv, sizeof(double));
+    const int ncv = nev + np, ldq = ncv;
+    memset(q, 0, sizeof(double) * (size_t)ncv * ncv);
     for (int i = 0; i < ncv; ++i) q[(size_t)i * ldq + i] = 1.0;
-    double *sub = h;         /* sub[i] = T(i, i-1), i >= 1 */
-    double *dia = h + ldh;   /* dia[i] = T(i, i) */
     int itop = 1;
     double c = 0.0, s = 0.0, r = 0.0;
     for (int jj = 1; jj <= np; ++jj) {
@@ -523,6 +521,18 @@ static void hor_dsapps(hor_state *st, int nev, int np, const double *shift, doub
         const double big = fabs(dia[i - 1]) + fabs(dia[i]);
         if (sub[i] <= kEps * big) sub[i] = 0.0;
     }
+}
+
+/* dsapps (HLanc.hpp:1484-1618): the shift sweeps, then V := V Q(:, 0:nev) and
+ * the new residual. */
+static void hor_dsapps(hor_state *st, int nev, int np, const double *shift, double *h, int ldh,
+                       double *resid)
+{
+    const int n = st->n, ncv = nev + np, ldq = ncv;
+    double *q = (double *)calloc((size_t)ncv * ncv, sizeof(double));
+    double *sub = h;         /* sub[i] = T(i, i-1), i >= 1 */
+    double *dia = h + ldh;   /* dia[i] = T(i, i) */
+    hor_dsapps_chase(nev, np, shift, sub, dia, q);
 
     /* Basis update (HLanc.hpp:1591-1614) written against full columns of Q:
      * entries of Q outside the Givens band are exact zeros, so

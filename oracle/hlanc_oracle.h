@@ -38,6 +38,9 @@ void hor_csr_op(void *ctx, const double *x, double *y);
 
 /* Givens rotation with MKL's dlartg sign convention (called at HLanc.hpp:1514,1536). */
 void hor_dlartg(double f, double g, double *c, double *s, double *r);
+/* The shift sweeps of dsapps (HLanc.hpp:1492-1589) on T = (dia, sub), sub[i] =
+ * T(i, i-1): dia/sub become Q^T T Q, q (ncv x ncv) receives Q. */
+void hor_dsapps_chase(int nev, int np, const double *shift, double *sub, double *dia, double *q);
 
 /* Symmetric tridiagonal eigen-decomposition, eigenvalues ascending, eigenvectors
  * in z (ldz >= n) -- the role of dsteqr("I") at HLanc.hpp:1691,1805.
