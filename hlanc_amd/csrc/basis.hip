@@ -495,6 +495,150 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
     }
 }
 
+// Wide passes (m > kNarrowCols, FP64 tiled solver bases): the fused pass
+// split in two launches over the same block numbering.  The subtracting kernel
+// forms a = w_in - V h_in for the block's tiles (V read in 128-column chunks),
+// writes w_out and the block's partial of ||a||^2 (column m of the partials);
+// the projecting kernel then reads a back and forms the partials of V^T a one
+// 128-column chunk after another.  Tile assignment, lane mapping, load masks and
+// reduction order are those of cgs_block, so a pass of m <= 128 columns run
+// this way gives the fused pass's sums.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void cgs_seg_sub_kernel(const SegMember *__restrict__ M,
+                                                             const SegStep *__restrict__ S, int ns, int sub)
+{
+    const int i = seg_find(sub ? &S[0].cgs0s : &S[0].cgs0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
+    const SegStep st = S[i];
+    const SegMember mb = M[st.mem];
+    if (MODE == EPI_PASS2 && mb.V32 && mb.ctl->exact == 0) return; // the FP32 launch's step
+    if ((MODE == EPI_PASS2 || MODE == EPI_PASS3) && !mb.ctl->need) return;
+    const PassPtrs p = pass_ptrs(mb, MODE);
+    const int bid = blockIdx.x - (sub ? st.cgs0s : st.cgs0);
+    const int n = mb.n, m = st.j + 1, nb = mb.nb;
+    const long long ldv = mb.vld;
+    const auto V = gptr(static_cast<const double *>(mb.V));
+    const auto w_in = gptr(p.w_in);
+    const auto w_out = gptr(p.w_out);
+    const auto partials = gptr(mb.partials);
+    __shared__ double hs[kMaxNcv + 4 * 32];
+    __shared__ double red[kBlock / kWave];
+    const bool has_h = p.h_in != nullptr;
+    for (int c = threadIdx.x; c < kMaxNcv + 4 * 32; c += kBlock) hs[c] = (has_h && c < m) ? p.h_in[c] : 0.0;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int g = lane / kTileRows;
+    const int rl = lane % kTileRows;
+    const long long ntiles = (n + kTileRows - 1) / kTileRows;
+    const long long W = (long long)nb * (kBlock / kWave);
+    double nrm = 0.0;
+    for (long long t = (long long)bid * (kBlock / kWave) + wv; t < ntiles; t += W) {
+        const long long r = t * kTileRows + rl;
+        const bool ok = r < n;
+        const double win = w_in[ok ? r : n - 1];
+        double a = win;
+        if (has_h) {
+            double pr = 0.0;
+            for (int k0 = 0; k0 < m; k0 += kNarrowCols) {
+                const int mk = m - k0 < kNarrowCols ? m - k0 : kNarrowCols;
+                const int cpg = (mk + kGroups - 1) / kGroups;
+                const int c0 = k0 + g * cpg;
+                const auto pv = V + (t * ldv + c0) * kTileRows + rl;
+                double v[32];
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    v[c] = c < cpg && (c + 3 < cpg || c0 + c < k0 + mk) ? __builtin_nontemporal_load(pv + c * kTileRows)
+                                                                       : 0.0;
+#pragma unroll
+                for (int c = 0; c < 32; ++c) pr += v[c] * hs[c0 + c];
+            }
+            pr += __shfl_xor(pr, 16);
+            pr += __shfl_xor(pr, 32);
+            a = win - pr;
+        }
+        if (!ok) a = 0.0;
+        if (g == 0 && ok) {
+            if (p.w_out) w_out[r] = a;
+            nrm += a * a;
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
+    if (lane == 0) red[wv] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s2 = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) s2 += red[w];
+        partials[(long long)m * nb + bid] = s2;
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void cgs_seg_proj_kernel(const SegMember *__restrict__ M,
+                                                              const SegStep *__restrict__ S, int ns, int sub)
+{
+    const int i = seg_find(sub ? &S[0].cgs0s : &S[0].cgs0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
+    const SegStep st = S[i];
+    const SegMember mb = M[st.mem];
+    if (MODE == EPI_PASS2 && mb.V32 && mb.ctl->exact == 0) return;
+    if ((MODE == EPI_PASS2 || MODE == EPI_PASS3) && !mb.ctl->need) return;
+    const PassPtrs p = pass_ptrs(mb, MODE);
+    if (!p.h_out) return; // the pass projects nothing
+    const int bid = blockIdx.x - (sub ? st.cgs0s : st.cgs0);
+    const int n = mb.n, m = st.j + 1, nb = mb.nb;
+    const long long ldv = mb.vld;
+    const auto V = gptr(static_cast<const double *>(mb.V));
+    const auto src = gptr(p.w_out ? p.w_out : p.w_in); // the pass's a, written by the subtracting kernel
+    const auto partials = gptr(mb.partials);
+    __shared__ double red[kBlock / kWave][kNarrowCols];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int g = lane / kTileRows;
+    const int rl = lane % kTileRows;
+    const long long ntiles = (n + kTileRows - 1) / kTileRows;
+    const long long W = (long long)nb * (kBlock / kWave);
+    for (int k0 = 0; k0 < m; k0 += kNarrowCols) {
+        const int mk = m - k0 < kNarrowCols ? m - k0 : kNarrowCols;
+        const int cpg = (mk + kGroups - 1) / kGroups;
+        const int c0 = g * cpg; // within the chunk
+        double acc[32];
+#pragma unroll
+        for (int c = 0; c < 32; ++c) acc[c] = 0.0;
+        for (long long t = (long long)bid * (kBlock / kWave) + wv; t < ntiles; t += W) {
+            const long long r = t * kTileRows + rl;
+            const bool ok = r < n;
+            const double a = ok ? src[r] : 0.0;
+            const auto pv = V + (t * ldv + k0 + c0) * kTileRows + rl;
+            double v[32];
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                v[c] = c < cpg && (c + 3 < cpg || c0 + c < mk) ? __builtin_nontemporal_load(pv + c * kTileRows) : 0.0;
+#pragma unroll
+            for (int c = 0; c < 32; ++c) acc[c] += v[c] * a;
+        }
+#pragma unroll
+        for (int c = 0; c < 32; ++c) {
+            double sv = acc[c];
+            sv += __shfl_xor(sv, 1);
+            sv += __shfl_xor(sv, 2);
+            sv += __shfl_xor(sv, 4);
+            sv += __shfl_xor(sv, 8);
+            acc[c] = sv;
+        }
+        __syncthreads(); // the previous chunk's readers of red are done
+        if (rl == 0) {
+#pragma unroll
+            for (int c = 0; c < 32; ++c)
+                if (c < cpg && c0 + c < mk) red[wv][c0 + c] = acc[c];
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < mk; c += kBlock) {
+            double sv = 0.0;
+            for (int w = 0; w < kBlock / kWave; ++w) sv += red[w][c];
+            partials[(long long)(k0 + c) * nb + bid] = sv;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMember *__restrict__ M,
                                                                   const SegStep *__restrict__ S, int ns, int mode)
 {
@@ -594,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, con
                                                         VT *__restrict__ Out, long long ldo, int c0,
                                                         int res_local, double *r, double qlast,
                                                         double beta, double *partials, unsigned *counter,
-                                                        Ctl *ctl)
+                                                        Ctl *ctl, int accum)
 {
     extern __shared__ double qs[]; // K x CB
     __shared__ double red[kBlock / kWave];
@@ -610,7 +754,8 @@ __global__ __launch_bounds__(kBlock) void vq_fma_kernel(int n, int K, int N, con
          row += (long long)gridDim.x * kBlock) {
         double acc[CB];
 #pragma unroll
-        for (int c = 0; c < CB; ++c) acc[c] = 0.0;
+        for (int c = 0; c < CB; ++c)
+            acc[c] = (accum && c0 + c < N) ? static_cast<double>(Out[vpos<TOUT>(row, c0 + c, ldo)]) : 0.0;
         for (int k = 0; k < K; ++k) {
             const double v = static_cast<double>(V[vpos<TIN>(row, k, ldv)]);
 #pragma unroll
@@ -875,6 +1020,43 @@ __global__ __launch_bounds__(kBlock) void resid_seg_kernel(const SegRes *__restr
     }
 }
 
+// The dsapps residual update of a restart whose V.Q ran unfused (bases wider
+// than the segmented restart kernels take): r = qlast rscale r + beta Out(:, k)
+// (res_col < 0: no beta term) on a tiled Out, and each block's partial ||r||^2.
+__global__ __launch_bounds__(kBlock) void vq_resid_kernel(int n, const double *__restrict__ Out, long long ld,
+                                                          int res_col, double *__restrict__ r, double qlast,
+                                                          double beta, const Ctl *ctl, double *partials)
+{
+    __shared__ double red[kBlock / kWave];
+    const double qrs = qlast * ctl->rscale;
+    double nrm = 0.0;
+    for (long long row = (long long)blockIdx.x * kBlock + threadIdx.x; row < n; row += (long long)gridDim.x * kBlock) {
+        const double rk = res_col >= 0 ? Out[vpos<true>(row, res_col, ld)] : 0.0;
+        const double rr = qrs * r[row] + beta * rk; // HLanc.hpp:1605-1614
+        r[row] = rr;
+        nrm += rr * rr;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) nrm += __shfl_xor(nrm, off);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += red[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
+// dst(:, 0:N) = FP32 copy of src(:, 0:N), both tiled with ld columns per tile.
+__global__ __launch_bounds__(kBlock) void basis_copy32_kernel(int n, int N, long long ld, const double *__restrict__ src,
+                                                              float *__restrict__ dst)
+{
+    const long long total = ((long long)n + 15) / 16 * ld * 16; // whole tiles, padding rows included
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < total; i += (long long)gridDim.x * kBlock) {
+        const long long c = (i >> 4) % ld;
+        if (c < N) dst[i] = static_cast<float>(src[i]);
+    }
+}
+
 // One block per listed solve: rnorm = sqrt(sum of its V.Q block partials).
 __global__ __launch_bounds__(kBlock) void vq_seg_finalize_kernel(const SegVQ *__restrict__ T)
 {
@@ -908,7 +1090,7 @@ __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, co
                                                          long long ldv, const double *__restrict__ Q, int ldq,
                                                          double *__restrict__ Out, long long ldo, int c0,
                                                          int res_local, double *r, double qlast, double beta,
-                                                         double *partials, unsigned *counter, Ctl *ctl)
+                                                         double *partials, unsigned *counter, Ctl *ctl, int accum)
 {
     extern __shared__ double qs[]; // Kp x (NT*16), row k
     __shared__ double red[kBlock / kWave];
@@ -933,7 +1115,15 @@ __global__ __launch_bounds__(kBlock) void vq_mfma_kernel(int n, int K, int N, co
         const bool ok = row < n;
         dbl4 acc[NT];
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int ct = 0; ct < NT; ++ct) {
+            acc[ct] = dbl4{0.0, 0.0, 0.0, 0.0};
+            if (accum) // K-chunked product: this launch adds to what the previous chunks wrote
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int cl = ct * 16 + lk + 4 * i;
+                    if (ok && c0 + cl < N) acc[ct][i] = Out[vpos<TOUT>(row, c0 + cl, ldo)];
+                }
+        }
         for (int k0 = 0; k0 < Kp; k0 += 32) { // 8 loads of V in flight per wave
             double b[8];
 #pragma unroll
@@ -1077,7 +1267,7 @@ int launch_cgs_t(int n, int m, const VT *V, long long ldv, const double *w_in, d
 template <typename VT, bool TIN, bool TOUT>
 int launch_vq_fma(int n, int K, int N, const VT *V, long long ldv, const double *Q, int ldq, VT *Out,
                   long long ldo, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
-                  hipStream_t s)
+                  hipStream_t s, int accum)
 {
     constexpr int CB = 32;
     const int nb = grid_for(n, kBlock * 4, 1024);
@@ -1089,7 +1279,8 @@ int launch_vq_fma(int n, int K, int N, const VT *V, long long ldv, const double 
         const int rl = (res_col >= c0 && res_col < c0 + CB) ? res_col - c0 : -1;
         const size_t lds = sizeof(double) * (size_t)K * CB;
         vq_fma_kernel<CB, VT, TIN, TOUT><<<nb, kBlock, lds, s>>>(n, K, N, V, ldv, Q, ldq, Out, ldo, c0, rl, r,
-                                                                  qlast, beta, red.partials, red.counter, ctl);
+                                                                  qlast, beta, red.partials, red.counter, ctl,
+                                                                  accum);
         HL_TRY_HIP(hipGetLastError());
     }
     if (res_col >= 0) {
@@ -1103,7 +1294,7 @@ int launch_vq_fma(int n, int K, int N, const VT *V, long long ldv, const double 
 template <bool TIN, bool TOUT>
 int launch_vq_mfma(int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq, double *Out,
                    long long ldo, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
-                   hipStream_t s)
+                   hipStream_t s, int accum)
 {
     const long long ntiles = (static_cast<long long>(n) + 15) / 16;
     const int nb = grid_for(ntiles, 4 * 4, 1024);
@@ -1123,7 +1314,7 @@ int launch_vq_mfma(int n, int K, int N, const double *V, long long ldv, const do
         const size_t lds = sizeof(double) * (size_t)Kp * nt * 16;
 #define HL_VQM(NTV)                                                                                          \
     vq_mfma_kernel<NTV, TIN, TOUT><<<nb, kBlock, lds, s>>>(n, K, N, V, ldv, Q, ldq, Out, ldo, c0, rl, r, qlast, \
-                                                           beta, red.partials, red.counter, ctl)
+                                                           beta, red.partials, red.counter, ctl, accum)
         if (nt == 1)
             HL_VQM(1);
         else if (nt == 2)
@@ -1260,20 +1451,41 @@ int launch_vq(int n, int K, int N, const double *V, VLayout Lin, const double *Q
               VLayout Lout, int use_mfma, int res_col, double *r, double qlast, double beta, Reduce &red, Ctl *ctl,
               hipStream_t s)
 {
-    if (K <= 0 || N <= 0 || K > 128) {
-        set_error("V.Q: K=%d N=%d unsupported (K <= 128)", K, N);
+    if (K <= 0 || N <= 0 || K > kMaxNcv) {
+        set_error("V.Q: K=%d N=%d unsupported (K <= %d)", K, N, kMaxNcv);
         return HL_ERR_ARG;
     }
-#define HL_VQ_DISPATCH(TI, TO)                                                                                   \
-    return use_mfma ? launch_vq_mfma<TI, TO>(n, K, N, V, Lin.ld, Q, ldq, Out, Lout.ld, res_col, r, qlast, beta, \
-                                             red, ctl, s)                                                       \
-                    : launch_vq_fma<double, TI, TO>(n, K, N, V, Lin.ld, Q, ldq, Out, Lout.ld, res_col, r, qlast, \
-                                                    beta, red, ctl, s)
-    if (Lin.tiled && Lout.tiled) HL_VQ_DISPATCH(true, true);
-    if (Lin.tiled) HL_VQ_DISPATCH(true, false);
-    if (Lout.tiled) HL_VQ_DISPATCH(false, true);
-    HL_VQ_DISPATCH(false, false);
+    // rows of Q per pass: the FMA kernel keeps a K x 32 slice of Q in 48 KB of LDS,
+    // the MFMA kernel a K x 16 one; wider K runs in chunks, each pass adding its
+    // chunk's product to what the previous passes wrote
+    const int kpass = use_mfma ? 384 : 192;
+    if (K > kpass && res_col >= 0) {
+        set_error("V.Q: the fused residual update needs K <= %d (K = %d)", kpass, K);
+        return HL_ERR_ARG;
+    }
+    for (int k0 = 0; k0 < K; k0 += kpass) {
+        const int kc = K - k0 < kpass ? K - k0 : kpass;
+        const double *Vk = V + (Lin.tiled ? 16LL * k0 : (long long)k0 * Lin.ld); // column k0 in either layout
+        const double *Qk = Q + k0;
+        const int acc = k0 > 0;
+        int rc;
+#define HL_VQ_DISPATCH(TI, TO)                                                                                      \
+    rc = use_mfma ? launch_vq_mfma<TI, TO>(n, kc, N, Vk, Lin.ld, Qk, ldq, Out, Lout.ld, res_col, r, qlast, beta, \
+                                           red, ctl, s, acc)                                                      \
+                  : launch_vq_fma<double, TI, TO>(n, kc, N, Vk, Lin.ld, Qk, ldq, Out, Lout.ld, res_col, r, qlast, \
+                                                  beta, red, ctl, s, acc)
+        if (Lin.tiled && Lout.tiled)
+            HL_VQ_DISPATCH(true, true);
+        else if (Lin.tiled)
+            HL_VQ_DISPATCH(true, false);
+        else if (Lout.tiled)
+            HL_VQ_DISPATCH(false, true);
+        else
+            HL_VQ_DISPATCH(false, false);
 #undef HL_VQ_DISPATCH
+        if (rc) return rc;
+    }
+    return HL_OK;
 }
 
 int launch_start_vector(int n, uint64_t seed, double *r, hipStream_t s)
@@ -1317,9 +1529,34 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
                    bool v32, bool sub, hipStream_t s)
 {
     if (ns <= 0) return HL_OK;
-    if (max_cpg > 32) {
-        set_error("CGS pass: more than 128 columns");
-        return HL_ERR_ARG;
+    if (max_cpg > kNarrowCols / kGroups) { // a step wider than the fused pass holds: two kernels for all
+        if (v32 || max_cpg > kMaxNcv / kGroups) {
+            set_error("CGS pass: %d columns per lane group unsupported%s", max_cpg, v32 ? " on an FP32 basis" : "");
+            return HL_ERR_ARG;
+        }
+#define HL_WIDE(MODE)                                                                         \
+    cgs_seg_sub_kernel<MODE><<<cgs_blocks_total, kBlock, 0, s>>>(M, S, ns, sub ? 1 : 0);     \
+    cgs_seg_proj_kernel<MODE><<<cgs_blocks_total, kBlock, 0, s>>>(M, S, ns, sub ? 1 : 0)
+        switch (mode) {
+        case EPI_WNORM:
+            HL_WIDE(EPI_WNORM);
+            break;
+        case EPI_PASS1:
+            HL_WIDE(EPI_PASS1);
+            break;
+        case EPI_PASS2:
+            HL_WIDE(EPI_PASS2);
+            break;
+        case EPI_PASS3:
+            HL_WIDE(EPI_PASS3);
+            break;
+        default:
+            set_error("CGS pass: bad mode %d", mode);
+            return HL_ERR_ARG;
+        }
+#undef HL_WIDE
+        HL_TRY_HIP(hipGetLastError());
+        return HL_OK;
     }
 #define HL_CGS_MODE(MODE)                                                                              \
     (!v32 ? cgs_seg_dispatch<MODE, double>(M, S, ns, max_cpg, cgs_blocks_total, sub, s)                \
@@ -1449,6 +1686,30 @@ int launch_basis_combine(int n, int K, const void *V, long long ld, bool v32, co
         basis_combine_kernel<float><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const float *>(V), ld, q, out);
     else
         basis_combine_kernel<double><<<blocks, kBlock, 0, s>>>(n, K, static_cast<const double *>(V), ld, q, out);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_vq_residual(int n, const double *Out, long long ld, int res_col, double *r, double qlast, double beta,
+                       Reduce &red, Ctl *ctl, hipStream_t s)
+{
+    const int nb = vq_seg_blocks(n);
+    if (nb > red.max_blocks) {
+        set_error("V.Q residual: reduction workspace too small");
+        return HL_ERR_STATE;
+    }
+    vq_resid_kernel<<<nb, kBlock, 0, s>>>(n, Out, ld, res_col, r, qlast, beta, ctl, red.partials);
+    HL_TRY_HIP(hipGetLastError());
+    cgs_finalize_kernel<<<1, kBlock, 0, s>>>(nb, 0, red.partials, nullptr, nullptr, ctl, nullptr, nullptr, EPI_RNORM,
+                                             0);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
+
+int launch_basis_copy32(int n, int N, long long ld, const double *src, float *dst, hipStream_t s)
+{
+    if (n <= 0 || N <= 0) return HL_OK;
+    basis_copy32_kernel<<<grid_for((long long)n * ld, kBlock, 4096), kBlock, 0, s>>>(n, N, ld, src, dst);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

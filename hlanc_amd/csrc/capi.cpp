@@ -63,14 +63,18 @@ bool is_device_ptr(const void *p, int dev)
 }
 
 // Argument rules of ds_solver (HLanc.hpp:1649-1660) plus this library's
-// limits (ncv <= 128: the CGS kernels' column capacity).
+// limits (ncv <= kMaxNcv; the two-phase FP32 mode ncv <= kNarrowCols).
 bool solve_args_ok(int n, int nev, int ncv, int ldv, const double *values, bool need_vectors,
                    const double *vectors, const hl_options &o, bool device_op, const char *who)
 {
-    if (n <= 0 || nev <= 0 || ncv <= nev || ncv > n || ncv > 128 || !values ||
-        (need_vectors && !vectors) || (vectors && ldv < n)) {
-        set_error("%s: need 0 < nev < ncv <= min(n, 128) and ldv >= n (n=%d nev=%d ncv=%d ldv=%d)", who, n, nev,
-                  ncv, ldv);
+    if (n <= 0 || nev <= 0 || ncv <= nev || ncv > n || !values || (need_vectors && !vectors) ||
+        (vectors && ldv < n)) {
+        set_error("%s: need 0 < nev < ncv <= n and ldv >= n (n=%d nev=%d ncv=%d ldv=%d)", who, n, nev, ncv, ldv);
+        return false;
+    }
+    if (ncv > kMaxNcv || (o.basis_prec == HL_PREC_F32 && ncv > kNarrowCols)) {
+        set_error("%s: ncv=%d exceeds this library's basis width (%d; %d for basis_prec HL_PREC_F32)", who, ncv,
+                  kMaxNcv, kNarrowCols);
         return false;
     }
     if (o.basis_prec != HL_PREC_F64 && o.basis_prec != HL_PREC_F32 && o.basis_prec != HL_PREC_F64_PROJ32) {
@@ -641,7 +645,7 @@ int hl_tune_get(const char *key, int *value)
 int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq, double *Out,
           long long ldo, int use_mfma, void *stream)
 {
-    if (n < 0 || K <= 0 || N <= 0 || K > 128 || !V || !Q || !Out || ldv < n || ldo < n || ldq < K) {
+    if (n < 0 || K <= 0 || N <= 0 || K > kMaxNcv || !V || !Q || !Out || ldv < n || ldo < n || ldq < K) {
         set_error("hl_vq: bad arguments");
         return HL_ERR_ARG;
     }

@@ -712,7 +712,10 @@ int Engine::sweep(std::vector<Member *> &ms)
                 nsub[st][gb] += hM[i].nb;
                 nfin[st] += S.j + 2; // j + 1 projections and the squared norm
                 const int cpg = (S.j + 1 + dev::kGroups - 1) / dev::kGroups; // columns per lane group
-                cpg_proj[st][gp] = std::max(cpg_proj[st][gp], cpg);
+                // PROJ32 steps in the FP32 launches: the first pass is a column dot and a
+                // step wider than kNarrowCols corrects on the FP64 basis, so neither sets the width
+                const bool f32_wide = gp == 0 && k == SEG_PROJ32 && S.j + 1 > kNarrowCols;
+                cpg_proj[st][gp] = std::max(cpg_proj[st][gp], f32_wide ? 1 : cpg);
                 cpg_sub[st][gb] = std::max(cpg_sub[st][gb], cpg);
                 if (k == SEG_PROJ32) { // leads its cgs0s group: the exact corrections launch over these alone
                     ++n32[st];
@@ -899,12 +902,16 @@ int Engine::after_sweep(std::vector<Member *> &act)
     // options, phase and shape alone, so a solve gives the same bits in any
     // batch; each basis type is one launch_vq_seg call.
     int nfinish = 0, nrephase = 0;
-    std::vector<int> cat[4];
+    std::vector<int> cat[4], wide; // wide: bases beyond the segmented restart kernels, one launch_vq each
     for (size_t i = 0; i < nm; ++i) {
         const Member &m = *act[i];
         nfinish += m.act == Member::ACT_FINISH;
         nrephase += m.act == Member::ACT_REPHASE;
         if (m.act != Member::ACT_RESTART) continue;
+        if (m.ncv > kNarrowCols) {
+            wide.push_back(static_cast<int>(i));
+            continue;
+        }
         const bool mf = m.opts.use_mfma && vq_seg_mfma_fits(m.ncv, m.vq_N);
         const bool v32 = m.kind == SEG_V32 || m.kind == SEG_V32_A32;
         cat[(v32 ? 2 : 0) + (mf ? 1 : 0)].push_back(static_cast<int>(i));
@@ -943,7 +950,7 @@ int Engine::after_sweep(std::vector<Member *> &act)
             bytes += vb * m.n * (t.K + t.N) + 16.0 * m.n + (t.Out32 ? 4.0 * m.n * t.N : 0.0);
         }
     int rc;
-    if (nres > 0 || nfinish > 0 || nrephase > 0)
+    if (nres > 0 || nfinish > 0 || nrephase > 0 || !wide.empty())
         HL_TRY_HIP(hipMemcpyAsync(dQ, hQ, sizeof(double) * nm * qstride, hipMemcpyHostToDevice, stream));
     if (nres > 0) {
         HL_TRY_HIP(hipMemcpyAsync(dV, hV, sizeof(SegVQ) * nres, hipMemcpyHostToDevice, stream));
@@ -957,9 +964,24 @@ int Engine::after_sweep(std::vector<Member *> &act)
                                               stream)))
             return rc;
         if ((rc = prof.end(stream, K_VQ, bytes))) return rc;
-        for (Member *m : act)
-            if (m->act == Member::ACT_RESTART) m->ws.cur = 1 - m->ws.cur;
     }
+    for (int i : wide) { // the K-chunked V.Q, then the dsapps residual and the FP32 copy on their own
+        Member &m = *act[i];
+        Workspace &W = m.ws;
+        if ((rc = prof.begin(stream))) return rc;
+        if ((rc = launch_vq(m.n, m.ncv, m.vq_N, W.V[W.cur], W.vl(), dQ + i * qstride, m.ncv, W.V[1 - W.cur], W.vl(),
+                            m.opts.use_mfma, -1, nullptr, 0.0, 0.0, W.red, W.ctl, stream)))
+            return rc;
+        if ((rc = launch_vq_residual(m.n, W.V[1 - W.cur], W.vl().ld, m.vq_res, W.r, m.vq_qlast, m.vq_beta, W.red,
+                                     W.ctl, stream)))
+            return rc;
+        if (m.kind == SEG_PROJ32 &&
+            (rc = launch_basis_copy32(m.n, m.vq_N, W.vl().ld, W.V[1 - W.cur], W.V32[1 - W.cur], stream)))
+            return rc;
+        if ((rc = prof.end(stream, K_VQ, 8.0 * m.n * (m.ncv + m.vq_N) + 16.0 * m.n))) return rc;
+    }
+    for (Member *m : act)
+        if (m->act == Member::ACT_RESTART) m->ws.cur = 1 - m->ws.cur;
     for (size_t i = 0; i < nm && nrephase > 0; ++i) { // FP32 phase done: FP64 start vector V q, then restart
         Member &m = *act[i];
         if (m.act != Member::ACT_REPHASE) continue;

@@ -34,7 +34,9 @@ __device__ __forceinline__ void run_epilogue(int mode, int j, double nrm2, const
         ctl->rnorm = sqrt(nrm2);
         if (ctl->dgks_always) {
             ctl->need = 1;
-            ctl->exact = ctl->rnorm < kProjExact * ctl->wnorm;
+            // a strongly cancelling step, or one wider than the FP32 passes take,
+            // corrects on the FP64 basis
+            ctl->exact = ctl->rnorm < kProjExact * ctl->wnorm || j + 1 > kNarrowCols;
             if (ctl->exact) {
                 ctl->nexact += 1;
                 ctl->mxsum += j + 1;

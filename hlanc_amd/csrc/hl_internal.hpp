@@ -90,6 +90,11 @@ struct Tuning {
 };
 Tuning &tuning();
 constexpr int kMaxReduceBlocks = 4096;
+// Basis widths: a fused CGS pass holds up to kNarrowCols columns of a tile in
+// registers (32 per lane group); wider passes (up to kMaxNcv columns) run as a
+// subtracting kernel and a projecting kernel over 128-column chunks.
+constexpr int kNarrowCols = HL_MAX_NCV_F32;
+constexpr int kMaxNcv = HL_MAX_NCV;
 
 // ------------------------------------------------------------ basis layout
 // Storage of a basis V (n rows): column-major with leading dimension ld, or
@@ -308,6 +313,15 @@ struct SegRes {
 constexpr int kResBlocksMax = 64; // blocks per checked pair
 int res_blocks(int n);
 int launch_resid_seg(const SegRes *T, int nt, int blocks_total, hipStream_t s);
+
+struct Reduce;
+// Restarts of bases wider than kNarrowCols run V.Q through launch_vq (K-chunked)
+// and then these: the dsapps residual update r = qlast rscale r + beta
+// Out(:, res_col) with ctl->rnorm := ||r|| (res_col < 0: no beta term), and the
+// FP32 copy of the new basis's first N columns (tiled, ld columns per tile).
+int launch_vq_residual(int n, const double *Out, long long ld, int res_col, double *r, double qlast, double beta,
+                       Reduce &red, Ctl *ctl, hipStream_t s);
+int launch_basis_copy32(int n, int N, long long ld, const double *src, float *dst, hipStream_t s);
 
 // out = V(:, 0:K) q (tiled basis, ld K, FP64 or FP32; q on the device).
 int launch_basis_combine(int n, int K, const void *V, long long ld, bool v32, const double *q, double *out,

@@ -270,10 +270,17 @@ inline int finish(int rc)
 } // namespace detail
 
 // Extended form: tolerance, restart cap, start vector, precision; statistics.
+// A basis wider than this build takes (HL_MAX_NCV, HL_MAX_NCV_F32) is not a hard
+// error: the call warns and returns 1, as for a solve that did not converge.
 template <typename dspmv_operator_t>
 int ds_solver_gpu(int n, int nev, int ncv, double *values, double *vectors, int ldv,
                   dspmv_operator_t const &dspmv_operator, int dev, hl_options const &opts, hl_info *info = nullptr)
 {
+    const int width = opts.basis_prec == HL_PREC_F32 ? HL_MAX_NCV_F32 : HL_MAX_NCV;
+    if (ncv > width && ncv <= n && nev < ncv) {
+        warn("ds_solver_gpu: ncv = %d exceeds this build's basis width (%d); returning 1\n", ncv, width);
+        return 1;
+    }
     return detail::finish(detail::solve(detail::is_device_operator<dspmv_operator_t>(), n, nev, ncv, values, vectors,
                                         ldv, dspmv_operator, dev, &opts, info));
 }

@@ -45,6 +45,11 @@ extern "C" {
  * Gram-Schmidt passes read (see hl_options.basis_prec). */
 #define HL_PREC_F64_PROJ32 2
 
+/* Widest Krylov basis the solvers take (ncv); HL_PREC_F32 solves take at most
+ * HL_MAX_NCV_F32.  The reference takes any ncv <= n (HLanc.hpp:1649-1676). */
+#define HL_MAX_NCV 512
+#define HL_MAX_NCV_F32 128
+
 typedef struct hl_csr_s *hl_csr_t;
 
 /* Text of the last error seen on this host thread (never NULL). */

@@ -1,13 +1,18 @@
 """Wide Krylov bases on the GPU: ncv = 96 (the 27-point workload's nev 32 /
-ncv 96, and the nev-32 requests of the serving stream) and ncv = 128, in the
-FP64 basis mode and with the FP32 copy for the projection passes.
+ncv 96, and the nev-32 requests of the serving stream), ncv = 128, and bases
+wider than the fused kernels hold (ncv = 160, 256, 400), in the FP64 basis mode
+and with the FP32 copy for the projection passes.
 
-These solves run the widest kernel instances: the CGS passes over 65..128
-columns (24 and 32 columns per lane group; FP32 passes of 12 and 16 column
-pairs), and restarts whose kept block spans 3..5 output tiles of 16 columns on
-the MFMA V.Q kernel next to restarts whose block of Q no longer fits its LDS
-(the FMA kernel).  Each is checked against the CPU oracle (values to 1e-10,
-residuals, orthogonality), alone and bitwise inside a batch.
+Up to 128 columns these solves run the widest fused instances: the CGS passes
+over 65..128 columns (24 and 32 columns per lane group; FP32 passes of 12 and 16
+column pairs), and restarts whose kept block spans 3..5 output tiles of 16
+columns on the MFMA V.Q kernel next to restarts whose block of Q no longer fits
+its LDS (the FMA kernel).  Beyond 128 columns a CGS pass runs as a subtracting
+and a projecting kernel over 128-column chunks, restarts run a K-chunked V.Q
+with the residual update on its own, and the Ritz-vector step of ncv = 400 runs
+in two K-chunks.  Each is checked against the CPU oracle (values to 1e-10,
+residuals, orthogonality), alone and bitwise inside a batch; narrow solves keep
+their bits when a wide one shares their launches.
 
 The full-size workloads (Chung-Lu n = 4.1 M, nev 20 / ncv 60, and the 27-point
 stencil 128^3, nev 32 / ncv 96) are checked by size-independent properties:
@@ -31,7 +36,13 @@ CASES = {
     "st27_24_ncv96": (lambda: W.stencil27(24, seed=3), 32, 96),
     "st27_16_ncv128": (lambda: W.stencil27(16, seed=4), 48, 128),
     "er6000_ncv128": (lambda: W.erdos_renyi(6000, 12, seed=5), 40, 128),
+    # wider than the fused CGS pass (two kernels per pass, 128-column chunks) and
+    # than the segmented restart kernels (K-chunked V.Q; K > 384 on the Ritz step)
+    "st27_16_ncv160": (lambda: W.stencil27(16, seed=4), 48, 160),
+    "er4000_ncv256": (lambda: W.erdos_renyi(4000, 12, seed=6), 80, 256),
+    "st27_14_ncv400": (lambda: W.stencil27(14, seed=2), 100, 400),
 }
+WIDE = ["st27_16_ncv160", "er4000_ncv256", "st27_14_ncv400"]
 
 _mats = {}
 
@@ -71,7 +82,7 @@ def test_wide_basis_matches_oracle(name, prec):
     assert r.info.restarts >= 2  # restarts ran (the V.Q kernels under test)
 
 
-@pytest.mark.parametrize("name", ["st27_24_ncv96", "er6000_ncv128"])
+@pytest.mark.parametrize("name", ["st27_24_ncv96", "er6000_ncv128", "st27_16_ncv160", "st27_14_ncv400"])
 def test_wide_basis_fma_restart_agrees(name):
     rp, col, val, nev, ncv, o = case(name)
     A = sp.csr_matrix((val, col, rp))
@@ -103,6 +114,35 @@ def test_wide_bases_in_one_batch_are_bitwise_their_standalone_solves():
         assert b.info.restarts == a.info.restarts and b.info.spmv == a.info.spmv
     for i, m in enumerate(mats):
         np.testing.assert_allclose(alone[i].values, m[5].values, rtol=TOL)
+
+
+def test_narrow_solves_beside_a_wide_one_keep_their_bits():
+    """A batch step of more than 128 columns runs every solve of its launch
+    through the two-kernel pass; a narrow solve's sums are the fused pass's, so
+    it stays bitwise its standalone solve."""
+    rp, col, val, nev, ncv, o = case("st27_16_ncv160")
+    wide = op_for(rp, col, val)
+    narrow = [op_for(*W.stencil27(10, seed=9)), op_for(*W.erdos_renyi(3000, 10, seed=4))]
+    for prec in (H.HL_PREC_F64, H.HL_PREC_F64_PROJ32):
+        alone = [H.ds_solver_gpu(op.nrows, 8, 24, op, tol=TOL, maxiter=3000, seed=11 + i, basis_prec=prec)
+                 for i, op in enumerate(narrow)]
+        batch = H.ds_solve_batch(narrow + [wide], [8, 8, nev], [24, 24, ncv], tol=TOL, maxiter=3000,
+                                 seeds=[11, 12, 0], basis_prec=prec)
+        for a, b in zip(alone, batch[:2]):
+            assert a.ret == 0 and b.status == 0
+            np.testing.assert_array_equal(b.values, a.values)
+            np.testing.assert_array_equal(b.vectors, a.vectors)
+        assert batch[2].status == 0
+        np.testing.assert_allclose(batch[2].values, o.values, rtol=TOL)
+
+
+def test_basis_beyond_the_build_width_is_an_argument_error():
+    rp, col, val = W.stencil27(12, seed=1)
+    op = op_for(rp, col, val)
+    with pytest.raises(H.HLancError):
+        H.ds_solver_gpu(op.nrows, 100, 600, op)
+    with pytest.raises(H.HLancError):  # the two-phase FP32 mode stops at 128
+        H.ds_solver_gpu(op.nrows, 50, 160, op, basis_prec=H.HL_PREC_F32)
 
 
 # ------------------------------------------------------------ full size
