@@ -5,24 +5,27 @@ the dominant kernel and the CPU oracle timed on a bounded sample beside it.
 Default workload (one "step" = one full eigensolve): normalized Laplacian of a
 symmetric Chung-Lu power-law graph (4,194,304 vertices, exponent 2.1, mean
 degree 16, max expected degree 2e4, weights U(0,1], giant component kept),
-FP64 CSR; nev = 20, ncv = 60, largest algebraic, tol 1e-10.
+FP64 CSR; nev = 20, ncv = 60, largest algebraic, tol 1e-10; the FP64 basis with
+its FP32 copy for the projection passes, Rayleigh-quotient values.  Every
+returned pair is checked by the library (true FP64 residual); a run whose worst
+residual exceeds 1e-10 prints its line and exits with status 3.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload chunglu|stencil27|laplace2d]
+The default run also reports, as sub-objects of the same line:
 
-Two more workloads measure the batched and served forms (not part of the
-default run):
+* ``batch``: 512 independent symmetric matrices, n log-uniform in [1e3, 2e5]
+  (a quarter 2-D 5-point stencils, a quarter 3-D 27-point stencils, half
+  Erdos-Renyi graphs of degree 8..32 with weights U(-1,1) and a diagonal
+  shift), nev = 8, ncv = 24: all solved as one continuously batched call in
+  the FP32-copy mode, against the plain FP64 ds_solver_gpu path one solve at a
+  time on the same GPU and matrices; eigensolves/s, p50/p99 completion.
+* ``serve``: a Poisson stream of requests submitted in real time to the
+  serving loop (hl_serve_*) at 50 and 200 requests/s, drawn from a pool of 32
+  matrices with n log-uniform in [4e3, 2e6] (27-point stencils, Chung-Lu
+  Laplacians, Erdos-Renyi graphs), nev uniform in {4, 8, 16, 32}, ncv = 3 nev;
+  each request's operator uploaded after admission; eigensolves/s and p50/p99
+  latency from submission to result.
 
-* ``--workload batch``: 512 independent symmetric matrices, n log-uniform in
-  [1e3, 2e5] (a quarter 2-D 5-point stencils, a quarter 3-D 27-point stencils,
-  half Erdos-Renyi graphs of degree 8..32 with weights U(-1,1) and a diagonal
-  shift), nev = 8, ncv = 24.  One step solves all 512: once one solve at a time
-  (the plain ds_solver_gpu path) and once as one batch (hl_ds_solve_batch); the
-  value is the batched eigensolves/s.
-* ``--workload serve``: Poisson arrivals at each of ``--rates`` requests/s of solves
-  drawn from a pool of matrices with n log-uniform in [4e3, 2e6], nev uniform
-  in {4, 8, 16, 32}, ncv = 3 nev; continuous batching under an HBM budget,
-  against first-come-first-served one at a time on the same arrival stream;
-  reports eigensolves/s and p50/p99 latency from arrival to result.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload chunglu|stencil27|laplace2d|batch|serve]
 """
 from __future__ import annotations
 
@@ -38,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+RESID_BAR = 1e-10
 WORKLOADS = {
     # name: (generator, kwargs, nev, ncv, label)
     "chunglu": ("chung_lu_laplacian", {"n": 1 << 22, "seed": 0}, 20, 60,
@@ -46,9 +50,10 @@ WORKLOADS = {
     "laplace2d": ("laplacian_2d", {"m": 256}, 10, 30, "2-D 5-point Laplacian 256x256 (Dirichlet)"),
 }
 
-
-BASIS_LABEL = {"fp64": "fp64", "fp32": "mixed: fp32 basis and values to 1e-6, then fp64 to tol",
+BASIS_LABEL = {"fp64": "fp64 basis, the reference's DGKS rule",
+               "fp32": "two phases: fp32 basis and values to 1e-6, then fp64 to tol",
                "proj32": "fp64 basis + fp32 copy for the projection passes (DGKS correction every step)"}
+DTYPE = {"fp64": "f64", "fp32": "f32 phase + f64", "proj32": "f64 (fp32 basis copy for projections)"}
 
 
 def basis_prec(H, basis):
@@ -63,22 +68,41 @@ def percentile(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else float("nan")
 
 
-def residuals(op, values, vectors, n):
-    """max_i ||A v_i - l_i v_i|| / ||A||, with ||A|| estimated by max |l_i| (a lower
-    bound, so the ratio is conservative), computed with the GPU SpMV."""
+def host_threads():
+    """Threads the CPU baseline uses: the box's CPU share (OMP_NUM_THREADS, else
+    the affinity mask), with what the host reports beside it."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", aff) or aff)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, min(share, aff)), {"nproc_affinity": aff, "os_cpu_count": os.cpu_count(),
+                                     "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
+
+
+def device_check(op, values, vecs_dev, nev):
+    """Cross-check of the library's residual check: max_i ||A v_i - l_i v_i|| /
+    max|l| with the GPU SpMV, and max |V^T V - I|."""
     import torch
 
-    V = torch.tensor(np.ascontiguousarray(vectors[:, : len(values)].T), device="cuda")
+    V = vecs_dev[:, :nev].T.contiguous()
     AV = torch.empty_like(V)
-    for i in range(len(values)):
+    for i in range(nev):
         op.spmv(V[i].data_ptr(), AV[i].data_ptr())
     torch.cuda.synchronize()
-    lam = torch.tensor(values, device="cuda")[:, None]
-    r = torch.linalg.norm(AV - lam * V, dim=1).cpu().numpy()
-    nrm = float(np.max(np.abs(values)))
-    Vh = np.ascontiguousarray(vectors[:, : len(values)])
-    ortho = float(np.abs(Vh.T @ Vh - np.eye(len(values))).max())  # on the host: a skinny product
-    return float(r.max() / nrm), ortho
+    lam = torch.tensor(values, device=V.device, dtype=torch.float64)[:, None]
+    res = torch.linalg.norm(AV - lam * V, dim=1).max().item() / float(np.max(np.abs(values)))
+    ortho = (V @ V.T - torch.eye(nev, device=V.device, dtype=torch.float64)).abs().max().item()
+    return res, ortho
 
 
 def spmv_timing(op, n, reps=50):
@@ -102,13 +126,13 @@ def spmv_timing(op, n, reps=50):
 
 
 def cpu_baseline(rp, col, val, nev, ncv, tol, gpu_spmv_per_solve):
-    """The CPU oracle (oracle/hlanc_oracle.c, a scalar-per-thread restatement of
-    ds_solver_mkl) on a bounded sample: its first restart cycle on the same matrix
-    (ncv Lanczos steps + one restart), extrapolated to a full solve by the number
-    of Lanczos steps the GPU solve needed."""
+    """The CPU oracle (oracle/hlanc_oracle.c: ds_solver_mkl restated with OpenMP
+    BLAS-2) on a bounded sample: its first restart cycle on the same matrix
+    (ncv Lanczos steps + one restart), scaled to a full solve by the number of
+    Lanczos steps the GPU solve needed."""
     import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, hostinfo = host_threads()
     A = O.Csr(rp, col, val)
     t = time.perf_counter()
     r = O.ds_solver(A, nev, ncv, tol=tol, maxiter=1, nthreads=threads, max_restarts_then_stop=1)
@@ -116,29 +140,19 @@ def cpu_baseline(rp, col, val, nev, ncv, tol, gpu_spmv_per_solve):
     steps = max(int(r.spmv), 1)
     per_solve = dt * gpu_spmv_per_solve / steps
     return {"value": 1.0 / per_solve, "unit": "eigensolves/s", "cores": threads, "kind": "port",
-            "sample": f"oracle first restart cycle ({steps} Lanczos steps, {dt:.1f} s on {threads} threads), "
-                      f"scaled to the {gpu_spmv_per_solve} steps of the GPU solve"}
-
-
-def check_pairs(ops, results, nevs):
-    """max over all returned pairs of ||A v - l v|| / max|l| (GPU SpMV)."""
-    worst = 0.0
-    for op, r, nev in zip(ops, results, nevs):
-        if r.vectors is None:
-            continue
-        res, _ = residuals(op, r.values[:nev], r.vectors, op.nrows)
-        worst = max(worst, res)
-    return worst
+            "sample": f"oracle first restart cycle ({steps} Lanczos steps, {dt:.1f} s on {threads} threads, the "
+                      f"box's CPU share), scaled to the {gpu_spmv_per_solve} steps of the GPU solve",
+            "host": hostinfo}
 
 
 def batch_cpu_baseline(host, sample, results, ns, nnzs, nev, ncv, tol, maxiter, budget_s=20.0):
     """The CPU oracle's full solves of a sample of the mix (smallest first, until
-    about budget_s seconds are spent), extrapolated to the whole mix with a per-solve
-    cost model: Lanczos steps x (nnz + 3 ncv n / 2), the SpMV and CGS work of a
-    step, using each matrix's step count from the GPU run."""
+    about budget_s seconds are spent), scaled to the whole mix by a per-solve
+    cost model: Lanczos steps x (nnz + 3 ncv n / 2), using each matrix's step
+    count from the GPU run."""
     import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, hostinfo = host_threads()
     cost = [r.info.spmv * (nnzs[i] + 1.5 * ncv * ns[i]) for i, r in enumerate(results)]
     t_cpu, c_done, done = 0.0, 0.0, 0
     for i in sorted(sample, key=lambda k: ns[k]):
@@ -154,7 +168,7 @@ def batch_cpu_baseline(host, sample, results, ns, nnzs, nev, ncv, tol, maxiter, 
     return {"value": len(results) / total, "unit": "eigensolves/s", "cores": threads, "kind": "port",
             "sample": f"oracle full solves of {done} matrices of the mix at evenly spaced size ranks, "
                       f"smallest first ({t_cpu:.1f} s on {threads} threads), scaled to all "
-                      f"{len(results)} by Lanczos steps x (nnz + 1.5 ncv n)"}
+                      f"{len(results)} by Lanczos steps x (nnz + 1.5 ncv n)", "host": hostinfo}
 
 
 def sell_where_even(H, ops, nnzs, limit=1.1):
@@ -173,176 +187,344 @@ def sell_where_even(H, ops, nnzs, limit=1.1):
     return taken
 
 
-def run_batch(args, H, torch):
+def gen_threads():
+    return max(1, min(16, host_threads()[0]))
+
+
+# ------------------------------------------------------------------ batch
+
+def run_batch(H, torch, count=512, steps=1, warmup=1, basis="proj32", tol=1e-10, maxiter=3000, max_active=0,
+              mfma=None, autotune=True, plain=True, cpu=True, profile=False):
     from hlanc_amd import workloads as W
 
-    nev, ncv, count = 8, 24, args.count
+    nev, ncv = 8, 24
     t0 = time.perf_counter()
-    mix = W.batch_mix(count, seed=0)
-    mixed = args.basis == "fp32"
+    mix = W.batch_mix(count, seed=0, threads=gen_threads())
+    mixed = basis == "fp32"
     ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=m[1][0], col=m[1][1], val=m[1][2]), fp32_values=mixed)
            for m in mix]
     ns = [op.nrows for op in ops]
     nnzs = [len(m[1][1]) for m in mix]
     nnz = sum(nnzs)
     # SpMV form per operator (the batched and the plain path share the operators)
-    n_sell = 0 if args.no_autotune or mixed else sell_where_even(H, ops, nnzs)
-    # host copies of 16 matrices at evenly spaced size ranks, for the CPU oracle's sample
+    n_sell = sell_where_even(H, ops, nnzs) if autotune and not mixed else 0
     sample = sorted(range(count), key=lambda i: ns[i])[:: max(1, count // 16)]
     host = {i: mix[i][1] for i in sample}
     del mix
     log(f"workload batch: {count} matrices, sum n {sum(ns)}, sum nnz {nnz}, built in "
         f"{time.perf_counter() - t0:.1f} s; {n_sell} operators in the sliced-ELL form")
     torch.cuda.init()
-    kw = dict(tol=args.tol, maxiter=args.maxiter, use_mfma=args.mfma)
-    prec = basis_prec(H, args.basis)
-
+    kw = dict(tol=tol, maxiter=maxiter, use_mfma=mfma)
+    prec = basis_prec(H, basis)
+    refine = H.HL_REFINE_RQ
     vec_out = [H.pinned_empty(op.nrows, ncv) for op in ops]  # page-locked outputs, allocated once
 
     def batched():
-        return H.ds_solve_batch(ops, nev, ncv, max_active=args.max_active, vectors_out=vec_out, basis_prec=prec,
-                                **kw)
+        return H.ds_solve_batch(ops, nev, ncv, max_active=max_active, vectors_out=vec_out, basis_prec=prec,
+                                refine=refine, **kw)
 
-    def plain():  # the plain FP64 ds_solver_gpu path, one solve after another
-        out = []
-        for op, v in zip(ops, vec_out):
-            out.append(H.ds_solver_gpu(op.nrows, nev, ncv, op, vectors_out=v, basis_prec=H.HL_PREC_F64, **kw))
-        return out
-
-    for i in range(args.warmup):
+    for _ in range(warmup):
         batched()
-    walls_b, walls_p = [], []
-    for i in range(args.steps):
+    walls_b = []
+    for i in range(steps):
         torch.cuda.synchronize()
         t = time.perf_counter()
         rb = batched()
         walls_b.append(time.perf_counter() - t)
         log(f"batched step {i}: {walls_b[-1]:.3f} s")
-    if args.no_plain:
-        rp, walls_p = None, [float("nan")]
-    else:
+    fails = sum(r.status != 0 for r in rb)
+    worst = max((r.info.max_rel_resid for r in rb if r.status == 0), default=float("nan"))
+    unchecked = sum(r.status == 0 and r.info.verified == 0 for r in rb)
+    vals_b = [r.values.copy() for r in rb]
+    lat = [r.t_done for r in rb]
+    rp_res, plain_value, err64 = None, None, None
+    if plain:  # the plain FP64 ds_solver_gpu path, one solve after another, same page-locked outputs
         t = time.perf_counter()
-        rp = plain()
-        walls_p.append(time.perf_counter() - t)
-        log(f"plain one-at-a-time: {walls_p[-1]:.3f} s")
+        rp_res = [H.ds_solver_gpu(op.nrows, nev, ncv, op, vectors_out=v, basis_prec=H.HL_PREC_F64, **kw)
+                  for op, v in zip(ops, vec_out)]
+        wall_p = time.perf_counter() - t
+        plain_value = count / wall_p
+        log(f"plain one-at-a-time FP64: {wall_p:.3f} s")
+        err64 = max(float(np.max(np.abs(a - b.values) / np.abs(b.values))) for a, b in zip(vals_b, rp_res))
     classes = None
-    if args.profile:
+    if profile:
         import ctypes
 
-        H.ds_solve_batch(ops, nev, ncv, max_active=args.max_active, flags=H.HL_FLAG_PROFILE, basis_prec=prec, **kw)
+        H.ds_solve_batch(ops, nev, ncv, max_active=max_active, flags=H.HL_FLAG_PROFILE, basis_prec=prec, **kw)
         pr = H.Profile()
         H.lib().hl_profile_last(ctypes.byref(pr))
         classes = {H.PROFILE_CLASSES[k]: {"launches": int(pr.launches[k]), "seconds": pr.seconds[k],
                                           "GBps": pr.bytes[k] / pr.seconds[k] / 1e9 if pr.seconds[k] else None}
                    for k in range(len(H.PROFILE_CLASSES)) if pr.launches[k]}
         log("profiled batch: " + ", ".join(f"{k} {v['launches']}x {v['seconds']:.3f}s" for k, v in classes.items()))
-    fails = sum(r.status != 0 for r in rb)
-    worst = check_pairs(ops, rb, [nev] * count)  # before the plain run's vectors overwrite the buffers
-    same = None  # bitwise identity with the plain solves holds for the same basis mode (FP64)
-    if rp is not None and args.basis == "fp64":
-        same = all(np.array_equal(a.values, b.values) and a.status == b.ret for a, b in zip(rb, rp))
-    err64 = None
-    if rp is not None:  # each batched value against the plain FP64 solve's
-        err64 = max(float(np.max(np.abs(a.values - b.values) / np.abs(b.values))) for a, b in zip(rb, rp))
-    lat = [r.t_done for r in rb]
     value = count / float(np.mean(walls_b))
-    plain_value = count / walls_p[0]
-    log(f"batched {value:.1f} solves/s vs plain {plain_value:.1f} solves/s ({value / plain_value:.2f}x); "
-        f"failures {fails}; bitwise same as plain: {same}; max rel value error vs plain FP64 {err64}; "
-        f"max rel residual {worst:.2e}")
+    log(f"batched {value:.1f} solves/s" + (f" vs plain FP64 {plain_value:.1f} solves/s ({value / plain_value:.2f}x)"
+                                          if plain_value else "") +
+        f"; failures {fails}; max rel value error vs plain FP64 {err64}; max rel residual (library) {worst:.2e}")
     restarts = [r.info.restarts for r in rb]
-    cpu = None
-    if not args.no_cpu:
-        cpu = batch_cpu_baseline(host, sample, rb, ns, nnzs, nev, ncv, args.tol, args.maxiter)
-        log(f"cpu baseline: {cpu}")
-    return {
+    cpu_rec = batch_cpu_baseline(host, sample, rb, ns, nnzs, nev, ncv, tol, maxiter) if cpu else None
+    if cpu_rec:
+        log(f"batch cpu baseline: {cpu_rec}")
+    out = {
         "metric": "eigensolves/s at rel.resid<=1e-10 (+ p50/p99 latency)",
-        "value": value, "unit": "eigensolves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * float(np.mean(walls_b)), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic (seeded generator)",
+        "value": value, "unit": "eigensolves/s", "ms_per_step": 1e3 * float(np.mean(walls_b)), "steps": steps,
+        "warmup": warmup, "dtype": DTYPE[basis],
         "config": {"workload": f"{count} uneven matrices, n log-uniform [1e3, 2e5], stencils + Erdos-Renyi",
-                   "sum_n": int(sum(ns)), "sum_nnz": int(nnz), "nev": nev, "ncv": ncv, "tol": args.tol,
-                   "max_active": args.max_active, "basis": BASIS_LABEL[args.basis],
-                   "sliced_ell_operators": n_sell},
-        "plain_value": plain_value, "plain_basis": "fp64", "speedup_vs_plain": value / plain_value,
+                   "sum_n": int(sum(ns)), "sum_nnz": int(nnz), "nev": nev, "ncv": ncv, "tol": tol,
+                   "max_active": max_active, "basis": BASIS_LABEL[basis], "values": "Rayleigh quotients",
+                   "sliced_ell_operators": n_sell, "vectors_out": "page-locked host memory"},
+        "plain_value": plain_value, "plain_basis": "fp64 (the reference's DGKS rule), one solve at a time",
+        "speedup_vs_plain": (value / plain_value) if plain_value else None,
         "max_rel_value_err_vs_fp64": err64,
         "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
-        "bitwise_same_as_plain": same, "failures": int(fails), "max_rel_resid": worst,
+        "failures": int(fails), "max_rel_resid": worst, "unchecked_pairs": int(unchecked),
+        "resid_bar_met": bool(fails == 0 and unchecked == 0 and worst <= RESID_BAR),
         "restarts": {"min": int(min(restarts)), "median": float(np.median(restarts)), "max": int(max(restarts))},
         "kernel_classes": classes,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_rec,
     }
+    for op in ops:
+        op.close()
+    return out
 
 
-def run_serve(args, H, torch):
+# ------------------------------------------------------------------ serve
+
+def run_serve(H, torch, rates=(50.0, 200.0), requests=48, pool_size=32, seed=0, basis="proj32", tol=1e-10,
+              maxiter=3000, hbm_budget=64e9, max_active=0, fcfs=False):
+    """Requests submitted in real time to the serving loop at each offered rate
+    (one Poisson sequence scaled to the rate); each request uploads its matrix
+    after admission and returns its values and per-pair residuals (its Ritz
+    vectors are formed and checked on the device, not copied out)."""
     from hlanc_amd import workloads as W
 
     t0 = time.perf_counter()
-    pool = W.serve_mix(args.pool, seed=0)
-    mixed = args.basis == "fp32"
-    prec = basis_prec(H, args.basis)
-    ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=m[1][0], col=m[1][1], val=m[1][2]), fp32_values=mixed)
-           for m in pool]
-    nnzs = [len(m[1][1]) for m in pool]
-    nnz = sum(nnzs)
-    del pool
-    n_sell = 0 if args.no_autotune or mixed else sell_where_even(H, ops, nnzs)
-    log(f"workload serve: pool of {len(ops)} matrices, sum n {sum(op.nrows for op in ops)}, sum nnz {nnz}, "
-        f"built in {time.perf_counter() - t0:.1f} s; {n_sell} operators in the sliced-ELL form")
+    pool = W.serve_mix(pool_size, seed=0, threads=gen_threads())
+    arrays = [m[1] for m in pool]
+    nnz = sum(len(a[1]) for a in arrays)
+    log(f"workload serve: pool of {len(arrays)} matrices, sum n {sum(len(a[0]) - 1 for a in arrays)}, sum nnz {nnz}, "
+        f"built in {time.perf_counter() - t0:.1f} s")
     torch.cuda.init()
-    rng = np.random.default_rng(args.seed)
-    R = args.requests
-    gaps = rng.exponential(1.0, R)  # one request sequence, scaled to each offered rate
-    pick = rng.integers(0, len(ops), R)
-    nevs = rng.choice([4, 8, 16, 32], R)
-    req_ops = [ops[k] for k in pick]
-    H.ds_solve_batch(req_ops[:2], list(nevs[:2]), list(3 * nevs[:2]), tol=args.tol, maxiter=args.maxiter,
-                     basis_prec=prec)
+    prec = basis_prec(H, basis)
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0, requests)
+    pick = rng.integers(0, len(arrays), requests)
+    nevs = rng.choice([4, 8, 16, 32], requests)
+    kw = dict(tol=tol, maxiter=maxiter, basis_prec=prec, refine=H.HL_REFINE_RQ, vectors=False)
 
-    def serve(arrivals, max_active, budget):
-        t = time.perf_counter()
-        res = H.ds_solve_batch(req_ops, list(nevs), list(3 * nevs), max_active=max_active, hbm_budget=budget,
-                               tol=args.tol, maxiter=args.maxiter, arrivals=arrivals, vectors=True,
-                               basis_prec=prec)
-        return res, time.perf_counter() - t
-
-    rates = [float(x) for x in str(args.rates).split(",")]
-    per_rate, worst, same, fails = [], 0.0, True, 0
-    for rate in rates:
+    def stream(rate, cap):
         arrivals = np.cumsum(gaps) / rate
-        rs, wall_s = serve(arrivals, args.max_active, args.hbm_budget)
-        worst = max(worst, check_pairs(req_ops, rs, list(nevs)))
-        for r in rs:
-            r.vectors = None  # host memory: the stream's vectors are checked, then dropped
-        rf, wall_f = serve(arrivals, 1, 0.0)  # first come, first served, one at a time
-        for r in rf:
-            r.vectors = None
-        lat_s = [r.latency for r in rs]
-        lat_f = [r.latency for r in rf]
-        same &= all(np.array_equal(a.values, b.values) and a.status == b.status for a, b in zip(rs, rf))
-        fails += sum(r.status != 0 for r in rs)
-        rec = {"offered_req_per_s": rate, "value": R / wall_s,
-               "latency_s": {"p50": percentile(lat_s, 50), "p99": percentile(lat_s, 99)},
-               "fcfs": {"value": R / wall_f, "latency_s": {"p50": percentile(lat_f, 50),
-                                                          "p99": percentile(lat_f, 99)}}}
+        with H.Server(max_active=cap, hbm_budget=hbm_budget, sell_even=True) as srv:
+            tickets = []
+            start = time.perf_counter()
+            for k in range(requests):
+                wait = start + arrivals[k] - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+                tickets.append(srv.submit(arrays[pick[k]], int(nevs[k]), 3 * int(nevs[k]), **kw))
+            outs = [srv.wait(t) for t in tickets]
+            st = srv.stats()
+        t_first = min(o.t_submit for o in outs)
+        wall = max(o.t_done for o in outs) - t_first
+        return outs, wall, st
+
+    with H.Server() as srv:  # warm-up: code objects loaded, allocator warm
+        srv.wait(srv.submit(arrays[0], 4, 12, **kw))
+    per_rate, worst, fails, unchecked = [], 0.0, 0, 0
+    values_by_rate = []
+    for rate in rates:
+        outs, wall, st = stream(rate, max_active)
+        lat = [o.latency for o in outs]
+        fails += sum(o.status != 0 for o in outs)
+        unchecked += sum(o.status == 0 and o.info.verified == 0 for o in outs)
+        worst = max([worst] + [o.info.max_rel_resid for o in outs if o.status == 0])
+        values_by_rate.append([o.values for o in outs])
+        rec = {"offered_req_per_s": rate, "value": requests / wall,
+               "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+               "queue_wait_s": {"p50": percentile([o.t_admit - o.t_submit for o in outs], 50),
+                                "p99": percentile([o.t_admit - o.t_submit for o in outs], 99)},
+               "peak_footprint_gb": max(o.footprint for o in outs) / 1e9}
+        if fcfs:  # one request in flight at a time, same stream
+            fo, fwall, _ = stream(rate, 1)
+            flat = [o.latency for o in fo]
+            rec["fcfs"] = {"value": requests / fwall, "latency_s": {"p50": percentile(flat, 50),
+                                                                    "p99": percentile(flat, 99)}}
+            same = all(np.array_equal(a.values, b.values) for a, b in zip(outs, fo))
+            rec["same_results_as_fcfs"] = bool(same)
         per_rate.append(rec)
-        log(f"rate {rate}: served {rec['value']:.2f} solves/s p50 {rec['latency_s']['p50']:.3f} s p99 "
-            f"{rec['latency_s']['p99']:.3f} s; FCFS {rec['fcfs']['value']:.2f} solves/s p50 "
-            f"{rec['fcfs']['latency_s']['p50']:.3f} s p99 {rec['fcfs']['latency_s']['p99']:.3f} s")
-    log(f"same results served and FCFS: {same}; failures {fails}; max rel residual {worst:.2e}")
+        log(f"serve at {rate:g} req/s: {rec['value']:.2f} solves/s, latency p50 {rec['latency_s']['p50']:.3f} s "
+            f"p99 {rec['latency_s']['p99']:.3f} s" + (f"; FCFS {rec['fcfs']['value']:.2f} solves/s p50 "
+                                                     f"{rec['fcfs']['latency_s']['p50']:.3f} s" if fcfs else ""))
+    # each result is the same whatever the arrival rate (so whatever ran beside it)
+    same_across = all(np.array_equal(a, b) for vr in values_by_rate[1:] for a, b in zip(values_by_rate[0], vr))
     top = per_rate[-1]
     return {
         "metric": "eigensolves/s at rel.resid<=1e-10 (+ p50/p99 latency)",
-        "value": top["value"], "unit": "eigensolves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": 1e3 * R / top["value"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+f64" if mixed else "f64", "data": "synthetic (seeded generator)",
-        "config": {"workload": f"Poisson stream, {R} requests, pool of {len(ops)} matrices n log-uniform "
-                               "[4e3, 2e6], nev in {4,8,16,32}, ncv = 3 nev",
-                   "offered_req_per_s": rates, "tol": args.tol, "max_active": args.max_active,
-                   "hbm_budget_bytes": args.hbm_budget, "basis": BASIS_LABEL[args.basis],
-                   "sliced_ell_operators": n_sell},
-        "latency_s": top["latency_s"], "fcfs": top["fcfs"], "rates": per_rate,
-        "same_results_as_fcfs": bool(same), "failures": int(fails), "max_rel_resid": worst,
+        "value": top["value"], "unit": "eigensolves/s", "dtype": DTYPE[basis],
+        "config": {"workload": f"Poisson stream, {requests} requests per rate, submitted in real time; pool of "
+                               f"{len(arrays)} matrices n log-uniform [4e3, 2e6], nev in {{4,8,16,32}}, ncv = 3 nev",
+                   "offered_req_per_s": list(rates), "tol": tol, "max_active": max_active,
+                   "hbm_budget_bytes": hbm_budget, "basis": BASIS_LABEL[basis], "values": "Rayleigh quotients",
+                   "results": "values + per-pair residuals (Ritz vectors formed and checked on the device)",
+                   "operators": "uploaded per request after admission (sliced-ELL where rows are even)"},
+        "latency_s": top["latency_s"], "rates": per_rate, "same_results_across_rates": bool(same_across),
+        "failures": int(fails), "max_rel_resid": worst, "unchecked_pairs": int(unchecked),
+        "resid_bar_met": bool(fails == 0 and unchecked == 0 and worst <= RESID_BAR),
+    }
+
+
+# ------------------------------------------------------------ single matrix
+
+def run_single(args, H, torch):
+    from hlanc_amd import workloads as W
+
+    gen, kw, nev, ncv, label = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    rp, col, val = getattr(W, gen)(**kw)
+    n, nnz = len(rp) - 1, len(val)
+    log(f"workload {args.workload}: n={n} nnz={nnz} generated in {time.perf_counter() - t0:.1f} s")
+    mixed = args.basis == "fp32"
+    prec = basis_prec(H, args.basis)
+    refine = H.HL_REFINE_NONE if args.basis == "fp64" else H.HL_REFINE_RQ
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val), fp32_values=mixed)
+    torch.cuda.init()
+    items = None
+    if not args.no_autotune:
+        nc, rc_, t_best = op.autotune(reps=10)
+        sw, npairs = op.slices()
+        sell_on, sell_entries = op.sell()
+        items = {"nnz_cap": nc, "rows_cap": rc_, "slice_columns": sw, "virtual_rows": npairs,
+                 "sliced_ell": sell_on, "sliced_ell_entries": sell_entries}
+        form = (f"sliced-ELL ({sell_entries} entries)" if sell_on else
+                f"items of <= {nc} nnz / <= {rc_} rows, column slices of {sw} ({npairs} virtual rows)")
+        log(f"SpMV autotune: {form}, {t_best * 1e6:.1f} us per launch")
+
+    # timed solves write their vectors to device memory (the result stays in HBM);
+    # one more solve into page-locked host memory gives the rate with the PCIe copy
+    vec_dev = torch.empty((ncv, n), dtype=torch.float64, device="cuda").T
+    solve = lambda out, p=prec, rf=refine: H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter,  # noqa
+                                                           vectors_out=out, use_mfma=args.mfma, basis_prec=p,
+                                                           refine=rf)
+    for i in range(args.warmup):
+        r = solve(vec_dev)
+        log(f"warmup {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} {r.info.seconds:.3f} s")
+    lat, worst = [], 0.0
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        t = time.perf_counter()
+        r = solve(vec_dev)
+        lat.append(time.perf_counter() - t)
+        worst = max(worst, r.info.max_rel_resid if r.ret == 0 else float("inf"))
+        log(f"solve {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} "
+            f"(fp32 phase {r.info.spmv_f32}) {lat[-1]:.3f} s, max rel residual {r.info.max_rel_resid:.2e}")
+        if r.ret != 0:
+            raise SystemExit(f"solve {i} did not converge (ret {r.ret})")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    value = args.steps / wall
+
+    vec_host = H.pinned_empty(n, ncv)
+    t = time.perf_counter()
+    rh = solve(vec_host)
+    t_pcie = time.perf_counter() - t
+    log(f"solve into page-locked host memory: {t_pcie:.3f} s ({8.0 * n * ncv / 1e9:.2f} GB of vectors over PCIe)")
+    if not np.array_equal(rh.vectors, r.vectors.cpu().numpy()) or not np.array_equal(rh.values, r.values):
+        raise SystemExit("host-output solve differs from the device-output solve")
+    del vec_host, rh
+    resid_x, ortho = device_check(op, r.values, vec_dev, nev)
+    log(f"library max rel residual {r.info.max_rel_resid:.2e} (||A||_est {r.info.anorm_est:.6g}); cross-check "
+        f"{resid_x:.2e}; orthogonality {ortho:.2e}; Rayleigh-quotient shift {r.info.rq_shift:.2e}")
+    fp64 = None
+    if args.basis != "fp64":  # the same solve in the reference's FP64 mode, vectors to HBM as well (untimed)
+        t = time.perf_counter()
+        r64 = solve(vec_dev, H.HL_PREC_F64, H.HL_REFINE_NONE)
+        t64 = time.perf_counter() - t
+        err64 = float(np.max(np.abs(r.values - r64.values) / np.abs(r64.values)))
+        fp64 = {"seconds": t64, "spmv": int(r64.info.spmv), "restarts": int(r64.info.restarts),
+                "max_rel_resid": r64.info.max_rel_resid, "max_rel_value_err": err64,
+                "speedup_of_headline": t64 / float(np.mean(lat))}
+        log(f"FP64 mode: {t64:.3f} s ({r64.info.spmv} SpMV), max rel value error of the headline vs it {err64:.2e}")
+
+    # per-kernel-class profile of one more solve (HIP events around each launch on the engine's stream)
+    import ctypes
+
+    o = H.options(args.tol, args.maxiter, use_mfma=args.mfma, basis_prec=prec, refine=refine)
+    o.flags = H.HL_FLAG_PROFILE
+    values = np.zeros(ncv)
+    info = H.Info()
+    vptr, ldv = H._vectors_args(vec_dev)
+    rc = H.lib().hl_ds_solve_csr(op.h, nev, ncv, values.ctypes.data_as(H._dp), vptr, ldv, ctypes.byref(o),
+                                 ctypes.byref(info))
+    if rc < 0:
+        raise SystemExit(f"profiled solve failed: {rc} {H.lib().hl_last_error().decode()}")
+    del vec_dev
+    pr = H.Profile()
+    H.lib().hl_profile_last(ctypes.byref(pr))
+    names = H.PROFILE_CLASSES
+    classes = {names[k]: {"launches": int(pr.launches[k]), "seconds": pr.seconds[k], "bytes": pr.bytes[k],
+                          "GBps": (pr.bytes[k] / pr.seconds[k] / 1e9) if pr.seconds[k] > 0 else None}
+               for k in range(len(names)) if pr.launches[k] > 0}
+    dom = max((0, 1, 2, 4), key=lambda k: pr.seconds[k])  # kernel classes with algorithmic bytes
+    log(f"profile (ret {rc}): " + ", ".join(f"{k} {v['launches']}x {v['seconds']:.3f}s {v['GBps'] or 0:.0f} GB/s"
+                                           for k, v in classes.items()))
+    achieved = pr.bytes[dom] / pr.seconds[dom] / 1e9
+    traffic = None  # measured HBM bytes per dispatch of the dominant kernel (rocprofv3 PMC runs)
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        key = args.workload + ("" if args.basis == "fp64" else "_" + args.basis)
+        rec = json.load(open(tpath)).get(key, {}).get(names[dom])
+        if rec:
+            traffic = rec["fetch_bytes"] + rec["write_bytes"]
+
+    t_spmv = spmv_timing(op, n)
+    spmv_bytes = op.spmv_bytes()
+    spmv_gbps = spmv_bytes / t_spmv / 1e9
+    log(f"SpMV {t_spmv * 1e6:.1f} us/launch, {spmv_gbps:.0f} GB/s ({spmv_bytes / 1e6:.1f} MB algorithmic)")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(rp, col, val, nev, ncv, args.tol, int(r.info.spmv))
+        log(f"cpu baseline: {cpu}")
+    op.close()
+    del rp, col, val
+    return {
+        "metric": "eigensolves/s at rel.resid<=1e-10 (+ p50/p99 latency); SpMV GB/s vs HBM peak",
+        "value": value,
+        "unit": "eigensolves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.basis],
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": label, "n": n, "nnz": int(nnz), "nev": nev, "ncv": ncv,
+                   "tol": args.tol, "which": "largest algebraic", "basis": BASIS_LABEL[args.basis],
+                   "values": "Ritz values" if refine == H.HL_REFINE_NONE else "Rayleigh quotients",
+                   "spmv_items": items, "vectors_out": "device memory (HBM)"},
+        "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+        "solve_s_with_pcie_vectors": t_pcie,
+        "max_rel_resid": worst,
+        "resid_check": "library: ||A v - l v|| / (||A||_est ||v||) per returned pair, FP64 SpMV",
+        "resid_cross_check": resid_x,
+        "anorm_est": r.info.anorm_est,
+        "rq_shift": r.info.rq_shift,
+        "resid_bar_met": bool(worst <= RESID_BAR),
+        "ritz_orthogonality": ortho,
+        "restarts": int(r.info.restarts),
+        "spmv_per_solve": int(r.info.spmv),
+        "spmv_fp32_phase": int(r.info.spmv_f32),
+        "cgs_passes_per_solve": int(r.info.cgs_passes),
+        "fp64_mode": fp64,
+        "spmv": {"GBps": spmv_gbps, "us_per_launch": t_spmv * 1e6, "bytes_per_launch": spmv_bytes,
+                 "frac_of_peak": spmv_gbps / HBM_PEAK_GBPS},
+        "kernel_classes": classes,
+        "roofline": {"kernel": names[dom], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic},
+        "cpu_baseline": cpu,
     }
 
 
@@ -352,13 +534,15 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="chunglu", choices=sorted(WORKLOADS) + ["batch", "serve"])
+    ap.add_argument("--no-sub", action="store_true", help="headline only: skip the batch and serve sub-objects")
     ap.add_argument("--count", type=int, default=512, help="batch: number of matrices")
     ap.add_argument("--max-active", type=int, default=0, help="batch/serve: solves in flight (0: no cap)")
-    ap.add_argument("--hbm-budget", type=float, default=64e9, help="serve: workspace budget in bytes")
-    ap.add_argument("--rates", default="2,200", help="serve: offered requests/s, comma-separated")
-    ap.add_argument("--requests", type=int, default=100, help="serve: requests in the stream")
+    ap.add_argument("--hbm-budget", type=float, default=64e9, help="serve: footprint budget in bytes")
+    ap.add_argument("--rates", default=None, help="serve: offered requests/s, comma-separated")
+    ap.add_argument("--requests", type=int, default=None, help="serve: requests per rate")
     ap.add_argument("--pool", type=int, default=32, help="serve: distinct matrices")
     ap.add_argument("--seed", type=int, default=0, help="serve: arrival stream seed")
+    ap.add_argument("--fcfs", action="store_true", help="serve: also run the stream one request at a time")
     ap.add_argument("--profile", action="store_true", help="batch: one more profiled batched pass")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV work-item shape")
     ap.add_argument("--no-plain", action="store_true", help="batch: skip the one-at-a-time comparison")
@@ -388,147 +572,37 @@ def main():
         return
 
     import hlanc_amd as H
-    from hlanc_amd import workloads as W
 
-    if args.workload in ("batch", "serve"):
-        if args.maxiter == 1000:
-            args.maxiter = 3000
-        out = (run_batch if args.workload == "batch" else run_serve)(args, H, torch)
-        print(json.dumps(out), flush=True)
-        return
-
-    gen, kw, nev, ncv, label = WORKLOADS[args.workload]
-    t0 = time.perf_counter()
-    rp, col, val = getattr(W, gen)(**kw)
-    n = len(rp) - 1
-    log(f"workload {args.workload}: n={n} nnz={len(val)} generated in {time.perf_counter() - t0:.1f} s")
-    mixed = args.basis == "fp32"
-    prec = basis_prec(H, args.basis)
-    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val), fp32_values=mixed)
-    torch.cuda.init()
-    items = None
-    if not args.no_autotune:
-        nc, rc_, t_best = op.autotune(reps=10)
-        sw, npairs = op.slices()
-        sell_on, sell_entries = op.sell()
-        items = {"nnz_cap": nc, "rows_cap": rc_, "slice_columns": sw, "virtual_rows": npairs,
-                 "sliced_ell": sell_on, "sliced_ell_entries": sell_entries}
-        form = (f"sliced-ELL ({sell_entries} entries)" if sell_on else
-                f"items of <= {nc} nnz / <= {rc_} rows, column slices of {sw} ({npairs} virtual rows)")
-        log(f"SpMV autotune: {form}, {t_best * 1e6:.1f} us per launch")
-
-    # timed solves write their vectors to device memory (the result stays in HBM);
-    # one more solve into page-locked host memory gives the rate with the PCIe copy
-    vec_dev = torch.empty((ncv, n), dtype=torch.float64, device="cuda").T
-    vec_out = H.pinned_empty(n, ncv)
-    for i in range(args.warmup):
-        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_dev,
-                            use_mfma=args.mfma, basis_prec=prec)
-        log(f"warmup {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} {r.info.seconds:.3f} s")
-
-    lat = []
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        t = time.perf_counter()
-        r = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_dev,
-                            use_mfma=args.mfma, basis_prec=prec)
-        lat.append(time.perf_counter() - t)
-        log(f"solve {i}: ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} "
-            f"(fp32 phase {r.info.spmv_f32}) {lat[-1]:.3f} s")
-        if r.ret != 0:
-            raise SystemExit(f"solve {i} did not converge (ret {r.ret})")
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    value = args.steps / wall
-
-    t = time.perf_counter()
-    rh = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, vectors_out=vec_out,
-                         use_mfma=args.mfma, basis_prec=prec)
-    t_pcie = time.perf_counter() - t
-    log(f"solve into page-locked host memory: {t_pcie:.3f} s ({8.0 * n * ncv / 1e9:.2f} GB of vectors over PCIe)")
-    resid, ortho = residuals(op, r.values, r.vectors.cpu().numpy(), n)
-    if not np.array_equal(rh.vectors, r.vectors.cpu().numpy()) or not np.array_equal(rh.values, r.values):
-        raise SystemExit("host-output solve differs from the device-output solve")
-    log(f"max rel residual {resid:.2e}, orthogonality {ortho:.2e}")
-    err64 = None
-    if args.basis != "fp64":  # the result's values against a plain FP64 solve's (untimed)
-        r64 = H.ds_solver_gpu(n, nev, ncv, op, tol=args.tol, maxiter=args.maxiter, use_mfma=args.mfma,
-                              basis_prec=H.HL_PREC_F64)
-        err64 = float(np.max(np.abs(r.values - r64.values) / np.abs(r64.values)))
-        log(f"max rel value error vs FP64 solve {err64:.2e} (FP64 solve {r64.info.seconds:.3f} s, "
-            f"{r64.info.spmv} SpMV)")
-
-    # per-kernel-class profile of one more solve (HIP events around each launch)
-    import ctypes
-
-    o = H.options(args.tol, args.maxiter, use_mfma=args.mfma, basis_prec=prec)
-    o.flags = H.HL_FLAG_PROFILE
-    values = np.zeros(ncv)
-    info = H.Info()
-    rc = H.lib().hl_ds_solve_csr(op.h, nev, ncv, values.ctypes.data_as(H._dp), vec_out.ctypes.data_as(H._dp), n,
-                                 ctypes.byref(o), ctypes.byref(info))
-    pr = H.Profile()
-    H.lib().hl_profile_last(ctypes.byref(pr))
-    names = H.PROFILE_CLASSES
-    classes = {names[k]: {"launches": int(pr.launches[k]), "seconds": pr.seconds[k], "bytes": pr.bytes[k],
-                          "GBps": (pr.bytes[k] / pr.seconds[k] / 1e9) if pr.seconds[k] > 0 else None}
-               for k in range(len(names)) if pr.launches[k] > 0}
-    dom = max((0, 1, 2, 4), key=lambda k: pr.seconds[k])  # kernel classes with algorithmic bytes
-    log(f"profile (ret {rc}): " + ", ".join(f"{k} {v['launches']}x {v['seconds']:.3f}s {v['GBps'] or 0:.0f} GB/s"
-                                           for k, v in classes.items()))
-    achieved = pr.bytes[dom] / pr.seconds[dom] / 1e9
-    traffic = None  # measured HBM bytes per dispatch of the dominant kernel (rocprofv3 PMC runs)
-    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(tpath):
-        key = args.workload + ("" if args.basis == "fp64" else "_" + args.basis)
-        rec = json.load(open(tpath)).get(key, {}).get(names[dom])
-        if rec:
-            traffic = rec["fetch_bytes"] + rec["write_bytes"]
-
-    t_spmv = spmv_timing(op, n)
-    spmv_bytes = op.spmv_bytes()
-    spmv_gbps = spmv_bytes / t_spmv / 1e9
-    log(f"SpMV {t_spmv * 1e6:.1f} us/launch, {spmv_gbps:.0f} GB/s ({spmv_bytes / 1e6:.1f} MB algorithmic)")
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(rp, col, val, nev, ncv, args.tol, int(r.info.spmv))
-        log(f"cpu baseline: {cpu}")
-
-    out = {
-        "metric": "eigensolves/s at rel.resid<=1e-10 (+ p50/p99 latency); SpMV GB/s vs HBM peak",
-        "value": value,
-        "unit": "eigensolves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64" if mixed else "f64",
-        "data": "synthetic (seeded generator)",
-        "config": {"workload": label, "n": n, "nnz": int(len(val)), "nev": nev, "ncv": ncv, "tol": args.tol,
-                   "which": "largest algebraic", "basis": BASIS_LABEL[args.basis], "spmv_items": items,
-                   "vectors_out": "device memory (HBM)"},
-        "latency_s": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
-        "solve_s_with_pcie_vectors": t_pcie,
-        "max_rel_resid": resid,
-        "ritz_orthogonality": ortho,
-        "restarts": int(r.info.restarts),
-        "spmv_per_solve": int(r.info.spmv),
-        "spmv_fp32_phase": int(r.info.spmv_f32),
-        "max_rel_value_err_vs_fp64": err64,
-        "cgs_passes_per_solve": int(r.info.cgs_passes),
-        "spmv": {"GBps": spmv_gbps, "us_per_launch": t_spmv * 1e6, "bytes_per_launch": spmv_bytes,
-                 "frac_of_peak": spmv_gbps / HBM_PEAK_GBPS},
-        "kernel_classes": classes,
-        "roofline": {"kernel": names[dom], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic},
-        "cpu_baseline": cpu,
-    }
+    maxiter_many = 3000 if args.maxiter == 1000 else args.maxiter
+    if args.workload == "batch":
+        out = run_batch(H, torch, count=args.count, steps=args.steps, warmup=args.warmup, basis=args.basis,
+                        tol=args.tol, maxiter=maxiter_many, max_active=args.max_active, mfma=args.mfma,
+                        autotune=not args.no_autotune, plain=not args.no_plain, cpu=not args.no_cpu,
+                        profile=args.profile)
+        out.update({"n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                    "data": "synthetic (seeded generator)"})
+    elif args.workload == "serve":
+        rates = tuple(float(x) for x in (args.rates or "50,200,500,1000,2000").split(","))
+        out = run_serve(H, torch, rates=rates, requests=args.requests or 100, pool_size=args.pool, seed=args.seed,
+                        basis=args.basis, tol=args.tol, maxiter=maxiter_many, hbm_budget=args.hbm_budget,
+                        max_active=args.max_active, fcfs=args.fcfs)
+        out.update({"n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1e3 / out["value"],
+                    "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                    "data": "synthetic (seeded generator)"})
+    else:
+        out = run_single(args, H, torch)
+        if args.workload == "chunglu" and not args.no_sub:
+            torch.cuda.empty_cache()
+            out["batch"] = run_batch(H, torch, count=args.count, steps=1, warmup=1, tol=args.tol,
+                                     maxiter=maxiter_many, autotune=not args.no_autotune, cpu=False)
+            rates = tuple(float(x) for x in (args.rates or "50,200").split(","))
+            out["serve"] = run_serve(H, torch, rates=rates, requests=args.requests or 48, pool_size=args.pool,
+                                     tol=args.tol, maxiter=maxiter_many, hbm_budget=args.hbm_budget)
     print(json.dumps(out), flush=True)
+    bars = [out.get("resid_bar_met", True)] + [out[k]["resid_bar_met"] for k in ("batch", "serve") if k in out]
+    if not all(bars):
+        log("a returned pair missed the 1e-10 residual bar")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -81,6 +81,21 @@ class SolveItem(ctypes.Structure):
                 ("arrival", ctypes.c_double), ("info", Info), ("status", ctypes.c_int), ("pad1", ctypes.c_int),
                 ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double)]
 
+class ServeResult(ctypes.Structure):
+    """hl_serve_result: a finished request of a server."""
+    _fields_ = [("status", ctypes.c_int), ("pad", ctypes.c_int), ("info", Info), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("footprint", ctypes.c_double)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("done", ctypes.c_longlong), ("queued", ctypes.c_int),
+                ("in_flight", ctypes.c_int), ("bytes_in_use", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+HL_PENDING = 2
+HL_SERVE_SELL_EVEN = 1
+
 _lib = None
 
 # (name, restype, argtypes) of every entry point of include/hlanc_capi.h
@@ -139,6 +154,20 @@ _SIGS = [
                                            ctypes.POINTER(Info)]),
     ("hl_ds_solve_batch", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(SolveItem), ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double]),
+    ("hl_serve_create", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_void_p)]),
+    ("hl_serve_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.POINTER(Options), _dp, _dp, ctypes.c_int,
+                                       ctypes.POINTER(ctypes.c_longlong)]),
+    ("hl_serve_submit_csr", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.POINTER(Options), _dp, _dp,
+                                           ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]),
+    ("hl_serve_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.POINTER(ServeResult)]),
+    ("hl_serve_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_double,
+                                     ctypes.POINTER(ServeResult)]),
+    ("hl_serve_release", ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
+    ("hl_serve_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServeStats)]),
+    ("hl_serve_destroy", ctypes.c_int, [ctypes.c_void_p]),
 ]
 
 
@@ -548,6 +577,122 @@ def ds_solve_batch(ops, nev: int, ncv: int, dev: int = 0, *, tol: float = 0.0, m
         out.append(BatchResult(it.status, vals[:it.nev].copy(), vecs, info, it.t_admit, it.t_done, it.arrival,
                                res[:it.nev].copy()))
     return out
+
+
+# ------------------------------------------------------------------ serving
+
+@dataclass
+class ServeOutcome:
+    status: int
+    values: np.ndarray
+    vectors: Optional[np.ndarray]
+    residuals: np.ndarray
+    info: Info
+    t_submit: float
+    t_admit: float
+    t_start: float
+    t_done: float
+    footprint: float
+
+    @property
+    def latency(self) -> float:
+        """Seconds from submission to completion."""
+        return self.t_done - self.t_submit
+
+
+class Server:
+    """The serving loop (hl_serve_*): requests submitted one by one while others
+    run, continuously batched on one device, admitted by HBM footprint.  Each
+    request's result is bitwise what ds_solver_gpu returns for the same operator
+    and options."""
+
+    def __init__(self, dev: int = 0, max_active: int = 0, hbm_budget: float = 0.0, sell_even: bool = False):
+        h = ctypes.c_void_p()
+        _check(lib().hl_serve_create(dev, int(max_active), float(hbm_budget),
+                                     HL_SERVE_SELL_EVEN if sell_even else 0, ctypes.byref(h)))
+        self.h = h
+        self._req = {}  # ticket -> (values, vectors, residuals, options, kept host arrays)
+
+    def submit(self, A, nev: int, ncv: int, *, tol: float = 0.0, maxiter: int = 200, seed: int = 0, v0=None,
+               use_mfma: Optional[bool] = None, basis_prec: Optional[int] = None, refine: int = HL_REFINE_NONE,
+               flags: int = 0, vectors: bool = True) -> int:
+        """A request on ``A``: a dcsrmv_operator_gpu on the server's device, or a host
+        matrix (dcsr_matrix_t or a (row_ptr, col, val) triple) that the server uploads
+        after admission.  Returns the ticket."""
+        res = np.full(max(nev, 1), np.nan)
+        o = options(tol, maxiter, v0, seed, use_mfma, basis_prec=basis_prec, flags=flags, refine=refine,
+                    residuals=res)
+        t = ctypes.c_longlong()
+        if isinstance(A, dcsrmv_operator_gpu):
+            n, keep = A.nrows, A
+            vals = np.zeros(max(ncv, 1))
+            vecs = np.zeros((max(ncv, 1), n)).T if vectors else None
+            rc = lib().hl_serve_submit(self.h, A.h, nev, ncv, ctypes.byref(o), _d(vals),
+                                       vecs.ctypes.data_as(_dp) if vecs is not None else None, n, ctypes.byref(t))
+        else:
+            if isinstance(A, dcsr_matrix_t):
+                rp, col, val, sym = A.row_ptr, A.col, A.val, A.is_symmetric
+            else:
+                (rp, col, val), sym = A, False
+            rp = np.ascontiguousarray(rp, np.int32)
+            col = np.ascontiguousarray(col, np.int32)
+            val = np.ascontiguousarray(val, np.float64)
+            n, keep = len(rp) - 1, (rp, col, val)
+            vals = np.zeros(max(ncv, 1))
+            vecs = np.zeros((max(ncv, 1), n)).T if vectors else None
+            rc = lib().hl_serve_submit_csr(self.h, n, len(val), _i(rp), _i(col), _d(val), int(bool(sym)), nev, ncv,
+                                           ctypes.byref(o), _d(vals),
+                                           vecs.ctypes.data_as(_dp) if vecs is not None else None, n,
+                                           ctypes.byref(t))
+        _check(rc)
+        self._req[t.value] = (vals, vecs, res, o, keep, nev, n)
+        return t.value
+
+    def _outcome(self, ticket, r: ServeResult) -> ServeOutcome:
+        vals, vecs, res, _, _, nev, n = self._req[ticket]
+        return ServeOutcome(r.status, vals[:nev].copy(), None if vecs is None else np.asarray(vecs[:n]),
+                            res[:nev].copy(), Info.from_buffer_copy(r.info), r.t_submit, r.t_admit, r.t_start,
+                            r.t_done, r.footprint)
+
+    def poll(self, ticket: int) -> Optional[ServeOutcome]:
+        r = ServeResult()
+        rc = _check(lib().hl_serve_poll(self.h, ticket, ctypes.byref(r)), allow=(HL_OK, HL_PENDING))
+        return None if rc == HL_PENDING else self._outcome(ticket, r)
+
+    def wait(self, ticket: int, timeout: Optional[float] = None) -> Optional[ServeOutcome]:
+        r = ServeResult()
+        rc = _check(lib().hl_serve_wait(self.h, ticket, -1.0 if timeout is None else float(timeout),
+                                        ctypes.byref(r)), allow=(HL_OK, HL_PENDING))
+        return None if rc == HL_PENDING else self._outcome(ticket, r)
+
+    def release(self, ticket: int) -> None:
+        """Forget a finished request (its host arrays and outputs are dropped)."""
+        _check(lib().hl_serve_release(self.h, ticket))
+        self._req.pop(ticket, None)
+
+    def stats(self) -> ServeStats:
+        st = ServeStats()
+        lib().hl_serve_stats(self.h, ctypes.byref(st))
+        return st
+
+    def close(self) -> None:
+        """Finish every submitted request, then stop the server."""
+        if getattr(self, "h", None):
+            rc = lib().hl_serve_destroy(self.h)
+            self.h = None
+            _check(rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------- basis operations

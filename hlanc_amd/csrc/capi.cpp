@@ -62,6 +62,8 @@ bool is_device_ptr(const void *p, int dev)
     return a.type == hipMemoryTypeDevice && a.device == dev;
 }
 
+} // namespace
+
 // Argument rules of ds_solver (HLanc.hpp:1649-1660) plus this library's
 // limits (ncv <= kMaxNcv; the two-phase FP32 mode ncv <= kNarrowCols).
 bool solve_args_ok(int n, int nev, int ncv, int ldv, const double *values, bool need_vectors,
@@ -95,6 +97,18 @@ bool solve_args_ok(int n, int nev, int ncv, int ldv, const double *values, bool 
     }
     return true;
 }
+
+const CsrPlan *csr_plan(hl_csr_t h) { return &h->plan; }
+
+void csr_release(hl_csr_t h)
+{
+    if (!h) return;
+    DeviceGuard g(h->plan.dev);
+    csr_plan_free(h->plan);
+    delete h;
+}
+
+namespace {
 
 // One solve is an engine with one member.
 int solve_common(hl::Member &m, int dev, int n, int nev, int ncv, double *values, double *vectors, int ldv,

@@ -833,9 +833,22 @@ int Engine::sweep(std::vector<Member *> &ms)
         if (!hit && std::find(gseen.begin(), gseen.end(), key) != gseen.end()) {
             // second sighting of this shape: capture it once, replay from now on
             hipGraph_t graph = nullptr;
+            std::unique_lock<std::mutex> cap_lk;
+            if (capture_mu) cap_lk = std::unique_lock<std::mutex>(*capture_mu);
             HL_TRY_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             rc = launch_steps();
             hipError_t ce = hipStreamEndCapture(stream, &graph);
+            if (cap_lk.owns_lock()) cap_lk.unlock();
+            const int inval = HL_ERR_HIP_BASE + static_cast<int>(hipErrorStreamCaptureInvalidated);
+            if (rc == inval || ce == hipErrorStreamCaptureInvalidated) {
+                // another host thread's HIP call overlapped the capture: nothing of it
+                // ran, so launch the sweep directly, and capture no more in this engine
+                if (graph) (void)hipGraphDestroy(graph);
+                (void)hipGetLastError();
+                graphs = false;
+                if ((rc = launch_steps())) return rc;
+                goto readback;
+            }
             if (rc) {
                 if (graph) (void)hipGraphDestroy(graph);
                 return rc;
@@ -867,6 +880,7 @@ int Engine::sweep(std::vector<Member *> &ms)
             if ((rc = launch_steps())) return rc;
         }
     }
+readback:
     // every member's alpha/beta entries and counters in one copy
     if ((rc = launch_seg_readback(dM, static_cast<int>(nm), stride, dRB, stream))) return rc;
     HL_TRY_HIP(hipMemcpyAsync(hRB, dRB, sizeof(double) * nm * stride, hipMemcpyDeviceToHost, stream));

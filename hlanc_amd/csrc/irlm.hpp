@@ -2,6 +2,7 @@
 #pragma once
 
 #include <algorithm>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -183,6 +184,10 @@ struct Engine {
     long long gclock = 0;
     long long graph_launches = 0, graph_captures = 0;
 
+    // Held while a sweep is captured into a graph, when another host thread of
+    // the same process issues HIP calls on the device meanwhile (the serving
+    // loop's loader): HIP invalidates a capture that such calls overlap.
+    std::mutex *capture_mu = nullptr;
     double t_start = 0.0;                // steady-clock seconds at run()
     int init(int dev);
     void release();
@@ -197,13 +202,22 @@ struct Engine {
     // is allocated on admission and freed when it is done.
     int run(std::vector<Member *> &queue, int max_active, double hbm_budget);
 
+    int poll_finished(bool wait);         // complete members whose vectors have arrived
+    void bury();                          // free detached workspaces (device idle)
+
   private:
     int grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd);
     int after_sweep(std::vector<Member *> &act);
     void drop_graphs();
-    int poll_finished(bool wait);         // complete members whose vectors have arrived
     int verify_pairs(std::vector<Member *> &ver); // residual check, enqueued after the Ritz vectors
-    void bury();                          // free detached workspaces (device idle)
 };
+
+// Shared with the serving loop (capi.cpp): argument rules of a solve, the plan
+// behind an operator handle, and the release of a handle without the device
+// synchronisation of hl_csr_destroy (the caller knows the device is done with it).
+bool solve_args_ok(int n, int nev, int ncv, int ldv, const double *values, bool need_vectors,
+                   const double *vectors, const hl_options &o, bool device_op, const char *who);
+const CsrPlan *csr_plan(hl_csr_t h);
+void csr_release(hl_csr_t h);
 
 }  // namespace hl

@@ -166,25 +166,33 @@ def mix_instance(kind: str, n: float, seed: int, deg: int = 16):
     return erdos_renyi(int(round(n)), deg, seed)
 
 
-def batch_mix(count: int = 512, seed: int = 0, nmin: float = 1e3, nmax: float = 2e5):
+def _build(kinds, ns, seeds, degs, threads):
+    """The instances, built on `threads` host threads (each from its own seed, so
+    the result does not depend on the thread count)."""
+    def one(i):
+        return kinds[i], mix_instance(kinds[i], ns[i], int(seeds[i]), int(degs[i]))
+
+    if threads <= 1:
+        return [one(i) for i in range(len(kinds))]
+    from concurrent.futures import ThreadPoolExecutor
+
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(one, range(len(kinds))))
+
+
+def batch_mix(count: int = 512, seed: int = 0, nmin: float = 1e3, nmax: float = 2e5, threads: int = 1):
     """[(kind, (row_ptr, col, val))] of the batched uneven workload."""
     rng = np.random.default_rng(seed)
     ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count))
     degs = rng.integers(8, 33, count)
-    out = []
-    for i in range(count):
-        kind = ("st5", "er", "st27", "er")[i % 4]
-        out.append((kind, mix_instance(kind, ns[i], seed * 100003 + i, int(degs[i]))))
-    return out
+    kinds = [("st5", "er", "st27", "er")[i % 4] for i in range(count)]
+    return _build(kinds, ns, [seed * 100003 + i for i in range(count)], degs, threads)
 
 
-def serve_mix(count: int = 32, seed: int = 0, nmin: float = 4e3, nmax: float = 2e6):
+def serve_mix(count: int = 32, seed: int = 0, nmin: float = 4e3, nmax: float = 2e6, threads: int = 1):
     """[(kind, (row_ptr, col, val))]: the request matrices of the serving stream."""
     rng = np.random.default_rng(seed + 7)
     ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count))
     degs = rng.integers(8, 33, count)
-    out = []
-    for i in range(count):
-        kind = ("st27", "er", "cl", "er")[i % 4]
-        out.append((kind, mix_instance(kind, ns[i], seed * 100019 + i, int(degs[i]))))
-    return out
+    kinds = [("st27", "er", "cl", "er")[i % 4] for i in range(count)]
+    return _build(kinds, ns, [seed * 100019 + i for i in range(count)], degs, threads)

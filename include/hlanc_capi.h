@@ -401,6 +401,73 @@ int hl_ds_solve_host_op(int dev, int n, hl_host_op_fn op, void *ctx, int nev, in
                         double *values, double *vectors, int ldv, const hl_options *opts,
                         hl_info *info);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop in place of the blocking call: no counterpart in the reference,
+ * which solves one operator per ds_solver_gpu call (HLanc.hpp:1868-1872).  A
+ * server owns a device and two host threads.  Requests are submitted one by one
+ * while others run; the serving thread advances every request in flight in
+ * lockstep (one segmented SpMV and four segmented CGS launches per Lanczos step
+ * for all of them, each request's basis, tridiagonal and convergence state
+ * resident on the device), and between sweeps retires finished requests and
+ * admits queued ones in submission order, while fewer than max_active (<= 0: no
+ * cap) are in flight and their footprints fit in hbm_budget bytes (<= 0: no cap;
+ * an idle server always admits one).  A request's footprint is its workspace
+ * (basis pair, vectors, partial sums) plus, for hl_serve_submit_csr, its
+ * operator: 12 nnz + 4 (n + 1) bytes and 16 n of staging, uploaded by the
+ * server's loader thread once the request is admitted (overlapping the sweeps
+ * of the requests already running) and freed when it is done.  Each request's
+ * results are bitwise those of hl_ds_solve_csr on the same operator and
+ * options, whatever runs beside it.  All entry points are thread-safe. */
+typedef struct hl_server_s *hl_server_t;
+
+#define HL_PENDING 2          /* hl_serve_poll / hl_serve_wait: not done yet */
+#define HL_SERVE_SELL_EVEN 1  /* flag: operators the server uploads take the sliced-ELL form
+                                 where it holds at most 1.1 nnz entries (stencils) */
+
+typedef struct {
+    int status;          /* HL_OK, HL_NOT_CONVERGED or an error code */
+    int pad;
+    hl_info info;
+    double t_submit;     /* seconds after hl_serve_create */
+    double t_admit;      /* admitted: counted against the budget */
+    double t_start;      /* operator on the device: joined the lockstep sweeps */
+    double t_done;       /* outputs written */
+    double footprint;    /* bytes counted against the budget */
+} hl_serve_result;
+
+typedef struct {
+    long long submitted, done;
+    int queued;          /* submitted, not admitted */
+    int in_flight;       /* admitted, not done */
+    double bytes_in_use; /* footprints of the requests in flight */
+    double seconds;      /* since hl_serve_create */
+} hl_serve_stats_t;
+
+int hl_serve_create(int dev, int max_active, double hbm_budget, int flags, hl_server_t *out);
+/* A request on an operator the caller keeps alive until the request is done.
+ * values (host, >= nev) and vectors (as hl_ds_solve_csr, or NULL: values only)
+ * are written when it is done; *ticket names it. */
+int hl_serve_submit(hl_server_t s, hl_csr_t A, int nev, int ncv, const hl_options *opts, double *values,
+                    double *vectors, int ldv, long long *ticket);
+/* A request on a host CSR matrix (dcsr_matrix_t's layout, square, sym_lower as
+ * hl_csr_create): the server uploads it after admission and frees it when the
+ * request is done.  The arrays are read, not copied: they must stay valid
+ * until the request is done. */
+int hl_serve_submit_csr(hl_server_t s, int nrows, int nnz, const int *row_ptr, const int *col,
+                        const double *val, int sym_lower, int nev, int ncv, const hl_options *opts,
+                        double *values, double *vectors, int ldv, long long *ticket);
+/* HL_OK with *out filled once the request is done, HL_PENDING before. */
+int hl_serve_poll(hl_server_t s, long long ticket, hl_serve_result *out);
+/* Blocks until the request is done (timeout_s < 0: no limit); HL_PENDING on timeout. */
+int hl_serve_wait(hl_server_t s, long long ticket, double timeout_s, hl_serve_result *out);
+/* Forgets a done request (its result can no longer be polled). */
+int hl_serve_release(hl_server_t s, long long ticket);
+/* Counters; returns the HIP failure that stopped the server, if any. */
+int hl_serve_stats(hl_server_t s, hl_serve_stats_t *out);
+/* Takes no more requests, finishes every submitted one, stops the threads and
+ * frees the server.  Returns the HIP failure that stopped the server, if any. */
+int hl_serve_destroy(hl_server_t s);
+
 #ifdef __cplusplus
 }
 #endif

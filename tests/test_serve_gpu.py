@@ -1,0 +1,186 @@
+"""The serving loop (hl_serve_*): requests submitted one by one while others are
+on the GPU, admitted as slots and HBM free up, continuously batched.  Each
+result must equal, bitwise, the standalone solve of the same matrix (an
+operator built from the same arrays) with the same options, whatever runs
+beside it and whenever it arrives; admission must honour the concurrency cap
+and the HBM budget (footprint = workspace + the operator's 12 nnz + 4 (n+1) +
+16 n bytes when the server uploads it)."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import hlanc_amd as H
+import oracle as O
+from hlanc_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def mats():
+    return [("st27_12", W.stencil27(12, seed=1), 8, 24), ("er_6000", W.erdos_renyi(6000, 12, seed=2), 6, 18),
+            ("cl_8k", W.chung_lu_laplacian(1 << 13, max_degree=2000, seed=3), 8, 24),
+            ("st27_20", W.stencil27(20, seed=4), 16, 48), ("st5_64", W.stencil5(64, seed=5), 4, 12),
+            ("er_20000", W.erdos_renyi(20000, 16, seed=6), 8, 24)]
+
+
+@pytest.fixture(scope="module")
+def pool():
+    ms = mats()
+    alone = {}
+    for name, (rp, col, val), nev, ncv in ms:
+        op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+        for prec in (H.HL_PREC_F64, H.HL_PREC_F64_PROJ32):
+            alone[(name, prec)] = H.ds_solver_gpu(op.nrows, nev, ncv, op, tol=TOL, maxiter=3000, seed=5,
+                                                  basis_prec=prec, refine=H.HL_REFINE_RQ)
+            assert alone[(name, prec)].ret == 0
+    return ms, alone
+
+
+def same(out, ref):
+    assert out.status == ref.ret
+    np.testing.assert_array_equal(out.values, ref.values)
+    if out.vectors is not None:
+        np.testing.assert_array_equal(out.vectors, ref.vectors)
+    np.testing.assert_array_equal(out.residuals, ref.residuals)
+    assert out.info.spmv == ref.info.spmv and out.info.restarts == ref.info.restarts
+
+
+@pytest.mark.parametrize("prec", [H.HL_PREC_F64, H.HL_PREC_F64_PROJ32])
+def test_requests_arriving_while_others_run_give_their_standalone_bits(pool, prec):
+    ms, alone = pool
+    with H.Server() as srv:
+        tickets = []
+        for k, (name, arrays, nev, ncv) in enumerate(ms * 2):
+            tickets.append((name, srv.submit(arrays, nev, ncv, tol=TOL, maxiter=3000, seed=5, basis_prec=prec,
+                                             refine=H.HL_REFINE_RQ, vectors=(k % 2 == 0))))
+            time.sleep(0.003 * (k % 3))  # staggered arrivals: later requests join running rounds
+        outs = [(name, srv.wait(t)) for name, t in tickets]
+        st = srv.stats()
+    assert st.done == len(tickets) and st.in_flight == 0 and st.queued == 0
+    overlap = 0
+    for (name, out) in outs:
+        same(out, alone[(name, prec)])
+        assert out.t_submit <= out.t_admit <= out.t_start <= out.t_done
+        assert out.info.max_rel_resid <= TOL
+    # the stream overlapped: some request started before an earlier one finished
+    for (_, a), (_, b) in zip(outs, outs[1:]):
+        overlap += b.t_start < a.t_done
+    assert overlap > 0
+
+
+def test_device_operators_and_host_matrices_share_the_rounds(pool):
+    ms, alone = pool
+    ops = [H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=a[0], col=a[1], val=a[2])) for _, a, _, _ in ms]
+    with H.Server() as srv:
+        tk = []
+        for k, ((name, arrays, nev, ncv), op) in enumerate(zip(ms, ops)):
+            A = op if k % 2 else arrays
+            tk.append((name, srv.submit(A, nev, ncv, tol=TOL, maxiter=3000, seed=5, refine=H.HL_REFINE_RQ)))
+        for name, t in tk:
+            same(srv.wait(t), alone[(name, H.HL_PREC_F64)])
+
+
+def test_hbm_budget_admits_by_footprint(pool):
+    ms, alone = pool
+    with H.Server(hbm_budget=1.0) as srv:  # below any footprint: one request in flight at a time
+        tk = [(name, srv.submit(arrays, nev, ncv, tol=TOL, maxiter=3000, seed=5, refine=H.HL_REFINE_RQ))
+              for name, arrays, nev, ncv in ms]
+        outs = [(name, srv.wait(t)) for name, t in tk]
+    for (name, out) in outs:
+        same(out, alone[(name, H.HL_PREC_F64)])
+    for (_, a), (_, b) in zip(outs, outs[1:]):
+        assert b.t_admit >= a.t_done
+    # the footprint counts the operator the server uploads beside the workspace
+    name, (rp, col, val), nev, ncv = ms[0]
+    n = len(rp) - 1
+    op_bytes = 12.0 * len(val) + 4.0 * (n + 1) + 16.0 * n
+    assert outs[0][1].footprint > op_bytes + 2 * 8.0 * n * ncv
+
+
+@pytest.mark.parametrize("cap", [1, 2])
+def test_concurrency_cap(pool, cap):
+    ms, alone = pool
+    with H.Server(max_active=cap) as srv:
+        tk = [(name, srv.submit(arrays, nev, ncv, tol=TOL, maxiter=3000, seed=5, refine=H.HL_REFINE_RQ))
+              for name, arrays, nev, ncv in ms]
+        outs = [(name, srv.wait(t)) for name, t in tk]
+    for name, out in outs:
+        same(out, alone[(name, H.HL_PREC_F64)])
+    events = sorted([(o.t_admit, 1) for _, o in outs] + [(o.t_done, -1) for _, o in outs],
+                    key=lambda e: (e[0], e[1]))
+    live = peak = 0
+    for _, d in events:
+        live += d
+        peak = max(peak, live)
+    assert peak <= cap
+
+
+def test_submissions_from_several_threads(pool):
+    ms, alone = pool
+    results = {}
+    with H.Server() as srv:
+        def client(k):
+            name, arrays, nev, ncv = ms[k % len(ms)]
+            t = srv.submit(arrays, nev, ncv, tol=TOL, maxiter=3000, seed=5, refine=H.HL_REFINE_RQ, vectors=False)
+            results[k] = (name, srv.wait(t))
+
+        th = [threading.Thread(target=client, args=(k,)) for k in range(12)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    assert len(results) == 12
+    for name, out in results.values():
+        same(out, alone[(name, H.HL_PREC_F64)])
+
+
+def test_poll_wait_release_and_local_failures(pool):
+    ms, alone = pool
+    name, arrays, nev, ncv = ms[3]
+    rp, col, val = arrays
+    with H.Server() as srv:
+        t = srv.submit(arrays, nev, ncv, tol=TOL, maxiter=3000, seed=5, refine=H.HL_REFINE_RQ)
+        assert srv.poll(t) is None  # just submitted
+        assert srv.wait(t, timeout=0.0) is None
+        bad = srv.submit(arrays, nev, ncv, tol=TOL, maxiter=1, seed=5)  # cannot converge in one restart
+        with pytest.raises(H.HLancError):
+            srv.submit(arrays, nev, len(rp), tol=TOL)  # ncv > n: refused at submission
+        out = srv.wait(t)
+        same(out, alone[(name, H.HL_PREC_F64)])
+        f = srv.wait(bad)
+        assert f.status == 1 and f.info.restarts == 1 and f.info.max_rel_resid == -1.0
+        assert srv.poll(t) is not None
+        srv.release(t)
+        with pytest.raises(H.HLancError):
+            srv.poll(t)
+        # symmetric-lower storage, expanded on the server's upload
+        lo = H.dcsr_matrix_t(row_ptr=rp, col=col, val=val)
+        import scipy.sparse as sp
+
+        L = sp.tril(sp.csr_matrix((val, col, rp))).tocsr()
+        L.sort_indices()
+        sym = H.dcsr_matrix_t(row_ptr=L.indptr, col=L.indices, val=L.data, is_symmetric=True)
+        s2 = srv.wait(srv.submit(sym, nev, ncv, tol=TOL, maxiter=3000, seed=5))
+        o = O.ds_solver(O.Csr(rp, col, val), nev, ncv, tol=TOL, maxiter=3000, seed=5)
+        assert s2.status == 0 and o.ret == 0
+        np.testing.assert_allclose(s2.values, o.values, rtol=TOL)
+        del lo
+
+
+def test_close_finishes_queued_requests(pool):
+    ms, alone = pool
+    srv = H.Server(max_active=1)
+    tk = [(name, srv.submit(arrays, nev, ncv, tol=TOL, maxiter=3000, seed=5, refine=H.HL_REFINE_RQ))
+          for name, arrays, nev, ncv in ms]
+    outs = {t: srv for _, t in tk}
+    # closing drains: every request completes before the server stops
+    keep = {t: srv._req[t] for _, t in tk}
+    srv.close()
+    for name, t in tk:
+        vals = keep[t][0][: keep[t][5]]
+        np.testing.assert_array_equal(vals, alone[(name, H.HL_PREC_F64)].values)
+    del outs
