@@ -555,6 +555,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (scale for HBM counters)")
+    ap.add_argument("--kernel-profile", type=int, default=0, metavar="RESTARTS",
+                    help="only build the workload, autotune its SpMV and run one solve capped at RESTARTS "
+                         "restarts (the command the rocprofv3 summaries in profiles/ run)")
+    ap.add_argument("--spmv-form", default=None, metavar="FORM",
+                    help="kernel profile: a fixed SpMV form instead of the autotuner's: "
+                         "'slices:WIDTH:NNZCAP:ROWSCAP', 'items:NNZCAP:ROWSCAP' or 'sell'")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no collectives); use --gpus 1", file=sys.stderr)
@@ -572,6 +578,31 @@ def main():
         return
 
     import hlanc_amd as H
+
+    if args.kernel_profile > 0:
+        from hlanc_amd import workloads as W
+
+        gen, kw, nev, ncv, _ = WORKLOADS[args.workload]
+        rp, col, val = getattr(W, gen)(**kw)
+        op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+        if args.spmv_form:
+            f = args.spmv_form.split(":")
+            if f[0] == "sell":
+                op.set_sell(True)
+            else:
+                if f[0] == "slices":
+                    op.set_slices(int(f[1]))
+                    f = f[1:]
+                op.set_items(int(f[1]), int(f[2]))
+        elif not args.no_autotune:
+            op.autotune(reps=10)
+        vec = torch.empty((ncv, len(rp) - 1), dtype=torch.float64, device="cuda").T
+        r = H.ds_solver_gpu(len(rp) - 1, nev, ncv, op, tol=args.tol, maxiter=args.kernel_profile, vectors_out=vec,
+                            basis_prec=basis_prec(H, args.basis), use_mfma=args.mfma)
+        torch.cuda.synchronize()
+        log(f"kernel profile: {args.workload} ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} "
+            f"slices {op.slices()} sell {op.sell()}")
+        return
 
     maxiter_many = 3000 if args.maxiter == 1000 else args.maxiter
     if args.workload == "batch":
