@@ -137,6 +137,8 @@ _SIGS = [
                                     ctypes.POINTER(ctypes.c_double)]),
     ("hl_bench_gather", ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int,
                                        ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    ("hl_bench_grid_barrier", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    ("hl_bench_graph_nodes", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     ("hl_bench_stream", ctypes.c_int, [ctypes.c_int, ctypes.c_longlong, ctypes.c_int,
                                        ctypes.POINTER(ctypes.c_double)]),
     ("hl_host_tridiag_eig", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
@@ -764,6 +766,20 @@ def tune_get(key: str) -> int:
     v = ctypes.c_int()
     _check(lib().hl_tune_get(key.encode(), ctypes.byref(v)))
     return v.value
+
+
+def bench_grid_barrier(blocks: int, barriers: int = 1000, dev: int = 0) -> float:
+    """Seconds per grid-wide barrier of a cooperative launch (hl_bench_grid_barrier)."""
+    t = ctypes.c_double()
+    _check(lib().hl_bench_grid_barrier(dev, int(blocks), int(barriers), ctypes.byref(t)))
+    return t.value
+
+
+def bench_graph_nodes(nodes: int = 20, blocks: int = 256, reps: int = 100, dev: int = 0) -> float:
+    """Seconds per node of a replayed graph of empty kernels (hl_bench_graph_nodes)."""
+    t = ctypes.c_double()
+    _check(lib().hl_bench_graph_nodes(dev, int(nodes), int(blocks), int(reps), ctypes.byref(t)))
+    return t.value
 
 
 def bench_cgs(n: int, m: int, tiled: bool = True, reps: int = 20, dev: int = 0) -> float:

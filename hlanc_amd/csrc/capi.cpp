@@ -567,6 +567,90 @@ int hl_bench_stream(int dev, long long bytes, int reps, double *seconds)
     return rc;
 }
 
+int hl_bench_grid_barrier(int dev, int blocks, int barriers, double *seconds)
+{
+    if (blocks < 1 || barriers < 1 || barriers > (1 << 20) || !seconds) return HL_ERR_ARG;
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    int coop = 0;
+    HL_TRY_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    const int cap = grid_barrier_blocks_max(dev);
+    if (!coop || blocks > cap) {
+        set_error("hl_bench_grid_barrier: %d blocks cannot all be co-resident (at most %d, cooperative launch %d)",
+                  blocks, cap, coop);
+        return HL_ERR_ARG;
+    }
+    if ((long long)barriers * blocks >= 0xffffffffLL) return HL_ERR_ARG; // counter range
+    unsigned *counter = nullptr;
+    double *buf = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HL_TRY_HIP(hipMalloc(&counter, 256));
+    HL_TRY_HIP(hipMalloc(&buf, sizeof(double) * 2 * (size_t)blocks * kBlock));
+    HL_TRY_HIP(hipEventCreate(&e0));
+    HL_TRY_HIP(hipEventCreate(&e1));
+    int *abort_flag = reinterpret_cast<int *>(counter + 16), *errors = reinterpret_cast<int *>(counter + 32);
+    double best = 1e30;
+    int host[2] = {0, 0};
+    for (int rep = 0; rep < 3 && !rc; ++rep) { // the best of three launches
+        if (hipMemset(counter, 0, 256) != hipSuccess) rc = HL_ERR_HIP_BASE;
+        if (!rc) rc = hipEventRecord(e0, nullptr) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+        if (!rc) rc = launch_grid_barrier(blocks, counter, abort_flag, buf, barriers, errors, nullptr);
+        if (!rc) rc = hipEventRecord(e1, nullptr) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+        if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+        float ms = 0.0f;
+        if (!rc) rc = hipEventElapsedTime(&ms, e0, e1) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+        if (!rc) rc = hipMemcpy(host, abort_flag, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+        if (!rc) rc = hipMemcpy(host + 1, errors, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+        if (!rc && (host[0] || host[1])) {
+            set_error("hl_bench_grid_barrier: %s", host[0] ? "a barrier wait expired" : "stale data after a barrier");
+            rc = HL_ERR_STATE;
+        }
+        if (!rc && ms < best) best = ms;
+    }
+    if (!rc) *seconds = 1e-3 * best / barriers;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(counter);
+    (void)hipFree(buf);
+    return rc;
+}
+
+int hl_bench_graph_nodes(int dev, int nodes, int blocks, int reps, double *seconds)
+{
+    if (nodes < 1 || nodes > 4096 || blocks < 1 || blocks > (1 << 20) || reps < 1 || !seconds) return HL_ERR_ARG;
+    int rc = check_device(dev);
+    if (rc) return rc;
+    DeviceGuard g(dev);
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HL_TRY_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HL_TRY_HIP(hipEventCreate(&e0));
+    HL_TRY_HIP(hipEventCreate(&e1));
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = HL_ERR_HIP_BASE;
+    for (int i = 0; i < nodes && !rc; ++i) rc = launch_empty(blocks, s);
+    if (hipStreamEndCapture(s, &graph) != hipSuccess && !rc) rc = HL_ERR_HIP_BASE;
+    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) rc = HL_ERR_HIP_BASE;
+    if (!rc && hipGraphLaunch(exec, s) != hipSuccess) rc = HL_ERR_HIP_BASE; // warm-up
+    if (!rc) rc = hipEventRecord(e0, s) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    for (int r = 0; r < reps && !rc; ++r)
+        if (hipGraphLaunch(exec, s) != hipSuccess) rc = HL_ERR_HIP_BASE;
+    if (!rc) rc = hipEventRecord(e1, s) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    float ms = 0.0f;
+    if (!rc) rc = hipEventElapsedTime(&ms, e0, e1) == hipSuccess ? 0 : HL_ERR_HIP_BASE;
+    if (!rc) *seconds = 1e-3 * ms / ((double)reps * nodes);
+    if (rc) set_error("hl_bench_graph_nodes: a HIP call failed");
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
 int hl_bench_gather(int dev, long long nx, long long m, int random, int reps, double *seconds)
 {
     if (nx <= 0 || nx > 0x7fffffffLL || m < 1024 || reps <= 0 || !seconds) return HL_ERR_ARG;
@@ -628,6 +712,8 @@ int hl_tune_set(const char *key, int value)
         t.cgs_f32_tiles = value;
     else if (!std::strcmp(key, "dgks_skip_exp") && value >= 0 && value <= 16)
         t.dgks_skip_exp = value;
+    else if (!std::strcmp(key, "fold_partials_max") && value >= 0)
+        t.fold_partials_max = value;
     else {
         set_error("hl_tune_set: unknown key or value out of range: %s=%d", key, value);
         return HL_ERR_ARG;
@@ -649,6 +735,8 @@ int hl_tune_get(const char *key, int *value)
         *value = t.cgs_f32_tiles;
     else if (!std::strcmp(key, "dgks_skip_exp"))
         *value = t.dgks_skip_exp;
+    else if (!std::strcmp(key, "fold_partials_max"))
+        *value = t.fold_partials_max;
     else {
         set_error("hl_tune_get: unknown key %s", key);
         return HL_ERR_ARG;

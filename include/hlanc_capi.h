@@ -219,24 +219,18 @@ int hl_host_restart_chase(int nev, int np, const double *shift, double *sub, dou
  * 10^-value, 3..9, default 6), "cgs_f32_tiles" (16-row tiles per trip of the
  * FP32-basis CGS passes, 1 or 2, default 1), "dgks_skip_exp" (HL_PREC_F64_PROJ32
  * solves skip a step's DGKS correction when its exact coefficients h satisfy
- * ||h|| <= 10^-value ||r||; 0..16, default 13, 0 = never skip).
+ * ||h|| <= 10^-value ||r||; 0..16, default 13, 0 = never skip),
+ * "fold_partials_max" (HL_PREC_F64 solves on device operators with at most 256
+ * CGS blocks and CGS blocks x (ncv + 1) <= value fold each pass's per-column
+ * finalize into the next launch: 5 launches per Lanczos step instead of 9, the
+ * same results bit for bit, 43 % less host time launching a sweep, the same
+ * GPU time; default 0 = never).
  * The solvers always run the split pass over a 16-row tiled basis; cgs_flags
- * selects the variant of hl_cgs_pass and hl_bench_cgs.  No counterpart in the
+ * selects the variant of hl_cgs_pass and of the CGS probe (hlanc_probes.h).  No counterpart in the
  * reference (its kernels are fixed). */
 int hl_tune_set(const char *key, int value);
 int hl_tune_get(const char *key, int *value);
-/* Average seconds per launch of one fused CGS pass (w -= V h; h' = V^T w) over
- * an n x m FP64 basis, layout tiled (1) or column-major (0), timed with HIP
- * events over `reps` launches on the default stream (own buffers). */
-int hl_bench_cgs(int dev, int n, int m, int tiled, int reps, double *seconds);
-/* Average seconds per pass of a read-only 16-byte-load stream over `bytes`
- * bytes of HBM (own buffer): the read rate this device reaches. */
-int hl_bench_stream(int dev, long long bytes, int reps, double *seconds);
-/* Average seconds per pass of a gather probe: m 4-byte indices streamed, each
- * followed by an 8-byte read of x[index] from a vector of nx doubles; indices
- * uniformly random (random = 1) or i mod nx (0).  The SpMV's x-gather pattern
- * without the matrix values. */
-int hl_bench_gather(int dev, long long nx, long long m, int random, int reps, double *seconds);
+/* The measurement probes (hl_bench_*) are declared in hlanc_probes.h. */
 
 /* ------------------------------------------------------------------ solver */
 

@@ -119,3 +119,19 @@ def test_reader_errors(tmp_path):
     bad.write_text("0:1 1:x\n")
     with pytest.raises(H.HLancError):
         H.dcsr_matrix_t("pp", str(bad))
+
+
+def test_tuning_keys_round_trip_and_refuse_bad_values():
+    # process-wide kernel parameters: no GPU needed to set or read them
+    for key, value in (("fold_partials_max", 0), ("fold_partials_max", 4096), ("dgks_skip_exp", 12)):
+        old = H.tune_get(key)
+        try:
+            H.tune_set(key, value)
+            assert H.tune_get(key) == value
+        finally:
+            H.tune_set(key, old)
+    assert H.tune_get("fold_partials_max") == 0  # opt-in
+    with pytest.raises(H.HLancError):
+        H.tune_set("fold_partials_max", -1)
+    with pytest.raises(H.HLancError):
+        H.tune_set("no_such_key", 1)
