@@ -531,8 +531,8 @@ def run_single(args, H, torch):
 def launch_probes(H):
     """Costs that decide how a Lanczos step is launched: a graph node (empty
     kernel), a grid-wide barrier of a cooperative launch at several grid sizes,
-    and the 2-D Laplacian 256^2 solve (nev 10, ncv 30, FP64 basis) with its
-    steps unfolded (9 launches per step) and folded (5)."""
+    and the 2-D Laplacian 256^2 solve (nev 10, ncv 30, FP64 basis, 9 launches
+    per Lanczos step) in microseconds per step."""
     from hlanc_amd import workloads as W
 
     out = {"graph_node_us": {}, "grid_barrier_us": {}}
@@ -545,24 +545,15 @@ def launch_probes(H):
     rp, col, val = W.laplacian_2d(256)
     op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
     op.autotune(reps=10)
-    old = H.tune_get("fold_partials_max")
-    out["laplace2d_fp64"] = {}
-    try:
-        for name, fold in (("unfolded", 0), ("folded", 8192)):
-            H.tune_set("fold_partials_max", fold)
-            ts = []
-            for _ in range(5):
-                t0 = time.perf_counter()
-                r = H.ds_solver_gpu(op.nrows, 10, 30, op, tol=1e-10, maxiter=1000, seed=0, basis_prec=H.HL_PREC_F64)
-                ts.append(time.perf_counter() - t0)
-            t = float(np.median(ts))
-            out["laplace2d_fp64"][name] = {"seconds": t, "spmv": int(r.info.spmv), "restarts": int(r.info.restarts),
-                                           "us_per_step": t / r.info.spmv * 1e6, "values": r.values.tolist()}
-            log(f"laplace2d fp64 {name}: {t * 1e3:.2f} ms, {t / r.info.spmv * 1e6:.1f} us per Lanczos step")
-    finally:
-        H.tune_set("fold_partials_max", old)
-    f, u = out["laplace2d_fp64"]["folded"], out["laplace2d_fp64"]["unfolded"]
-    out["laplace2d_fp64"]["same_values"] = f.pop("values") == u.pop("values")
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        r = H.ds_solver_gpu(op.nrows, 10, 30, op, tol=1e-10, maxiter=1000, seed=0, basis_prec=H.HL_PREC_F64)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    out["laplace2d_fp64"] = {"seconds": t, "spmv": int(r.info.spmv), "restarts": int(r.info.restarts),
+                             "us_per_step": t / r.info.spmv * 1e6}
+    log(f"laplace2d fp64: {t * 1e3:.2f} ms, {t / r.info.spmv * 1e6:.1f} us per Lanczos step")
     return out
 
 
@@ -598,7 +589,7 @@ def main():
                          "restarts (the command the rocprofv3 summaries in profiles/ run)")
     ap.add_argument("--launch-probes", action="store_true",
                     help="only measure launch costs: graph nodes, grid-wide barriers, and the 2-D Laplacian "
-                         "256^2 solve with folded and unfolded Lanczos steps (one JSON line)")
+                         "256^2 solve per Lanczos step (one JSON line)")
     ap.add_argument("--spmv-form", default=None, metavar="FORM",
                     help="kernel profile: a fixed SpMV form instead of the autotuner's: "
                          "'slices:WIDTH:NNZCAP:ROWSCAP', 'items:NNZCAP:ROWSCAP' or 'sell'")

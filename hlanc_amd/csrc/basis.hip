@@ -53,18 +53,11 @@ __device__ __forceinline__ T ld_stream(P p)
 // from one address.  Only the last 3 columns of a lane group can lie past m
 // (in the last group), so only their loads carry a per-lane mask.  Padding
 // rows past n are read and multiplied by a = 0.
-struct NoPre {
-    __device__ bool operator()() const { return true; }
-};
-
-// `pre` runs after the first tile's loads are issued (lean loads) and before
-// h_in is read: a folded pass closes the previous pass there, so that its loads
-// overlap the basis loads.  It returns false (uniformly) to end the pass.
-template <int MAXC, typename VT, bool TILED, int FL, typename PRE = NoPre>
+template <int MAXC, typename VT, bool TILED, int FL>
 __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const VT *__restrict__ V_, long long ldv,
                                           const double *w_in_, double *w_out_, const double *__restrict__ h_in,
                                           double *h_out, double *partials_, unsigned *counter, Ctl *ctl,
-                                          double *h_sub, double *h_dia, int mode, int j, PRE pre = PRE{})
+                                          double *h_sub, double *h_dia, int mode, int j)
 {
     if ((mode == EPI_PASS2 || mode == EPI_PASS3) && !ctl->need) return; // DGKS decided on the device
     const auto V = gptr(V_);
@@ -84,6 +77,8 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
     const int cpg = (m + kGroups - 1) / kGroups;
     const int c0 = g * cpg;
     const bool has_h = h_in != nullptr;
+    for (int c = threadIdx.x; c < kGroups * MAXC; c += kBlock) hs[c] = (has_h && c < m) ? h_in[c] : 0.0;
+    __syncthreads();
 
     double acc[MAXC];
 #pragma unroll
@@ -121,25 +116,12 @@ __device__ __forceinline__ void cgs_block(int bid, int nb, int n, int m, const V
                         ? ((FL & 1) ? __builtin_nontemporal_load(pv + c * kTileRows) : pv[c * kTileRows]) \
                         : 0.0;                                                                   \
     }
-    // lean loads: the first tile is in flight while h is read
-    bool fetched = false;
-    if constexpr (LEAN) {
-        if (t < ntiles) {
-            HL_CGS_FETCH_LEAN(t)
-            fetched = true;
-        }
-    }
-    if (!pre()) return;
-    for (int c = threadIdx.x; c < kGroups * MAXC; c += kBlock) hs[c] = (has_h && c < m) ? h_in[c] : 0.0;
-    __syncthreads();
     if (PF) HL_CGS_FETCH(t);
     for (; t < ntiles; t += W) {
-        if constexpr (LEAN) {
-            if (!fetched) HL_CGS_FETCH_LEAN(t)
-            fetched = false;
-        } else if (!PF) {
+        if constexpr (LEAN)
+            HL_CGS_FETCH_LEAN(t)
+        else if (!PF)
             HL_CGS_FETCH(t);
-        }
         double v[MAXC];
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) v[c] = vn[c];
@@ -477,66 +459,6 @@ __device__ __forceinline__ PassPtrs pass_ptrs(const SegMember &mb, int mode)
     }
 }
 
-// A pass of a folded step (SegMember::fold): close the previous pass of the
-// step first (its block-order sums and epilogue, computed by every block on a
-// private copy of the step state; block 0 publishes the projection, the
-// tridiagonal entries and the state), then run this pass on the sums held in
-// LDS.  Bitwise the split pass plus the finalize of the pass before it.
-template <int MAXC, int MODE>
-__device__ __forceinline__ void cgs_fold_block(int bid, const SegMember &mb, int j, const PassPtrs &p)
-{
-    const int m = j + 1;
-    double *const Pa = mb.partials;                                  // EPI_WNORM, EPI_PASS2
-    double *const Pb = mb.partials + (long long)mb.nb * (mb.ncv + 1); // EPI_PASS1, EPI_PASS3
-    const double *V = static_cast<const double *>(mb.V);
-    if constexpr (MODE == EPI_WNORM) { // right after the SpMV: nothing to close
-        cgs_block<MAXC, double, true, 25>(bid, mb.nb, mb.n, m, V, mb.vld, p.w_in, nullptr, nullptr, nullptr, Pa,
-                                         nullptr, mb.ctl, nullptr, nullptr, MODE, j);
-        return;
-    } else {
-        __shared__ double hf[kGroups * MAXC + 1];
-        constexpr int PREV = MODE - 1; // the pass this launch closes
-        const Ctl *cin = MODE == EPI_PASS2 ? mb.ctl2 : mb.ctl;
-        Ctl *cout = MODE == EPI_PASS2 ? mb.ctl : mb.ctl2;
-        // every thread keeps its own copy of the state and runs the (identical)
-        // epilogue on it; block 0's thread 0 publishes
-        Ctl lc = *cin;
-        const bool pub = bid == 0 && threadIdx.x == 0;
-        // loads of the closing issued together with the partials'
-        const double hin_j = PREV == EPI_PASS1 ? mb.h0[j] : PREV == EPI_PASS2 ? mb.h1[j] : 0.0;
-        const double dia0 = (pub && PREV != EPI_PASS1 && PREV != EPI_WNORM) ? mb.hdia[j] : 0.0;
-        auto close = [&]() -> bool {
-            fold_sums(PREV == EPI_PASS1 ? Pb : Pa, mb.nb, m + 1, hf);
-            double sub_j = 0.0, dia_j = dia0; // T entries: only the publishing thread writes them
-            run_epilogue_at(PREV, j, hf[m], hin_j, &lc, &sub_j, &dia_j);
-            if (pub) {
-                if (PREV == EPI_PASS1) mb.hsub[j] = sub_j;
-                if (PREV != EPI_WNORM) mb.hdia[j] = dia_j;
-                store_ctl(cout, lc);
-            }
-            if (bid == 0) { // the closed pass's projection, as its finalize writes it
-                double *hout = PREV == EPI_PASS1 ? mb.h1 : mb.h0;
-                for (int c = threadIdx.x; c < m; c += kBlock) hout[c] = hf[c];
-            }
-            return MODE == EPI_PASS1 || lc.need; // DGKS decided
-        };
-        if constexpr (MODE == EPI_PASS3) { // closes the first correction only where it ran (uniform)
-            if (lc.need) {
-                if (!close()) return;
-            } else {
-                if (pub) store_ctl(cout, lc);
-                return;
-            }
-            cgs_block<MAXC, double, true, 25>(bid, mb.nb, mb.n, m, V, mb.vld, p.w_in, p.w_out, hf, nullptr, Pb,
-                                             nullptr, nullptr, nullptr, nullptr, EPI_NONE, j);
-        } else {
-            cgs_block<MAXC, double, true, 25>(bid, mb.nb, mb.n, m, V, mb.vld, p.w_in, p.w_out, hf, nullptr,
-                                             MODE == EPI_PASS2 ? Pa : Pb, nullptr, nullptr, nullptr, nullptr, EPI_NONE,
-                                             j, close);
-        }
-    }
-}
-
 // One CGS pass of every listed solve in a single launch (split form).  The
 // pass (MODE) is a template argument so that profiles tell the always-run
 // passes (EPI_WNORM, EPI_PASS1) from the DGKS corrections.
@@ -566,8 +488,6 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
         }
         cgs_block_f32<(MAXC + 1) / 2, 17, TB>(bid, mb.nb, mb.n, st.j + 1, V, mb.vld, p.w_in, p.w_out, p.h_in,
                                              p.h_out, mb.partials);
-    } else if (mb.fold) {
-        cgs_fold_block<MAXC, MODE>(bid, mb, st.j, p);
     } else {
         cgs_block<MAXC, double, true, 25>(bid, mb.nb, mb.n, st.j + 1, static_cast<const double *>(mb.V), mb.vld,
                                          p.w_in, p.w_out, p.h_in, p.h_out, mb.partials, nullptr, mb.ctl, mb.hsub,
@@ -769,16 +689,6 @@ __global__ __launch_bounds__(kBlock) void dgks_decide_seg_kernel(const SegMember
             ctl->nskip += 1;
         }
     }
-}
-
-// After a sweep of folded solves: the last step's second correction, one
-// block per listed step.
-__global__ __launch_bounds__(kBlock) void fold_tail_kernel(const SegMember *__restrict__ M,
-                                                           const SegStep *__restrict__ S)
-{
-    const SegStep st = S[blockIdx.x];
-    const SegMember mb = M[st.mem];
-    (void)fold_close_pass3(mb, st.j, true);
 }
 
 typedef double dv2 __attribute__((ext_vector_type(2)));
@@ -1685,26 +1595,10 @@ int launch_dgks_decide_seg(const SegMember *M, const SegStep *S, int ns, double 
 int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fin_blocks_total, int mode,
                             hipStream_t s)
 {
-    if (ns <= 0 || fin_blocks_total <= 0) return HL_OK;
+    if (ns <= 0) return HL_OK;
     cgs_finalize_seg_kernel<<<fin_blocks_total, kBlock, 0, s>>>(M, S, ns, mode);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
-}
-
-int launch_fold_tail(const SegMember *M, const SegStep *S, int ns, hipStream_t s)
-{
-    if (ns <= 0) return HL_OK;
-    fold_tail_kernel<<<ns, kBlock, 0, s>>>(M, S);
-    HL_TRY_HIP(hipGetLastError());
-    return HL_OK;
-}
-
-bool fold_fits(int nb, int ncv)
-{
-    const long long cap = tuning().fold_partials_max;
-    // two partial buffers of nb (ncv + 1) in the workspace's kMaxReduceBlocks (ncv + 1)
-    // (and one leaf per lane of the folded reduction: nb <= kBlock)
-    return cap > 0 && ncv <= kNarrowCols && nb <= kBlock && (long long)nb * (ncv + 1) <= cap;
 }
 
 int vq_seg_blocks(int n) { return grid_for(n, kBlock * 4, 1024); }

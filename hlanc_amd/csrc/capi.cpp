@@ -712,8 +712,6 @@ int hl_tune_set(const char *key, int value)
         t.cgs_f32_tiles = value;
     else if (!std::strcmp(key, "dgks_skip_exp") && value >= 0 && value <= 16)
         t.dgks_skip_exp = value;
-    else if (!std::strcmp(key, "fold_partials_max") && value >= 0)
-        t.fold_partials_max = value;
     else {
         set_error("hl_tune_set: unknown key or value out of range: %s=%d", key, value);
         return HL_ERR_ARG;
@@ -735,8 +733,6 @@ int hl_tune_get(const char *key, int *value)
         *value = t.cgs_f32_tiles;
     else if (!std::strcmp(key, "dgks_skip_exp"))
         *value = t.dgks_skip_exp;
-    else if (!std::strcmp(key, "fold_partials_max"))
-        *value = t.fold_partials_max;
     else {
         set_error("hl_tune_get: unknown key %s", key);
         return HL_ERR_ARG;

@@ -647,10 +647,7 @@ int Engine::sweep(std::vector<Member *> &ms)
     }
     const size_t nm = act.size();
     const int stride = 2 * ncvmax + kCtlWords;
-    // a wide step sends whole FP64 launches to the two-kernel passes, which do not fold
-    bool wide64 = false;
-    for (Member *m : act) wide64 |= (m->kind == SEG_F64 || m->kind == SEG_PROJ32) && m->ncv > kNarrowCols;
-    if ((rc = grow(nm, nm * (size_t)(std::max(maxsteps, 1) + 1), nm, nm * (size_t)stride, nm * (size_t)ncvmax * ncvmax)))
+    if ((rc = grow(nm, nm * (size_t)std::max(maxsteps, 1), nm, nm * (size_t)stride, nm * (size_t)ncvmax * ncvmax)))
         return rc;
     for (size_t i = 0; i < nm; ++i) {
         Member &m = *act[i];
@@ -669,13 +666,9 @@ int Engine::sweep(std::vector<Member *> &ms)
         M.hdia = W.hdia;
         M.partials = W.red.partials;
         M.ctl = W.ctl;
-        M.ctl2 = W.ctl + 1;
         M.n = m.n;
         M.nb = cgs_blocks(m.n);
         M.ncv = m.ncv;
-        // folding changes the launches, not the arithmetic: a solve gives the same
-        // bits folded or not, so the choice may depend on the batch
-        M.fold = m.kind == SEG_F64 && m.A != nullptr && !wide64 && fold_fits(M.nb, m.ncv);
     }
     // every step index's entries grouped by precision kind (kSegOrder): one SpMV
     // launch per value/basis type, one CGS partials launch per basis type read,
@@ -713,12 +706,11 @@ int Engine::sweep(std::vector<Member *> &ms)
                 S.cgs0s = nsub[st][gb];
                 S.fin0 = nfin[st];
                 S.red0 = nred[st];
-                S.fold = hM[i].fold && st > 0;
                 nspmv[st][gs] += m.spmv_blocks();
                 if (m.sliced()) nred[st] += (m.n + kRowsPerSumBlock - 1) / kRowsPerSumBlock;
                 nproj[st][gp] += hM[i].nb;
                 nsub[st][gb] += hM[i].nb;
-                if (!hM[i].fold) nfin[st] += S.j + 2; // j + 1 projections and the squared norm
+                nfin[st] += S.j + 2; // j + 1 projections and the squared norm
                 const int cpg = (S.j + 1 + dev::kGroups - 1) / dev::kGroups; // columns per lane group
                 // PROJ32 steps in the FP32 launches: the first pass is a column dot and a
                 // step wider than kNarrowCols corrects on the FP64 basis, so neither sets the width
@@ -737,17 +729,8 @@ int Engine::sweep(std::vector<Member *> &ms)
         for (; cur_sb < 2; ++cur_sb) goff_sub[st][cur_sb + 1] = static_cast<int>(ns);
     }
     off[maxsteps] = static_cast<int>(ns);
-    // the folded solves' last steps, closed after the sweep
-    int ntail = 0;
-    for (size_t i = 0; i < nm; ++i) {
-        if (!hM[i].fold || act[i]->sweep_np <= 0) continue;
-        SegStep &S = hS[ns + ntail++];
-        S = SegStep{};
-        S.mem = static_cast<int>(i);
-        S.j = act[i]->sweep_k + act[i]->sweep_np - 1;
-    }
     HL_TRY_HIP(hipMemcpyAsync(dM, hM, sizeof(SegMember) * nm, hipMemcpyHostToDevice, stream));
-    HL_TRY_HIP(hipMemcpyAsync(dS, hS, sizeof(SegStep) * (ns + ntail), hipMemcpyHostToDevice, stream));
+    HL_TRY_HIP(hipMemcpyAsync(dS, hS, sizeof(SegStep) * ns, hipMemcpyHostToDevice, stream));
     double host_s = now_s() - th0; // host time of this sweep (profiled runs report it)
 
     const int modes[4] = {EPI_WNORM, EPI_PASS1, EPI_PASS2, EPI_PASS3};
@@ -805,21 +788,13 @@ int Engine::sweep(std::vector<Member *> &ms)
                                         stream)))
                     return r;
                 if ((r = prof.end(stream, p < 2 ? K_CGS : K_DGKS, bytes))) return r;
-                // finalize of the unfolded steps (folded ones close the pass in the next launch)
-                const bool decide = modes[p] == EPI_PASS1 && any_proj32 && tuning().dgks_skip_exp > 0;
-                if (nfin[st] == 0 && !decide) continue;
                 if ((r = prof.begin(stream))) return r;
                 if ((r = launch_cgs_seg_finalize(dM, S, cnt, nfin[st], modes[p], stream))) return r;
-                if (decide &&
+                if (modes[p] == EPI_PASS1 && any_proj32 && tuning().dgks_skip_exp > 0 &&
                     (r = launch_dgks_decide_seg(dM, S, cnt, std::pow(10.0, -tuning().dgks_skip_exp), stream)))
                     return r;
                 if ((r = prof.end(stream, K_MISC, 0.0))) return r;
             }
-        }
-        if (ntail > 0) {
-            if ((r = prof.begin(stream))) return r;
-            if ((r = launch_fold_tail(dM, dS + off[maxsteps], ntail, stream))) return r;
-            if ((r = prof.end(stream, K_MISC, 0.0))) return r;
         }
         return HL_OK;
     };
@@ -851,7 +826,6 @@ int Engine::sweep(std::vector<Member *> &ms)
             key.push_back(nb32[st]);
             key.push_back(cpg32[st]);
         }
-        key.push_back(ntail);
         ++gclock;
         GraphEntry *hit = nullptr;
         for (GraphEntry &g : gcache)

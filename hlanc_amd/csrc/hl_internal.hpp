@@ -86,8 +86,6 @@ struct Tuning {
     int cgs_f32_tiles = 1;      // tiles per trip of the FP32-basis CGS passes (1 or 2; equal speed, 1 fewer VGPRs)
     int dgks_skip_exp = 13;     // FP32-copy projections: skip the correction when ||h1|| <= 10^-exp ||r|| (0: never)
     int cgs_blocks_max = 2048; // grid cap of the CGS passes (<= kMaxReduceBlocks)
-    int fold_partials_max = 0;  // fold FP64 solves whose CGS blocks x (ncv + 1) <= this (0: never):
-                                // 43 % less host launch work per sweep, the same GPU time
     int cgs_flags = 9;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile,
                                // bit 3: split (partials kernel + finalize kernel); bit 2 unused
 };
@@ -249,19 +247,8 @@ struct SegMember {               // per solve, fixed during a sweep
     long long vld;
     double *r, *w, *h0, *h1, *hsub, *hdia, *partials;
     Ctl *ctl;
-    Ctl *ctl2;                   // folded steps: the second copy of the step state (below)
     int n, nb;                   // rows; CGS blocks = cgs_blocks(n)
     int ncv;                     // basis width (read-back size)
-    // Folded steps (FP64 bases, device operators, few partials): no finalize
-    // launches; every block of a pass first reduces the previous pass's partials
-    // itself, in the finalize's order, and runs its epilogue on a private copy of
-    // the step state.  The state alternates between ctl and ctl2 so that no
-    // launch reads what it writes: the first subtracting pass reads ctl and its
-    // block 0 writes ctl2, the first correction ctl2 -> ctl, the second
-    // correction ctl -> ctl2, the next step's SpMV (or the tail launch after
-    // the sweep) ctl2 -> ctl.  Partials alternate the same way between
-    // partials and partials + nb (ncv + 1).
-    int fold;
 };
 struct SegStep {                 // per solve and Lanczos step
     int mem;                     // index into the SegMember table
@@ -271,8 +258,6 @@ struct SegStep {                 // per solve and Lanczos step
     int cgs0s;                   // first block in the launches of EPI_PASS1 and EPI_PASS3
     int fin0;                    // first block in the finalize launch
     int red0;                    // first block in the row-sum launch of sliced SpMVs
-    int fold;                    // folded solve past its sweep's first step: the SpMV first
-                                 // closes the previous step's second correction
 };
 // Partials kernel of CGS pass `mode` (EPI_WNORM .. EPI_PASS3) for every listed
 // step, then the per-column finalize kernel (j + 2 blocks per step).
@@ -285,11 +270,6 @@ int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fi
 // Skips the pending first correction of FP32-copy solves whose exact projection
 // h1 satisfies ||h1|| <= tau ||r|| (one block per listed step; tau <= 0: none).
 int launch_dgks_decide_seg(const SegMember *M, const SegStep *S, int ns, double tau, hipStream_t s);
-// After a sweep of folded solves: closes the second correction of each listed
-// step (the member's last of the sweep) into SegMember::ctl, one block each.
-int launch_fold_tail(const SegMember *M, const SegStep *S, int ns, hipStream_t s);
-// Whether a solve's steps fold (FP64 basis, nb CGS blocks, ncv columns).
-bool fold_fits(int nb, int ncv);
 
 // Launch-cost probes (probes.hip): the co-resident block count of the grid
 // barrier kernel, its cooperative launch (buf: 2 blocks x kBlock doubles;

@@ -254,7 +254,7 @@ struct WsLayout {
         hsub = c.take(8 * (size_t)ncv);
         hdia = c.take(8 * (size_t)ncv);
         hbuf = c.take(16 * (size_t)ncv);
-        ctl = c.take(2 * sizeof(Ctl)); // the step state and its second copy (folded steps)
+        ctl = c.take(sizeof(Ctl));
         xv = host_op ? c.take(8 * ldv) : 0;
         spart = npairs > 0 ? c.take(8 * (size_t)npairs) : 0;
         cpart = nslots > 0 ? c.take(8 * (size_t)nslots) : 0;

@@ -260,22 +260,18 @@ __global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__res
     const int i = seg_find_spmv(S, ns, blockIdx.x);
     const SegStep st = S[i];
     const SegMember mb = M[st.mem];
-    // folded solves: the previous step's second correction closes here first
-    // (the launch's first block of the solve publishes it for the later kernels)
-    const double *xs = (mb.fold && st.fold) ? dev::fold_close_pass3(mb, st.j - 1, blockIdx.x == st.spmv0)
-                                            : &mb.ctl->xscale;
     if constexpr (std::is_same<AT, double>::value) {
         if (mb.e_ptr) { // sliced-ELL form: a wave per 64-row chunk
             const int c = (blockIdx.x - st.spmv0) * (kBlock / kWave) + threadIdx.x / kWave;
             if (c < mb.nchunks)
-                sell_chunk<VT>(c, mb.n, mb.e_ptr, mb.e_col, mb.e_val, mb.rp, mb.r, xs, mb.w,
+                sell_chunk<VT>(c, mb.n, mb.e_ptr, mb.e_col, mb.e_val, mb.rp, mb.r, &mb.ctl->xscale, mb.w,
                                static_cast<VT *>(mb.V), mb.V32, mb.vld, st.j, true);
             return;
         }
     }
     const bool sliced = mb.part != nullptr; // partials now; rows and the basis column in the row-sum kernel
     spmv_item<AT, double, VT>(mb.blks[blockIdx.x - st.spmv0], mb.rp, mb.col, seg_values<AT>(mb), mb.r,
-                              xs, sliced ? mb.part : mb.w, sliced ? nullptr : static_cast<VT *>(mb.V),
+                              &mb.ctl->xscale, sliced ? mb.part : mb.w, sliced ? nullptr : static_cast<VT *>(mb.V),
                               sliced ? nullptr : mb.V32, mb.vld, st.j, true, mb.chunk_part, mb.chunk_cnt);
 }
 

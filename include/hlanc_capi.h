@@ -219,12 +219,7 @@ int hl_host_restart_chase(int nev, int np, const double *shift, double *sub, dou
  * 10^-value, 3..9, default 6), "cgs_f32_tiles" (16-row tiles per trip of the
  * FP32-basis CGS passes, 1 or 2, default 1), "dgks_skip_exp" (HL_PREC_F64_PROJ32
  * solves skip a step's DGKS correction when its exact coefficients h satisfy
- * ||h|| <= 10^-value ||r||; 0..16, default 13, 0 = never skip),
- * "fold_partials_max" (HL_PREC_F64 solves on device operators with at most 256
- * CGS blocks and CGS blocks x (ncv + 1) <= value fold each pass's per-column
- * finalize into the next launch: 5 launches per Lanczos step instead of 9, the
- * same results bit for bit, 43 % less host time launching a sweep, the same
- * GPU time; default 0 = never).
+ * ||h|| <= 10^-value ||r||; 0..16, default 13, 0 = never skip).
  * The solvers always run the split pass over a 16-row tiled basis; cgs_flags
  * selects the variant of hl_cgs_pass and of the CGS probe (hlanc_probes.h).  No counterpart in the
  * reference (its kernels are fixed). */

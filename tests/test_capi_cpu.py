@@ -123,15 +123,15 @@ def test_reader_errors(tmp_path):
 
 def test_tuning_keys_round_trip_and_refuse_bad_values():
     # process-wide kernel parameters: no GPU needed to set or read them
-    for key, value in (("fold_partials_max", 0), ("fold_partials_max", 4096), ("dgks_skip_exp", 12)):
+    for key, value in (("cgs_blocks_max", 1024), ("dgks_skip_exp", 12), ("f32_phase_tol_exp", 7)):
         old = H.tune_get(key)
         try:
             H.tune_set(key, value)
             assert H.tune_get(key) == value
         finally:
             H.tune_set(key, old)
-    assert H.tune_get("fold_partials_max") == 0  # opt-in
+    assert H.tune_get("dgks_skip_exp") == 13
     with pytest.raises(H.HLancError):
-        H.tune_set("fold_partials_max", -1)
+        H.tune_set("dgks_skip_exp", -1)
     with pytest.raises(H.HLancError):
         H.tune_set("no_such_key", 1)
