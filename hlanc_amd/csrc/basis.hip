@@ -13,8 +13,8 @@
 //     repeatable, the same for a solve alone or beside others.  The segmented
 //     kernels (cgs_seg_kernel, cgs_finalize_seg_kernel) run one pass of many
 //     solves in one launch; cgs_kernel is the standalone form (hl_cgs_pass).
-//   * cgs_block_f32 / coldot_block / dgks_decide_seg_kernel: the default basis
-//     mode (an FP32 copy of the FP64 basis for the projection-only passes):
+//   * cgs_block_f32 / coldot_block / dgks_decide_seg_kernel: the FP32-copy
+//     basis mode (an FP32 copy of the FP64 basis for the projection-only passes):
 //     the FP32 pass loads row pairs of adjacent column pairs (full 128-byte
 //     lines per 16 lanes), the first pass is the Lanczos three-term projection
 //     (one FP64 column), and a correction whose exact coefficients are

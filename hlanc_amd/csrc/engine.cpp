@@ -9,8 +9,8 @@
 //   * the basis V, the residual and the alpha/beta recurrence stay in HBM; the
 //     DGKS decisions are taken on the device (a correction pass that is not
 //     needed exits at once);
-//   * a step's solves are grouped by precision kind (FP64; FP64 with an FP32
-//     copy for the projection passes, the default; FP32 basis), each kind's
+//   * a step's solves are grouped by precision kind (FP64, the default; FP64
+//     with an FP32 copy for the projection passes; FP32 basis), each kind's
 //     launches reading its own basis type;
 //   * after a sweep one kernel packs every member's alpha/beta and Ctl and one
 //     copy brings them home; the members' ncv x ncv work runs on host threads

@@ -2,7 +2,7 @@
 //
 // The restart logic of ds_solver (HLanc.hpp:1376-1401, 1492-1589) on the
 // ncv x ncv tridiagonal problem, the per-solve device workspace and the HIP
-// event profiler.  The driver that sequences sweeps, restarts and the Ritz
+// event profiler.  The code that sequences sweeps, restarts and the Ritz
 // vector step over these pieces is engine.cpp.
 #include <algorithm>
 #include <chrono>
