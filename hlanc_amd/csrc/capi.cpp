@@ -2,7 +2,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <mutex>
+#include <thread>
 #include <new>
 #include <vector>
 
@@ -712,6 +714,8 @@ int hl_tune_set(const char *key, int value)
         t.cgs_f32_tiles = value;
     else if (!std::strcmp(key, "dgks_skip_exp") && value >= 0 && value <= 16)
         t.dgks_skip_exp = value;
+    else if (!std::strcmp(key, "batch_engines") && value >= 1 && value <= 8)
+        t.batch_engines = value;
     else {
         set_error("hl_tune_set: unknown key or value out of range: %s=%d", key, value);
         return HL_ERR_ARG;
@@ -733,6 +737,8 @@ int hl_tune_get(const char *key, int *value)
         *value = t.cgs_f32_tiles;
     else if (!std::strcmp(key, "dgks_skip_exp"))
         *value = t.dgks_skip_exp;
+    else if (!std::strcmp(key, "batch_engines"))
+        *value = t.batch_engines;
     else {
         set_error("hl_tune_get: unknown key %s", key);
         return HL_ERR_ARG;
@@ -862,12 +868,43 @@ int hl_ds_solve_batch(int dev, hl_solve_item *items, int count, int max_active, 
         graphs &= (it.opts.flags & HL_FLAG_NO_GRAPH) == 0;
         queue.push_back(&m);
     }
-    hl::Engine eng;
-    if ((rc = eng.init(dev))) return rc;
-    eng.prof.on = profile;
-    eng.graphs = graphs;
-    rc = eng.run(queue, max_active, hbm_budget);
-    if (profile && !rc) thread_profile() = eng.prof.tot;
+    // several engines, each on its own stream and host thread, share the
+    // queue: while one runs its restart logic on the host, the others' sweeps
+    // keep the device busy (a profiled batch runs on one engine)
+    const int engines =
+        profile ? 1 : std::max(1, std::min(tuning().batch_engines, static_cast<int>(queue.size())));
+    if (engines == 1) {
+        hl::Engine eng;
+        if ((rc = eng.init(dev))) return rc;
+        eng.prof.on = profile;
+        eng.graphs = graphs;
+        rc = eng.run(queue, max_active, hbm_budget);
+        if (profile && !rc) thread_profile() = eng.prof.tot;
+    } else {
+        hl::Admission adm;
+        adm.queue = &queue;
+        adm.max_active = max_active;
+        adm.budget = hbm_budget;
+        adm.t_start = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+        adm.engines = engines;
+        std::mutex hip_mu; // captures vs. the other engines' allocations
+        std::vector<int> rcs(static_cast<size_t>(engines), HL_OK);
+        auto serve = [&](int k) {
+            DeviceGuard gk(dev);
+            hl::Engine eng;
+            eng.capture_mu = &hip_mu;
+            eng.graphs = graphs;
+            int r = eng.init(dev); // a full host pool each: their restart phases rarely coincide
+            if (!r) r = eng.run_shared(adm);
+            rcs[static_cast<size_t>(k)] = r;
+        };
+        std::vector<std::thread> th;
+        for (int k = 1; k < engines; ++k) th.emplace_back(serve, k);
+        serve(0);
+        for (std::thread &t : th) t.join();
+        for (int r : rcs)
+            if (r && !rc) rc = r;
+    }
     for (int i = 0; i < count; ++i) {
         const hl::Member &m = ms[i];
         if (!m.A) continue;

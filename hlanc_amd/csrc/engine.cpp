@@ -481,7 +481,7 @@ struct Engine::Pool {
     }
 };
 
-int Engine::init(int dev_)
+int Engine::init(int dev_, int host_threads)
 {
     dev = dev_;
     HL_TRY_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -490,7 +490,7 @@ int Engine::init(int dev_)
     HL_TRY_HIP(hipMalloc(&dOne, sizeof(Ctl)));
     HL_TRY_HIP(hipMemcpy(dOne, &one, sizeof(Ctl), hipMemcpyHostToDevice));
     const unsigned hw = std::thread::hardware_concurrency();
-    pool = new Pool(static_cast<int>(std::min(15u, hw > 1 ? hw - 1 : 0u)));
+    pool = new Pool(host_threads >= 0 ? host_threads : static_cast<int>(std::min(15u, hw > 1 ? hw - 1 : 0u)));
     return HL_OK;
 }
 
@@ -504,9 +504,10 @@ void Engine::drop_graphs()
 void Engine::release()
 {
     if (stream) (void)hipStreamSynchronize(stream);
+    bury(); // takes the lock itself
+    auto lk = hip_lock();
     for (Pending &p : pend) (void)hipEventDestroy(p.ev);
     pend.clear();
-    bury();
     drop_graphs();
     void *d[] = {dM, dS, dV, dRB, dQ, dMv, dSv, dRv, dOne};
     for (void *p : d)
@@ -552,7 +553,9 @@ int grow_set(T *&d, T **h, int nh, size_t &cap, size_t need)
 
 int Engine::grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd)
 {
+    std::unique_lock<std::mutex> lk;
     if (nm > capM || ns > capS || nv > capV || rb > capRB || qd > capQ) {
+        lk = hip_lock();
         HL_TRY_HIP(hipStreamSynchronize(stream)); // uploads from the old buffers may be in flight
         if (nm > capM || ns > capS) drop_graphs(); // captured launches name the old tables
     }
@@ -574,6 +577,8 @@ int Engine::grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd)
 
 void Engine::bury()
 {
+    if (graveyard.empty()) return;
+    auto lk = hip_lock();
     for (auto &p : graveyard) {
         if (p.first) (void)hipFree(p.first);
         if (p.second) (void)hipHostFree(p.second);
@@ -591,7 +596,10 @@ int Engine::poll_finished(bool wait)
             pend[keep++] = p;
             continue;
         }
-        (void)hipEventDestroy(p.ev);
+        {
+            auto lk = hip_lock();
+            (void)hipEventDestroy(p.ev);
+        }
         if (e != hipSuccess) {
             for (size_t r = k + 1; r < pend.size(); ++r) pend[keep++] = pend[r];
             pend.resize(keep);
@@ -838,7 +846,8 @@ int Engine::sweep(std::vector<Member *> &ms)
             HL_TRY_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             rc = launch_steps();
             hipError_t ce = hipStreamEndCapture(stream, &graph);
-            if (cap_lk.owns_lock()) cap_lk.unlock();
+            // the lock stays held through instantiation and eviction (both allocate
+            // or free device memory) and ends with this block
             const int inval = HL_ERR_HIP_BASE + static_cast<int>(hipErrorStreamCaptureInvalidated);
             if (rc == inval || ce == hipErrorStreamCaptureInvalidated) {
                 // another host thread's HIP call overlapped the capture: nothing of it
@@ -1034,7 +1043,10 @@ int Engine::after_sweep(std::vector<Member *> &act)
     if (!ver.empty() && (rc = verify_pairs(ver))) return rc;
     if (fin.ms.empty()) return HL_OK;
     // completed when this event has passed: the next sweep is enqueued meanwhile
-    HL_TRY_HIP(hipEventCreateWithFlags(&fin.ev, hipEventDisableTiming));
+    {
+        auto lk = hip_lock();
+        HL_TRY_HIP(hipEventCreateWithFlags(&fin.ev, hipEventDisableTiming));
+    }
     HL_TRY_HIP(hipEventRecord(fin.ev, stream));
     pend.push_back(std::move(fin));
     return HL_OK;
@@ -1057,6 +1069,7 @@ int Engine::verify_pairs(std::vector<Member *> &ver)
         maxnev = std::max(maxnev, m->nev0);
     }
     if (npairs > capMv) {
+        auto lk = hip_lock();
         if ((rc = grow_set(dMv, &hMv, 1, capMv, npairs))) return rc;
         if ((rc = grow_set(dSv, &hSv, 1, capSv, npairs))) return rc;
         if ((rc = grow_set(dRv, &hRv, 1, capRv, npairs))) return rc;
@@ -1144,59 +1157,102 @@ int Engine::verify_pairs(std::vector<Member *> &ver)
 
 int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
 {
-    t_start = now_s();
+    Admission adm;
+    adm.queue = &queue;
+    adm.max_active = max_active;
+    adm.budget = hbm_budget;
+    adm.t_start = now_s();
+    return run_shared(adm);
+}
+
+int Engine::run_shared(Admission &adm)
+{
+    t_start = adm.t_start;
+    std::vector<Member *> &queue = *adm.queue;
     std::vector<Member *> active;
-    size_t next = 0;
-    double in_use = 0.0;
     int rc = HL_OK;
     auto need = [](const Member *m) {
         return Workspace::bytes_needed(m->n, m->ncv, m->A == nullptr, m->nslots(), m->npairs(), m->copy32());
     };
-    // members done since the last call leave the active set; their workspaces
-    // are freed at the next point where the device is idle
+    // members done since the last call leave the active set (and the shared
+    // counts); their workspaces are freed at the next point where the device is idle
     auto retire = [&]() {
         size_t keep = 0;
+        int gone = 0;
+        double freed = 0.0;
         for (Member *m : active) {
             if (m->phase == Member::DONE) {
                 m->t_done = now_s() - t_start;
                 graveyard.push_back(m->ws.detach());
-                in_use -= need(m);
+                freed += need(m);
+                ++gone;
             } else {
                 active[keep++] = m;
             }
         }
         active.resize(keep);
+        if (gone) {
+            std::lock_guard<std::mutex> lk(adm.mu);
+            adm.active -= gone;
+            adm.in_use -= freed;
+            adm.cv.notify_all();
+        }
     };
     auto sweeping = [&]() {
         for (Member *m : active)
             if (m->phase != Member::FINISHING) return true;
         return false;
     };
-    while (next < queue.size() || !active.empty()) {
+    for (;;) {
         if ((rc = poll_finished(false))) break; // Ritz vectors that have arrived
         retire();
         // admission at the sweep boundary, in queue order
-        double t = now_s() - t_start;
-        if (active.empty() && next < queue.size() && queue[next]->arrival > t) {
-            std::this_thread::sleep_for(std::chrono::duration<double>(queue[next]->arrival - t));
-            t = now_s() - t_start;
+        std::vector<Member *> admitted;
+        bool queue_done = false;
+        {
+            std::unique_lock<std::mutex> lk(adm.mu);
+            for (;;) {
+                const double t = now_s() - t_start;
+                auto fits = [&](const Member *m) {
+                    return (adm.max_active <= 0 || adm.active < adm.max_active) &&
+                           (adm.budget <= 0.0 || adm.active == 0 || adm.in_use + need(m) <= adm.budget);
+                };
+                // this engine's share: its part of what is in flight or has arrived
+                size_t arrived = adm.next;
+                while (arrived < queue.size() && queue[arrived]->arrival <= t) ++arrived;
+                const size_t avail = static_cast<size_t>(adm.active) + (arrived - adm.next);
+                const size_t share = (avail + adm.engines - 1) / adm.engines;
+                while (adm.next < arrived && fits(queue[adm.next]) &&
+                       (adm.engines == 1 || active.size() + admitted.size() < share)) {
+                    Member *m = queue[adm.next++];
+                    m->t_admit = t;
+                    adm.active += 1;
+                    adm.in_use += need(m);
+                    admitted.push_back(m);
+                }
+                queue_done = adm.next >= queue.size();
+                if (!active.empty() || !admitted.empty() || !pend.empty() || queue_done) break;
+                // nothing to do here: wait for the next arrival, or for another
+                // engine to finish a member (nothing in flight means it fits)
+                const Member *q = queue[adm.next];
+                if (q->arrival > t && fits(q))
+                    adm.cv.wait_for(lk, std::chrono::duration<double>(q->arrival - t));
+                else
+                    adm.cv.wait(lk);
+            }
         }
-        while (next < queue.size() && queue[next]->arrival <= t &&
-               (max_active <= 0 || (int)active.size() < max_active) &&
-               (hbm_budget <= 0.0 || active.empty() || in_use + need(queue[next]) <= hbm_budget)) {
-            Member *m = queue[next++];
-            m->t_admit = t;
+        if (queue_done && active.empty() && admitted.empty() && pend.empty()) break;
+        for (Member *m : admitted) {
             m->phase = Member::NEW;
+            auto hk = hip_lock(); // allocations: not while another engine captures a sweep
             if ((m->err = m->ws.init(dev, m->n, m->ncv, m->A == nullptr, m->nslots(), m->npairs(), m->copy32(),
                                      m->verifies() ? m->nev0 : 0, stream))) {
                 m->ws.release();
-                m->phase = Member::DONE;
-                m->t_done = now_s() - t_start;
-                continue;
+                m->phase = Member::DONE; // retired below, out of the shared counts
             }
-            in_use += need(m);
             active.push_back(m);
         }
+        retire();
         if (sweeping()) {
             if ((rc = sweep(active))) break;
             retire();
@@ -1211,15 +1267,26 @@ int Engine::run(std::vector<Member *> &queue, int max_active, double hbm_budget)
         HL_TRY_HIP(hipStreamSynchronize(stream));
         rc = prof.flush();
     }
-    if (rc) { // a HIP failure ends every member in flight
+    if (rc) { // a HIP failure ends every member in flight on this engine
         (void)hipStreamSynchronize(stream);
-        for (Pending &p : pend) (void)hipEventDestroy(p.ev);
+        {
+            auto lk = hip_lock();
+            for (Pending &p : pend) (void)hipEventDestroy(p.ev);
+        }
         pend.clear();
+        double freed = 0.0;
         for (Member *m : active) {
             m->err = rc;
+            freed += need(m);
+            auto lk = hip_lock();
             m->ws.release();
             m->phase = Member::DONE;
         }
+        std::lock_guard<std::mutex> lk(adm.mu);
+        adm.active -= static_cast<int>(active.size());
+        adm.in_use -= freed;
+        adm.cv.notify_all();
+        active.clear();
     }
     HL_TRY_HIP(hipStreamSynchronize(stream));
     bury();

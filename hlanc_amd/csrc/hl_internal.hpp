@@ -86,6 +86,7 @@ struct Tuning {
     int cgs_f32_tiles = 1;      // tiles per trip of the FP32-basis CGS passes (1 or 2; equal speed, 1 fewer VGPRs)
     int dgks_skip_exp = 13;     // FP32-copy projections: skip the correction when ||h1|| <= 10^-exp ||r|| (0: never)
     int cgs_blocks_max = 2048; // grid cap of the CGS passes (<= kMaxReduceBlocks)
+    int batch_engines = 3;     // hl_ds_solve_batch: engines (streams and host threads) sharing a batch
     int cgs_flags = 9;         // bit 0: non-temporal basis loads, bit 1: prefetch next tile,
                                // bit 3: split (partials kernel + finalize kernel); bit 2 unused
 };

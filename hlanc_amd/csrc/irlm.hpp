@@ -2,6 +2,7 @@
 #pragma once
 
 #include <algorithm>
+#include <condition_variable>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -136,6 +137,26 @@ struct Member {
 // segmented SpMV launch and four segmented CGS passes per Lanczos step serve
 // every member that has a step left in its current sweep; one segmented V.Q
 // launch restarts every member that restarts after the sweep.
+//
+// Admission state of one batch, shared by the engines that serve it (each on
+// its own stream and host thread): members are taken from the queue in order,
+// once their arrival time has passed, while fewer than max_active (<= 0: no
+// cap) are in flight over all engines and their workspaces fit in budget bytes
+// (<= 0: no cap; when nothing is in flight one member is always admitted).
+struct Admission {
+    std::mutex mu;
+    std::condition_variable cv;     // a member finished: an idle engine may admit now
+    std::vector<Member *> *queue = nullptr;
+    size_t next = 0;                // first member not admitted yet
+    int active = 0;                 // admitted and not done, all engines
+    double in_use = 0.0;            // their workspace bytes
+    int max_active = 0;
+    double budget = 0.0;
+    double t_start = 0.0;           // steady-clock seconds when the batch started
+    int engines = 1;                // engines sharing the queue: each admits up to its share of the
+                                    // members in flight or arrived, so that all of them get work
+};
+
 struct Engine {
     int dev = 0;
     hipStream_t stream = nullptr;
@@ -184,12 +205,18 @@ struct Engine {
     long long gclock = 0;
     long long graph_launches = 0, graph_captures = 0;
 
-    // Held while a sweep is captured into a graph, when another host thread of
-    // the same process issues HIP calls on the device meanwhile (the serving
-    // loop's loader): HIP invalidates a capture that such calls overlap.
+    // Held while a sweep is captured into a graph and around the engine's
+    // allocations and frees, when other host threads of the same process issue
+    // HIP calls on the device meanwhile (the serving loop's loader, the other
+    // engines of a batch): HIP invalidates a capture that such calls overlap.
     std::mutex *capture_mu = nullptr;
+    std::unique_lock<std::mutex> hip_lock() const
+    {
+        return capture_mu ? std::unique_lock<std::mutex>(*capture_mu) : std::unique_lock<std::mutex>();
+    }
     double t_start = 0.0;                // steady-clock seconds at run()
-    int init(int dev);
+    // host_threads: workers of the decide() pool (< 0: one per host core, up to 15)
+    int init(int dev, int host_threads = -1);
     void release();
     ~Engine() { release(); }
     // Start NEW members (start vector, norm) and run one sweep of every ACTIVE
@@ -201,6 +228,8 @@ struct Engine {
     // always admitted).  Admission happens between sweeps; a member's workspace
     // is allocated on admission and freed when it is done.
     int run(std::vector<Member *> &queue, int max_active, double hbm_budget);
+    // The same loop over admission state shared with other engines.
+    int run_shared(Admission &adm);
 
     int poll_finished(bool wait);         // complete members whose vectors have arrived
     void bury();                          // free detached workspaces (device idle)

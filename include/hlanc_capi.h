@@ -219,7 +219,10 @@ int hl_host_restart_chase(int nev, int np, const double *shift, double *sub, dou
  * 10^-value, 3..9, default 6), "cgs_f32_tiles" (16-row tiles per trip of the
  * FP32-basis CGS passes, 1 or 2, default 1), "dgks_skip_exp" (HL_PREC_F64_PROJ32
  * solves skip a step's DGKS correction when its exact coefficients h satisfy
- * ||h|| <= 10^-value ||r||; 0..16, default 13, 0 = never skip).
+ * ||h|| <= 10^-value ||r||; 0..16, default 13, 0 = never skip), "batch_engines"
+ * (hl_ds_solve_batch: engines, each on its own stream and host thread, that
+ * share the batch's queue and caps; 1..8, default 3; the results are the same
+ * bit for bit).
  * The solvers always run the split pass over a 16-row tiled basis; cgs_flags
  * selects the variant of hl_cgs_pass and of the CGS probe (hlanc_probes.h).  No counterpart in the
  * reference (its kernels are fixed). */
@@ -378,7 +381,11 @@ typedef struct {
  * array order at restart boundaries, once their arrival time has passed,
  * while fewer than max_active (<= 0: no cap) are in flight and their
  * workspaces fit in hbm_budget bytes (<= 0: no cap; one solve is always
- * admitted).  A workspace is allocated on admission and freed when its solve
+ * admitted when nothing is in flight).  The solves in flight are spread over
+ * "batch_engines" engines (hl_tune_set; default 3), each advancing its own in
+ * lockstep on its own stream and host thread, so that one engine's host work
+ * between sweeps overlaps the others' kernels; the caps hold over all of them.
+ * A workspace is allocated on admission and freed when its solve
  * ends.  Each item's results are bitwise those of hl_ds_solve_csr on the same
  * operator and options.  Returns HL_OK once every valid item has run (see each
  * item's status), or the HIP error that stopped the batch. */

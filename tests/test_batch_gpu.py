@@ -204,3 +204,26 @@ def test_many_similar_solves():
         assert r.status == a.ret == 0
         np.testing.assert_array_equal(r.values, a.values)
         np.testing.assert_array_equal(r.vectors, a.vectors)
+
+
+@pytest.mark.parametrize("engines", [1, 2, 5])
+def test_engine_count_changes_no_bits(mixed, engines):
+    """A batch shares its queue among several engines (streams and host
+    threads; tuning key "batch_engines", default 3): each solve's result is
+    bitwise its standalone one whatever engine serves it, and the caps hold
+    over all engines."""
+    mats, ops, alone = mixed
+    old = H.tune_get("batch_engines")
+    H.tune_set("batch_engines", engines)
+    try:
+        assert_same(run_batch(ops), alone)
+        res = run_batch(ops, max_active=2)
+    finally:
+        H.tune_set("batch_engines", old)
+    assert_same(res, alone)
+    events = sorted([(r.t_admit, 1) for r in res] + [(r.t_done, -1) for r in res], key=lambda e: (e[0], e[1]))
+    live = peak = 0
+    for _, d in events:
+        live += d
+        peak = max(peak, live)
+    assert peak <= 2

@@ -123,14 +123,17 @@ def test_reader_errors(tmp_path):
 
 def test_tuning_keys_round_trip_and_refuse_bad_values():
     # process-wide kernel parameters: no GPU needed to set or read them
-    for key, value in (("cgs_blocks_max", 1024), ("dgks_skip_exp", 12), ("f32_phase_tol_exp", 7)):
+    for key, value in (("cgs_blocks_max", 1024), ("dgks_skip_exp", 12), ("f32_phase_tol_exp", 7),
+                       ("batch_engines", 1)):
         old = H.tune_get(key)
         try:
             H.tune_set(key, value)
             assert H.tune_get(key) == value
         finally:
             H.tune_set(key, old)
-    assert H.tune_get("dgks_skip_exp") == 13
+    assert H.tune_get("dgks_skip_exp") == 13 and H.tune_get("batch_engines") == 3
+    with pytest.raises(H.HLancError):
+        H.tune_set("batch_engines", 9)
     with pytest.raises(H.HLancError):
         H.tune_set("dgks_skip_exp", -1)
     with pytest.raises(H.HLancError):
