@@ -393,9 +393,14 @@ def run_single(args, H, torch):
         nc, rc_, t_best = op.autotune(reps=10)
         sw, npairs = op.slices()
         sell_on, sell_entries = op.sell()
+        stg_on, stg_pieces, stg_entries, stg_bins = op.staged()
         items = {"nnz_cap": nc, "rows_cap": rc_, "slice_columns": sw, "virtual_rows": npairs,
-                 "sliced_ell": sell_on, "sliced_ell_entries": sell_entries}
+                 "sliced_ell": sell_on, "sliced_ell_entries": sell_entries,
+                 "staged_slices": stg_on, "staged_pieces": stg_pieces, "staged_entries": stg_entries,
+                 "staged_bins": stg_bins}
         form = (f"sliced-ELL ({sell_entries} entries)" if sell_on else
+                f"staged slices of 16384 columns ({stg_pieces} pieces, {stg_entries} entries, {stg_bins} row bins)"
+                if stg_on else
                 f"items of <= {nc} nnz / <= {rc_} rows, column slices of {sw} ({npairs} virtual rows)")
         log(f"SpMV autotune: {form}, {t_best * 1e6:.1f} us per launch")
 
@@ -592,7 +597,7 @@ def main():
                          "256^2 solve per Lanczos step (one JSON line)")
     ap.add_argument("--spmv-form", default=None, metavar="FORM",
                     help="kernel profile: a fixed SpMV form instead of the autotuner's: "
-                         "'slices:WIDTH:NNZCAP:ROWSCAP', 'items:NNZCAP:ROWSCAP' or 'sell'")
+                         "'slices:WIDTH:NNZCAP:ROWSCAP', 'items:NNZCAP:ROWSCAP', 'sell' or 'staged'")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no collectives); use --gpus 1", file=sys.stderr)
@@ -625,6 +630,8 @@ def main():
             f = args.spmv_form.split(":")
             if f[0] == "sell":
                 op.set_sell(True)
+            elif f[0] == "staged":
+                op.set_staged(True)
             else:
                 if f[0] == "slices":
                     op.set_slices(int(f[1]))
@@ -637,7 +644,7 @@ def main():
                             basis_prec=basis_prec(H, args.basis), use_mfma=args.mfma)
         torch.cuda.synchronize()
         log(f"kernel profile: {args.workload} ret {r.ret} restarts {r.info.restarts} spmv {r.info.spmv} "
-            f"slices {op.slices()} sell {op.sell()}")
+            f"slices {op.slices()} sell {op.sell()} staged {op.staged()}")
         return
 
     maxiter_many = 3000 if args.maxiter == 1000 else args.maxiter

@@ -122,6 +122,9 @@ _SIGS = [
     ("hl_csr_get_slices", ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
     ("hl_csr_set_sell", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hl_csr_get_sell", ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong)]),
+    ("hl_csr_set_staged", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("hl_csr_get_staged", ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong),
+                                         ctypes.POINTER(ctypes.c_longlong), _ip]),
     ("hl_cgs_pass", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p]),
@@ -363,6 +366,18 @@ class dcsrmv_operator_gpu:
         on, ne = ctypes.c_int(), ctypes.c_longlong()
         _check(lib().hl_csr_get_sell(self.h, ctypes.byref(on), ctypes.byref(ne)))
         return bool(on.value), ne.value
+
+    def set_staged(self, on: bool) -> None:
+        """Staged-slice SpMV (x slices of 16384 columns in LDS, partials per row bin) on or off
+        (hl_csr_set_staged)."""
+        _check(lib().hl_csr_set_staged(self.h, int(bool(on))))
+
+    def staged(self):
+        """(on, pieces, stored entries, row bins) of the staged-slice form."""
+        on, npc, ne, nb = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_int()
+        _check(lib().hl_csr_get_staged(self.h, ctypes.byref(on), ctypes.byref(npc), ctypes.byref(ne),
+                                       ctypes.byref(nb)))
+        return bool(on.value), npc.value, ne.value, nb.value
 
 
 class dcsrmv_operator_mgpu(dcsrmv_operator_gpu):

@@ -342,6 +342,28 @@ int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries)
     return HL_OK;
 }
 
+int hl_csr_set_staged(hl_csr_t h, int on)
+{
+    if (!h) return HL_ERR_ARG;
+    DeviceGuard g(h->plan.dev);
+    std::lock_guard<std::mutex> lk(h->mu);
+    return plan_staged(h->plan, on != 0);
+}
+
+int hl_csr_get_staged(hl_csr_t h, int *on, long long *pieces, long long *entries, int *bins)
+{
+    if (!h) return HL_ERR_ARG;
+    const CsrPlan &p = h->plan;
+    long long ne = 0;
+    if (p.stg_on)
+        HL_TRY_HIP(hipMemcpy(&ne, p.s_cptr + p.stg_nchunks, sizeof(long long), hipMemcpyDeviceToHost));
+    if (on) *on = p.stg_on ? 1 : 0;
+    if (pieces) *pieces = p.stg_npieces;
+    if (entries) *entries = ne;
+    if (bins) *bins = p.stg_nbins;
+    return HL_OK;
+}
+
 int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *seconds)
 {
     if (!h || reps <= 0) return HL_ERR_ARG;
@@ -420,10 +442,29 @@ int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *s
             rc = src;
         }
     }
+    // then the staged-slice form, which must win by 2 % (it takes more memory)
+    bool best_stg = false;
+    if (!rc && p.nnz >= (1 << 16)) {
+        const int src = plan_staged(p, true);
+        double t = 0.0;
+        if (src == HL_OK) {
+            if (!(rc = time_form(t)) && t < 0.98 * best_t) {
+                best_t = t;
+                best_stg = true;
+            }
+        } else if (src != HL_ERR_ARG) {
+            rc = src;
+        }
+    }
     cleanup();
     if (rc) return rc;
-    if (!best_sell && (rc = plan_sell(p, false, 0.0))) return rc;
-    if (!best_sell && (rc = plan_slices(p, best_w))) return rc;
+    if (best_stg) {
+        // the staged form is in place
+    } else if (best_sell) {
+        if ((rc = plan_sell(p, true, 0.0))) return rc;
+    } else if ((rc = plan_staged(p, false)) || (rc = plan_sell(p, false, 0.0)) || (rc = plan_slices(p, best_w))) {
+        return rc;
+    }
     if ((rc = csr_items(h, best_n, best_r))) return rc; // (FP32-value SpMVs use the items)
     if (nnz_cap) *nnz_cap = best_n;
     if (rows_cap) *rows_cap = best_r;

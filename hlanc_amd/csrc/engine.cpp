@@ -131,6 +131,7 @@ int Member::spmv_blocks() const
 {
     if (!A) return 0;
     if (sell()) return (A->nchunks + kBlock / kWave - 1) / (kBlock / kWave); // a wave per chunk
+    if (staged()) return 0;                                                    // kernels of their own
     return sliced() ? A->sitems.nblk : A->items.nblk;
 }
 
@@ -617,13 +618,13 @@ namespace {
 void spmv_fields(SegMember &M, const Member &m)
 {
     const Workspace &W = m.ws;
-    const bool sl = m.sliced();
+    const bool sl = m.sliced(), sg = m.staged();
     M.blks = !m.A ? nullptr : sl ? m.A->sitems.blks : m.A->items.blks;
     M.rp = !m.A ? nullptr : sl ? m.A->s_rp : m.A->row_ptr;
     M.col = !m.A ? nullptr : sl ? m.A->s_col : m.A->col;
     M.val = !m.A ? nullptr : sl ? m.A->s_val : m.A->val;
     M.val32 = m.A ? m.A->val32 : nullptr;
-    M.part = sl ? W.part : nullptr;
+    M.part = sl || sg ? W.part : nullptr;
     M.r_pp = sl ? m.A->r_pp : nullptr;
     M.r_pairs = sl ? m.A->r_pairs : nullptr;
     M.chunk_part = W.chunk_part;
@@ -633,6 +634,16 @@ void spmv_fields(SegMember &M, const Member &m)
     M.e_col = el ? m.A->e_col : nullptr;
     M.e_val = el ? m.A->e_val : nullptr;
     M.nchunks = el ? m.A->nchunks : 0;
+    M.s_sup = sg ? m.A->s_sup : nullptr;
+    M.s_cptr = sg ? m.A->s_cptr : nullptr;
+    M.s_plen = sg ? m.A->s_plen : nullptr;
+    M.s_ecol = sg ? m.A->s_ecol : nullptr;
+    M.s_lst = sg ? m.A->s_lst : nullptr;
+    M.s_eval = sg ? m.A->s_eval : nullptr;
+    M.s_brow = sg ? m.A->s_brow : nullptr;
+    M.s_run = sg ? m.A->s_run : nullptr;
+    M.s_lpp = sg ? m.A->s_lpp : nullptr;
+    M.stg_nsl = sg ? m.A->stg_nsl : 0;
 }
 } // namespace
 
@@ -689,7 +700,7 @@ int Engine::sweep(std::vector<Member *> &ms)
     std::vector<std::array<int, 3>> nspmv(maxsteps);
     std::vector<std::array<int, 2>> nproj(maxsteps), nsub(maxsteps), cpg_proj(maxsteps), cpg_sub(maxsteps);
     std::vector<int> off(maxsteps + 1, 0), nfin(maxsteps, 0), n32(maxsteps, 0), nb32(maxsteps, 0), cpg32(maxsteps, 0),
-        nred(maxsteps, 0);
+        nred(maxsteps, 0), nstg(maxsteps, 0), nbin(maxsteps, 0), nslx(maxsteps, 0);
     size_t ns = 0;
     for (int st = 0; st < maxsteps; ++st) {
         off[st] = static_cast<int>(ns);
@@ -714,8 +725,15 @@ int Engine::sweep(std::vector<Member *> &ms)
                 S.cgs0s = nsub[st][gb];
                 S.fin0 = nfin[st];
                 S.red0 = nred[st];
+                S.stg0 = nstg[st];
+                S.bin0 = nbin[st];
                 nspmv[st][gs] += m.spmv_blocks();
                 if (m.sliced()) nred[st] += (m.n + kRowsPerSumBlock - 1) / kRowsPerSumBlock;
+                if (m.staged()) {
+                    nstg[st] += m.A->stg_nsup;
+                    nbin[st] += m.A->stg_nbins;
+                    nslx[st] = std::max(nslx[st], m.A->stg_nsl);
+                }
                 nproj[st][gp] += hM[i].nb;
                 nsub[st][gb] += hM[i].nb;
                 nfin[st] += S.j + 2; // j + 1 projections and the squared norm
@@ -761,7 +779,7 @@ int Engine::sweep(std::vector<Member *> &ms)
                              8.0 * m.A->ncols + 8.0 * m.A->nrows +
                              (m.kind == SEG_PROJ32 ? 12.0 : m.vbytes()) * m.A->nrows;
             }
-            if (nspmv[st][0] + nspmv[st][1] + nspmv[st][2] > 0) {
+            if (nspmv[st][0] + nspmv[st][1] + nspmv[st][2] + nstg[st] > 0) {
                 if ((r = prof.begin(stream))) return r;
                 for (int g = 0; g < 3; ++g)
                     if (nspmv[st][g] > 0 &&
@@ -771,6 +789,10 @@ int Engine::sweep(std::vector<Member *> &ms)
                 if (nred[st] > 0 && // row sums of the sliced SpMVs (FP64 bases: the last group)
                     (r = launch_spmv_rows_seg(dM, dS + goff_spmv[st][2], goff_spmv[st][3] - goff_spmv[st][2],
                                               nred[st], stream)))
+                    return r;
+                if (nstg[st] > 0 && // staged-slice SpMVs (FP64 bases: the last group)
+                    (r = launch_spmv_staged_seg(dM, dS + goff_spmv[st][2], goff_spmv[st][3] - goff_spmv[st][2],
+                                                nstg[st], nbin[st], nslx[st], stream)))
                     return r;
                 if ((r = prof.end(stream, K_SPMV, bytes))) return r;
             }
@@ -830,6 +852,9 @@ int Engine::sweep(std::vector<Member *> &ms)
             }
             key.push_back(nfin[st]);
             key.push_back(nred[st]);
+            key.push_back(nstg[st]);
+            key.push_back(nbin[st]);
+            key.push_back(nslx[st]);
             key.push_back(n32[st]);
             key.push_back(nb32[st]);
             key.push_back(cpg32[st]);
@@ -1092,7 +1117,8 @@ int Engine::verify_pairs(std::vector<Member *> &ver)
         HL_TRY_HIP(hipStreamSynchronize(stream)); // yh is reused by the next member
     }
     // device operators: entries ordered by pair index, so launch i is one run
-    std::vector<int> off(maxnev + 1, 0), nblk(maxnev, 0), nred(maxnev, 0);
+    std::vector<int> off(maxnev + 1, 0), nblk(maxnev, 0), nred(maxnev, 0), nstg(maxnev, 0), nbin(maxnev, 0),
+        nslx(maxnev, 0);
     size_t ne = 0;
     for (int i = 0; i < maxnev; ++i) {
         off[i] = static_cast<int>(ne);
@@ -1112,8 +1138,15 @@ int Engine::verify_pairs(std::vector<Member *> &ver)
             S.mem = static_cast<int>(ne);
             S.spmv0 = nblk[i];
             S.red0 = nred[i];
+            S.stg0 = nstg[i];
+            S.bin0 = nbin[i];
             nblk[i] += m->spmv_blocks();
             if (m->sliced()) nred[i] += (m->n + kRowsPerSumBlock - 1) / kRowsPerSumBlock;
+            if (m->staged()) {
+                nstg[i] += m->A->stg_nsup;
+                nbin[i] += m->A->stg_nbins;
+                nslx[i] = std::max(nslx[i], m->A->stg_nsl);
+            }
             ++ne;
         }
     }
@@ -1126,6 +1159,9 @@ int Engine::verify_pairs(std::vector<Member *> &ver)
             if (cnt <= 0) continue;
             if (nblk[i] > 0 && (rc = launch_spmv_seg(dMv, dSv + off[i], cnt, nblk[i], SEG_F64, stream))) return rc;
             if (nred[i] > 0 && (rc = launch_spmv_rows_seg(dMv, dSv + off[i], cnt, nred[i], stream))) return rc;
+            if (nstg[i] > 0 &&
+                (rc = launch_spmv_staged_seg(dMv, dSv + off[i], cnt, nstg[i], nbin[i], nslx[i], stream)))
+                return rc;
         }
     }
     // the residual sums of every pair in one launch

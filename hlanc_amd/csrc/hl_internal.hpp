@@ -182,7 +182,41 @@ struct CsrPlan {
     long long *e_ptr = nullptr;  // nchunks + 1
     int *e_col = nullptr;        // e_ptr[nchunks] entries (padding: column 0)
     double *e_val = nullptr;     // (padding: 0)
+    // Staged-slice form of the FP64 matrix (stg_on; exclusive with the two
+    // forms above): column slices of 2^kStagedLw columns, small enough that a
+    // block holds the slice's x in LDS.  Each row's entries of a slice, cut
+    // into pieces of <= kStagedPiece, are one partial sum.  Pieces are stored
+    // slice-major in ELL chunks of 64 (a lane per piece, column-major, 16-bit
+    // slice-local columns); within a slice they are grouped by row bin, and by
+    // descending length inside a (slice, bin) run.  Row bins hold <= kBinCap
+    // pieces; the second kernel copies a bin's runs into LDS and adds each
+    // row's partials in slice order through a per-row list of LDS offsets.
+    bool stg_on = false;
+    int stg_nsl = 0;                       // slices
+    int stg_nchunks = 0;                   // 64-piece chunks (each slice padded to whole chunks)
+    int stg_nsup = 0;                      // blocks of the first kernel
+    int stg_nbins = 0;                     // blocks of the second kernel
+    long long stg_npieces = 0;             // pieces, without the padding
+    long long *s_cptr = nullptr;           // nchunks + 1: first entry of each chunk
+    unsigned short *s_plen = nullptr;      // 64 nchunks: entries per piece (padding 0)
+    unsigned short *s_ecol = nullptr;      // s_cptr[nchunks]: slice-local columns
+    double *s_eval = nullptr;              // s_cptr[nchunks]: values
+    int4 *s_sup = nullptr;                 // nsup: {slice, first chunk, end chunk, 0}
+    int *s_brow = nullptr;                 // nbins + 1: first row of each bin
+    int *s_run = nullptr;                  // (nbins + 1) x nsl: first piece of (bin, slice)
+    int *s_lpp = nullptr;                  // nrows + 1: each row's pieces in s_lst
+    unsigned short *s_lst = nullptr;       // npieces: LDS offset of each piece, slice order
+    double *s_spart = nullptr;             // 64 nchunks: partials of standalone calls
 };
+
+// Staged-slice form limits (the form refuses matrices beyond them).
+constexpr int kStagedLw = 14;            // slices of 2^14 columns (x of a slice: 128 KB of LDS)
+constexpr int kStagedThreads = 1024;     // threads per block of the first staged kernel
+constexpr int kBinThreads = 512;         // threads per block of the second (bins) kernel
+constexpr int kStagedPiece = 64;         // entries per piece
+constexpr int kBinCap = 4096;            // pieces per row bin (their partials: 32 KB of LDS)
+constexpr int kStagedMaxSlices = 4000;   // slices (the bins kernel keeps two ints per slice in LDS)
+constexpr int kStagedSupEntries = 65536; // entries per block of the first kernel (about)
 
 int csr_plan_build(CsrPlan &p, int nrows, int ncols, const int *rp, const int *col,
                    const double *val, int sym_lower, int val_prec);
@@ -196,6 +230,9 @@ int plan_slices(CsrPlan &p, int width);
 // Builds (on) or drops the sliced-ELL form; building it drops the column slices.
 // max_fill > 0 refuses (HL_ERR_ARG) a form of more than max_fill x nnz entries.
 int plan_sell(CsrPlan &p, bool on, double max_fill);
+// Builds (on) or drops the staged-slice form; building it drops the other two
+// forms.
+int plan_staged(CsrPlan &p, bool on);
 
 // y = A (x * xscale) with the optional fused write vcol(r) = x[r] * xscale.
 // xscale == nullptr means 1.  val_prec selects the FP64 or FP32 value array.
@@ -242,6 +279,14 @@ struct SegMember {               // per solve, fixed during a sweep
     const int *e_col;
     const double *e_val;
     int nchunks;
+    // staged-slice form (FP64 values and basis), else s_sup null: its first
+    // kernel writes the pieces' partials into part, its bins kernel adds them
+    const int4 *s_sup;
+    const long long *s_cptr;
+    const unsigned short *s_plen, *s_ecol, *s_lst;
+    const double *s_eval;
+    const int *s_brow, *s_run, *s_lpp;
+    int stg_nsl;
     void *V;                     // tiled basis in use (double, or float for SEG_V32*)
     const void *Vproj;           // basis read by the projection-only CGS passes (V, or the FP32 copy)
     float *V32;                  // FP32 copy written beside V (SEG_PROJ32), else null
@@ -259,6 +304,8 @@ struct SegStep {                 // per solve and Lanczos step
     int cgs0s;                   // first block in the launches of EPI_PASS1 and EPI_PASS3
     int fin0;                    // first block in the finalize launch
     int red0;                    // first block in the row-sum launch of sliced SpMVs
+    int stg0;                    // first block in the first launch of staged-slice SpMVs
+    int bin0;                    // first block in their bins launch
 };
 // Partials kernel of CGS pass `mode` (EPI_WNORM .. EPI_PASS3) for every listed
 // step, then the per-column finalize kernel (j + 2 blocks per step).
@@ -283,6 +330,12 @@ int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_tot
 // Second kernel of the sliced SpMVs among S[0, ns): w = row sums of the
 // partials, V(:, j) = r xscale (and its FP32 copy).
 int launch_spmv_rows_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s);
+// The two kernels of the staged-slice SpMVs among S[0, ns): the pieces'
+// partials (stg_blocks blocks), then the row bins (bin_blocks; nsl_max: the
+// most slices of a listed operator) writing w and V(:, j) = r xscale (and its
+// FP32 copy).
+int launch_spmv_staged_seg(const SegMember *M, const SegStep *S, int ns, int stg_blocks, int bin_blocks, int nsl_max,
+                           hipStream_t s);
 constexpr int kRowsPerSumBlock = 256; // rows per block of the row-sum kernels
 
 // Restart V.Q of one solve within a segmented launch (tiled bases, ld K).

@@ -111,11 +111,14 @@ struct Member {
     bool copy32() const { return A != nullptr && opts.basis_prec == HL_PREC_F64_PROJ32; } // FP32 basis copy
     // the operator's column-sliced form serves the SpMV (FP64 values and basis)
     bool sliced() const { return A != nullptr && A->slice_w > 0 && (kind == SEG_F64 || kind == SEG_PROJ32); }
+    // the operator's staged-slice form serves the SpMV (FP64 values and basis)
+    bool staged() const { return A != nullptr && A->stg_on && (kind == SEG_F64 || kind == SEG_PROJ32); }
     bool sell() const { return A != nullptr && A->sell_on && kind != SEG_V32_A32; } // FP64 values only
     bool verifies() const { return (opts.flags & HL_FLAG_NO_VERIFY) == 0; } // residual check requested
     int spmv_blocks() const; // blocks of this solve's SpMV launch segment
     int nslots() const { return A ? std::max(A->items.nslots, A->sitems.nslots) : 0; }
-    int npairs() const { return A ? A->npairs : 0; }
+    // partials of a sliced or staged SpMV (the forms exclude each other)
+    int npairs() const { return A ? (A->stg_on ? kWave * A->stg_nchunks : A->npairs) : 0; }
     void absorb(const double *rb);       // hsub, hdia (ncv each) and the Ctl words
     // Host-only (thread-safe across members): restart logic; writes the restart
     // Q (ncv x vq_N) or the Ritz-vector Q (ncv x ncv) to qslot.

@@ -343,6 +343,286 @@ __global__ __launch_bounds__(kBlock) void spmv_rows_kernel(int n, const int *__r
               vld, vj, vtiled);
 }
 
+// ------------------------------------------------------------ staged slices
+// First kernel of the staged-slice form, one block per run of chunks within a
+// slice (su = {slice, first chunk, end chunk}): x of the slice into LDS, then
+// each wave takes kStagedGroup chunks at a time (their loads in flight
+// together), a lane per piece summing its entries in order (lanes stop at
+// their own length, so padding is neither loaded nor added), the partial
+// written at the piece's index.  A chunk is column-major: entry k of lane l
+// at cptr[c] + 64 k + l, so a wave's k-th loads are one contiguous stretch.
+constexpr int kStagedGroup = 4;
+constexpr int kStagedUnroll = 4; // entries of a piece in flight per trip
+__device__ __forceinline__ void staged_block(const int4 su, long long ncols, const long long *__restrict__ cptr_,
+                                             const unsigned short *__restrict__ plen_,
+                                             const unsigned short *__restrict__ ecol_,
+                                             const double *__restrict__ eval_, const double *__restrict__ x_,
+                                             double s, double *__restrict__ part_, double *xs)
+{
+    const auto cptr = dev::gptr(cptr_);
+    const auto plen = dev::gptr(plen_);
+    const auto ecol = dev::gptr(ecol_);
+    const auto eval = dev::gptr(eval_);
+    const auto x = dev::gptr(x_);
+    const auto part = dev::gptr(part_);
+    constexpr int W = 1 << kStagedLw;
+    constexpr int G = kStagedGroup;
+    constexpr int U = kStagedUnroll;
+    constexpr int NW = kStagedThreads / kWave;
+    const long long c0 = static_cast<long long>(su.x) << kStagedLw;
+    const int nx = static_cast<int>(min(static_cast<long long>(W), ncols - c0));
+    for (int i = threadIdx.x; i < W; i += kStagedThreads) xs[i] = i < nx ? x[c0 + i] : 0.0;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    // chunk bounds and piece lengths of a wave's group of chunks; the next
+    // group's are loaded while this one's entries are in flight
+    long long nb[G], ne[G];
+    int nl[G];
+    auto meta = [&](int cb) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int c = cb + g * NW;
+            nb[g] = ne[g] = 0;
+            nl[g] = 0;
+            if (c < su.z) {
+                nb[g] = cptr[c];
+                ne[g] = cptr[c + 1];
+                nl[g] = plen[static_cast<long long>(c) * kWave + lane];
+            }
+        }
+    };
+    meta(su.y + wave);
+    __syncthreads();
+    for (int cb = su.y + wave; cb < su.z; cb += G * NW) {
+        long long base[G];
+        int len[G];
+        int wmax = 0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            base[g] = nb[g];
+            len[g] = nl[g];
+            wmax = max(wmax, static_cast<int>((ne[g] - nb[g]) / kWave));
+        }
+        meta(cb + G * NW);
+        double acc[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) acc[g] = 0.0;
+        for (int k0 = 0; k0 < wmax; k0 += U) {
+            int ci[G][U];
+            double av[G][U];
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (k0 + u < len[g]) {
+                        const long long e = base[g] + static_cast<long long>(k0 + u) * kWave + lane;
+                        ci[g][u] = __builtin_nontemporal_load(&ecol[e]);
+                        av[g][u] = __builtin_nontemporal_load(&eval[e]);
+                    }
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (k0 + u < len[g]) acc[g] += av[g][u] * (xs[ci[g][u]] * s);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int c = cb + g * NW;
+            if (c < su.z) part[static_cast<long long>(c) * kWave + lane] = acc[g];
+        }
+    }
+}
+
+// Second kernel of the staged-slice form, one block (kBinThreads) per row bin
+// b of at most kBinCap pieces: the bin's run of every slice (run[b nsl + sl] ..
+// run[(b + 1) nsl + sl]) copied into LDS back to back (ps), and beside it the
+// bin's stretch of the per-row offset lists (lsh); then y(r) = the row's
+// partials added in slice order, and the basis column v(r) = x(r) s beside.
+// roff/rdel: 2 nsl + 1 ints of LDS.
+template <typename VT>
+__device__ __forceinline__ void bins_block(int b, int nsl, const int *__restrict__ brow_, const int *__restrict__ run_,
+                                           const int *__restrict__ lpp_, const unsigned short *__restrict__ lst_,
+                                           const double *__restrict__ part_, const double *__restrict__ x_, double s,
+                                           double *__restrict__ y_, VT *__restrict__ vout_,
+                                           float *__restrict__ vout32_, long long vld, int vj, bool vtiled,
+                                           double *ps, unsigned short *lsh, int *roff, int *rdel, int *wtot)
+{
+    const auto brow = dev::gptr(brow_);
+    const auto run = dev::gptr(run_);
+    const auto lpp = dev::gptr(lpp_);
+    const auto lst = dev::gptr(lst_);
+    const auto part = dev::gptr(part_);
+    const auto x = dev::gptr(x_);
+    const auto y = dev::gptr(y_);
+    const auto vout = dev::gptr(vout_);
+    const auto vout32 = dev::gptr(vout32_);
+    constexpr int T = kBinThreads;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int r0 = brow[b], r1 = brow[b + 1];
+    const int l0 = lpp[r0], nl = lpp[r1] - l0; // the bin's offsets: lst[l0, l0 + nl)
+    // this thread's first row: its bounds and x(r), independent of the copy
+    const int rf = r0 + tid;
+    int f0 = 0, f1 = 0;
+    double xf = 0.0;
+    if (rf < r1) {
+        f0 = lpp[rf] - l0;
+        f1 = lpp[rf + 1] - l0;
+        if (vout) xf = x[rf];
+    }
+    const auto rs = run + static_cast<long long>(b) * nsl; // starts; rs + nsl: ends
+    // LDS offsets of the runs: exclusive scan of their lengths, kPer slices per
+    // thread; rdel[sl] = (first piece of the run) - (its LDS offset)
+    constexpr int kPer = (kStagedMaxSlices + T - 1) / T;
+    int len[kPer], st[kPer], sum = 0;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int sl = tid * kPer + u;
+        st[u] = sl < nsl ? rs[sl] : 0;
+        len[u] = sl < nsl ? rs[nsl + sl] - st[u] : 0;
+        sum += len[u];
+    }
+    int incl = sum;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+    }
+    if (lane == kWave - 1) wtot[wave] = incl;
+    __syncthreads();
+    int ex = incl - sum;
+    for (int w = 0; w < wave; ++w) ex += wtot[w];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int sl = tid * kPer + u;
+        if (sl < nsl) {
+            roff[sl] = ex;
+            rdel[sl] = st[u] - ex;
+        }
+        ex += len[u];
+    }
+    if (tid == T - 1) roff[nsl] = ex; // the bin's total (= nl)
+    __syncthreads();
+    // the copy: LDS index i comes from run sl (the last run starting at or
+    // before i; empty runs never are); all of a thread's loads in flight, the
+    // bin's offsets with them
+    constexpr int kCopy = kBinCap / T;
+    const int total = roff[nsl];
+    double v[kCopy];
+    int li[kCopy];
+#pragma unroll
+    for (int u = 0; u < kCopy; ++u) {
+        const int i = tid + u * T;
+        if (i < total) {
+            int lo = 0, hi = nsl - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (roff[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            v[u] = part[rdel[lo] + i];
+        }
+        if (i < nl) li[u] = lst[l0 + i];
+    }
+#pragma unroll
+    for (int u = 0; u < kCopy; ++u) {
+        const int i = tid + u * T;
+        if (i < total) ps[i] = v[u];
+        if (i < nl) lsh[i] = static_cast<unsigned short>(li[u]);
+    }
+    __syncthreads();
+    for (int r = rf; r < r1; r += T) {
+        int k0 = f0, k1 = f1;
+        double xr = xf;
+        if (r != rf) {
+            k0 = lpp[r] - l0;
+            k1 = lpp[r + 1] - l0;
+            xr = vout ? x[r] : 0.0;
+        }
+        double acc = 0.0;
+        for (int k = k0; k < k1; ++k) acc += ps[lsh[k]];
+        y[r] = acc;
+        if (vout) {
+            const double xs = xr * s;
+            vout[vcol_pos(r, vld, vj, vtiled)] = static_cast<VT>(xs);
+            if (vout32) vout32[vcol_pos(r, vld, vj, vtiled)] = static_cast<float>(xs);
+        }
+    }
+}
+
+// Dynamic LDS of the bins kernels: the run offsets and deltas (2 nsl + 1 ints).
+inline size_t bins_dyn_lds(int nsl) { return sizeof(int) * (2 * static_cast<size_t>(nsl) + 1); }
+
+__global__ __launch_bounds__(kStagedThreads) void spmv_staged_kernel(const int4 *__restrict__ sup, long long ncols,
+                                                                     const long long *__restrict__ cptr,
+                                                                     const unsigned short *__restrict__ plen,
+                                                                     const unsigned short *__restrict__ ecol,
+                                                                     const double *__restrict__ eval,
+                                                                     const double *__restrict__ x,
+                                                                     const double *__restrict__ xscale,
+                                                                     double *__restrict__ part)
+{
+    __shared__ double xs[1 << kStagedLw];
+    staged_block(sup[blockIdx.x], ncols, cptr, plen, ecol, eval, x, xscale ? *xscale : 1.0, part, xs);
+}
+
+__global__ __launch_bounds__(kBinThreads) void spmv_bins_kernel(int nsl, const int *__restrict__ brow,
+                                                                const int *__restrict__ run,
+                                                                const int *__restrict__ lpp,
+                                                                const unsigned short *__restrict__ lst,
+                                                                const double *__restrict__ part,
+                                                                const double *__restrict__ x,
+                                                                const double *__restrict__ xscale,
+                                                                double *__restrict__ y, double *__restrict__ vout,
+                                                                long long vld, int vj, bool vtiled)
+{
+    __shared__ double ps[kBinCap];
+    __shared__ unsigned short lsh[kBinCap];
+    __shared__ int wtot[kBinThreads / kWave];
+    extern __shared__ int bins_dyn[];
+    bins_block<double>(blockIdx.x, nsl, brow, run, lpp, lst, part, x, xscale ? *xscale : 1.0, y, vout, nullptr,
+                            vld, vj, vtiled, ps, lsh, bins_dyn, bins_dyn + nsl + 1, wtot);
+}
+
+// The staged-slice SpMVs of every listed solve: blocks numbered by stg0
+// (first kernel) and bin0 (bins kernel), as spmv_seg_kernel numbers by spmv0.
+__device__ __forceinline__ int seg_find_stg(const SegStep *S, int ns, int b, bool bins)
+{
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((bins ? S[mid].bin0 : S[mid].stg0) <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kStagedThreads) void spmv_staged_seg_kernel(const SegMember *__restrict__ M,
+                                                                         const SegStep *__restrict__ S, int ns)
+{
+    __shared__ double xs[1 << kStagedLw];
+    const SegStep st = S[seg_find_stg(S, ns, blockIdx.x, false)];
+    const SegMember mb = M[st.mem];
+    staged_block(mb.s_sup[blockIdx.x - st.stg0], mb.n, mb.s_cptr, mb.s_plen, mb.s_ecol, mb.s_eval, mb.r,
+                 mb.ctl->xscale, mb.part, xs);
+}
+
+__global__ __launch_bounds__(kBinThreads) void spmv_bins_seg_kernel(const SegMember *__restrict__ M,
+                                                                    const SegStep *__restrict__ S, int ns)
+{
+    __shared__ double ps[kBinCap];
+    __shared__ unsigned short lsh[kBinCap];
+    __shared__ int wtot[kBinThreads / kWave];
+    extern __shared__ int bins_dyn[];
+    const SegStep st = S[seg_find_stg(S, ns, blockIdx.x, true)];
+    const SegMember mb = M[st.mem];
+    bins_block<double>(blockIdx.x - st.bin0, mb.stg_nsl, mb.s_brow, mb.s_run, mb.s_lpp, mb.s_lst, mb.part, mb.r,
+                       mb.ctl->xscale, mb.w, static_cast<double *>(mb.V), mb.V32, mb.vld, st.j, true, ps, lsh,
+                       bins_dyn, bins_dyn + mb.stg_nsl + 1, wtot);
+}
+
 template <typename T>
 int dev_upload(T **dst, const T *src, size_t count)
 {
@@ -532,6 +812,21 @@ void slices_free(CsrPlan &p)
     p.npairs = 0;
     items_free(p.sitems);
 }
+
+void staged_free(CsrPlan &p)
+{
+    void *ptrs[] = {p.s_cptr, p.s_plen, p.s_ecol, p.s_eval, p.s_sup, p.s_brow, p.s_run, p.s_lpp, p.s_lst, p.s_spart};
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    p.s_cptr = nullptr;
+    p.s_plen = p.s_ecol = p.s_lst = nullptr;
+    p.s_eval = p.s_spart = nullptr;
+    p.s_sup = nullptr;
+    p.s_brow = p.s_run = p.s_lpp = nullptr;
+    p.stg_on = false;
+    p.stg_nsl = p.stg_nchunks = p.stg_nsup = p.stg_nbins = 0;
+    p.stg_npieces = 0;
+}
 } // namespace
 
 // Virtual rows (row, slice): slice-major, rows ascending within a slice, each
@@ -546,6 +841,7 @@ int plan_slices(CsrPlan &p, int width)
     slices_free(p);
     if (width == 0 || p.nrows == 0) return HL_OK;
     sell_free(p);
+    staged_free(p);
     const int n = p.nrows;
     const long long nnz = p.nnz;
     std::vector<int> R(static_cast<size_t>(n) + 1), C(static_cast<size_t>(nnz));
@@ -680,8 +976,217 @@ int plan_sell(CsrPlan &p, bool on, double max_fill)
         return rc;
     }
     slices_free(p);
+    staged_free(p);
     p.nchunks = nch;
     p.sell_on = true;
+    return HL_OK;
+}
+
+// The staged-slice form (see CsrPlan).  Host work: pieces generated row by row
+// (slices ascending, each slice's entries in the row's own order, cut every
+// kStagedPiece entries; this is also each row's summation order), row bins cut
+// greedily at kBinCap pieces, pieces bucketed by slice (row order kept) and
+// sorted by descending length within each (slice, bin) run, each slice padded
+// to whole 64-piece chunks.
+int plan_staged(CsrPlan &p, bool on)
+{
+    staged_free(p);
+    if (!on || p.nrows == 0) return HL_OK;
+    constexpr int lw = kStagedLw, bincap = kBinCap;
+    const int n = p.nrows;
+    const long long nnz = p.nnz;
+    const long long nsl_ll = (static_cast<long long>(p.ncols) + (1LL << lw) - 1) >> lw;
+    if (nsl_ll > kStagedMaxSlices) {
+        set_error("SpMV staged slices: %lld slices of 2^%d columns (at most %d)", nsl_ll, lw, kStagedMaxSlices);
+        return HL_ERR_ARG;
+    }
+    const int nsl = static_cast<int>(std::max(nsl_ll, 1LL));
+    std::vector<int> R(static_cast<size_t>(n) + 1), C(static_cast<size_t>(nnz));
+    std::vector<double> A(static_cast<size_t>(nnz));
+    HL_TRY_HIP(hipMemcpy(R.data(), p.row_ptr, sizeof(int) * R.size(), hipMemcpyDeviceToHost));
+    if (nnz > 0) {
+        HL_TRY_HIP(hipMemcpy(C.data(), p.col, sizeof(int) * C.size(), hipMemcpyDeviceToHost));
+        HL_TRY_HIP(hipMemcpy(A.data(), p.val, sizeof(double) * A.size(), hipMemcpyDeviceToHost));
+    }
+    // pass 1: pieces per row and per slice
+    std::vector<int> last(static_cast<size_t>(nsl), -1), cnt(static_cast<size_t>(nsl), 0), touched;
+    std::vector<long long> lpp(static_cast<size_t>(n) + 1, 0), per_slice(static_cast<size_t>(nsl), 0);
+    for (int r = 0; r < n; ++r) {
+        touched.clear();
+        for (int q = R[r]; q < R[r + 1]; ++q) {
+            const int sl = C[q] >> lw;
+            if (last[sl] != r) {
+                last[sl] = r;
+                cnt[sl] = 0;
+                touched.push_back(sl);
+            }
+            ++cnt[sl];
+        }
+        long long k = 0;
+        for (int sl : touched) {
+            const int pc = (cnt[sl] + kStagedPiece - 1) / kStagedPiece;
+            k += pc;
+            per_slice[sl] += pc;
+        }
+        if (k > bincap) {
+            set_error("SpMV staged slices: row %d makes %lld pieces (at most %d)", r, k, bincap);
+            return HL_ERR_ARG;
+        }
+        lpp[r + 1] = lpp[r] + k;
+    }
+    const long long np = lpp[n];
+    // row bins of <= bincap pieces
+    std::vector<int> brow{0};
+    for (int r = 0; r < n; ++r)
+        if (lpp[r + 1] - lpp[brow.back()] > bincap) brow.push_back(r);
+    brow.push_back(n);
+    const int nbins = static_cast<int>(brow.size()) - 1;
+    // padded piece index ranges of the slices
+    std::vector<long long> sbase(static_cast<size_t>(nsl) + 1, 0);
+    for (int sl = 0; sl < nsl; ++sl) sbase[sl + 1] = sbase[sl] + (per_slice[sl] + kWave - 1) / kWave * kWave;
+    const long long npad = sbase[nsl];
+    if (npad > 0x7fffffffLL) {
+        set_error("SpMV staged slices: %lld pieces exceed int32", npad);
+        return HL_ERR_ARG;
+    }
+    // pass 2: pieces in generation order g (row, slice, cut), entries regrouped
+    // so that each piece's entries are contiguous
+    std::vector<unsigned short> g_sl(static_cast<size_t>(np)), g_len(static_cast<size_t>(np));
+    std::vector<long long> g_e(static_cast<size_t>(np));
+    std::vector<int> gc(static_cast<size_t>(nnz));
+    std::vector<double> ga(static_cast<size_t>(nnz));
+    std::vector<int> start(static_cast<size_t>(nsl), 0);
+    std::fill(last.begin(), last.end(), -1);
+    for (int r = 0; r < n; ++r) {
+        touched.clear();
+        for (int q = R[r]; q < R[r + 1]; ++q) {
+            const int sl = C[q] >> lw;
+            if (last[sl] != r) {
+                last[sl] = r;
+                cnt[sl] = 0;
+                touched.push_back(sl);
+            }
+            ++cnt[sl];
+        }
+        std::sort(touched.begin(), touched.end());
+        int at = R[r];
+        long long g = lpp[r];
+        for (int sl : touched) {
+            start[sl] = at;
+            for (int k = 0; k < cnt[sl]; k += kStagedPiece) {
+                g_sl[g] = static_cast<unsigned short>(sl);
+                g_len[g] = static_cast<unsigned short>(std::min(kStagedPiece, cnt[sl] - k));
+                g_e[g] = at + k;
+                ++g;
+            }
+            at += cnt[sl];
+        }
+        for (int q = R[r]; q < R[r + 1]; ++q) {
+            const int sl = C[q] >> lw;
+            gc[start[sl]] = C[q] - (sl << lw);
+            ga[start[sl]++] = A[q];
+        }
+    }
+    // bucket by slice (generation order, hence row order, kept), then order each
+    // (slice, bin) run by descending length (ties: generation order)
+    std::vector<long long> fill(sbase.begin(), sbase.end() - 1);
+    std::vector<int> at_pos(static_cast<size_t>(npad), -1); // padded index -> generation index
+    for (long long g = 0; g < np; ++g) at_pos[fill[g_sl[g]]++] = static_cast<int>(g);
+    // run table: first padded index of (bin, slice); row b = nbins holds the ends
+    std::vector<int> run(static_cast<size_t>(nbins + 1) * nsl);
+    for (int sl = 0; sl < nsl; ++sl) {
+        long long i = sbase[sl];
+        const long long e = sbase[sl] + per_slice[sl];
+        for (int b = 0; b < nbins; ++b) {
+            run[static_cast<size_t>(b) * nsl + sl] = static_cast<int>(i);
+            const long long i0 = i;
+            // pieces of rows below brow[b + 1]: their generation index is below lpp[brow[b + 1]]
+            while (i < e && at_pos[i] < lpp[brow[b + 1]]) ++i;
+            std::stable_sort(at_pos.begin() + i0, at_pos.begin() + i,
+                             [&](int x, int y) { return g_len[x] > g_len[y]; });
+        }
+        run[static_cast<size_t>(nbins) * nsl + sl] = static_cast<int>(e);
+    }
+    // LDS offset of every piece within its bin, listed in generation order
+    std::vector<unsigned short> lst(static_cast<size_t>(np));
+    for (int b = 0; b < nbins; ++b) {
+        int o = 0;
+        for (int sl = 0; sl < nsl; ++sl) {
+            const int i0 = run[static_cast<size_t>(b) * nsl + sl], i1 = run[static_cast<size_t>(b + 1) * nsl + sl];
+            for (int i = i0; i < i1; ++i) lst[at_pos[i]] = static_cast<unsigned short>(o + (i - i0));
+            o += i1 - i0;
+        }
+        if (o > bincap) {
+            set_error("SpMV staged slices: bin %d holds %d pieces", b, o);
+            return HL_ERR_ARG;
+        }
+    }
+    // chunks (column-major, as wide as their longest piece), lengths and entries
+    const long long nch = npad / kWave;
+    std::vector<long long> cptr(static_cast<size_t>(nch) + 1, 0);
+    std::vector<unsigned short> plen(static_cast<size_t>(npad), 0);
+    for (long long c = 0; c < nch; ++c) {
+        int w = 0;
+        for (int l = 0; l < kWave; ++l) {
+            const int g = at_pos[c * kWave + l];
+            if (g >= 0) {
+                plen[c * kWave + l] = g_len[g];
+                w = std::max(w, static_cast<int>(g_len[g]));
+            }
+        }
+        cptr[c + 1] = cptr[c] + static_cast<long long>(kWave) * w;
+    }
+    std::vector<unsigned short> ec(static_cast<size_t>(cptr[nch]), 0);
+    std::vector<double> ev(static_cast<size_t>(cptr[nch]), 0.0);
+    for (long long c = 0; c < nch; ++c)
+        for (int l = 0; l < kWave; ++l) {
+            const int g = at_pos[c * kWave + l];
+            if (g < 0) continue;
+            for (int k = 0; k < g_len[g]; ++k) {
+                const long long e = cptr[c] + static_cast<long long>(k) * kWave + l;
+                ec[e] = static_cast<unsigned short>(gc[g_e[g] + k]);
+                ev[e] = ga[g_e[g] + k];
+            }
+        }
+    // blocks of the first kernel: runs of chunks within a slice, about
+    // kStagedSupEntries entries each
+    std::vector<int4> sup;
+    for (int sl = 0; sl < nsl; ++sl) {
+        const long long c0 = sbase[sl] / kWave, c1 = sbase[sl + 1] / kWave;
+        long long c = c0;
+        while (c < c1) {
+            const long long a = c;
+            long long ent = 0;
+            while (c < c1 && (c == a || ent + (cptr[c + 1] - cptr[c]) <= kStagedSupEntries)) {
+                ent += cptr[c + 1] - cptr[c];
+                ++c;
+            }
+            sup.push_back(make_int4(sl, static_cast<int>(a), static_cast<int>(c), 0));
+        }
+    }
+    std::vector<int> lpp32(static_cast<size_t>(n) + 1);
+    for (int r = 0; r <= n; ++r) lpp32[r] = static_cast<int>(lpp[r]);
+    int rc;
+    if ((rc = dev_upload(&p.s_cptr, cptr.data(), cptr.size())) ||
+        (rc = dev_upload(&p.s_plen, plen.data(), plen.size())) ||
+        (rc = dev_upload(&p.s_ecol, ec.data(), ec.size())) || (rc = dev_upload(&p.s_eval, ev.data(), ev.size())) ||
+        (rc = dev_upload(&p.s_sup, sup.data(), sup.size())) ||
+        (rc = dev_upload(&p.s_brow, brow.data(), brow.size())) ||
+        (rc = dev_upload(&p.s_run, run.data(), run.size())) ||
+        (rc = dev_upload(&p.s_lpp, lpp32.data(), lpp32.size())) ||
+        (rc = dev_upload(&p.s_lst, lst.data(), lst.size())) ||
+        (rc = dev_upload<double>(&p.s_spart, nullptr, static_cast<size_t>(npad)))) {
+        staged_free(p);
+        return rc;
+    }
+    sell_free(p);
+    slices_free(p);
+    p.stg_on = true;
+    p.stg_nsl = nsl;
+    p.stg_nchunks = static_cast<int>(nch);
+    p.stg_nsup = static_cast<int>(sup.size());
+    p.stg_nbins = nbins;
+    p.stg_npieces = np;
     return HL_OK;
 }
 
@@ -689,6 +1194,7 @@ void csr_plan_free(CsrPlan &p)
 {
     sell_free(p);
     slices_free(p);
+    staged_free(p);
     items_free(p.items);
     void *ptrs[] = {p.row_ptr, p.col, p.val, p.val32, p.x_stage, p.y_stage};
     for (void *q : ptrs)
@@ -713,6 +1219,15 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
             spmv_sell_kernel<<<(p.nchunks + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, s>>>(
                 p.nchunks, p.nrows, p.e_ptr, p.e_col, p.e_val, p.row_ptr, x, xscale, y, vcol.base, vcol.L.ld, vcol.j,
                 vcol.L.tiled);
+    } else if (p.stg_on) { // partials of the pieces (x slices in LDS), then the row bins
+        if (p.stg_nsup > 0)
+            spmv_staged_kernel<<<p.stg_nsup, kStagedThreads, 0, s>>>(p.s_sup, p.ncols, p.s_cptr, p.s_plen, p.s_ecol,
+                                                                     p.s_eval, x, xscale, p.s_spart);
+        HL_TRY_HIP(hipGetLastError());
+        if (p.stg_nbins > 0)
+            spmv_bins_kernel<<<p.stg_nbins, kBinThreads, bins_dyn_lds(p.stg_nsl), s>>>(
+                p.stg_nsl, p.s_brow, p.s_run, p.s_lpp, p.s_lst, p.s_spart, x, xscale, y, vcol.base, vcol.L.ld, vcol.j,
+                vcol.L.tiled);
     } else if (p.slice_w > 0) { // partials of the virtual rows, then the row sums
         if (p.sitems.nblk > 0)
             spmv_adaptive<double, double><<<p.sitems.nblk, kBlock, 0, s>>>(
@@ -735,6 +1250,17 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
 }  // namespace hl
 
 namespace hl {
+
+int launch_spmv_staged_seg(const SegMember *M, const SegStep *S, int ns, int stg_blocks, int bin_blocks, int nsl_max,
+                           hipStream_t s)
+{
+    if (ns <= 0) return HL_OK;
+    if (stg_blocks > 0) spmv_staged_seg_kernel<<<stg_blocks, kStagedThreads, 0, s>>>(M, S, ns);
+    HL_TRY_HIP(hipGetLastError());
+    if (bin_blocks > 0) spmv_bins_seg_kernel<<<bin_blocks, kBinThreads, bins_dyn_lds(nsl_max), s>>>(M, S, ns);
+    HL_TRY_HIP(hipGetLastError());
+    return HL_OK;
+}
 
 int launch_spmv_rows_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s)
 {

@@ -143,14 +143,33 @@ int hl_csr_get_slices(hl_csr_t h, int *width, int *npairs); /* outputs may be NU
  * reference. */
 int hl_csr_set_sell(hl_csr_t h, int on);
 int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries); /* outputs may be NULL */
+/* Staged-slice SpMV: on != 0 cuts the FP64 matrix into column slices of 16384
+ * columns, few enough that a block holds a slice's x (128 KB) in LDS, so that
+ * no x gather leaves the CU.  Each row's entries of a slice (cut every 64
+ * entries) form a piece; pieces are stored slice-major in ELL chunks of 64, a
+ * lane per piece (16-bit slice-local columns), and a first kernel writes one
+ * partial per piece.  A second kernel takes a bin of rows holding at most 4096
+ * pieces, copies the bin's partials into LDS and adds each row's partials in
+ * slice order.  Suits graphs whose gathers miss the caches (power-law and
+ * random graphs).  on = 0 drops the form; building it drops the column slices
+ * and the sliced-ELL form (and they drop it).  Refused (HL_ERR_ARG) beyond
+ * 4000 slices (65.5 M columns) or for a row of more than 4096 pieces.  Used
+ * by every FP64-value SpMV of the operator; costs 10 bytes per stored entry
+ * (entries are padded to their chunk's longest piece) plus about 12 per piece
+ * on the device and 8 bytes per piece per solve.  Same rules as
+ * hl_csr_set_items.  No counterpart in the reference. */
+int hl_csr_set_staged(hl_csr_t h, int on);
+/* on, pieces, stored entries (nnz plus padding), row bins; outputs may be NULL. */
+int hl_csr_get_staged(hl_csr_t h, int *on, long long *pieces, long long *entries, int *bins);
 /* Autotuner: times `reps` SpMV launches for every shape of a small grid
  * (nnz_cap in {512, 1024, 2048} x rows_cap in {64 .. 2048}), and, for
  * operators of >= 2^20 columns, every shape again under column slices of 2^19,
  * 2^18 and 2^17 columns (widest first; a narrower width must win by 2 %), then
- * the sliced-ELL form where it holds at most 1.5 nnz entries, with HIP events
- * on the null stream; keeps the fastest and reports its item shape (outputs
- * may be NULL; hl_csr_get_slices and hl_csr_get_sell tell the form).  Same
- * rules as hl_csr_set_items. */
+ * the sliced-ELL form where it holds at most 1.5 nnz entries, then, for
+ * operators of >= 65536 nonzeros, the staged-slice form (which must win by
+ * 2 %), with HIP events on the null stream; keeps the fastest and reports its
+ * item shape (outputs may be NULL; hl_csr_get_slices, hl_csr_get_sell and
+ * hl_csr_get_staged tell the form).  Same rules as hl_csr_set_items. */
 int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *seconds);
 
 /* ------------------------------------------------------ basis operations
