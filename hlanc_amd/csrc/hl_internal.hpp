@@ -215,6 +215,7 @@ constexpr int kStagedThreads = 1024;     // threads per block of the first stage
 constexpr int kBinThreads = 512;         // threads per block of the second (bins) kernel
 constexpr int kStagedPiece = 64;         // entries per piece
 constexpr int kBinCap = 4096;            // pieces per row bin (their partials: 32 KB of LDS)
+constexpr int kBinStride = 16;           // the bins kernel looks up the run of every 16th partial
 constexpr int kStagedMaxSlices = 4000;   // slices (the bins kernel keeps two ints per slice in LDS)
 constexpr int kStagedSupEntries = 65536; // entries per block of the first kernel (about)
 

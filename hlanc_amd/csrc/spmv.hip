@@ -445,7 +445,8 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int *__restrict
                                            const double *__restrict__ part_, const double *__restrict__ x_, double s,
                                            double *__restrict__ y_, VT *__restrict__ vout_,
                                            float *__restrict__ vout32_, long long vld, int vj, bool vtiled,
-                                           double *ps, unsigned short *lsh, int *roff, int *rdel, int *wtot)
+                                           double *ps, unsigned short *lsh, unsigned short *coarse, int *roff,
+                                           int *rdel, int *wtot)
 {
     const auto brow = dev::gptr(brow_);
     const auto run = dev::gptr(run_);
@@ -501,26 +502,34 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int *__restrict
     }
     if (tid == T - 1) roff[nsl] = ex; // the bin's total (= nl)
     __syncthreads();
-    // the copy: LDS index i comes from run sl (the last run starting at or
-    // before i; empty runs never are); all of a thread's loads in flight, the
-    // bin's offsets with them
+    // the copy: LDS index i comes from run sl, the last run starting at or
+    // before i (empty runs never are).  A coarse table gives the run of every
+    // 16th index (a binary search each), and each index walks on from there;
+    // all of a thread's loads in flight, the bin's offsets with them
     constexpr int kCopy = kBinCap / T;
     const int total = roff[nsl];
+    for (int q = tid; q < (total + kBinStride - 1) / kBinStride; q += T) {
+        const int i = q * kBinStride;
+        int lo = 0, hi = nsl - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (roff[mid] <= i)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        coarse[q] = static_cast<unsigned short>(lo);
+    }
+    __syncthreads();
     double v[kCopy];
     int li[kCopy];
 #pragma unroll
     for (int u = 0; u < kCopy; ++u) {
         const int i = tid + u * T;
         if (i < total) {
-            int lo = 0, hi = nsl - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (roff[mid] <= i)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            v[u] = part[rdel[lo] + i];
+            int sl = coarse[i / kBinStride];
+            while (roff[sl + 1] <= i) ++sl;
+            v[u] = part[rdel[sl] + i];
         }
         if (i < nl) li[u] = lst[l0 + i];
     }
@@ -578,10 +587,11 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_kernel(int nsl, const i
 {
     __shared__ double ps[kBinCap];
     __shared__ unsigned short lsh[kBinCap];
+    __shared__ unsigned short coarse[kBinCap / kBinStride];
     __shared__ int wtot[kBinThreads / kWave];
     extern __shared__ int bins_dyn[];
-    bins_block<double>(blockIdx.x, nsl, brow, run, lpp, lst, part, x, xscale ? *xscale : 1.0, y, vout, nullptr,
-                            vld, vj, vtiled, ps, lsh, bins_dyn, bins_dyn + nsl + 1, wtot);
+    bins_block<double>(blockIdx.x, nsl, brow, run, lpp, lst, part, x, xscale ? *xscale : 1.0, y, vout, nullptr, vld,
+                       vj, vtiled, ps, lsh, coarse, bins_dyn, bins_dyn + nsl + 1, wtot);
 }
 
 // The staged-slice SpMVs of every listed solve: blocks numbered by stg0
@@ -614,12 +624,13 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_seg_kernel(const SegMem
 {
     __shared__ double ps[kBinCap];
     __shared__ unsigned short lsh[kBinCap];
+    __shared__ unsigned short coarse[kBinCap / kBinStride];
     __shared__ int wtot[kBinThreads / kWave];
     extern __shared__ int bins_dyn[];
     const SegStep st = S[seg_find_stg(S, ns, blockIdx.x, true)];
     const SegMember mb = M[st.mem];
     bins_block<double>(blockIdx.x - st.bin0, mb.stg_nsl, mb.s_brow, mb.s_run, mb.s_lpp, mb.s_lst, mb.part, mb.r,
-                       mb.ctl->xscale, mb.w, static_cast<double *>(mb.V), mb.V32, mb.vld, st.j, true, ps, lsh,
+                       mb.ctl->xscale, mb.w, static_cast<double *>(mb.V), mb.V32, mb.vld, st.j, true, ps, lsh, coarse,
                        bins_dyn, bins_dyn + mb.stg_nsl + 1, wtot);
 }
 
