@@ -371,7 +371,16 @@ __device__ __forceinline__ void staged_block(const int4 su, long long ncols, con
     constexpr int NW = kStagedThreads / kWave;
     const long long c0 = static_cast<long long>(su.x) << kStagedLw;
     const int nx = static_cast<int>(min(static_cast<long long>(W), ncols - c0));
-    for (int i = threadIdx.x; i < W; i += kStagedThreads) xs[i] = i < nx ? x[c0 + i] : 0.0;
+    if (nx == W && (reinterpret_cast<uintptr_t>(x_ + c0) & 15) == 0) { // a whole, aligned slice: 16-byte loads
+        const double2 *x2 = reinterpret_cast<const double2 *>(x_ + c0);
+        for (int i = threadIdx.x; i < W / 2; i += kStagedThreads) {
+            const double2 t = x2[i];
+            xs[2 * i] = t.x;
+            xs[2 * i + 1] = t.y;
+        }
+    } else {
+        for (int i = threadIdx.x; i < W; i += kStagedThreads) xs[i] = i < nx ? x[c0 + i] : 0.0;
+    }
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = static_cast<int>(threadIdx.x) / kWave;
     // chunk bounds and piece lengths of a wave's group of chunks; the next
@@ -428,7 +437,7 @@ __device__ __forceinline__ void staged_block(const int4 su, long long ncols, con
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int c = cb + g * NW;
-            if (c < su.z) part[static_cast<long long>(c) * kWave + lane] = acc[g];
+            if (c < su.z) __builtin_nontemporal_store(acc[g], &part[static_cast<long long>(c) * kWave + lane]);
         }
     }
 }
@@ -529,7 +538,7 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int *__restrict
         if (i < total) {
             int sl = coarse[i / kBinStride];
             while (roff[sl + 1] <= i) ++sl;
-            v[u] = part[rdel[sl] + i];
+            v[u] = __builtin_nontemporal_load(&part[rdel[sl] + i]); // read once
         }
         if (i < nl) li[u] = lst[l0 + i];
     }
