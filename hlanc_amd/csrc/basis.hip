@@ -469,12 +469,12 @@ __device__ __forceinline__ PassPtrs pass_ptrs(const SegMember &mb, int mode)
 // step cancelled strongly (ctl->exact): then an FP64 launch of its own runs it.
 template <int MAXC, int MODE, typename VT, int TB = 2>
 __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__restrict__ M,
-                                                         const SegStep *__restrict__ S, int ns, int sub)
+                                                         const SegStep *__restrict__ S, int ns, int sub, int solo)
 {
     constexpr bool F32 = std::is_same<VT, float>::value;
     const int i = seg_find(sub ? &S[0].cgs0s : &S[0].cgs0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
     const SegStep st = S[i];
-    const SegMember mb = M[st.mem];
+    const SegMember mb = M[solo ? 0 : st.mem];
     if (MODE == EPI_PASS2 && mb.V32 && (mb.ctl->exact != 0) == F32) return; // the other launch's step
     const PassPtrs p = pass_ptrs(mb, MODE);
     const int bid = blockIdx.x - (sub ? st.cgs0s : st.cgs0);
@@ -505,11 +505,11 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_kernel(const SegMember *__rest
 // this way gives the fused pass's sums.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void cgs_seg_sub_kernel(const SegMember *__restrict__ M,
-                                                             const SegStep *__restrict__ S, int ns, int sub)
+                                                             const SegStep *__restrict__ S, int ns, int sub, int solo)
 {
     const int i = seg_find(sub ? &S[0].cgs0s : &S[0].cgs0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
     const SegStep st = S[i];
-    const SegMember mb = M[st.mem];
+    const SegMember mb = M[solo ? 0 : st.mem];
     if (MODE == EPI_PASS2 && mb.V32 && mb.ctl->exact == 0) return; // the FP32 launch's step
     if ((MODE == EPI_PASS2 || MODE == EPI_PASS3) && !mb.ctl->need) return;
     const PassPtrs p = pass_ptrs(mb, MODE);
@@ -574,11 +574,11 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_sub_kernel(const SegMember *__
 
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void cgs_seg_proj_kernel(const SegMember *__restrict__ M,
-                                                              const SegStep *__restrict__ S, int ns, int sub)
+                                                              const SegStep *__restrict__ S, int ns, int sub, int solo)
 {
     const int i = seg_find(sub ? &S[0].cgs0s : &S[0].cgs0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
     const SegStep st = S[i];
-    const SegMember mb = M[st.mem];
+    const SegMember mb = M[solo ? 0 : st.mem];
     if (MODE == EPI_PASS2 && mb.V32 && mb.ctl->exact == 0) return;
     if ((MODE == EPI_PASS2 || MODE == EPI_PASS3) && !mb.ctl->need) return;
     const PassPtrs p = pass_ptrs(mb, MODE);
@@ -640,11 +640,12 @@ __global__ __launch_bounds__(kBlock) void cgs_seg_proj_kernel(const SegMember *_
 }
 
 __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMember *__restrict__ M,
-                                                                  const SegStep *__restrict__ S, int ns, int mode)
+                                                                  const SegStep *__restrict__ S, int ns, int mode,
+                                                                  int solo)
 {
     const int i = seg_find(&S[0].fin0, sizeof(SegStep) / sizeof(int), ns, blockIdx.x);
     const SegStep st = S[i];
-    const SegMember mb = M[st.mem];
+    const SegMember mb = M[solo ? 0 : st.mem];
     const PassPtrs p = pass_ptrs(mb, mode);
     const int c = blockIdx.x - st.fin0;
     if (mode == EPI_WNORM && mb.V32) {
@@ -669,10 +670,10 @@ __global__ __launch_bounds__(kBlock) void cgs_finalize_seg_kernel(const SegMembe
 // would move r by at most tau ||r||, so it is skipped (ctl->need = 0).  One
 // block per listed step.
 __global__ __launch_bounds__(kBlock) void dgks_decide_seg_kernel(const SegMember *__restrict__ M,
-                                                                 const SegStep *__restrict__ S, double tau2)
+                                                                 const SegStep *__restrict__ S, double tau2, int solo)
 {
     const SegStep st = S[blockIdx.x];
-    const SegMember mb = M[st.mem];
+    const SegMember mb = M[solo ? 0 : st.mem];
     if (!mb.V32 || tau2 <= 0.0) return;
     __shared__ double red[kBlock / kWave];
     double s2 = 0.0;
@@ -1506,9 +1507,10 @@ int launch_scale_copy(int n, const double *r, const Ctl *ctl, double *v, VColumn
 
 namespace {
 template <int MODE, typename VT, int TB = 2>
-void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg, int blocks, int sub, hipStream_t s)
+void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg, int blocks, int sub, int solo,
+                      hipStream_t s)
 {
-#define HL_SEG(MC) cgs_seg_kernel<MC, MODE, VT, TB><<<blocks, kBlock, 0, s>>>(M, S, ns, sub)
+#define HL_SEG(MC) cgs_seg_kernel<MC, MODE, VT, TB><<<blocks, kBlock, 0, s>>>(M, S, ns, sub, solo)
     if (max_cpg <= 2)
         HL_SEG(2);
     else if (max_cpg <= 4)
@@ -1526,7 +1528,7 @@ void cgs_seg_dispatch(const SegMember *M, const SegStep *S, int ns, int max_cpg,
 } // namespace
 
 int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total, int mode,
-                   bool v32, bool sub, hipStream_t s)
+                   bool v32, bool sub, bool solo, hipStream_t s)
 {
     if (ns <= 0) return HL_OK;
     if (max_cpg > kNarrowCols / kGroups) { // a step wider than the fused pass holds: two kernels for all
@@ -1535,8 +1537,8 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
             return HL_ERR_ARG;
         }
 #define HL_WIDE(MODE)                                                                         \
-    cgs_seg_sub_kernel<MODE><<<cgs_blocks_total, kBlock, 0, s>>>(M, S, ns, sub ? 1 : 0);     \
-    cgs_seg_proj_kernel<MODE><<<cgs_blocks_total, kBlock, 0, s>>>(M, S, ns, sub ? 1 : 0)
+    cgs_seg_sub_kernel<MODE><<<cgs_blocks_total, kBlock, 0, s>>>(M, S, ns, sub ? 1 : 0, solo ? 1 : 0); \
+    cgs_seg_proj_kernel<MODE><<<cgs_blocks_total, kBlock, 0, s>>>(M, S, ns, sub ? 1 : 0, solo ? 1 : 0)
         switch (mode) {
         case EPI_WNORM:
             HL_WIDE(EPI_WNORM);
@@ -1559,9 +1561,10 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
         return HL_OK;
     }
 #define HL_CGS_MODE(MODE)                                                                              \
-    (!v32 ? cgs_seg_dispatch<MODE, double>(M, S, ns, max_cpg, cgs_blocks_total, sub, s)                \
-     : tuning().cgs_f32_tiles == 1 ? cgs_seg_dispatch<MODE, float, 1>(M, S, ns, max_cpg, cgs_blocks_total, sub, s) \
-                                   : cgs_seg_dispatch<MODE, float, 2>(M, S, ns, max_cpg, cgs_blocks_total, sub, s))
+    (!v32 ? cgs_seg_dispatch<MODE, double>(M, S, ns, max_cpg, cgs_blocks_total, sub, solo, s)           \
+     : tuning().cgs_f32_tiles == 1                                                                    \
+         ? cgs_seg_dispatch<MODE, float, 1>(M, S, ns, max_cpg, cgs_blocks_total, sub, solo, s)        \
+         : cgs_seg_dispatch<MODE, float, 2>(M, S, ns, max_cpg, cgs_blocks_total, sub, solo, s))
     switch (mode) {
     case EPI_WNORM:
         HL_CGS_MODE(EPI_WNORM);
@@ -1584,19 +1587,19 @@ int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, in
     return HL_OK;
 }
 
-int launch_dgks_decide_seg(const SegMember *M, const SegStep *S, int ns, double tau, hipStream_t s)
+int launch_dgks_decide_seg(const SegMember *M, const SegStep *S, int ns, double tau, bool solo, hipStream_t s)
 {
     if (ns <= 0 || tau <= 0.0) return HL_OK;
-    dgks_decide_seg_kernel<<<ns, kBlock, 0, s>>>(M, S, tau * tau);
+    dgks_decide_seg_kernel<<<ns, kBlock, 0, s>>>(M, S, tau * tau, solo ? 1 : 0);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }
 
 int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fin_blocks_total, int mode,
-                            hipStream_t s)
+                            bool solo, hipStream_t s)
 {
     if (ns <= 0) return HL_OK;
-    cgs_finalize_seg_kernel<<<fin_blocks_total, kBlock, 0, s>>>(M, S, ns, mode);
+    cgs_finalize_seg_kernel<<<fin_blocks_total, kBlock, 0, s>>>(M, S, ns, mode, solo ? 1 : 0);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }

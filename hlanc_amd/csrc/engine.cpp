@@ -765,6 +765,7 @@ int Engine::sweep(std::vector<Member *> &ms)
         any_host |= (m->A == nullptr);
         any_proj32 |= m->kind == SEG_PROJ32;
     }
+    const bool solo = nm == 1; // every entry's member is entry 0 of the table
     auto launch_steps = [&]() -> int {
         int r;
         for (int st = 0; st < maxsteps; ++st) {
@@ -784,15 +785,15 @@ int Engine::sweep(std::vector<Member *> &ms)
                 for (int g = 0; g < 3; ++g)
                     if (nspmv[st][g] > 0 &&
                         (r = launch_spmv_seg(dM, dS + goff_spmv[st][g], goff_spmv[st][g + 1] - goff_spmv[st][g],
-                                             nspmv[st][g], kSpmvKind[g], stream)))
+                                             nspmv[st][g], kSpmvKind[g], solo, stream)))
                         return r;
                 if (nred[st] > 0 && // row sums of the sliced SpMVs (FP64 bases: the last group)
                     (r = launch_spmv_rows_seg(dM, dS + goff_spmv[st][2], goff_spmv[st][3] - goff_spmv[st][2],
-                                              nred[st], stream)))
+                                              nred[st], solo, stream)))
                     return r;
                 if (nstg[st] > 0 && // staged-slice SpMVs (FP64 bases: the last group)
                     (r = launch_spmv_staged_seg(dM, dS + goff_spmv[st][2], goff_spmv[st][3] - goff_spmv[st][2],
-                                                nstg[st], nbin[st], nslx[st], stream)))
+                                                nstg[st], nbin[st], nslx[st], solo, stream)))
                     return r;
                 if ((r = prof.end(stream, K_SPMV, bytes))) return r;
             }
@@ -811,17 +812,17 @@ int Engine::sweep(std::vector<Member *> &ms)
                 if ((r = prof.begin(stream))) return r;
                 for (int g = 0; g < 2; ++g) // group 0 reads FP32 bases, group 1 FP64
                     if (nb[g] > 0 && (r = launch_cgs_seg(dM, dS + go[g], go[g + 1] - go[g], cp[g], nb[g], modes[p],
-                                                         g == 0, sub, stream)))
+                                                         g == 0, sub, solo, stream)))
                         return r;
                 if (modes[p] == EPI_PASS2 && n32[st] > 0 && // strongly cancelling PROJ32 steps, on FP64
                     (r = launch_cgs_seg(dM, dS + goff_sub[st][1], n32[st], cpg32[st], nb32[st], modes[p], false, true,
-                                        stream)))
+                                        solo, stream)))
                     return r;
                 if ((r = prof.end(stream, p < 2 ? K_CGS : K_DGKS, bytes))) return r;
                 if ((r = prof.begin(stream))) return r;
-                if ((r = launch_cgs_seg_finalize(dM, S, cnt, nfin[st], modes[p], stream))) return r;
+                if ((r = launch_cgs_seg_finalize(dM, S, cnt, nfin[st], modes[p], solo, stream))) return r;
                 if (modes[p] == EPI_PASS1 && any_proj32 && tuning().dgks_skip_exp > 0 &&
-                    (r = launch_dgks_decide_seg(dM, S, cnt, std::pow(10.0, -tuning().dgks_skip_exp), stream)))
+                    (r = launch_dgks_decide_seg(dM, S, cnt, std::pow(10.0, -tuning().dgks_skip_exp), solo, stream)))
                     return r;
                 if ((r = prof.end(stream, K_MISC, 0.0))) return r;
             }
@@ -836,6 +837,7 @@ int Engine::sweep(std::vector<Member *> &ms)
         key.reserve(2 + 28 * (size_t)maxsteps);
         key.push_back(reinterpret_cast<long long>(dM));
         key.push_back(reinterpret_cast<long long>(dS));
+        key.push_back(solo ? 1 : 0);
         for (int st = 0; st < maxsteps; ++st) {
             key.push_back(off[st]);
             for (int g = 0; g < 4; ++g) {
@@ -1157,10 +1159,12 @@ int Engine::verify_pairs(std::vector<Member *> &ver)
         for (int i = 0; i < maxnev; ++i) {
             const int cnt = off[i + 1] - off[i];
             if (cnt <= 0) continue;
-            if (nblk[i] > 0 && (rc = launch_spmv_seg(dMv, dSv + off[i], cnt, nblk[i], SEG_F64, stream))) return rc;
-            if (nred[i] > 0 && (rc = launch_spmv_rows_seg(dMv, dSv + off[i], cnt, nred[i], stream))) return rc;
+            const bool solo = ne == 1; // one pair: the table's only entry
+            if (nblk[i] > 0 && (rc = launch_spmv_seg(dMv, dSv + off[i], cnt, nblk[i], SEG_F64, solo, stream)))
+                return rc;
+            if (nred[i] > 0 && (rc = launch_spmv_rows_seg(dMv, dSv + off[i], cnt, nred[i], solo, stream))) return rc;
             if (nstg[i] > 0 &&
-                (rc = launch_spmv_staged_seg(dMv, dSv + off[i], cnt, nstg[i], nbin[i], nslx[i], stream)))
+                (rc = launch_spmv_staged_seg(dMv, dSv + off[i], cnt, nstg[i], nbin[i], nslx[i], solo, stream)))
                 return rc;
         }
     }

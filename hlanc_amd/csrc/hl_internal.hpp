@@ -246,7 +246,9 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
 // alone (same block count, same partial-sum order), so results are bitwise the
 // same alone and in a batch.
 // Precision kinds of a segment: each segmented launch serves solves of one
-// kind (the engine groups a step's solves by kind).
+// kind (the engine groups a step's solves by kind).  `solo` launches serve
+// member 0 of the table alone (an engine with one solve): their blocks read
+// the member without waiting for their step entry.
 enum SegKind {
     SEG_F64 = 0,     // FP64 basis and values
     SEG_V32 = 1,     // FP32 basis, FP64 values
@@ -313,12 +315,12 @@ struct SegStep {                 // per solve and Lanczos step
 // v32: the launch reads FP32 bases (SegMember::Vproj); sub: its blocks are
 // numbered by SegStep::cgs0s (else cgs0).
 int launch_cgs_seg(const SegMember *M, const SegStep *S, int ns, int max_cpg, int cgs_blocks_total, int mode,
-                   bool v32, bool sub, hipStream_t s);
+                   bool v32, bool sub, bool solo, hipStream_t s);
 int launch_cgs_seg_finalize(const SegMember *M, const SegStep *S, int ns, int fin_blocks_total, int mode,
-                            hipStream_t s);
+                            bool solo, hipStream_t s);
 // Skips the pending first correction of FP32-copy solves whose exact projection
 // h1 satisfies ||h1|| <= tau ||r|| (one block per listed step; tau <= 0: none).
-int launch_dgks_decide_seg(const SegMember *M, const SegStep *S, int ns, double tau, hipStream_t s);
+int launch_dgks_decide_seg(const SegMember *M, const SegStep *S, int ns, double tau, bool solo, hipStream_t s);
 
 // Launch-cost probes (probes.hip): the co-resident block count of the grid
 // barrier kernel, its cooperative launch (buf: 2 blocks x kBlock doubles;
@@ -327,16 +329,16 @@ int grid_barrier_blocks_max(int dev);
 int launch_grid_barrier(int blocks, unsigned *counter, int *abort, double *buf, int barriers, int *errors,
                         hipStream_t s);
 int launch_empty(int blocks, hipStream_t s);
-int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, hipStream_t s);
+int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, bool solo, hipStream_t s);
 // Second kernel of the sliced SpMVs among S[0, ns): w = row sums of the
 // partials, V(:, j) = r xscale (and its FP32 copy).
-int launch_spmv_rows_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s);
+int launch_spmv_rows_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, bool solo, hipStream_t s);
 // The two kernels of the staged-slice SpMVs among S[0, ns): the pieces'
 // partials (stg_blocks blocks), then the row bins (bin_blocks; nsl_max: the
 // most slices of a listed operator) writing w and V(:, j) = r xscale (and its
 // FP32 copy).
 int launch_spmv_staged_seg(const SegMember *M, const SegStep *S, int ns, int stg_blocks, int bin_blocks, int nsl_max,
-                           hipStream_t s);
+                           bool solo, hipStream_t s);
 constexpr int kRowsPerSumBlock = 256; // rows per block of the row-sum kernels
 
 // Restart V.Q of one solve within a segmented launch (tiled bases, ld K).

@@ -255,11 +255,11 @@ template <> __device__ __forceinline__ const float *seg_values<float>(const SegM
 // AT: the listed solves' matrix values, VT: their basis.
 template <typename AT, typename VT>
 __global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__restrict__ M,
-                                                          const SegStep *__restrict__ S, int ns)
+                                                          const SegStep *__restrict__ S, int ns, int solo)
 {
     const int i = seg_find_spmv(S, ns, blockIdx.x);
     const SegStep st = S[i];
-    const SegMember mb = M[st.mem];
+    const SegMember mb = M[solo ? 0 : st.mem];
     if constexpr (std::is_same<AT, double>::value) {
         if (mb.e_ptr) { // sliced-ELL form: a wave per 64-row chunk
             const int c = (blockIdx.x - st.spmv0) * (kBlock / kWave) + threadIdx.x / kWave;
@@ -315,7 +315,7 @@ __device__ __forceinline__ void spmv_rows(int r0, int r1, const int *__restrict_
 }
 
 __global__ __launch_bounds__(kBlock) void spmv_rows_seg_kernel(const SegMember *__restrict__ M,
-                                                               const SegStep *__restrict__ S, int ns)
+                                                               const SegStep *__restrict__ S, int ns, int solo)
 {
     int lo = 0, hi = ns - 1; // the step owning this block (by red0)
     while (lo < hi) {
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void spmv_rows_seg_kernel(const SegMember *
             hi = mid - 1;
     }
     const SegStep st = S[lo];
-    const SegMember mb = M[st.mem];
+    const SegMember mb = M[solo ? 0 : st.mem];
     const int r0 = (blockIdx.x - st.red0) * kRowsPerSumBlock;
     spmv_rows(r0, min(r0 + kRowsPerSumBlock, mb.n), mb.r_pp, mb.r_pairs, mb.part, mb.r, mb.ctl->xscale, mb.w,
               static_cast<double *>(mb.V), mb.V32, mb.vld, st.j, true);
@@ -619,17 +619,18 @@ __device__ __forceinline__ int seg_find_stg(const SegStep *S, int ns, int b, boo
 }
 
 __global__ __launch_bounds__(kStagedThreads) void spmv_staged_seg_kernel(const SegMember *__restrict__ M,
-                                                                         const SegStep *__restrict__ S, int ns)
+                                                                         const SegStep *__restrict__ S, int ns,
+                                                                         int solo)
 {
     __shared__ double xs[1 << kStagedLw];
     const SegStep st = S[seg_find_stg(S, ns, blockIdx.x, false)];
-    const SegMember mb = M[st.mem];
+    const SegMember mb = M[solo ? 0 : st.mem];
     staged_block(mb.s_sup[blockIdx.x - st.stg0], mb.n, mb.s_cptr, mb.s_plen, mb.s_ecol, mb.s_eval, mb.r,
                  mb.ctl->xscale, mb.part, xs);
 }
 
 __global__ __launch_bounds__(kBinThreads) void spmv_bins_seg_kernel(const SegMember *__restrict__ M,
-                                                                    const SegStep *__restrict__ S, int ns)
+                                                                    const SegStep *__restrict__ S, int ns, int solo)
 {
     __shared__ double ps[kBinCap];
     __shared__ unsigned short lsh[kBinCap];
@@ -637,7 +638,7 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_seg_kernel(const SegMem
     __shared__ int wtot[kBinThreads / kWave];
     extern __shared__ int bins_dyn[];
     const SegStep st = S[seg_find_stg(S, ns, blockIdx.x, true)];
-    const SegMember mb = M[st.mem];
+    const SegMember mb = M[solo ? 0 : st.mem];
     bins_block<double>(blockIdx.x - st.bin0, mb.stg_nsl, mb.s_brow, mb.s_run, mb.s_lpp, mb.s_lst, mb.part, mb.r,
                        mb.ctl->xscale, mb.w, static_cast<double *>(mb.V), mb.V32, mb.vld, st.j, true, ps, lsh, coarse,
                        bins_dyn, bins_dyn + mb.stg_nsl + 1, wtot);
@@ -1272,37 +1273,38 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
 namespace hl {
 
 int launch_spmv_staged_seg(const SegMember *M, const SegStep *S, int ns, int stg_blocks, int bin_blocks, int nsl_max,
-                           hipStream_t s)
+                           bool solo, hipStream_t s)
 {
     if (ns <= 0) return HL_OK;
-    if (stg_blocks > 0) spmv_staged_seg_kernel<<<stg_blocks, kStagedThreads, 0, s>>>(M, S, ns);
+    if (stg_blocks > 0) spmv_staged_seg_kernel<<<stg_blocks, kStagedThreads, 0, s>>>(M, S, ns, solo ? 1 : 0);
     HL_TRY_HIP(hipGetLastError());
-    if (bin_blocks > 0) spmv_bins_seg_kernel<<<bin_blocks, kBinThreads, bins_dyn_lds(nsl_max), s>>>(M, S, ns);
+    if (bin_blocks > 0)
+        spmv_bins_seg_kernel<<<bin_blocks, kBinThreads, bins_dyn_lds(nsl_max), s>>>(M, S, ns, solo ? 1 : 0);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }
 
-int launch_spmv_rows_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, hipStream_t s)
+int launch_spmv_rows_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, bool solo, hipStream_t s)
 {
     if (ns <= 0 || blocks_total <= 0) return HL_OK;
-    spmv_rows_seg_kernel<<<blocks_total, kBlock, 0, s>>>(M, S, ns);
+    spmv_rows_seg_kernel<<<blocks_total, kBlock, 0, s>>>(M, S, ns, solo ? 1 : 0);
     HL_TRY_HIP(hipGetLastError());
     return HL_OK;
 }
 
-int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, hipStream_t s)
+int launch_spmv_seg(const SegMember *M, const SegStep *S, int ns, int blocks_total, int kind, bool solo, hipStream_t s)
 {
     if (ns <= 0 || blocks_total <= 0) return HL_OK;
     switch (kind) {
     case SEG_F64:
     case SEG_PROJ32: // (writes the FP32 copy of the basis column too)
-        spmv_seg_kernel<double, double><<<blocks_total, kBlock, 0, s>>>(M, S, ns);
+        spmv_seg_kernel<double, double><<<blocks_total, kBlock, 0, s>>>(M, S, ns, solo ? 1 : 0);
         break;
     case SEG_V32:
-        spmv_seg_kernel<double, float><<<blocks_total, kBlock, 0, s>>>(M, S, ns);
+        spmv_seg_kernel<double, float><<<blocks_total, kBlock, 0, s>>>(M, S, ns, solo ? 1 : 0);
         break;
     case SEG_V32_A32:
-        spmv_seg_kernel<float, float><<<blocks_total, kBlock, 0, s>>>(M, S, ns);
+        spmv_seg_kernel<float, float><<<blocks_total, kBlock, 0, s>>>(M, S, ns, solo ? 1 : 0);
         break;
     default:
         set_error("SpMV: bad segment kind %d", kind);
