@@ -198,7 +198,7 @@ struct CsrPlan {
     int stg_nbins = 0;                     // blocks of the second kernel
     long long stg_npieces = 0;             // pieces, without the padding
     long long *s_cptr = nullptr;           // nchunks + 1: first entry of each chunk
-    unsigned short *s_plen = nullptr;      // 64 nchunks: entries per piece (padding 0)
+    unsigned char *s_plen = nullptr;       // 64 nchunks: entries per piece (padding 0)
     unsigned short *s_ecol = nullptr;      // s_cptr[nchunks]: slice-local columns
     double *s_eval = nullptr;              // s_cptr[nchunks]: values
     int4 *s_sup = nullptr;                 // nsup: {slice, first chunk, end chunk, 0}
@@ -213,7 +213,7 @@ struct CsrPlan {
 constexpr int kStagedLw = 14;            // slices of 2^14 columns (x of a slice: 128 KB of LDS)
 constexpr int kStagedThreads = 1024;     // threads per block of the first staged kernel
 constexpr int kBinThreads = 512;         // threads per block of the second (bins) kernel
-constexpr int kStagedPiece = 64;         // entries per piece
+constexpr int kStagedPiece = 64;         // entries per piece (a piece's length fits a byte)
 constexpr int kBinCap = 4096;            // pieces per row bin (their partials: 32 KB of LDS)
 constexpr int kBinStride = 16;           // the bins kernel looks up the run of every 16th partial
 constexpr int kStagedMaxSlices = 4000;   // slices (the bins kernel keeps two ints per slice in LDS)
@@ -286,7 +286,8 @@ struct SegMember {               // per solve, fixed during a sweep
     // kernel writes the pieces' partials into part, its bins kernel adds them
     const int4 *s_sup;
     const long long *s_cptr;
-    const unsigned short *s_plen, *s_ecol, *s_lst;
+    const unsigned char *s_plen;
+    const unsigned short *s_ecol, *s_lst;
     const double *s_eval;
     const int *s_brow, *s_run, *s_lpp;
     int stg_nsl;

@@ -354,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void spmv_rows_kernel(int n, const int *__r
 constexpr int kStagedGroup = 4;
 constexpr int kStagedUnroll = 4; // entries of a piece in flight per trip
 __device__ __forceinline__ void staged_block(const int4 su, long long ncols, const long long *__restrict__ cptr_,
-                                             const unsigned short *__restrict__ plen_,
+                                             const unsigned char *__restrict__ plen_,
                                              const unsigned short *__restrict__ ecol_,
                                              const double *__restrict__ eval_, const double *__restrict__ x_,
                                              double s, double *__restrict__ part_, double *xs)
@@ -573,7 +573,7 @@ inline size_t bins_dyn_lds(int nsl) { return sizeof(int) * (2 * static_cast<size
 
 __global__ __launch_bounds__(kStagedThreads) void spmv_staged_kernel(const int4 *__restrict__ sup, long long ncols,
                                                                      const long long *__restrict__ cptr,
-                                                                     const unsigned short *__restrict__ plen,
+                                                                     const unsigned char *__restrict__ plen,
                                                                      const unsigned short *__restrict__ ecol,
                                                                      const double *__restrict__ eval,
                                                                      const double *__restrict__ x,
@@ -840,7 +840,8 @@ void staged_free(CsrPlan &p)
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p.s_cptr = nullptr;
-    p.s_plen = p.s_ecol = p.s_lst = nullptr;
+    p.s_plen = nullptr;
+    p.s_ecol = p.s_lst = nullptr;
     p.s_eval = p.s_spart = nullptr;
     p.s_sup = nullptr;
     p.s_brow = p.s_run = p.s_lpp = nullptr;
@@ -1145,13 +1146,13 @@ int plan_staged(CsrPlan &p, bool on)
     // chunks (column-major, as wide as their longest piece), lengths and entries
     const long long nch = npad / kWave;
     std::vector<long long> cptr(static_cast<size_t>(nch) + 1, 0);
-    std::vector<unsigned short> plen(static_cast<size_t>(npad), 0);
+    std::vector<unsigned char> plen(static_cast<size_t>(npad), 0); // lengths <= kStagedPiece fit a byte
     for (long long c = 0; c < nch; ++c) {
         int w = 0;
         for (int l = 0; l < kWave; ++l) {
             const int g = at_pos[c * kWave + l];
             if (g >= 0) {
-                plen[c * kWave + l] = g_len[g];
+                plen[c * kWave + l] = static_cast<unsigned char>(g_len[g]);
                 w = std::max(w, static_cast<int>(g_len[g]));
             }
         }
