@@ -640,7 +640,7 @@ void spmv_fields(SegMember &M, const Member &m)
     M.s_ecol = sg ? m.A->s_ecol : nullptr;
     M.s_lst = sg ? m.A->s_lst : nullptr;
     M.s_eval = sg ? m.A->s_eval : nullptr;
-    M.s_brow = sg ? m.A->s_brow : nullptr;
+    M.s_bmeta = sg ? m.A->s_bmeta : nullptr;
     M.s_run = sg ? m.A->s_run : nullptr;
     M.s_lpp = sg ? m.A->s_lpp : nullptr;
     M.stg_nsl = sg ? m.A->stg_nsl : 0;

@@ -202,7 +202,7 @@ struct CsrPlan {
     unsigned short *s_ecol = nullptr;      // s_cptr[nchunks]: slice-local columns
     double *s_eval = nullptr;              // s_cptr[nchunks]: values
     int4 *s_sup = nullptr;                 // nsup: {slice, first chunk, end chunk, 0}
-    int *s_brow = nullptr;                 // nbins + 1: first row of each bin
+    int4 *s_bmeta = nullptr;               // nbins: {first row, end row, first list offset, end list offset}
     int *s_run = nullptr;                  // (nbins + 1) x nsl: first piece of (bin, slice)
     int *s_lpp = nullptr;                  // nrows + 1: each row's pieces in s_lst
     unsigned short *s_lst = nullptr;       // npieces: LDS offset of each piece, slice order
@@ -289,7 +289,8 @@ struct SegMember {               // per solve, fixed during a sweep
     const unsigned char *s_plen;
     const unsigned short *s_ecol, *s_lst;
     const double *s_eval;
-    const int *s_brow, *s_run, *s_lpp;
+    const int4 *s_bmeta;
+    const int *s_run, *s_lpp;
     int stg_nsl;
     void *V;                     // tiled basis in use (double, or float for SEG_V32*)
     const void *Vproj;           // basis read by the projection-only CGS passes (V, or the FP32 copy)

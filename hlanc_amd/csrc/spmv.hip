@@ -449,7 +449,7 @@ __device__ __forceinline__ void staged_block(const int4 su, long long ncols, con
 // partials added in slice order, and the basis column v(r) = x(r) s beside.
 // roff/rdel: 2 nsl + 1 ints of LDS.
 template <typename VT>
-__device__ __forceinline__ void bins_block(int b, int nsl, const int *__restrict__ brow_, const int *__restrict__ run_,
+__device__ __forceinline__ void bins_block(int b, int nsl, const int4 *__restrict__ bmeta_, const int *__restrict__ run_,
                                            const int *__restrict__ lpp_, const unsigned short *__restrict__ lst_,
                                            const double *__restrict__ part_, const double *__restrict__ x_, double s,
                                            double *__restrict__ y_, VT *__restrict__ vout_,
@@ -457,7 +457,6 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int *__restrict
                                            double *ps, unsigned short *lsh, unsigned short *coarse, int *roff,
                                            int *rdel, int *wtot)
 {
-    const auto brow = dev::gptr(brow_);
     const auto run = dev::gptr(run_);
     const auto lpp = dev::gptr(lpp_);
     const auto lst = dev::gptr(lst_);
@@ -468,8 +467,8 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int *__restrict
     const auto vout32 = dev::gptr(vout32_);
     constexpr int T = kBinThreads;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
-    const int r0 = brow[b], r1 = brow[b + 1];
-    const int l0 = lpp[r0], nl = lpp[r1] - l0; // the bin's offsets: lst[l0, l0 + nl)
+    const int4 bm = bmeta_[b]; // rows [r0, r1), offsets lst[l0, l0 + nl)
+    const int r0 = bm.x, r1 = bm.y, l0 = bm.z, nl = bm.w - bm.z;
     // this thread's first row: its bounds and x(r), independent of the copy
     const int rf = r0 + tid;
     int f0 = 0, f1 = 0;
@@ -584,7 +583,7 @@ __global__ __launch_bounds__(kStagedThreads) void spmv_staged_kernel(const int4 
     staged_block(sup[blockIdx.x], ncols, cptr, plen, ecol, eval, x, xscale ? *xscale : 1.0, part, xs);
 }
 
-__global__ __launch_bounds__(kBinThreads) void spmv_bins_kernel(int nsl, const int *__restrict__ brow,
+__global__ __launch_bounds__(kBinThreads) void spmv_bins_kernel(int nsl, const int4 *__restrict__ bmeta,
                                                                 const int *__restrict__ run,
                                                                 const int *__restrict__ lpp,
                                                                 const unsigned short *__restrict__ lst,
@@ -599,7 +598,7 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_kernel(int nsl, const i
     __shared__ unsigned short coarse[kBinCap / kBinStride];
     __shared__ int wtot[kBinThreads / kWave];
     extern __shared__ int bins_dyn[];
-    bins_block<double>(blockIdx.x, nsl, brow, run, lpp, lst, part, x, xscale ? *xscale : 1.0, y, vout, nullptr, vld,
+    bins_block<double>(blockIdx.x, nsl, bmeta, run, lpp, lst, part, x, xscale ? *xscale : 1.0, y, vout, nullptr, vld,
                        vj, vtiled, ps, lsh, coarse, bins_dyn, bins_dyn + nsl + 1, wtot);
 }
 
@@ -639,7 +638,7 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_seg_kernel(const SegMem
     extern __shared__ int bins_dyn[];
     const SegStep st = S[seg_find_stg(S, ns, blockIdx.x, true)];
     const SegMember mb = M[solo ? 0 : st.mem];
-    bins_block<double>(blockIdx.x - st.bin0, mb.stg_nsl, mb.s_brow, mb.s_run, mb.s_lpp, mb.s_lst, mb.part, mb.r,
+    bins_block<double>(blockIdx.x - st.bin0, mb.stg_nsl, mb.s_bmeta, mb.s_run, mb.s_lpp, mb.s_lst, mb.part, mb.r,
                        mb.ctl->xscale, mb.w, static_cast<double *>(mb.V), mb.V32, mb.vld, st.j, true, ps, lsh, coarse,
                        bins_dyn, bins_dyn + mb.stg_nsl + 1, wtot);
 }
@@ -836,7 +835,7 @@ void slices_free(CsrPlan &p)
 
 void staged_free(CsrPlan &p)
 {
-    void *ptrs[] = {p.s_cptr, p.s_plen, p.s_ecol, p.s_eval, p.s_sup, p.s_brow, p.s_run, p.s_lpp, p.s_lst, p.s_spart};
+    void *ptrs[] = {p.s_cptr, p.s_plen, p.s_ecol, p.s_eval, p.s_sup, p.s_bmeta, p.s_run, p.s_lpp, p.s_lst, p.s_spart};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p.s_cptr = nullptr;
@@ -844,7 +843,8 @@ void staged_free(CsrPlan &p)
     p.s_ecol = p.s_lst = nullptr;
     p.s_eval = p.s_spart = nullptr;
     p.s_sup = nullptr;
-    p.s_brow = p.s_run = p.s_lpp = nullptr;
+    p.s_bmeta = nullptr;
+    p.s_run = p.s_lpp = nullptr;
     p.stg_on = false;
     p.stg_nsl = p.stg_nchunks = p.stg_nsup = p.stg_nbins = 0;
     p.stg_npieces = 0;
@@ -1188,12 +1188,15 @@ int plan_staged(CsrPlan &p, bool on)
     }
     std::vector<int> lpp32(static_cast<size_t>(n) + 1);
     for (int r = 0; r <= n; ++r) lpp32[r] = static_cast<int>(lpp[r]);
+    std::vector<int4> bmeta(static_cast<size_t>(nbins)); // {first row, end row, first offset, end offset}
+    for (int b = 0; b < nbins; ++b)
+        bmeta[b] = make_int4(brow[b], brow[b + 1], lpp32[brow[b]], lpp32[brow[b + 1]]);
     int rc;
     if ((rc = dev_upload(&p.s_cptr, cptr.data(), cptr.size())) ||
         (rc = dev_upload(&p.s_plen, plen.data(), plen.size())) ||
         (rc = dev_upload(&p.s_ecol, ec.data(), ec.size())) || (rc = dev_upload(&p.s_eval, ev.data(), ev.size())) ||
         (rc = dev_upload(&p.s_sup, sup.data(), sup.size())) ||
-        (rc = dev_upload(&p.s_brow, brow.data(), brow.size())) ||
+        (rc = dev_upload(&p.s_bmeta, bmeta.data(), bmeta.size())) ||
         (rc = dev_upload(&p.s_run, run.data(), run.size())) ||
         (rc = dev_upload(&p.s_lpp, lpp32.data(), lpp32.size())) ||
         (rc = dev_upload(&p.s_lst, lst.data(), lst.size())) ||
@@ -1248,7 +1251,7 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
         HL_TRY_HIP(hipGetLastError());
         if (p.stg_nbins > 0)
             spmv_bins_kernel<<<p.stg_nbins, kBinThreads, bins_dyn_lds(p.stg_nsl), s>>>(
-                p.stg_nsl, p.s_brow, p.s_run, p.s_lpp, p.s_lst, p.s_spart, x, xscale, y, vcol.base, vcol.L.ld, vcol.j,
+                p.stg_nsl, p.s_bmeta, p.s_run, p.s_lpp, p.s_lst, p.s_spart, x, xscale, y, vcol.base, vcol.L.ld, vcol.j,
                 vcol.L.tiled);
     } else if (p.slice_w > 0) { // partials of the virtual rows, then the row sums
         if (p.sitems.nblk > 0)
