@@ -214,7 +214,8 @@ constexpr int kStagedLw = 14;            // slices of 2^14 columns (x of a slice
 constexpr int kStagedThreads = 1024;     // threads per block of the first staged kernel
 constexpr int kBinThreads = 512;         // threads per block of the second (bins) kernel
 constexpr int kStagedPiece = 64;         // entries per piece (a piece's length fits a byte)
-constexpr int kBinCap = 4096;            // pieces per row bin (their partials: 32 KB of LDS)
+constexpr int kBinCap = 3584;            // pieces per row bin (28 KB of partials and 7 KB of offsets in LDS:
+                                         // four bins per CU)
 constexpr int kBinStride = 16;           // the bins kernel looks up the run of every 16th partial
 constexpr int kStagedMaxSlices = 4000;   // slices (the bins kernel keeps two ints per slice in LDS)
 constexpr int kStagedSupEntries = 65536; // entries per block of the first kernel (about)

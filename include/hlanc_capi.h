@@ -148,12 +148,12 @@ int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries); /* outputs may be 
  * no x gather leaves the CU.  Each row's entries of a slice (cut every 64
  * entries) form a piece; pieces are stored slice-major in ELL chunks of 64, a
  * lane per piece (16-bit slice-local columns), and a first kernel writes one
- * partial per piece.  A second kernel takes a bin of rows holding at most 4096
+ * partial per piece.  A second kernel takes a bin of rows holding at most 3584
  * pieces, copies the bin's partials into LDS and adds each row's partials in
  * slice order.  Suits graphs whose gathers miss the caches (power-law and
  * random graphs).  on = 0 drops the form; building it drops the column slices
  * and the sliced-ELL form (and they drop it).  Refused (HL_ERR_ARG) beyond
- * 4000 slices (65.5 M columns) or for a row of more than 4096 pieces.  Used
+ * 4000 slices (65.5 M columns) or for a row of more than 3584 pieces.  Used
  * by every FP64-value SpMV of the operator; costs 10 bytes per stored entry
  * (entries are padded to their chunk's longest piece) plus about 12 per piece
  * on the device and 8 bytes per piece per solve.  Same rules as

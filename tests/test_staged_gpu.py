@@ -78,8 +78,8 @@ def test_staged_spmv_matches_oracle(case):
     op.set_staged(True)
     on, pieces, entries, bins = op.staged()
     assert on and pieces == pieces_of(rp, col)
-    # every bin holds at most 4096 pieces, so there are at least pieces / 4096 of them
-    assert bins >= max(1, -(-pieces // 4096))
+    # every bin holds at most 3584 pieces, so there are at least pieces / 3584 of them
+    assert bins >= max(1, -(-pieces // 3584))
     assert entries >= rp[-1] and entries % 64 == 0
     x = np.random.default_rng(7).uniform(-1, 1, ncols)
     y_ref = O.spmv(O.Csr(rp, col, val, ncols, False), x)
@@ -140,9 +140,9 @@ def test_staged_forms_exclude_each_other_and_limits():
     with pytest.raises(H.HLancError):
         wide.set_staged(True)
     assert not wide.staged()[0]
-    # a row of more pieces than a bin holds: 270,000 entries in one slice make
-    # 4219 pieces; 260,000 make 4063
-    for m, ok in ((270_000, False), (260_000, True)):
+    # a row of more pieces than a bin holds (3584): 230,000 entries in one slice
+    # make 3594 pieces; 229,000 make 3579
+    for m, ok in ((230_000, False), (229_000, True)):
         long_row = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=np.array([0, m], dtype=np.int32),
                                                          col=(np.arange(m) % 16384).astype(np.int32),
                                                          val=np.ones(m), ncols=16384))
@@ -152,7 +152,7 @@ def test_staged_forms_exclude_each_other_and_limits():
             assert not long_row.staged()[0]
         else:
             long_row.set_staged(True)
-            assert long_row.staged()[1] == 4063
+            assert long_row.staged()[1] == 3579
             assert long_row(np.ones(16384))[0] == float(m)
 
 
