@@ -93,6 +93,23 @@ def test_staged_spmv_matches_oracle(case):
     assert np.linalg.norm(op(x) - y_ref) <= TOL * scale
 
 
+def test_staged_spmv_matches_mkl_golden_vectors(golden):
+    """The staged form against the mkl_dcsrmv outputs recorded in
+    tests/golden/ (simple.pp, Journals, a random 1000 x 1500 CSR)."""
+    cases = []
+    for tag, (rp, col, val, nc, sym) in (("simple", R.simple()), ("journals", R.journals())):
+        cases.append((rp, col, val, nc, sym, golden[f"spmv_{tag}_x"], golden[f"spmv_{tag}_y"]))
+    x = golden["spmv_rand_x"]
+    cases.append((golden["spmv_rand_rp"], golden["spmv_rand_col"], golden["spmv_rand_val"], len(x), False, x,
+                  golden["spmv_rand_y"]))
+    for rp, col, val, nc, sym, x, y_mkl in cases:
+        op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, ncols=nc, is_symmetric=sym))
+        op.set_staged(True)
+        assert op.staged()[0]
+        y = op(x)
+        assert np.linalg.norm(y - y_mkl) <= TOL * np.linalg.norm(y_mkl)
+
+
 def test_staged_rows_sum_in_slice_order():
     """A row's partials are added in slice order, each piece (64 entries of a
     slice, in entry order) summed in entry order: the result is bitwise that
