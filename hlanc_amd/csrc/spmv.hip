@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void spmv_rows_kernel(int n, const int *__r
 // at cptr[c] + 64 k + l, so a wave's k-th loads are one contiguous stretch.
 constexpr int kStagedGroup = 4;
 constexpr int kStagedUnroll = 4; // entries of a piece in flight per trip
-__device__ __forceinline__ void staged_block(const int4 su, long long ncols, const long long *__restrict__ cptr_,
+__device__ __forceinline__ void staged_block(int4 su, long long ncols, const long long *__restrict__ cptr_,
                                              const unsigned char *__restrict__ plen_,
                                              const unsigned short *__restrict__ ecol_,
                                              const double *__restrict__ eval_, const double *__restrict__ x_,
@@ -369,6 +369,11 @@ __device__ __forceinline__ void staged_block(const int4 su, long long ncols, con
     constexpr int G = kStagedGroup;
     constexpr int U = kStagedUnroll;
     constexpr int NW = kStagedThreads / kWave;
+    // block-uniform values (su may come through a generic pointer), so chunk
+    // indices are known uniform and the chunk metadata loads are scalar
+    su.x = __builtin_amdgcn_readfirstlane(su.x);
+    su.y = __builtin_amdgcn_readfirstlane(su.y);
+    su.z = __builtin_amdgcn_readfirstlane(su.z);
     const long long c0 = static_cast<long long>(su.x) << kStagedLw;
     const int nx = static_cast<int>(min(static_cast<long long>(W), ncols - c0));
     if (nx == W && (reinterpret_cast<uintptr_t>(x_ + c0) & 15) == 0) { // a whole, aligned slice: 16-byte loads
@@ -382,7 +387,7 @@ __device__ __forceinline__ void staged_block(const int4 su, long long ncols, con
         for (int i = threadIdx.x; i < W; i += kStagedThreads) xs[i] = i < nx ? x[c0 + i] : 0.0;
     }
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = static_cast<int>(threadIdx.x) / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
     // chunk bounds and piece lengths of a wave's group of chunks; the next
     // group's are loaded while this one's entries are in flight
     long long nb[G], ne[G];
