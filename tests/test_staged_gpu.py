@@ -93,6 +93,46 @@ def test_staged_spmv_matches_oracle(case):
     assert np.linalg.norm(op(x) - y_ref) <= TOL * scale
 
 
+@pytest.mark.parametrize("shape", ["no_entries", "one_row", "five_rows", "one_entry_per_slice"])
+def test_staged_spmv_small_and_empty(shape):
+    """Edge shapes of the staged form: a matrix with no entries (one bin, no
+    chunks), a single row, fewer rows than a chunk's 64 lanes, and a row with
+    one entry in each of 200 slices (200 one-entry pieces added in slice
+    order)."""
+    rng = np.random.default_rng(11)
+    if shape == "no_entries":
+        n, ncols = 100, 70_000
+        rp, col = np.zeros(n + 1, np.int32), np.zeros(0, np.int32)
+    elif shape == "one_row":
+        n, ncols = 1, 20
+        rp, col = np.array([0, 3], np.int32), np.array([19, 0, 7], np.int32)
+    elif shape == "five_rows":
+        n, ncols = 5, 3
+        rp = np.array([0, 2, 2, 5, 6, 9], np.int32)
+        col = rng.integers(0, ncols, 9).astype(np.int32)
+    else:
+        n, ncols = 3, 200 << 14
+        rp = np.array([0, 200, 201, 201], np.int32)
+        col = np.concatenate([(np.arange(200) << 14) + rng.integers(0, 1 << 14, 200), [5]]).astype(np.int32)
+    val = rng.uniform(-1, 1, len(col))
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, ncols=ncols))
+    op.set_staged(True)
+    on, pieces, entries, bins = op.staged()
+    assert on and pieces == (pieces_of(rp, col) if len(col) else 0) and bins >= 1
+    x = rng.uniform(-1, 1, ncols)
+    y = op(x)
+    if shape == "one_entry_per_slice":  # row 0's pieces: one product each, added in slice order
+        prods = val[:200] * x[col[:200]]
+        acc = 0.0
+        for p_ in prods:
+            acc += p_
+        assert y[0] == acc and y[1] == val[200] * x[5] and y[2] == 0.0
+    y_ref = O.spmv(O.Csr(rp, col, val, ncols, False), x)
+    assert np.linalg.norm(y - y_ref) <= TOL * max(np.linalg.norm(y_ref), 1e-300)
+    if not len(col):
+        assert (y == 0.0).all()
+
+
 def test_staged_spmv_matches_mkl_golden_vectors(golden):
     """The staged form against the mkl_dcsrmv outputs recorded in
     tests/golden/ (simple.pp, Journals, a random 1000 x 1500 CSR)."""
