@@ -122,6 +122,7 @@ _SIGS = [
     ("hl_csr_get_slices", ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
     ("hl_csr_set_sell", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hl_csr_get_sell", ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong)]),
+    ("hl_csr_get_sell_col_bytes", ctypes.c_int, [ctypes.c_void_p, _ip]),
     ("hl_csr_set_staged", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hl_csr_get_staged", ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong),
                                          ctypes.POINTER(ctypes.c_longlong), _ip]),
@@ -366,6 +367,13 @@ class dcsrmv_operator_gpu:
         on, ne = ctypes.c_int(), ctypes.c_longlong()
         _check(lib().hl_csr_get_sell(self.h, ctypes.byref(on), ctypes.byref(ne)))
         return bool(on.value), ne.value
+
+    def sell_col_bytes(self) -> int:
+        """Bytes per stored column index of the sliced-ELL form: 2 for banded operators (offsets
+        from the row), 4 otherwise, 0 when the form is off."""
+        b = ctypes.c_int()
+        _check(lib().hl_csr_get_sell_col_bytes(self.h, ctypes.byref(b)))
+        return b.value
 
     def set_staged(self, on: bool) -> None:
         """Staged-slice SpMV (x slices of 16384 columns in LDS, partials per row bin) on or off

@@ -342,6 +342,13 @@ int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries)
     return HL_OK;
 }
 
+int hl_csr_get_sell_col_bytes(hl_csr_t h, int *bytes)
+{
+    if (!h || !bytes) return HL_ERR_ARG;
+    *bytes = !h->plan.sell_on ? 0 : h->plan.e_d16 ? 2 : 4;
+    return HL_OK;
+}
+
 int hl_csr_set_staged(hl_csr_t h, int on)
 {
     if (!h) return HL_ERR_ARG;

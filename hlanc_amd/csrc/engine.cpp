@@ -632,6 +632,7 @@ void spmv_fields(SegMember &M, const Member &m)
     const bool el = m.sell();
     M.e_ptr = el ? m.A->e_ptr : nullptr;
     M.e_col = el ? m.A->e_col : nullptr;
+    M.e_d16 = el ? m.A->e_d16 : nullptr;
     M.e_val = el ? m.A->e_val : nullptr;
     M.nchunks = el ? m.A->nchunks : 0;
     M.s_sup = sg ? m.A->s_sup : nullptr;

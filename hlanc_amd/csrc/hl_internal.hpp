@@ -180,7 +180,8 @@ struct CsrPlan {
     bool sell_on = false;
     int nchunks = 0;
     long long *e_ptr = nullptr;  // nchunks + 1
-    int *e_col = nullptr;        // e_ptr[nchunks] entries (padding: column 0)
+    int *e_col = nullptr;        // e_ptr[nchunks] entries (padding: column 0); null when banded:
+    short *e_d16 = nullptr;      // banded operators (every |col - row| < 2^15): col - row (padding: 0)
     double *e_val = nullptr;     // (padding: 0)
     // Staged-slice form of the FP64 matrix (stg_on; exclusive with the two
     // forms above): column slices of 2^kStagedLw columns, small enough that a
@@ -281,6 +282,7 @@ struct SegMember {               // per solve, fixed during a sweep
     unsigned *chunk_cnt;
     const long long *e_ptr;      // sliced-ELL form (FP64 values), else null: then blks/rp/col/val
     const int *e_col;
+    const short *e_d16;          // banded sliced-ELL: columns relative to the row (else null, e_col set)
     const double *e_val;
     int nchunks;
     // staged-slice form (FP64 values and basis), else s_sup null: its first

@@ -156,9 +156,11 @@ __device__ __forceinline__ void spmv_item(const SpmvBlk b, const int *__restrict
 // Chunk c of the sliced-ELL form, one wave: lane l sums row 64 c + l in its
 // own column order (U entries of the row in flight at once), y(r) = that sum,
 // and writes the basis column v(r) = x(r) s beside.
-template <typename VT>
+// CT int: absolute columns; CT short: columns relative to the row (banded
+// operators: every |col - row| < 2^15), 2 bytes an entry instead of 4.
+template <typename VT, typename CT = int>
 __device__ __forceinline__ void sell_chunk(int c, int n, const long long *__restrict__ e_ptr_,
-                                           const int *__restrict__ e_col_, const double *__restrict__ e_val_,
+                                           const CT *__restrict__ e_col_, const double *__restrict__ e_val_,
                                            const int *__restrict__ rp_, const double *__restrict__ x_,
                                            const double *__restrict__ xscale, double *__restrict__ y_,
                                            VT *__restrict__ vout_, float *__restrict__ vout32_, long long vld,
@@ -196,7 +198,7 @@ __device__ __forceinline__ void sell_chunk(int c, int n, const long long *__rest
         double xv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            if (k0 + u < len) xv[u] = x[ci[u]];
+            if (k0 + u < len) xv[u] = x[std::is_same<CT, short>::value ? r + ci[u] : ci[u]];
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (k0 + u < len) acc += av[u] * (xv[u] * s);
@@ -213,6 +215,7 @@ __device__ __forceinline__ void sell_chunk(int c, int n, const long long *__rest
 
 __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(int nchunks, int n, const long long *__restrict__ e_ptr,
                                                            const int *__restrict__ e_col,
+                                                           const short *__restrict__ e_d16,
                                                            const double *__restrict__ e_val,
                                                            const int *__restrict__ rp, const double *__restrict__ x,
                                                            const double *__restrict__ xscale, double *__restrict__ y,
@@ -220,7 +223,11 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(int nchunks, int n, c
                                                            bool vtiled)
 {
     const int c = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    if (c < nchunks) sell_chunk<double>(c, n, e_ptr, e_col, e_val, rp, x, xscale, y, vout, nullptr, vld, vj, vtiled);
+    if (c >= nchunks) return;
+    if (e_d16)
+        sell_chunk<double, short>(c, n, e_ptr, e_d16, e_val, rp, x, xscale, y, vout, nullptr, vld, vj, vtiled);
+    else
+        sell_chunk<double>(c, n, e_ptr, e_col, e_val, rp, x, xscale, y, vout, nullptr, vld, vj, vtiled);
 }
 
 template <typename AT, typename XT>
@@ -263,9 +270,14 @@ __global__ __launch_bounds__(kBlock) void spmv_seg_kernel(const SegMember *__res
     if constexpr (std::is_same<AT, double>::value) {
         if (mb.e_ptr) { // sliced-ELL form: a wave per 64-row chunk
             const int c = (blockIdx.x - st.spmv0) * (kBlock / kWave) + threadIdx.x / kWave;
-            if (c < mb.nchunks)
-                sell_chunk<VT>(c, mb.n, mb.e_ptr, mb.e_col, mb.e_val, mb.rp, mb.r, &mb.ctl->xscale, mb.w,
-                               static_cast<VT *>(mb.V), mb.V32, mb.vld, st.j, true);
+            if (c < mb.nchunks) {
+                if (mb.e_d16)
+                    sell_chunk<VT, short>(c, mb.n, mb.e_ptr, mb.e_d16, mb.e_val, mb.rp, mb.r, &mb.ctl->xscale, mb.w,
+                                          static_cast<VT *>(mb.V), mb.V32, mb.vld, st.j, true);
+                else
+                    sell_chunk<VT>(c, mb.n, mb.e_ptr, mb.e_col, mb.e_val, mb.rp, mb.r, &mb.ctl->xscale, mb.w,
+                                   static_cast<VT *>(mb.V), mb.V32, mb.vld, st.j, true);
+            }
             return;
         }
     }
@@ -816,11 +828,12 @@ void items_free(ItemSet &t)
 
 void sell_free(CsrPlan &p)
 {
-    void *ptrs[] = {p.e_ptr, p.e_col, p.e_val};
+    void *ptrs[] = {p.e_ptr, p.e_col, p.e_d16, p.e_val};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     p.e_ptr = nullptr;
     p.e_col = nullptr;
+    p.e_d16 = nullptr;
     p.e_val = nullptr;
     p.nchunks = 0;
     p.sell_on = false;
@@ -986,18 +999,31 @@ int plan_sell(CsrPlan &p, bool on, double max_fill)
         HL_TRY_HIP(hipMemcpy(C.data(), p.col, sizeof(int) * C.size(), hipMemcpyDeviceToHost));
         HL_TRY_HIP(hipMemcpy(A.data(), p.val, sizeof(double) * A.size(), hipMemcpyDeviceToHost));
     }
-    std::vector<int> ec(static_cast<size_t>(ne), 0);
+    // banded (every |col - row| < 2^15): columns as 16-bit offsets from the row
+    bool banded = true;
+    for (int r = 0; r < n && banded; ++r)
+        for (int q = R[r]; q < R[r + 1]; ++q)
+            if (C[q] - r < -32768 || C[q] - r > 32767) {
+                banded = false;
+                break;
+            }
+    std::vector<int> ec(banded ? 0 : static_cast<size_t>(ne), 0);
+    std::vector<short> ed(banded ? static_cast<size_t>(ne) : 0, 0);
     std::vector<double> ev(static_cast<size_t>(ne), 0.0);
     for (int r = 0; r < n; ++r) {
         const int c = r / kWave, l = r % kWave;
         for (int q = R[r]; q < R[r + 1]; ++q) {
             const long long at = ptr[c] + static_cast<long long>(kWave) * (q - R[r]) + l;
-            ec[at] = C[q];
+            if (banded)
+                ed[at] = static_cast<short>(C[q] - r);
+            else
+                ec[at] = C[q];
             ev[at] = A[q];
         }
     }
     int rc;
-    if ((rc = dev_upload(&p.e_ptr, ptr.data(), ptr.size())) || (rc = dev_upload(&p.e_col, ec.data(), ec.size())) ||
+    if ((rc = dev_upload(&p.e_ptr, ptr.data(), ptr.size())) ||
+        (rc = banded ? dev_upload(&p.e_d16, ed.data(), ed.size()) : dev_upload(&p.e_col, ec.data(), ec.size())) ||
         (rc = dev_upload(&p.e_val, ev.data(), ev.size()))) {
         sell_free(p);
         return rc;
@@ -1247,7 +1273,8 @@ int launch_spmv(const CsrPlan &p, const double *x, const double *xscale, double 
     } else if (p.sell_on) { // a wave per 64-row chunk
         if (p.nchunks > 0)
             spmv_sell_kernel<<<(p.nchunks + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, s>>>(
-                p.nchunks, p.nrows, p.e_ptr, p.e_col, p.e_val, p.row_ptr, x, xscale, y, vcol.base, vcol.L.ld, vcol.j,
+                p.nchunks, p.nrows, p.e_ptr, p.e_col, p.e_d16, p.e_val, p.row_ptr, x, xscale, y, vcol.base, vcol.L.ld,
+                vcol.j,
                 vcol.L.tiled);
     } else if (p.stg_on) { // partials of the pieces (x slices in LDS), then the row bins
         if (p.stg_nsup > 0)

@@ -139,10 +139,15 @@ int hl_csr_get_slices(hl_csr_t h, int *width, int *npairs); /* outputs may be NU
  * even row lengths (stencils).  It drops the column slices (and they drop it).
  * Refused (HL_ERR_ARG) when the chunks would hold more than twice nnz entries
  * or a row holds more than 2048 nonzeros (one wave would walk it alone).  Used by every FP64-value SpMV of the operator; costs 12 bytes per
- * entry on the device.  Same rules as hl_csr_set_items.  No counterpart in the
- * reference. */
+ * entry on the device, 10 when the operator is banded (every |col - row| <
+ * 32768: columns are then stored as 16-bit offsets from the row).  Same rules
+ * as hl_csr_set_items.  No counterpart in the reference. */
 int hl_csr_set_sell(hl_csr_t h, int on);
 int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries); /* outputs may be NULL */
+/* Bytes per stored column index of the sliced-ELL form: 2 (banded, offsets
+ * from the row), 4 (absolute columns), 0 when the form is off.  No
+ * counterpart in the reference. */
+int hl_csr_get_sell_col_bytes(hl_csr_t h, int *bytes);
 /* Staged-slice SpMV: on != 0 cuts the FP64 matrix into column slices of 16384
  * columns, few enough that a block holds a slice's x (128 KB) in LDS, so that
  * no x gather leaves the CU.  Each row's entries of a slice (cut every 64

@@ -276,11 +276,36 @@ def even_matrix(seed=4, n=1000, ncols=1200):
     return rp, col, val, ncols
 
 
-@pytest.mark.parametrize("case", ["even", "stencil", "lap2d"])
+def band_matrix(n=100_000, far=False, seed=6):
+    """Rows of 8..12 entries whose columns sit at row offsets in [-32768, 32767]
+    (the 16-bit column form's extremes included), or, with far, one more entry
+    per 7th row at offset +32768 or -32769 (absolute columns)."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for r in range(n):
+        k = rng.integers(8, 13)
+        off = rng.integers(-40, 41, k)
+        if r % 1000 == 0:
+            off = np.concatenate([off, [-32768, 32767]])
+        if far and r % 7 == 0:
+            off = np.concatenate([off, [32768 if r < n // 2 else -32769]])
+        c = r + off
+        c = c[(c >= 0) & (c < n)]
+        rows.append(np.full(len(c), r))
+        cols.append(c)
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int32)
+    return rp, cols.astype(np.int32), rng.uniform(-1, 1, len(cols)), n
+
+
+@pytest.mark.parametrize("case", ["even", "stencil", "lap2d", "band", "far"])
 def test_sell_spmv_matches_oracle(case):
-    """hl_csr_set_sell: 64-row chunks, a lane per row summing in column order."""
+    """hl_csr_set_sell: 64-row chunks, a lane per row summing in column order;
+    banded operators store columns as 16-bit offsets from the row."""
     if case == "even":
         rp, col, val, ncols = even_matrix()
+    elif case in ("band", "far"):
+        rp, col, val, ncols = band_matrix(far=case == "far")
     elif case == "stencil":
         rp, col, val = W.stencil27(17, seed=3)
         ncols = len(rp) - 1
@@ -297,12 +322,17 @@ def test_sell_spmv_matches_oracle(case):
     lens = np.diff(rp)
     widths = [lens[c:c + 64].max() for c in range(0, n, 64)]
     assert on and entries == 64 * sum(widths)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    offs = col.astype(np.int64) - rows
+    banded = len(col) == 0 or (offs.min() >= -32768 and offs.max() <= 32767)
+    assert op.sell_col_bytes() == (2 if banded else 4)
+    assert banded == (case != "far")
     y = op(x)
     assert np.linalg.norm(y - y_ref) <= TOL * scale
     np.testing.assert_array_equal(op(x), y)  # repeatable
     assert (y[lens == 0] == 0.0).all()
     op.set_sell(False)
-    assert op.sell() == (False, 0)
+    assert op.sell() == (False, 0) and op.sell_col_bytes() == 0
     assert np.linalg.norm(op(x) - y_ref) <= TOL * scale
 
 
