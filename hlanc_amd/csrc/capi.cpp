@@ -5,6 +5,7 @@
 #include <chrono>
 #include <mutex>
 #include <thread>
+#include <exception>
 #include <new>
 #include <vector>
 
@@ -28,6 +29,22 @@ void set_error(const char *fmt, ...)
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
+}
+
+int exception_code() noexcept
+{
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        set_error("out of host memory");
+        return HL_ERR_OOM;
+    } catch (const std::exception &e) {
+        set_error("internal error: %s", e.what());
+        return HL_ERR_STATE;
+    } catch (...) {
+        set_error("internal error: unknown exception");
+        return HL_ERR_STATE;
+    }
 }
 
 int hip_status(hipError_t e, const char *what)
@@ -159,25 +176,31 @@ const char *hl_last_error(void) { return g_err; }
 const char *hl_version(void) { return "1.0"; }
 
 int hl_host_alloc(size_t bytes, void **ptr)
-{
+try {
     if (!ptr) return HL_ERR_ARG;
     *ptr = nullptr;
     HL_TRY_HIP(hipHostMalloc(ptr, bytes > 0 ? bytes : 1, hipHostMallocDefault));
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_host_free(void *ptr)
-{
+try {
     if (ptr) HL_TRY_HIP(hipHostFree(ptr));
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_device_count(int *count)
-{
+try {
     int c = 0;
     if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
     if (count) *count = c;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 void hl_options_default(hl_options *o)
@@ -195,7 +218,7 @@ void hl_options_default(hl_options *o)
 
 int hl_csr_create(int dev, int nrows, int ncols, int nnz, const int *row_ptr, const int *col,
                   const double *val, int sym_lower, int val_prec, hl_csr_t *out)
-{
+try {
     if (!out) return HL_ERR_ARG;
     *out = nullptr;
     if (nrows < 0 || ncols < 0 || (nrows > 0 && (!row_ptr || row_ptr[0] != 0 || row_ptr[nrows] != nnz)) ||
@@ -222,30 +245,36 @@ int hl_csr_create(int dev, int nrows, int ncols, int nnz, const int *row_ptr, co
     }
     *out = h;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_destroy(hl_csr_t h)
-{
+try {
     if (!h) return HL_OK;
     DeviceGuard g(h->plan.dev);
     (void)hipDeviceSynchronize();
     csr_plan_free(h->plan);
     delete h;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_info(hl_csr_t h, int *dev, int *nrows, int *ncols, long long *nnz_general)
-{
+try {
     if (!h) return HL_ERR_ARG;
     if (dev) *dev = h->plan.dev;
     if (nrows) *nrows = h->plan.nrows;
     if (ncols) *ncols = h->plan.ncols;
     if (nnz_general) *nnz_general = h->plan.nnz;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_apply(hl_csr_t h, const double *x, double *y)
-{
+try {
     if (!h || (h->plan.ncols > 0 && !x) || (h->plan.nrows > 0 && !y)) return HL_ERR_ARG;
     std::lock_guard<std::mutex> lk(h->mu);
     CsrPlan &p = h->plan;
@@ -261,20 +290,26 @@ int hl_csr_apply(hl_csr_t h, const double *x, double *y)
     if (rc) return rc;
     if (!yd) HL_TRY_HIP(hipMemcpy(y, yy, sizeof(double) * p.nrows, hipMemcpyDefault));
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_spmv(hl_csr_t h, const double *x_dev, double *y_dev, void *stream)
-{
+try {
     if (!h) return HL_ERR_ARG;
     DeviceGuard g(h->plan.dev);
     return launch_spmv(h->plan, x_dev, nullptr, y_dev, VColumn{}, HL_PREC_F64, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_spmv_f32val(hl_csr_t h, const double *x_dev, double *y_dev, void *stream)
-{
+try {
     if (!h) return HL_ERR_ARG;
     DeviceGuard g(h->plan.dev);
     return launch_spmv(h->plan, x_dev, nullptr, y_dev, VColumn{}, HL_PREC_F32, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return exception_code();
 }
 
 double hl_csr_spmv_bytes(hl_csr_t h, int val_prec)
@@ -300,39 +335,47 @@ int csr_items(hl_csr_t h, int nnz_cap, int rows_cap)
 } // namespace
 
 int hl_csr_set_items(hl_csr_t h, int nnz_cap, int rows_cap)
-{
+try {
     if (!h) return HL_ERR_ARG;
     DeviceGuard g(h->plan.dev);
     std::lock_guard<std::mutex> lk(h->mu);
     return csr_items(h, nnz_cap, rows_cap);
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_set_slices(hl_csr_t h, int width)
-{
+try {
     if (!h) return HL_ERR_ARG;
     DeviceGuard g(h->plan.dev);
     std::lock_guard<std::mutex> lk(h->mu);
     return plan_slices(h->plan, width);
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_get_slices(hl_csr_t h, int *width, int *npairs)
-{
+try {
     if (!h) return HL_ERR_ARG;
     if (width) *width = h->plan.slice_w;
     if (npairs) *npairs = h->plan.npairs;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_set_sell(hl_csr_t h, int on)
-{
+try {
     if (!h) return HL_ERR_ARG;
     DeviceGuard g(h->plan.dev);
     std::lock_guard<std::mutex> lk(h->mu);
     return plan_sell(h->plan, on != 0, 2.0);
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries)
-{
+try {
     if (!h) return HL_ERR_ARG;
     long long ne = 0;
     if (h->plan.sell_on)
@@ -340,25 +383,31 @@ int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries)
     if (on) *on = h->plan.sell_on ? 1 : 0;
     if (entries) *entries = ne;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_get_sell_col_bytes(hl_csr_t h, int *bytes)
-{
+try {
     if (!h || !bytes) return HL_ERR_ARG;
     *bytes = !h->plan.sell_on ? 0 : h->plan.e_d16 ? 2 : 4;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_set_staged(hl_csr_t h, int on)
-{
+try {
     if (!h) return HL_ERR_ARG;
     DeviceGuard g(h->plan.dev);
     std::lock_guard<std::mutex> lk(h->mu);
     return plan_staged(h->plan, on != 0);
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_get_staged(hl_csr_t h, int *on, long long *pieces, long long *entries, int *bins)
-{
+try {
     if (!h) return HL_ERR_ARG;
     const CsrPlan &p = h->plan;
     long long ne = 0;
@@ -369,10 +418,12 @@ int hl_csr_get_staged(hl_csr_t h, int *on, long long *pieces, long long *entries
     if (entries) *entries = ne;
     if (bins) *bins = p.stg_nbins;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *seconds)
-{
+try {
     if (!h || reps <= 0) return HL_ERR_ARG;
     DeviceGuard g(h->plan.dev);
     std::lock_guard<std::mutex> lk(h->mu);
@@ -477,6 +528,8 @@ int hl_csr_autotune(hl_csr_t h, int reps, int *nnz_cap, int *rows_cap, double *s
     if (rows_cap) *rows_cap = best_r;
     if (seconds) *seconds = best_t;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 namespace {
@@ -507,7 +560,7 @@ int pass_ws(int dev, PassWs *&out)
 
 int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
                 const double *h_in, double *h_out, double *nrm2_out, void *stream)
-{
+try {
     if (n < 0 || m < 0 || m > 128 || (m > 0 && (!V || ldv < n)) || !w_in) {
         set_error("hl_cgs_pass: bad arguments");
         return HL_ERR_ARG;
@@ -525,11 +578,13 @@ int hl_cgs_pass(int dev, int n, int m, const double *V, long long ldv, const dou
         HL_TRY_HIP(hipMemcpyAsync(h_out, ws->hout, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
     if (nrm2_out) HL_TRY_HIP(hipMemcpyAsync(nrm2_out, &ws->ctl->nrm2, sizeof(double), hipMemcpyDeviceToDevice, s));
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_cgs_pass_noalloc(int dev, int n, int m, const double *V, long long ldv, const double *w_in, double *w_out,
                         const double *h_in, double *h_out, double *nrm2_out, void *stream)
-{
+try {
     if (n < 0 || m < 0 || (m > 0 && (!V || ldv < n)) || !w_in) {
         set_error("hl_cgs_pass_noalloc: bad arguments");
         return HL_ERR_ARG;
@@ -545,10 +600,12 @@ int hl_cgs_pass_noalloc(int dev, int n, int m, const double *V, long long ldv, c
     }
     DeviceGuard g(dev);
     return launch_cgs_noalloc(dev, n, m, V, ldv, w_in, w_out, h_in, h_out, nrm2_out, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return exception_code();
 }
 
 int hl_bench_cgs(int dev, int n, int m, int tiled, int reps, double *seconds)
-{
+try {
     if (n <= 0 || m <= 0 || m > 128 || reps <= 0 || !seconds) return HL_ERR_ARG;
     int rc = check_device(dev);
     if (rc) return rc;
@@ -585,10 +642,12 @@ int hl_bench_cgs(int dev, int n, int m, int tiled, int reps, double *seconds)
     (void)hipFree(w2);
     (void)hipFree(h);
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_bench_stream(int dev, long long bytes, int reps, double *seconds)
-{
+try {
     if (bytes < 1024 || reps <= 0 || !seconds) return HL_ERR_ARG;
     int rc = check_device(dev);
     if (rc) return rc;
@@ -615,10 +674,12 @@ int hl_bench_stream(int dev, long long bytes, int reps, double *seconds)
     (void)hipFree(a);
     (void)hipFree(out);
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_bench_grid_barrier(int dev, int blocks, int barriers, double *seconds)
-{
+try {
     if (blocks < 1 || barriers < 1 || barriers > (1 << 20) || !seconds) return HL_ERR_ARG;
     int rc = check_device(dev);
     if (rc) return rc;
@@ -664,10 +725,12 @@ int hl_bench_grid_barrier(int dev, int blocks, int barriers, double *seconds)
     (void)hipFree(counter);
     (void)hipFree(buf);
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_bench_graph_nodes(int dev, int nodes, int blocks, int reps, double *seconds)
-{
+try {
     if (nodes < 1 || nodes > 4096 || blocks < 1 || blocks > (1 << 20) || reps < 1 || !seconds) return HL_ERR_ARG;
     int rc = check_device(dev);
     if (rc) return rc;
@@ -699,10 +762,12 @@ int hl_bench_graph_nodes(int dev, int nodes, int blocks, int reps, double *secon
     (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(s);
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_bench_gather(int dev, long long nx, long long m, int random, int reps, double *seconds)
-{
+try {
     if (nx <= 0 || nx > 0x7fffffffLL || m < 1024 || reps <= 0 || !seconds) return HL_ERR_ARG;
     int rc = check_device(dev);
     if (rc) return rc;
@@ -746,10 +811,12 @@ int hl_bench_gather(int dev, long long nx, long long m, int random, int reps, do
     (void)hipFree(di);
     (void)hipFree(out);
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_tune_set(const char *key, int value)
-{
+try {
     if (!key) return HL_ERR_ARG;
     Tuning &t = tuning();
     if (!std::strcmp(key, "cgs_blocks_max") && value >= 1 && value <= kMaxReduceBlocks)
@@ -769,10 +836,12 @@ int hl_tune_set(const char *key, int value)
         return HL_ERR_ARG;
     }
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_tune_get(const char *key, int *value)
-{
+try {
     if (!key || !value) return HL_ERR_ARG;
     const Tuning &t = tuning();
     if (!std::strcmp(key, "cgs_blocks_max"))
@@ -792,11 +861,13 @@ int hl_tune_get(const char *key, int *value)
         return HL_ERR_ARG;
     }
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const double *Q, int ldq, double *Out,
           long long ldo, int use_mfma, void *stream)
-{
+try {
     if (n < 0 || K <= 0 || N <= 0 || K > kMaxNcv || !V || !Q || !Out || ldv < n || ldo < n || ldq < K) {
         set_error("hl_vq: bad arguments");
         return HL_ERR_ARG;
@@ -807,18 +878,24 @@ int hl_vq(int dev, int n, int K, int N, const double *V, long long ldv, const do
     Reduce red;
     return launch_vq(n, K, N, V, VLayout{ldv, false}, Q, ldq, Out, VLayout{ldo, false}, use_mfma, -1, nullptr, 0.0,
                      0.0, red, nullptr, static_cast<hipStream_t>(stream));
+} catch (...) {
+    return exception_code();
 }
 
 int hl_host_tridiag_eig(int n, double *d, double *e, double *z, int ldz)
-{
+try {
     if (n < 0 || (n > 0 && (!d || (n > 1 && !e) || !z || ldz < n))) return HL_ERR_ARG;
     return tridiag_eig(n, d, e, z, ldz);
+} catch (...) {
+    return exception_code();
 }
 
 int hl_host_tridiag_eig_last(int n, double *d, double *e, double *zlast)
-{
+try {
     if (n < 0 || (n > 0 && (!d || (n > 1 && !e) || !zlast))) return HL_ERR_ARG;
     return tridiag_eig_last(n, d, e, zlast);
+} catch (...) {
+    return exception_code();
 }
 
 void hl_host_dlartg(double f, double g, double *c, double *s, double *r)
@@ -831,29 +908,35 @@ void hl_host_dlartg(double f, double g, double *c, double *s, double *r)
 }
 
 int hl_host_householder_q(int m, int k, double *a, int lda, double *tau, double *q, int ldq)
-{
+try {
     if (m < 0 || k < 0 || k > m || (m > 0 && (!a || !tau || !q || lda < m || ldq < m))) return HL_ERR_ARG;
     householder_q(m, k, a, lda, tau, q, ldq);
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_host_restart_chase(int nev, int np, const double *shift, double *sub, double *dia, double *q)
-{
+try {
     if (nev <= 0 || np < 0 || (np > 0 && !shift) || !sub || !dia || !q) return HL_ERR_ARG;
     restart_chase(nev, np, shift, sub, dia, q);
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_profile_last(hl_profile *out)
-{
+try {
     if (!out) return HL_ERR_ARG;
     *out = thread_profile();
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_ds_solve_csr(hl_csr_t A, int nev, int ncv, double *values, double *vectors, int ldv,
                     const hl_options *opts, hl_info *info)
-{
+try {
     if (!A) return HL_ERR_ARG;
     if (A->plan.nrows != A->plan.ncols) {
         set_error("ds_solver: the operator must be square");
@@ -862,20 +945,24 @@ int hl_ds_solve_csr(hl_csr_t A, int nev, int ncv, double *values, double *vector
     hl::Member m;
     m.A = &A->plan;
     return solve_common(m, A->plan.dev, A->plan.nrows, nev, ncv, values, vectors, ldv, opts, info);
+} catch (...) {
+    return exception_code();
 }
 
 int hl_ds_solve_host_op(int dev, int n, hl_host_op_fn op, void *ctx, int nev, int ncv, double *values,
                         double *vectors, int ldv, const hl_options *opts, hl_info *info)
-{
+try {
     if (!op) return HL_ERR_ARG;
     hl::Member m;
     m.host_op = op;
     m.host_ctx = ctx;
     return solve_common(m, dev, n, nev, ncv, values, vectors, ldv, opts, info);
+} catch (...) {
+    return exception_code();
 }
 
 int hl_ds_solve_batch(int dev, hl_solve_item *items, int count, int max_active, double hbm_budget)
-{
+try {
     if (count < 0 || (count > 0 && !items)) return HL_ERR_ARG;
     int rc = check_device(dev);
     if (rc) return rc;
@@ -964,6 +1051,8 @@ int hl_ds_solve_batch(int dev, hl_solve_item *items, int count, int max_active, 
         it.t_done = m.t_done;
     }
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 } // extern "C"

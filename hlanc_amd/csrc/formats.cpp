@@ -201,7 +201,7 @@ using namespace hl;
 extern "C" {
 
 int hl_matrix_read(const char *format, const char *filename, int is_symmetric, hl_matrix *out)
-{
+try {
     if (!format || !filename || !out) return HL_ERR_ARG;
     std::memset(out, 0, sizeof(*out));
     int rc;
@@ -217,6 +217,8 @@ int hl_matrix_read(const char *format, const char *filename, int is_symmetric, h
     }
     if (rc) hl_matrix_free(out);
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 void hl_matrix_free(hl_matrix *m)
@@ -229,7 +231,7 @@ void hl_matrix_free(hl_matrix *m)
 }
 
 int hl_matrix_write(const char *format, const char *filename, const hl_matrix *m)
-{
+try {
     if (!format || !filename || !m) return HL_ERR_ARG;
     const int nr = m->nrows;
     if (!std::strcmp(format, "bin")) {
@@ -309,6 +311,8 @@ int hl_matrix_write(const char *format, const char *filename, const hl_matrix *m
         return HL_ERR_IO;
     }
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 } // extern "C"

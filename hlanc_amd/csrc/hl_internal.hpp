@@ -21,6 +21,11 @@ constexpr double kSafmin = 2.2250738585072014e-308; // dlamch("S")
 
 // ----------------------------------------------------------------- errors
 void set_error(const char *fmt, ...);
+// The error code of the exception being handled (called from a catch block):
+// std::bad_alloc is HL_ERR_OOM, anything else HL_ERR_STATE, with the message
+// set.  Every extern "C" entry point ends in such a handler, so no C++
+// exception crosses the C ABI.
+int exception_code() noexcept;
 int hip_status(hipError_t e, const char *what);
 
 #define HL_TRY_HIP(call)                                           \

@@ -347,7 +347,7 @@ using namespace hl;
 extern "C" {
 
 int hl_serve_create(int dev, int max_active, double hbm_budget, int flags, hl_server_t *out)
-{
+try {
     if (!out) return HL_ERR_ARG;
     *out = nullptr;
     int count = 0;
@@ -370,11 +370,13 @@ int hl_serve_create(int dev, int max_active, double hbm_budget, int flags, hl_se
     s->load_th = std::thread(load_loop, s);
     *out = s;
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_serve_submit(hl_server_t s, hl_csr_t A, int nev, int ncv, const hl_options *opts, double *values,
                     double *vectors, int ldv, long long *ticket)
-{
+try {
     if (!s || !A || !ticket) return HL_ERR_ARG;
     const CsrPlan *p = csr_plan(A);
     if (p->dev != s->dev || p->nrows != p->ncols) {
@@ -386,12 +388,14 @@ int hl_serve_submit(hl_server_t s, hl_csr_t A, int nev, int ncv, const hl_option
     const int rc = hl::submit(s, q, A, nev, ncv, opts, values, vectors, ldv, ticket);
     if (rc) delete q;
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_serve_submit_csr(hl_server_t s, int nrows, int nnz, const int *row_ptr, const int *col, const double *val,
                         int sym_lower, int nev, int ncv, const hl_options *opts, double *values, double *vectors,
                         int ldv, long long *ticket)
-{
+try {
     if (!s || !ticket || nrows <= 0 || nnz < 0 || !row_ptr || (nnz > 0 && (!col || !val))) return HL_ERR_ARG;
     if (row_ptr[0] != 0 || row_ptr[nrows] != nnz) {
         set_error("hl_serve_submit_csr: inconsistent CSR (row_ptr[0] must be 0 and row_ptr[nrows] == nnz)");
@@ -408,10 +412,12 @@ int hl_serve_submit_csr(hl_server_t s, int nrows, int nnz, const int *row_ptr, c
     const int rc = hl::submit(s, q, nullptr, nev, ncv, opts, values, vectors, ldv, ticket);
     if (rc) delete q;
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_serve_poll(hl_server_t s, long long ticket, hl_serve_result *out)
-{
+try {
     if (!s) return HL_ERR_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
@@ -422,10 +428,12 @@ int hl_serve_poll(hl_server_t s, long long ticket, hl_serve_result *out)
     if (it->second->state != Request::DONE) return HL_PENDING;
     fill_result(*it->second, out);
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_serve_wait(hl_server_t s, long long ticket, double timeout_s, hl_serve_result *out)
-{
+try {
     if (!s) return HL_ERR_ARG;
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
@@ -441,10 +449,12 @@ int hl_serve_wait(hl_server_t s, long long ticket, double timeout_s, hl_serve_re
         return HL_PENDING;
     fill_result(*q, out);
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_serve_release(hl_server_t s, long long ticket)
-{
+try {
     if (!s) return HL_ERR_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
@@ -454,10 +464,12 @@ int hl_serve_release(hl_server_t s, long long ticket)
     }
     s->reqs.erase(it);
     return HL_OK;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_serve_stats(hl_server_t s, hl_serve_stats_t *out)
-{
+try {
     if (!s || !out) return HL_ERR_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
     std::memset(out, 0, sizeof(*out));
@@ -468,10 +480,12 @@ int hl_serve_stats(hl_server_t s, hl_serve_stats_t *out)
     out->bytes_in_use = s->in_use;
     out->seconds = now_s() - s->t0;
     return s->fatal;
+} catch (...) {
+    return exception_code();
 }
 
 int hl_serve_destroy(hl_server_t s)
-{
+try {
     if (!s) return HL_OK;
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -489,6 +503,8 @@ int hl_serve_destroy(hl_server_t s)
     const int rc = s->fatal;
     delete s;
     return rc;
+} catch (...) {
+    return exception_code();
 }
 
 } // extern "C"
