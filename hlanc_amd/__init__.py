@@ -95,6 +95,7 @@ class ServeStats(ctypes.Structure):
 
 HL_PENDING = 2
 HL_SERVE_SELL_EVEN = 1
+HL_FORM_ITEMS, HL_FORM_SLICES, HL_FORM_SELL, HL_FORM_STAGED = 0, 1, 2, 3
 
 _lib = None
 
@@ -123,6 +124,8 @@ _SIGS = [
     ("hl_csr_set_sell", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hl_csr_get_sell", ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong)]),
     ("hl_csr_get_sell_col_bytes", ctypes.c_int, [ctypes.c_void_p, _ip]),
+    ("hl_csr_pick_form", ctypes.c_int, [ctypes.c_void_p, _ip]),
+    ("hl_csr_device_bytes", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     ("hl_csr_set_staged", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("hl_csr_get_staged", ctypes.c_int, [ctypes.c_void_p, _ip, ctypes.POINTER(ctypes.c_longlong),
                                          ctypes.POINTER(ctypes.c_longlong), _ip]),
@@ -367,6 +370,19 @@ class dcsrmv_operator_gpu:
         on, ne = ctypes.c_int(), ctypes.c_longlong()
         _check(lib().hl_csr_get_sell(self.h, ctypes.byref(on), ctypes.byref(ne)))
         return bool(on.value), ne.value
+
+    def pick_form(self) -> int:
+        """Builds the SpMV form chosen from the matrix's structure, without timing
+        (hl_csr_pick_form): HL_FORM_SELL, HL_FORM_STAGED or HL_FORM_ITEMS."""
+        f = ctypes.c_int()
+        _check(lib().hl_csr_pick_form(self.h, ctypes.byref(f)))
+        return f.value
+
+    def device_bytes(self) -> float:
+        """Device bytes the operator holds, its SpMV form's arrays included (hl_csr_device_bytes)."""
+        b = ctypes.c_double()
+        _check(lib().hl_csr_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
 
     def sell_col_bytes(self) -> int:
         """Bytes per stored column index of the sliced-ELL form: 2 for banded operators (offsets

@@ -396,6 +396,26 @@ try {
     return exception_code();
 }
 
+int hl_csr_pick_form(hl_csr_t h, int *form)
+try {
+    if (!h || !form) return HL_ERR_ARG;
+    DeviceGuard g(h->plan.dev);
+    std::lock_guard<std::mutex> lk(h->mu);
+    return csr_pick_form(h->plan, form);
+} catch (...) {
+    return exception_code();
+}
+
+int hl_csr_device_bytes(hl_csr_t h, double *bytes)
+try {
+    if (!h || !bytes) return HL_ERR_ARG;
+    std::lock_guard<std::mutex> lk(h->mu);
+    *bytes = csr_device_bytes(h->plan);
+    return HL_OK;
+} catch (...) {
+    return exception_code();
+}
+
 int hl_csr_set_staged(hl_csr_t h, int on)
 try {
     if (!h) return HL_ERR_ARG;

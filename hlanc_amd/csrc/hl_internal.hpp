@@ -184,6 +184,7 @@ struct CsrPlan {
     // a row's own length.
     bool sell_on = false;
     int nchunks = 0;
+    long long sell_entries = 0;  // e_ptr[nchunks]
     long long *e_ptr = nullptr;  // nchunks + 1
     int *e_col = nullptr;        // e_ptr[nchunks] entries (padding: column 0); null when banded:
     short *e_d16 = nullptr;      // banded operators (every |col - row| < 2^15): col - row (padding: 0)
@@ -203,6 +204,7 @@ struct CsrPlan {
     int stg_nsup = 0;                      // blocks of the first kernel
     int stg_nbins = 0;                     // blocks of the second kernel
     long long stg_npieces = 0;             // pieces, without the padding
+    long long stg_entries = 0;             // s_cptr[nchunks]: stored entries (ELL padding included)
     long long *s_cptr = nullptr;           // nchunks + 1: first entry of each chunk
     unsigned char *s_plen = nullptr;       // 64 nchunks: entries per piece (padding 0)
     unsigned short *s_ecol = nullptr;      // s_cptr[nchunks]: slice-local columns
@@ -241,6 +243,14 @@ int plan_sell(CsrPlan &p, bool on, double max_fill);
 // Builds (on) or drops the staged-slice form; building it drops the other two
 // forms.
 int plan_staged(CsrPlan &p, bool on);
+// Device bytes the plan holds: the CSR, its FP32 values, staging, work items
+// and the arrays of the form in use.
+double csr_device_bytes(const CsrPlan &p);
+// The SpMV form chosen from the matrix's structure, without timing (see
+// hl_csr_pick_form): *form is HL_FORM_SELL, HL_FORM_STAGED or HL_FORM_ITEMS.
+int csr_pick_form(CsrPlan &p, int *form);
+constexpr double kPickSellFill = 1.1;        // sliced-ELL where chunks hold <= 1.1 nnz entries
+constexpr long long kPickStagedCols = 786432; // staged slices where x spans >= 6 MB (beyond one XCD's L2)
 
 // y = A (x * xscale) with the optional fused write vcol(r) = x[r] * xscale.
 // xscale == nullptr means 1.  val_prec selects the FP64 or FP32 value array.

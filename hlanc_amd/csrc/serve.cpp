@@ -91,6 +91,12 @@ double operator_bytes(int n, long long nnz_general)
     return 12.0 * static_cast<double>(nnz_general) + 4.0 * (n + 1.0) + 16.0 * n;
 }
 
+// A request's workspace bytes, once its operator is known.
+double workspace_bytes(const Member &m)
+{
+    return Workspace::bytes_needed(m.n, m.ncv, false, m.nslots(), m.npairs(), m.copy32());
+}
+
 // The loader thread: uploads admitted requests' operators in admission order.
 void load_loop(hl_server_s *s)
 {
@@ -118,6 +124,12 @@ void load_loop(hl_server_s *s)
             }
         }
         std::lock_guard<std::mutex> lk(s->mu);
+        if (h) { // the footprint again: the operator's device bytes and the workspace its form needs
+            q->m.A = csr_plan(h);
+            const double fp = workspace_bytes(q->m) + csr_device_bytes(*csr_plan(h));
+            s->in_use += fp - q->footprint;
+            q->footprint = fp;
+        }
         q->A = h;
         q->owned = h != nullptr;
         q->load_rc = rc;
@@ -125,12 +137,6 @@ void load_loop(hl_server_s *s)
         s->ready.push_back(q);
         s->cv_work.notify_all();
     }
-}
-
-// A request's workspace bytes, once its operator is known.
-double workspace_bytes(const Member &m)
-{
-    return Workspace::bytes_needed(m.n, m.ncv, false, m.nslots(), m.npairs(), m.copy32());
 }
 
 void serve_loop(hl_server_s *s)

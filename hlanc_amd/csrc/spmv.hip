@@ -836,6 +836,7 @@ void sell_free(CsrPlan &p)
     p.e_d16 = nullptr;
     p.e_val = nullptr;
     p.nchunks = 0;
+    p.sell_entries = 0;
     p.sell_on = false;
 }
 
@@ -866,6 +867,7 @@ void staged_free(CsrPlan &p)
     p.stg_on = false;
     p.stg_nsl = p.stg_nchunks = p.stg_nsup = p.stg_nbins = 0;
     p.stg_npieces = 0;
+    p.stg_entries = 0;
 }
 } // namespace
 
@@ -1031,6 +1033,7 @@ int plan_sell(CsrPlan &p, bool on, double max_fill)
     slices_free(p);
     staged_free(p);
     p.nchunks = nch;
+    p.sell_entries = ne;
     p.sell_on = true;
     return HL_OK;
 }
@@ -1243,6 +1246,58 @@ int plan_staged(CsrPlan &p, bool on)
     p.stg_nsup = static_cast<int>(sup.size());
     p.stg_nbins = nbins;
     p.stg_npieces = np;
+    p.stg_entries = cptr[nch];
+    return HL_OK;
+}
+
+double csr_device_bytes(const CsrPlan &p)
+{
+    const double n = p.nrows, nnz = static_cast<double>(p.nnz);
+    auto items = [](const ItemSet &t) {
+        return static_cast<double>(sizeof(SpmvBlk)) * std::max(t.nblk, 1) + 12.0 * std::max(t.nslots, 1);
+    };
+    // row_ptr, col and values, FP32 values, x and y staging, work items
+    double b = 4.0 * (n + 1) + 12.0 * nnz + (p.val32 ? 4.0 * nnz : 0.0) + 8.0 * (p.ncols + n) + items(p.items);
+    if (p.slice_w > 0) // virtual-row CSR, row lists, partials
+        b += 4.0 * (p.npairs + 1.0) + 12.0 * nnz + 4.0 * (n + 1) + 12.0 * p.npairs + items(p.sitems);
+    if (p.sell_on) // chunk pointers, columns (2 or 4 bytes), values
+        b += 8.0 * (p.nchunks + 1.0) + (p.e_d16 ? 10.0 : 12.0) * static_cast<double>(p.sell_entries);
+    if (p.stg_on) { // chunks, piece lengths and partials, entries, blocks, bins, run table, lists
+        const double pad = static_cast<double>(kWave) * p.stg_nchunks;
+        b += 8.0 * (p.stg_nchunks + 1.0) + 9.0 * pad + 10.0 * static_cast<double>(p.stg_entries) +
+             16.0 * (p.stg_nsup + p.stg_nbins) + 4.0 * (p.stg_nbins + 1.0) * p.stg_nsl + 4.0 * (n + 1) +
+             2.0 * static_cast<double>(p.stg_npieces);
+    }
+    return b;
+}
+
+// Structural choice of the SpMV form: the sliced-ELL form where its chunks
+// hold at most kPickSellFill x nnz entries (even rows: stencils, meshes), else
+// the staged-slice form where x spans at least kPickStagedCols columns (too
+// large for one XCD's L2, so gathers from global memory would miss), else the
+// stream items.  A form that refuses the matrix passes to the next choice.
+// Measured on one MI355X (standalone SpMV): Erdos-Renyi 1e6 / 2e6 rows staged
+// 196 / 215 us against items 253 / 353; Chung-Lu 1e6 / 2e6 staged 97 / 185
+// against 121 / 265; at 5e5 rows items win (66 against 81, 56 against 58).
+int csr_pick_form(CsrPlan &p, int *form)
+{
+    int rc;
+    if ((rc = plan_slices(p, 0)) || (rc = plan_staged(p, false)) || (rc = plan_sell(p, false, 0.0))) return rc;
+    rc = plan_sell(p, true, kPickSellFill);
+    if (rc == HL_OK) {
+        *form = HL_FORM_SELL;
+        return HL_OK;
+    }
+    if (rc != HL_ERR_ARG) return rc;
+    if (static_cast<long long>(p.ncols) >= kPickStagedCols) {
+        rc = plan_staged(p, true);
+        if (rc == HL_OK) {
+            *form = HL_FORM_STAGED;
+            return HL_OK;
+        }
+        if (rc != HL_ERR_ARG) return rc;
+    }
+    *form = HL_FORM_ITEMS;
     return HL_OK;
 }
 

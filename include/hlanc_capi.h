@@ -148,6 +148,25 @@ int hl_csr_get_sell(hl_csr_t h, int *on, long long *entries); /* outputs may be 
  * from the row), 4 (absolute columns), 0 when the form is off.  No
  * counterpart in the reference. */
 int hl_csr_get_sell_col_bytes(hl_csr_t h, int *bytes);
+/* SpMV forms (hl_csr_pick_form). */
+#define HL_FORM_ITEMS 0   /* stream items over the rows (hl_csr_set_items) */
+#define HL_FORM_SLICES 1  /* column slices (hl_csr_set_slices) */
+#define HL_FORM_SELL 2    /* sliced-ELL (hl_csr_set_sell) */
+#define HL_FORM_STAGED 3  /* staged slices (hl_csr_set_staged) */
+/* The SpMV form chosen from the matrix's structure, without timing anything
+ * (for callers that cannot time forms, for instance while other solves
+ * run): sliced-ELL where its chunks hold at most 1.1 nnz entries, else
+ * staged slices where x spans at least 786,432 columns (6 MB, more than one
+ * XCD's L2 holds), else stream items; a form that refuses the matrix passes
+ * to the next.  The chosen form is built (the others dropped) and *form
+ * names it.  The staged form's host planning takes about 45 ns per nonzero
+ * (3.3 s at 70 M), more than its faster SpMV saves in one solve of a few
+ * hundred steps: it pays off for operators that serve many solves.
+ * hl_csr_autotune times the forms instead.  No counterpart in the reference. */
+int hl_csr_pick_form(hl_csr_t h, int *form);
+/* Device bytes the operator holds: its CSR (and FP32 values), staging, work
+ * items and the arrays of its SpMV form.  No counterpart in the reference. */
+int hl_csr_device_bytes(hl_csr_t h, double *bytes);
 /* Staged-slice SpMV: on != 0 cuts the FP64 matrix into column slices of 16384
  * columns, few enough that a block holds a slice's x (128 KB) in LDS, so that
  * no x gather leaves the CU.  Each row's entries of a slice (cut every 64
@@ -435,7 +454,10 @@ int hl_ds_solve_host_op(int dev, int n, hl_host_op_fn op, void *ctx, int nev, in
  * (basis pair, vectors, partial sums) plus, for hl_serve_submit_csr, its
  * operator: 12 nnz + 4 (n + 1) bytes and 16 n of staging, uploaded by the
  * server's loader thread once the request is admitted (overlapping the sweeps
- * of the requests already running) and freed when it is done.  Each request's
+ * of the requests already running) and freed when it is done.  Once the
+ * loader has built such an operator, the request's footprint is counted
+ * again from the operator's device bytes (its SpMV form's arrays included,
+ * hl_csr_device_bytes) and the workspace its form needs.  Each request's
  * results are bitwise those of hl_ds_solve_csr on the same operator and
  * options, whatever runs beside it.  All entry points are thread-safe. */
 typedef struct hl_server_s *hl_server_t;

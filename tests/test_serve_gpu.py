@@ -4,7 +4,8 @@ result must equal, bitwise, the standalone solve of the same matrix (an
 operator built from the same arrays) with the same options, whatever runs
 beside it and whenever it arrives; admission must honour the concurrency cap
 and the HBM budget (footprint = workspace + the operator's 12 nnz + 4 (n+1) +
-16 n bytes when the server uploads it)."""
+16 n bytes when the server uploads it, recounted from the operator's device
+bytes once it is built)."""
 import threading
 import time
 
@@ -184,3 +185,65 @@ def test_close_finishes_queued_requests(pool):
         vals = keep[t][0][: keep[t][5]]
         np.testing.assert_array_equal(vals, alone[(name, H.HL_PREC_F64)].values)
     del outs
+
+
+def test_pick_form_follows_structure():
+    """hl_csr_pick_form: sliced-ELL for even rows, staged slices where x spans
+    at least 786,432 columns, stream items otherwise, and the next choice when
+    a form refuses the matrix; hl_csr_device_bytes counts the form's arrays."""
+    cases = [("stencil5", W.stencil5(64, seed=5), None, H.HL_FORM_SELL),  # fill 1.006
+             ("stencil27", W.stencil27(12, seed=1), None, H.HL_FORM_ITEMS),  # fill 1.13: boundary rows
+             ("er_small", W.erdos_renyi(6000, 12, seed=2), None, H.HL_FORM_ITEMS),
+             ("cl_large", W.chung_lu_laplacian(1 << 20, seed=3), None, H.HL_FORM_STAGED)]
+    # uneven rows (no sliced-ELL) over 70 M columns (more slices than the staged form takes)
+    rng = np.random.default_rng(4)
+    lens = np.where(np.arange(200) % 10 == 0, 300, 1)
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = rng.integers(0, 70_000_000, rp[-1]).astype(np.int32)
+    cases.append(("wide_uneven", (rp, col, rng.uniform(-1, 1, rp[-1])), 70_000_000, H.HL_FORM_ITEMS))
+    for name, (rp, col, val), ncols, want in cases:
+        ncols = ncols or len(rp) - 1
+        op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val, ncols=ncols))
+        n = len(rp) - 1
+        base = op.device_bytes()
+        assert base >= 12.0 * len(val) + 4.0 * (n + 1) + 8.0 * (n + ncols), name
+        assert op.pick_form() == want, name
+        assert op.sell()[0] == (want == H.HL_FORM_SELL) and op.staged()[0] == (want == H.HL_FORM_STAGED)
+        assert op.slices()[0] == 0
+        if want == H.HL_FORM_SELL:
+            assert op.device_bytes() >= base + 10.0 * op.sell()[1]
+        if want == H.HL_FORM_STAGED:
+            _, pieces, entries, _ = op.staged()
+            assert op.device_bytes() >= base + 10.0 * entries + 8.0 * pieces
+        if want == H.HL_FORM_ITEMS:
+            assert op.device_bytes() == base
+        x = np.random.default_rng(5).uniform(-1, 1, ncols)
+        y_ref = O.spmv(O.Csr(rp, col, val, ncols, False), x)
+        assert np.linalg.norm(op(x) - y_ref) <= 1e-14 * np.linalg.norm(y_ref), name
+
+
+def test_server_footprints_count_the_operator_form():
+    """With HL_SERVE_SELL_EVEN the server uploads even-row operators in the
+    sliced-ELL form (the form hl_csr_pick_form picks for these small
+    matrices); each result is bitwise the standalone solve on an operator given
+    the same form, and each footprint is recounted from the operator's device
+    bytes, its form's arrays included."""
+    reqs = [("st27_12", W.stencil27(12, seed=1), 8, 24), ("er_6000", W.erdos_renyi(6000, 12, seed=2), 6, 18),
+            ("st5_64", W.stencil5(64, seed=5), 4, 12)]
+    ref, forms, dev_bytes = {}, {}, {}
+    for name, (rp, col, val), nev, ncv in reqs:
+        op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+        forms[name] = op.pick_form()
+        dev_bytes[name] = op.device_bytes()
+        ref[name] = H.ds_solver_gpu(op.nrows, nev, ncv, op, tol=TOL, maxiter=3000, seed=5,
+                                    basis_prec=H.HL_PREC_F64_PROJ32, refine=H.HL_REFINE_RQ)
+        assert ref[name].ret == 0
+    assert forms == {"st27_12": H.HL_FORM_ITEMS, "er_6000": H.HL_FORM_ITEMS, "st5_64": H.HL_FORM_SELL}
+    with H.Server(sell_even=True) as srv:
+        tk = [(name, srv.submit(arrays, nev, ncv, tol=TOL, maxiter=3000, seed=5, basis_prec=H.HL_PREC_F64_PROJ32,
+                                refine=H.HL_REFINE_RQ, vectors=False))
+              for name, arrays, nev, ncv in reqs]
+        outs = [(name, srv.wait(t)) for name, t in tk]
+    for name, out in outs:
+        same(out, ref[name])
+        assert out.footprint > dev_bytes[name], name
