@@ -198,6 +198,9 @@ struct CsrPlan {
     // descending length inside a (slice, bin) run.  Row bins hold <= kBinCap
     // pieces; the second kernel copies a bin's runs into LDS and adds each
     // row's partials in slice order through a per-row list of LDS offsets.
+    // Within each group of chunks a wave of the first kernel takes at once,
+    // lanes are re-ordered by piece length; a lane's code names the slot its
+    // partial goes to.
     bool stg_on = false;
     int stg_nsl = 0;                       // slices
     int stg_nchunks = 0;                   // 64-piece chunks (each slice padded to whole chunks)
@@ -206,7 +209,8 @@ struct CsrPlan {
     long long stg_npieces = 0;             // pieces, without the padding
     long long stg_entries = 0;             // s_cptr[nchunks]: stored entries (ELL padding included)
     long long *s_cptr = nullptr;           // nchunks + 1: first entry of each chunk
-    unsigned char *s_plen = nullptr;       // 64 nchunks: entries per piece (padding 0)
+    unsigned short *s_plen = nullptr;      // 64 nchunks: lane codes, entries of the piece (padding 0) | slot << 7
+                                           // (the piece's slot in its wave's group of chunks)
     unsigned short *s_ecol = nullptr;      // s_cptr[nchunks]: slice-local columns
     double *s_eval = nullptr;              // s_cptr[nchunks]: values
     int4 *s_sup = nullptr;                 // nsup: {slice, first chunk, end chunk, 0}
@@ -304,7 +308,7 @@ struct SegMember {               // per solve, fixed during a sweep
     // kernel writes the pieces' partials into part, its bins kernel adds them
     const int4 *s_sup;
     const long long *s_cptr;
-    const unsigned char *s_plen;
+    const unsigned short *s_plen;
     const unsigned short *s_ecol, *s_lst;
     const double *s_eval;
     const int4 *s_bmeta;

@@ -360,13 +360,16 @@ __global__ __launch_bounds__(kBlock) void spmv_rows_kernel(int n, const int *__r
 // slice (su = {slice, first chunk, end chunk}): x of the slice into LDS, then
 // each wave takes kStagedGroup chunks at a time (their loads in flight
 // together), a lane per piece summing its entries in order (lanes stop at
-// their own length, so padding is neither loaded nor added), the partial
-// written at the piece's index.  A chunk is column-major: entry k of lane l
-// at cptr[c] + 64 k + l, so a wave's k-th loads are one contiguous stretch.
+// their own length, so padding is neither loaded nor added).  Within a wave's
+// group, lanes are ordered by descending piece length (chunks only as wide as
+// their own longest piece) and each lane writes its partial to its piece's
+// slot in the group (plen: length | slot << 7), so partials keep the run
+// order the bins kernel reads.  A chunk is column-major: entry k of lane l at
+// cptr[c] + 64 k + l, so a wave's k-th loads are one contiguous stretch.
 constexpr int kStagedGroup = 4;
 constexpr int kStagedUnroll = 4; // entries of a piece in flight per trip
 __device__ __forceinline__ void staged_block(int4 su, long long ncols, const long long *__restrict__ cptr_,
-                                             const unsigned char *__restrict__ plen_,
+                                             const unsigned short *__restrict__ plen_,
                                              const unsigned short *__restrict__ ecol_,
                                              const double *__restrict__ eval_, const double *__restrict__ x_,
                                              double s, double *__restrict__ part_, double *xs)
@@ -400,8 +403,8 @@ __device__ __forceinline__ void staged_block(int4 su, long long ncols, const lon
     }
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
-    // chunk bounds and piece lengths of a wave's group of chunks; the next
-    // group's are loaded while this one's entries are in flight
+    // chunk bounds and lane codes (piece length | slot << 7) of a wave's group
+    // of chunks; the next group's are loaded while this one's entries are in flight
     long long nb[G], ne[G];
     int nl[G];
     auto meta = [&](int cb) {
@@ -421,12 +424,13 @@ __device__ __forceinline__ void staged_block(int4 su, long long ncols, const lon
     __syncthreads();
     for (int cb = su.y + wave; cb < su.z; cb += G * NW) {
         long long base[G];
-        int len[G];
+        int len[G], slot[G];
         int wmax = 0;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             base[g] = nb[g];
-            len[g] = nl[g];
+            len[g] = nl[g] & 127;
+            slot[g] = nl[g] >> 7;
             wmax = max(wmax, static_cast<int>((ne[g] - nb[g]) / kWave));
         }
         meta(cb + G * NW);
@@ -451,10 +455,15 @@ __device__ __forceinline__ void staged_block(int4 su, long long ncols, const lon
                 for (int u = 0; u < U; ++u)
                     if (k0 + u < len[g]) acc[g] += av[g][u] * (xs[ci[g][u]] * s);
         }
+        // each partial to its piece's slot in the group (the group's stores
+        // together write every slot of its chunks; non-temporal, as the bins
+        // kernel reads them once)
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int c = cb + g * NW;
-            if (c < su.z) __builtin_nontemporal_store(acc[g], &part[static_cast<long long>(c) * kWave + lane]);
+            if (c < su.z)
+                __builtin_nontemporal_store(acc[g], &part[static_cast<long long>(cb + (slot[g] >> 6) * NW) * kWave +
+                                                          (slot[g] & (kWave - 1))]);
         }
     }
 }
@@ -589,7 +598,7 @@ inline size_t bins_dyn_lds(int nsl) { return sizeof(int) * (2 * static_cast<size
 
 __global__ __launch_bounds__(kStagedThreads) void spmv_staged_kernel(const int4 *__restrict__ sup, long long ncols,
                                                                      const long long *__restrict__ cptr,
-                                                                     const unsigned char *__restrict__ plen,
+                                                                     const unsigned short *__restrict__ plen,
                                                                      const unsigned short *__restrict__ ecol,
                                                                      const double *__restrict__ eval,
                                                                      const double *__restrict__ x,
@@ -1177,35 +1186,17 @@ int plan_staged(CsrPlan &p, bool on)
             return HL_ERR_ARG;
         }
     }
-    // chunks (column-major, as wide as their longest piece), lengths and entries
+    // blocks of the first kernel: runs of chunks within a slice, about
+    // kStagedSupEntries entries each (chunk widths of the order above)
     const long long nch = npad / kWave;
-    std::vector<long long> cptr(static_cast<size_t>(nch) + 1, 0);
-    std::vector<unsigned char> plen(static_cast<size_t>(npad), 0); // lengths <= kStagedPiece fit a byte
-    for (long long c = 0; c < nch; ++c) {
+    auto width0 = [&](long long c) {
         int w = 0;
         for (int l = 0; l < kWave; ++l) {
             const int g = at_pos[c * kWave + l];
-            if (g >= 0) {
-                plen[c * kWave + l] = static_cast<unsigned char>(g_len[g]);
-                w = std::max(w, static_cast<int>(g_len[g]));
-            }
+            if (g >= 0) w = std::max(w, static_cast<int>(g_len[g]));
         }
-        cptr[c + 1] = cptr[c] + static_cast<long long>(kWave) * w;
-    }
-    std::vector<unsigned short> ec(static_cast<size_t>(cptr[nch]), 0);
-    std::vector<double> ev(static_cast<size_t>(cptr[nch]), 0.0);
-    for (long long c = 0; c < nch; ++c)
-        for (int l = 0; l < kWave; ++l) {
-            const int g = at_pos[c * kWave + l];
-            if (g < 0) continue;
-            for (int k = 0; k < g_len[g]; ++k) {
-                const long long e = cptr[c] + static_cast<long long>(k) * kWave + l;
-                ec[e] = static_cast<unsigned short>(gc[g_e[g] + k]);
-                ev[e] = ga[g_e[g] + k];
-            }
-        }
-    // blocks of the first kernel: runs of chunks within a slice, about
-    // kStagedSupEntries entries each
+        return w;
+    };
     std::vector<int4> sup;
     for (int sl = 0; sl < nsl; ++sl) {
         const long long c0 = sbase[sl] / kWave, c1 = sbase[sl + 1] / kWave;
@@ -1213,13 +1204,71 @@ int plan_staged(CsrPlan &p, bool on)
         while (c < c1) {
             const long long a = c;
             long long ent = 0;
-            while (c < c1 && (c == a || ent + (cptr[c + 1] - cptr[c]) <= kStagedSupEntries)) {
-                ent += cptr[c + 1] - cptr[c];
+            while (c < c1) {
+                const long long e = static_cast<long long>(kWave) * width0(c);
+                if (c > a && ent + e > kStagedSupEntries) break;
+                ent += e;
                 ++c;
             }
             sup.push_back(make_int4(sl, static_cast<int>(a), static_cast<int>(c), 0));
         }
     }
+    // within each wave's group of chunks (the kStagedGroup chunks a wave of
+    // the first kernel takes at once: cb, cb + NW, ..), lanes are re-ordered by
+    // descending piece length, so that a chunk is only as wide as its own
+    // longest piece; each lane keeps the slot of its piece in the group (the
+    // partial is written there, so partials stay in run order)
+    constexpr int NW = kStagedThreads / kWave;
+    std::vector<int> lane_g(static_cast<size_t>(npad), -1);      // piece of each lane
+    std::vector<unsigned short> plen(static_cast<size_t>(npad), 0); // length | slot << 7
+    {
+        std::vector<std::pair<int, int>> grp; // (length, slot in the group)
+        std::vector<long long> pos;           // the group's lane positions, slot order
+        for (const int4 &su : sup)
+            for (long long base = su.y; base < su.z; base += static_cast<long long>(kStagedGroup) * NW)
+                for (int w = 0; w < NW && base + w < su.z; ++w) {
+                    grp.clear();
+                    pos.clear();
+                    for (int g = 0; g < kStagedGroup; ++g) {
+                        const long long c = base + w + static_cast<long long>(g) * NW;
+                        if (c >= su.z) break;
+                        for (int l = 0; l < kWave; ++l) {
+                            const long long q = c * kWave + l;
+                            const int pg = at_pos[q];
+                            grp.emplace_back(pg >= 0 ? g_len[pg] : 0, static_cast<int>(pos.size()));
+                            pos.push_back(q);
+                        }
+                    }
+                    std::stable_sort(grp.begin(), grp.end(),
+                                     [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                                         return x.first > y.first;
+                                     });
+                    for (size_t i = 0; i < grp.size(); ++i) {
+                        const int slot = grp[i].second;
+                        lane_g[pos[i]] = at_pos[pos[slot]];
+                        plen[pos[i]] = static_cast<unsigned short>(grp[i].first | (slot << 7));
+                    }
+                }
+    }
+    // chunks (column-major, as wide as their longest piece) and entries
+    std::vector<long long> cptr(static_cast<size_t>(nch) + 1, 0);
+    for (long long c = 0; c < nch; ++c) {
+        int w = 0;
+        for (int l = 0; l < kWave; ++l) w = std::max(w, plen[c * kWave + l] & 127);
+        cptr[c + 1] = cptr[c] + static_cast<long long>(kWave) * w;
+    }
+    std::vector<unsigned short> ec(static_cast<size_t>(cptr[nch]), 0);
+    std::vector<double> ev(static_cast<size_t>(cptr[nch]), 0.0);
+    for (long long c = 0; c < nch; ++c)
+        for (int l = 0; l < kWave; ++l) {
+            const int g = lane_g[c * kWave + l];
+            if (g < 0) continue;
+            for (int k = 0; k < g_len[g]; ++k) {
+                const long long e = cptr[c] + static_cast<long long>(k) * kWave + l;
+                ec[e] = static_cast<unsigned short>(gc[g_e[g] + k]);
+                ev[e] = ga[g_e[g] + k];
+            }
+        }
     std::vector<int> lpp32(static_cast<size_t>(n) + 1);
     for (int r = 0; r <= n; ++r) lpp32[r] = static_cast<int>(lpp[r]);
     std::vector<int4> bmeta(static_cast<size_t>(nbins)); // {first row, end row, first offset, end offset}
@@ -1262,9 +1311,9 @@ double csr_device_bytes(const CsrPlan &p)
         b += 4.0 * (p.npairs + 1.0) + 12.0 * nnz + 4.0 * (n + 1) + 12.0 * p.npairs + items(p.sitems);
     if (p.sell_on) // chunk pointers, columns (2 or 4 bytes), values
         b += 8.0 * (p.nchunks + 1.0) + (p.e_d16 ? 10.0 : 12.0) * static_cast<double>(p.sell_entries);
-    if (p.stg_on) { // chunks, piece lengths and partials, entries, blocks, bins, run table, lists
+    if (p.stg_on) { // chunks, lane codes and partials, entries, blocks, bins, run table, lists
         const double pad = static_cast<double>(kWave) * p.stg_nchunks;
-        b += 8.0 * (p.stg_nchunks + 1.0) + 9.0 * pad + 10.0 * static_cast<double>(p.stg_entries) +
+        b += 8.0 * (p.stg_nchunks + 1.0) + 10.0 * pad + 10.0 * static_cast<double>(p.stg_entries) +
              16.0 * (p.stg_nsup + p.stg_nbins) + 4.0 * (p.stg_nbins + 1.0) * p.stg_nsl + 4.0 * (n + 1) +
              2.0 * static_cast<double>(p.stg_npieces);
     }
