@@ -215,7 +215,9 @@ struct CsrPlan {
     double *s_eval = nullptr;              // s_cptr[nchunks]: values
     int4 *s_sup = nullptr;                 // nsup: {slice, first chunk, end chunk, 0}
     int4 *s_bmeta = nullptr;               // nbins: {first row, end row, first list offset, end list offset}
-    int *s_run = nullptr;                  // (nbins + 1) x nsl: first piece of (bin, slice)
+    int *s_run = nullptr;                  // nbins x (2 nsl + 1 + kBinCap / kBinStride): each bin's run table
+                                           // (LDS offset of each slice's run and the total, first piece minus
+                                           // LDS offset, the run of every kBinStride-th LDS index)
     int *s_lpp = nullptr;                  // nrows + 1: each row's pieces in s_lst
     unsigned short *s_lst = nullptr;       // npieces: LDS offset of each piece, slice order
     double *s_spart = nullptr;             // 64 nchunks: partials of standalone calls
@@ -228,7 +230,7 @@ constexpr int kBinThreads = 512;         // threads per block of the second (bin
 constexpr int kStagedPiece = 64;         // entries per piece (a piece's length fits a byte)
 constexpr int kBinCap = 3584;            // pieces per row bin (28 KB of partials and 7 KB of offsets in LDS:
                                          // four bins per CU)
-constexpr int kBinStride = 16;           // the bins kernel looks up the run of every 16th partial
+constexpr int kBinStride = 16;           // the run table names the run of every 16th partial
 constexpr int kStagedMaxSlices = 4000;   // slices (the bins kernel keeps two ints per slice in LDS)
 constexpr int kStagedSupEntries = 65536; // entries per block of the first kernel (about)
 

@@ -469,11 +469,12 @@ __device__ __forceinline__ void staged_block(int4 su, long long ncols, const lon
 }
 
 // Second kernel of the staged-slice form, one block (kBinThreads) per row bin
-// b of at most kBinCap pieces: the bin's run of every slice (run[b nsl + sl] ..
-// run[(b + 1) nsl + sl]) copied into LDS back to back (ps), and beside it the
-// bin's stretch of the per-row offset lists (lsh); then y(r) = the row's
-// partials added in slice order, and the basis column v(r) = x(r) s beside.
-// roff/rdel: 2 nsl + 1 ints of LDS.
+// b of at most kBinCap pieces: the bin's run of every slice copied into LDS
+// back to back (ps), and beside it the bin's stretch of the per-row offset
+// lists (lsh); then y(r) = the row's partials added in slice order, and the
+// basis column v(r) = x(r) s beside.  The bin's run table (built with the
+// plan: 2 nsl + 1 ints of LDS offsets and deltas, then the coarse table) is
+// read first.
 template <typename VT>
 __device__ __forceinline__ void bins_block(int b, int nsl, const int4 *__restrict__ bmeta_, const int *__restrict__ run_,
                                            const int *__restrict__ lpp_, const unsigned short *__restrict__ lst_,
@@ -481,7 +482,7 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int4 *__restric
                                            double *__restrict__ y_, VT *__restrict__ vout_,
                                            float *__restrict__ vout32_, long long vld, int vj, bool vtiled,
                                            double *ps, unsigned short *lsh, unsigned short *coarse, int *roff,
-                                           int *rdel, int *wtot)
+                                           int *rdel)
 {
     const auto run = dev::gptr(run_);
     const auto lpp = dev::gptr(lpp_);
@@ -492,7 +493,7 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int4 *__restric
     const auto vout = dev::gptr(vout_);
     const auto vout32 = dev::gptr(vout32_);
     constexpr int T = kBinThreads;
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const int tid = threadIdx.x;
     const int4 bm = bmeta_[b]; // rows [r0, r1), offsets lst[l0, l0 + nl)
     const int r0 = bm.x, r1 = bm.y, l0 = bm.z, nl = bm.w - bm.z;
     // this thread's first row: its bounds and x(r), independent of the copy
@@ -504,57 +505,19 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int4 *__restric
         f1 = lpp[rf + 1] - l0;
         if (vout) xf = x[rf];
     }
-    const auto rs = run + static_cast<long long>(b) * nsl; // starts; rs + nsl: ends
-    // LDS offsets of the runs: exclusive scan of their lengths, kPer slices per
-    // thread; rdel[sl] = (first piece of the run) - (its LDS offset)
-    constexpr int kPer = (kStagedMaxSlices + T - 1) / T;
-    int len[kPer], st[kPer], sum = 0;
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int sl = tid * kPer + u;
-        st[u] = sl < nsl ? rs[sl] : 0;
-        len[u] = sl < nsl ? rs[nsl + sl] - st[u] : 0;
-        sum += len[u];
-    }
-    int incl = sum;
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int t = __shfl_up(incl, off);
-        if (lane >= off) incl += t;
-    }
-    if (lane == kWave - 1) wtot[wave] = incl;
-    __syncthreads();
-    int ex = incl - sum;
-    for (int w = 0; w < wave; ++w) ex += wtot[w];
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-        const int sl = tid * kPer + u;
-        if (sl < nsl) {
-            roff[sl] = ex;
-            rdel[sl] = st[u] - ex;
-        }
-        ex += len[u];
-    }
-    if (tid == T - 1) roff[nsl] = ex; // the bin's total (= nl)
+    // the bin's run table into LDS: roff[sl] (LDS offset of run sl, roff[nsl]
+    // = the bin's total) and rdel[sl] (first piece of run sl - roff[sl]),
+    // then the coarse table: the run of every kBinStride-th LDS index
+    const auto tb = run + static_cast<long long>(b) * (2 * nsl + 1 + kBinCap / kBinStride);
+    for (int i = tid; i < 2 * nsl + 1; i += T) roff[i] = tb[i]; // rdel = roff + nsl + 1
+    for (int q = tid; q < kBinCap / kBinStride; q += T) coarse[q] = static_cast<unsigned short>(tb[2 * nsl + 1 + q]);
     __syncthreads();
     // the copy: LDS index i comes from run sl, the last run starting at or
-    // before i (empty runs never are).  A coarse table gives the run of every
-    // 16th index (a binary search each), and each index walks on from there;
-    // all of a thread's loads in flight, the bin's offsets with them
+    // before i (empty runs never are): from the coarse table's run of the
+    // index's group, a short walk; all of a thread's loads in flight, the
+    // bin's offsets with them
     constexpr int kCopy = kBinCap / T;
     const int total = roff[nsl];
-    for (int q = tid; q < (total + kBinStride - 1) / kBinStride; q += T) {
-        const int i = q * kBinStride;
-        int lo = 0, hi = nsl - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (roff[mid] <= i)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        coarse[q] = static_cast<unsigned short>(lo);
-    }
-    __syncthreads();
     double v[kCopy];
     int li[kCopy];
 #pragma unroll
@@ -622,10 +585,9 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_kernel(int nsl, const i
     __shared__ double ps[kBinCap];
     __shared__ unsigned short lsh[kBinCap];
     __shared__ unsigned short coarse[kBinCap / kBinStride];
-    __shared__ int wtot[kBinThreads / kWave];
     extern __shared__ int bins_dyn[];
     bins_block<double>(blockIdx.x, nsl, bmeta, run, lpp, lst, part, x, xscale ? *xscale : 1.0, y, vout, nullptr, vld,
-                       vj, vtiled, ps, lsh, coarse, bins_dyn, bins_dyn + nsl + 1, wtot);
+                       vj, vtiled, ps, lsh, coarse, bins_dyn, bins_dyn + nsl + 1);
 }
 
 // The staged-slice SpMVs of every listed solve: blocks numbered by stg0
@@ -660,13 +622,12 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_seg_kernel(const SegMem
     __shared__ double ps[kBinCap];
     __shared__ unsigned short lsh[kBinCap];
     __shared__ unsigned short coarse[kBinCap / kBinStride];
-    __shared__ int wtot[kBinThreads / kWave];
     extern __shared__ int bins_dyn[];
     const SegStep st = S[seg_find_stg(S, ns, blockIdx.x, true)];
     const SegMember mb = M[solo ? 0 : st.mem];
     bins_block<double>(blockIdx.x - st.bin0, mb.stg_nsl, mb.s_bmeta, mb.s_run, mb.s_lpp, mb.s_lst, mb.part, mb.r,
                        mb.ctl->xscale, mb.w, static_cast<double *>(mb.V), mb.V32, mb.vld, st.j, true, ps, lsh, coarse,
-                       bins_dyn, bins_dyn + mb.stg_nsl + 1, wtot);
+                       bins_dyn, bins_dyn + mb.stg_nsl + 1);
 }
 
 template <typename T>
@@ -1186,6 +1147,27 @@ int plan_staged(CsrPlan &p, bool on)
             return HL_ERR_ARG;
         }
     }
+    // each bin's run table for the bins kernel: roff[0..nsl] (LDS offset of
+    // each slice's run, then the bin's total), rdel[0..nsl) (first piece of the
+    // run minus its LDS offset), and the run of every kBinStride-th LDS index
+    constexpr int kCoarse = kBinCap / kBinStride;
+    const size_t tstride = 2 * static_cast<size_t>(nsl) + 1 + kCoarse;
+    std::vector<int> rtab(static_cast<size_t>(nbins) * tstride, 0);
+    for (int b = 0; b < nbins; ++b) {
+        int *ro = rtab.data() + static_cast<size_t>(b) * tstride, *rd = ro + nsl + 1, *co = rd + nsl;
+        int o = 0;
+        for (int sl = 0; sl < nsl; ++sl) {
+            const int i0 = run[static_cast<size_t>(b) * nsl + sl], i1 = run[static_cast<size_t>(b + 1) * nsl + sl];
+            ro[sl] = o;
+            rd[sl] = i0 - o;
+            o += i1 - i0;
+        }
+        ro[nsl] = o;
+        for (int q = 0, sl = 0; q < kCoarse && q * kBinStride < o; ++q) {
+            while (ro[sl + 1] <= q * kBinStride) ++sl; // the last run starting at or before the index
+            co[q] = sl;
+        }
+    }
     // blocks of the first kernel: runs of chunks within a slice, about
     // kStagedSupEntries entries each (chunk widths of the order above)
     const long long nch = npad / kWave;
@@ -1280,7 +1262,7 @@ int plan_staged(CsrPlan &p, bool on)
         (rc = dev_upload(&p.s_ecol, ec.data(), ec.size())) || (rc = dev_upload(&p.s_eval, ev.data(), ev.size())) ||
         (rc = dev_upload(&p.s_sup, sup.data(), sup.size())) ||
         (rc = dev_upload(&p.s_bmeta, bmeta.data(), bmeta.size())) ||
-        (rc = dev_upload(&p.s_run, run.data(), run.size())) ||
+        (rc = dev_upload(&p.s_run, rtab.data(), rtab.size())) ||
         (rc = dev_upload(&p.s_lpp, lpp32.data(), lpp32.size())) ||
         (rc = dev_upload(&p.s_lst, lst.data(), lst.size())) ||
         (rc = dev_upload<double>(&p.s_spart, nullptr, static_cast<size_t>(npad)))) {
@@ -1314,7 +1296,8 @@ double csr_device_bytes(const CsrPlan &p)
     if (p.stg_on) { // chunks, lane codes and partials, entries, blocks, bins, run table, lists
         const double pad = static_cast<double>(kWave) * p.stg_nchunks;
         b += 8.0 * (p.stg_nchunks + 1.0) + 10.0 * pad + 10.0 * static_cast<double>(p.stg_entries) +
-             16.0 * (p.stg_nsup + p.stg_nbins) + 4.0 * (p.stg_nbins + 1.0) * p.stg_nsl + 4.0 * (n + 1) +
+             16.0 * (p.stg_nsup + p.stg_nbins) + 4.0 * p.stg_nbins * (2.0 * p.stg_nsl + 1 + kBinCap / kBinStride) +
+             4.0 * (n + 1) +
              2.0 * static_cast<double>(p.stg_npieces);
     }
     return b;
