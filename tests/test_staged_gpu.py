@@ -245,6 +245,47 @@ def test_staged_operator_solves_match_oracle_alone_and_batched():
     np.testing.assert_array_equal(b[3].values, r_plain.values)
 
 
+def test_staged_solve_with_an_empty_slice():
+    """A square operator whose middle slice of 16384 columns (and so, being
+    symmetric, those rows) holds no entry: no piece and no chunk there, while
+    the basis column is still written over those rows.  Values and vectors
+    against the oracle and dense residuals, alone and batched bitwise."""
+    rng = np.random.default_rng(12)
+    n, blk = 3 << 14, 1 << 14
+    keep = np.r_[0:blk, 2 * blk:3 * blk]  # rows/columns of slices 0 and 2
+    m = len(keep)
+    a = rng.integers(0, m, 8 * m)
+    b = rng.integers(0, m, 8 * m)
+    v = rng.uniform(-1, 1, 8 * m)
+    B = sp.coo_matrix((np.r_[v, v], (np.r_[a, b], np.r_[b, a])), shape=(m, m)).tocsr()
+    d = np.linspace(0.0, 10.0, m)
+    d[-8:] = 30.0 + 2.0 * np.arange(8)  # separated top of the spectrum: a quick solve
+    B = B + sp.diags(d)
+    P = sp.csr_matrix((np.ones(m), (keep, np.arange(m))), shape=(n, m))
+    A = (P @ B @ P.T).tocsr()
+    A.sort_indices()
+    rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    op = H.dcsrmv_operator_gpu(H.dcsr_matrix_t(row_ptr=rp, col=col, val=val))
+    op.set_staged(True)
+    assert op.staged()[0]
+    o = O.ds_solver(O.Csr(rp, col, val), 4, 16, tol=1e-10, maxiter=2000)
+    assert o.ret == 0
+    alone = []
+    for prec in (H.HL_PREC_F64, H.HL_PREC_F64_PROJ32):
+        r = H.ds_solver_gpu(n, 4, 16, op, tol=1e-10, maxiter=2000, basis_prec=prec)
+        assert r.ret == 0
+        np.testing.assert_allclose(r.values, o.values, rtol=1e-10)
+        for i in range(4):
+            x = r.vectors[:, i]
+            assert np.linalg.norm(A @ x - r.values[i] * x) <= 1e-9 * np.abs(r.values).max()
+        alone.append(r)
+    bt = H.ds_solve_batch([op, op], 4, 16, tol=1e-10, maxiter=2000,
+                          basis_prec=[H.HL_PREC_F64, H.HL_PREC_F64_PROJ32])
+    for k in range(2):
+        np.testing.assert_array_equal(bt[k].values, alone[k].values)
+        np.testing.assert_array_equal(bt[k].vectors, alone[k].vectors)
+
+
 def test_staged_ncv96_solve_matches_oracle():
     """A wider basis (ncv 96) over a staged operator, FP32-copy mode, against
     the oracle and bitwise the same batched with a second staged operator."""
