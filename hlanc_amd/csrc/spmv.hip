@@ -1012,8 +1012,11 @@ int plan_sell(CsrPlan &p, bool on, double max_fill)
 // (slices ascending, each slice's entries in the row's own order, cut every
 // kStagedPiece entries; this is also each row's summation order), row bins cut
 // greedily at kBinCap pieces, pieces bucketed by slice (row order kept) and
-// sorted by descending length within each (slice, bin) run, each slice padded
-// to whole 64-piece chunks.
+// sorted by descending length within each (slice, bin) run (the partials'
+// order), each slice padded to whole 64-piece chunks; the first kernel's
+// blocks cut from that order; then, within each wave's group of chunks, the
+// lanes re-ordered by length with each lane's slot recorded; each bin's run
+// table for the bins kernel.
 int plan_staged(CsrPlan &p, bool on)
 {
     staged_free(p);
