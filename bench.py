@@ -570,6 +570,8 @@ def main():
     ap.add_argument("--workload", default="chunglu", choices=sorted(WORKLOADS) + ["batch", "serve"])
     ap.add_argument("--no-sub", action="store_true", help="headline only: skip the batch and serve sub-objects")
     ap.add_argument("--count", type=int, default=512, help="batch: number of matrices")
+    ap.add_argument("--batch-engines", type=int, default=0,
+                    help="batch: engines sharing the admission queue (0: the library's default)")
     ap.add_argument("--max-active", type=int, default=0, help="batch/serve: solves in flight (0: no cap)")
     ap.add_argument("--hbm-budget", type=float, default=64e9, help="serve: footprint budget in bytes")
     ap.add_argument("--rates", default=None, help="serve: offered requests/s, comma-separated")
@@ -649,6 +651,8 @@ def main():
 
     maxiter_many = 3000 if args.maxiter == 1000 else args.maxiter
     if args.workload == "batch":
+        if args.batch_engines > 0:
+            H.tune_set("batch_engines", args.batch_engines)
         out = run_batch(H, torch, count=args.count, steps=args.steps, warmup=args.warmup, basis=args.basis,
                         tol=args.tol, maxiter=maxiter_many, max_active=args.max_active, mfma=args.mfma,
                         autotune=not args.no_autotune, plain=not args.no_plain, cpu=not args.no_cpu,
