@@ -4,6 +4,7 @@
 #include <cstring>
 #include <chrono>
 #include <mutex>
+#include <shared_mutex>
 #include <thread>
 #include <exception>
 #include <new>
@@ -1042,12 +1043,14 @@ try {
         adm.budget = hbm_budget;
         adm.t_start = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
         adm.engines = engines;
-        std::mutex hip_mu; // captures vs. the other engines' allocations
+        std::mutex hip_mu;            // captures vs. the other engines' allocations
+        std::shared_mutex sync_rw;    // captures vs. the other engines' waits
         std::vector<int> rcs(static_cast<size_t>(engines), HL_OK);
         auto serve = [&](int k) {
             DeviceGuard gk(dev);
             hl::Engine eng;
             eng.capture_mu = &hip_mu;
+            eng.sync_rw = &sync_rw;
             eng.graphs = graphs;
             int r = eng.init(dev); // a full host pool each: their restart phases rarely coincide
             if (!r) r = eng.run_shared(adm);

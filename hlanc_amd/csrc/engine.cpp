@@ -485,6 +485,7 @@ struct Engine::Pool {
 int Engine::init(int dev_, int host_threads)
 {
     dev = dev_;
+    auto lk = hip_lock(); // stream and allocations: not while another engine captures
     HL_TRY_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     Ctl one{};
     one.xscale = one.rscale = 1.0;
@@ -504,7 +505,7 @@ void Engine::drop_graphs()
 
 void Engine::release()
 {
-    if (stream) (void)hipStreamSynchronize(stream);
+    if (stream) (void)sync_stream();
     bury(); // takes the lock itself
     auto lk = hip_lock();
     for (Pending &p : pend) (void)hipEventDestroy(p.ev);
@@ -557,7 +558,7 @@ int Engine::grow(size_t nm, size_t ns, size_t nv, size_t rb, size_t qd)
     std::unique_lock<std::mutex> lk;
     if (nm > capM || ns > capS || nv > capV || rb > capRB || qd > capQ) {
         lk = hip_lock();
-        HL_TRY_HIP(hipStreamSynchronize(stream)); // uploads from the old buffers may be in flight
+        HL_TRY_HIP(sync_stream()); // uploads from the old buffers may be in flight
         if (nm > capM || ns > capS) drop_graphs(); // captured launches name the old tables
     }
     int rc;
@@ -592,7 +593,11 @@ int Engine::poll_finished(bool wait)
     size_t keep = 0;
     for (size_t k = 0; k < pend.size(); ++k) {
         Pending &p = pend[k];
-        hipError_t e = wait ? hipEventSynchronize(p.ev) : hipEventQuery(p.ev);
+        hipError_t e;
+        {
+            auto wl = wait_lock();
+            e = wait ? hipEventSynchronize(p.ev) : hipEventQuery(p.ev);
+        }
         if (e == hipErrorNotReady) {
             pend[keep++] = p;
             continue;
@@ -653,7 +658,10 @@ int Engine::sweep(std::vector<Member *> &ms)
     int rc;
     const double th0 = now_s();
     for (Member *m : ms)
-        if (m->phase == Member::NEW && (rc = m->start(stream))) return rc;
+        if (m->phase == Member::NEW) { // may copy a caller's pageable v0: not while another engine captures
+            auto wl = wait_lock();
+            if ((rc = m->start(stream))) return rc;
+        }
     std::vector<Member *> act;
     for (Member *m : ms)
         if (m->phase == Member::ACTIVE) act.push_back(m);
@@ -775,7 +783,10 @@ int Engine::sweep(std::vector<Member *> &ms)
             double bytes = 0.0;
             for (int q = off[st]; q < off[st + 1]; ++q) {
                 Member &m = *act[hS[q].mem];
-                if (!m.A && (r = m.host_apply(hS[q].j, stream))) return r;
+                if (!m.A) { // waits on the stream inside: not while another engine captures
+                    auto wl = wait_lock();
+                    if ((r = m.host_apply(hS[q].j, stream))) return r;
+                }
                 if (m.A) // values + indices, row pointers, x, y, the basis column (and its FP32 copy)
                     bytes += (m.kind == SEG_V32_A32 ? 8.0 : 12.0) * m.A->nnz + 4.0 * (m.A->nrows + 1) +
                              8.0 * m.A->ncols + 8.0 * m.A->nrows +
@@ -871,6 +882,8 @@ int Engine::sweep(std::vector<Member *> &ms)
             hipGraph_t graph = nullptr;
             std::unique_lock<std::mutex> cap_lk;
             if (capture_mu) cap_lk = std::unique_lock<std::mutex>(*capture_mu);
+            std::unique_lock<std::shared_mutex> cap_wl; // no other engine waits meanwhile
+            if (sync_rw) cap_wl = std::unique_lock<std::shared_mutex>(*sync_rw);
             HL_TRY_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             rc = launch_steps();
             hipError_t ce = hipStreamEndCapture(stream, &graph);
@@ -921,7 +934,7 @@ readback:
     // every member's alpha/beta entries and counters in one copy
     if ((rc = launch_seg_readback(dM, static_cast<int>(nm), stride, dRB, stream))) return rc;
     HL_TRY_HIP(hipMemcpyAsync(hRB, dRB, sizeof(double) * nm * stride, hipMemcpyDeviceToHost, stream));
-    HL_TRY_HIP(hipStreamSynchronize(stream));
+    HL_TRY_HIP(sync_stream());
     const double th1 = now_s();
     bury(); // the device is idle: frees here cost no stall
     if (prof.on && (rc = prof.flush())) return rc;
@@ -1062,10 +1075,12 @@ int Engine::after_sweep(std::vector<Member *> &act)
         if ((rc = prof.end(stream, K_VQ, 16.0 * m.n * m.ncv))) return rc;
         // sync_host (HLanc.hpp:1063-1071): all ncv columns to the caller's
         // buffer, host (pageable or page-locked) or device memory
-        if (m.with_vectors)
+        if (m.with_vectors) { // pageable host memory makes the copy wait: not while another engine captures
+            auto wl = wait_lock();
             HL_TRY_HIP(hipMemcpy2DAsync(m.vectors, sizeof(double) * (size_t)m.ldv_host, W.V[nxt],
                                         sizeof(double) * (size_t)W.ldv, sizeof(double) * (size_t)m.n, m.ncv,
                                         hipMemcpyDefault, stream));
+        }
         if (m.verify) ver.push_back(&m);
     }
     if (!ver.empty() && (rc = verify_pairs(ver))) return rc;
@@ -1112,12 +1127,12 @@ int Engine::verify_pairs(std::vector<Member *> &ver)
         for (int i = 0; i < m->nev0; ++i) {
             HL_TRY_HIP(hipMemcpyAsync(W.xh, Vr + (size_t)i * W.ldv, sizeof(double) * m->n, hipMemcpyDeviceToHost,
                                       stream));
-            HL_TRY_HIP(hipStreamSynchronize(stream));
+            HL_TRY_HIP(sync_stream());
             m->host_op(m->host_ctx, W.xh, W.yh);
             HL_TRY_HIP(hipMemcpyAsync(Y + (size_t)i * W.ldv, W.yh, sizeof(double) * m->n, hipMemcpyHostToDevice,
                                       stream));
         }
-        HL_TRY_HIP(hipStreamSynchronize(stream)); // yh is reused by the next member
+        HL_TRY_HIP(sync_stream()); // yh is reused by the next member
     }
     // device operators: entries ordered by pair index, so launch i is one run
     std::vector<int> off(maxnev + 1, 0), nblk(maxnev, 0), nred(maxnev, 0), nstg(maxnev, 0), nbin(maxnev, 0),
@@ -1305,11 +1320,11 @@ int Engine::run_shared(Admission &adm)
     if (!rc) rc = poll_finished(true);
     if (!rc) retire();
     if (!rc && prof.on) { // the V.Q launches after the last sweep
-        HL_TRY_HIP(hipStreamSynchronize(stream));
+        HL_TRY_HIP(sync_stream());
         rc = prof.flush();
     }
     if (rc) { // a HIP failure ends every member in flight on this engine
-        (void)hipStreamSynchronize(stream);
+        (void)sync_stream();
         {
             auto lk = hip_lock();
             for (Pending &p : pend) (void)hipEventDestroy(p.ev);
@@ -1329,7 +1344,7 @@ int Engine::run_shared(Admission &adm)
         adm.cv.notify_all();
         active.clear();
     }
-    HL_TRY_HIP(hipStreamSynchronize(stream));
+    HL_TRY_HIP(sync_stream());
     bury();
     return rc;
 }

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <condition_variable>
 #include <mutex>
+#include <shared_mutex>
 #include <utility>
 #include <vector>
 
@@ -216,6 +217,20 @@ struct Engine {
     std::unique_lock<std::mutex> hip_lock() const
     {
         return capture_mu ? std::unique_lock<std::mutex>(*capture_mu) : std::unique_lock<std::mutex>();
+    }
+    // Shared between the engines of a batch: each engine's waits on its own
+    // stream or events hold it shared, a capture holds it exclusively, so that
+    // no wait of one engine overlaps another engine's capture (HIP refuses
+    // such a wait with hipErrorStreamCaptureUnsupported).
+    std::shared_mutex *sync_rw = nullptr;
+    std::shared_lock<std::shared_mutex> wait_lock() const
+    {
+        return sync_rw ? std::shared_lock<std::shared_mutex>(*sync_rw) : std::shared_lock<std::shared_mutex>();
+    }
+    hipError_t sync_stream() const
+    {
+        auto lk = wait_lock();
+        return hipStreamSynchronize(stream);
     }
     double t_start = 0.0;                // steady-clock seconds at run()
     // host_threads: workers of the decide() pool (< 0: one per host core, up to 15)
