@@ -1204,6 +1204,7 @@ int plan_staged(CsrPlan &p, bool on)
     // longest piece; each lane keeps the slot of its piece in the group (the
     // partial is written there, so partials stay in run order)
     constexpr int NW = kStagedThreads / kWave;
+    static_assert(kStagedPiece < 128 && kStagedGroup * kWave <= 512, "lane code: 7-bit length, 9-bit slot");
     std::vector<int> lane_g(static_cast<size_t>(npad), -1);      // piece of each lane
     std::vector<unsigned short> plen(static_cast<size_t>(npad), 0); // length | slot << 7
     {
