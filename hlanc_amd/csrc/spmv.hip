@@ -515,7 +515,10 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int4 *__restric
     // the copy: LDS index i comes from run sl, the last run starting at or
     // before i (empty runs never are): from the coarse table's run of the
     // index's group, a short walk; all of a thread's loads in flight, the
-    // bin's offsets with them
+    // bin's offsets with them.  The partials are read with ordinary (cached)
+    // loads: a run averages under two 128-byte lines, and its first and last
+    // lines hold the neighbouring bins' runs of the same slice, which other
+    // blocks read at about the same time
     constexpr int kCopy = kBinCap / T;
     const int total = roff[nsl];
     double v[kCopy];
@@ -526,7 +529,7 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int4 *__restric
         if (i < total) {
             int sl = coarse[i / kBinStride];
             while (roff[sl + 1] <= i) ++sl;
-            v[u] = __builtin_nontemporal_load(&part[rdel[sl] + i]); // read once
+            v[u] = part[rdel[sl] + i];
         }
         if (i < nl) li[u] = lst[l0 + i];
     }
