@@ -456,8 +456,8 @@ __device__ __forceinline__ void staged_block(int4 su, long long ncols, const lon
                     if (k0 + u < len[g]) acc[g] += av[g][u] * (xs[ci[g][u]] * s);
         }
         // each partial to its piece's slot in the group (the group's stores
-        // together write every slot of its chunks; non-temporal, as the bins
-        // kernel reads them once)
+        // together write every slot of its chunks); non-temporal stores, which
+        // took less time over a whole solve than plain ones
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int c = cb + g * NW;
