@@ -559,6 +559,23 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int4 *__restric
     }
 }
 
+// The bin of bins-kernel block i of nb.  Blocks are dispatched to the 8 XCDs
+// in turn (block i to XCD i % 8), so consecutive bins would run on different
+// XCDs; here each XCD takes groups of kBinGroup consecutive bins instead, and
+// a run's end lines, shared with the neighbouring bins' runs of the same
+// slice, are read again from that XCD's L2.  Groups still go out in bin
+// order, so every XCD gets its share of the heavy (hub-row) bins at the
+// front.  A bijection on [0, nb): the last nb mod (8 kBinGroup) bins keep
+// their own block.
+constexpr int kBinGroup = 8;
+__device__ __forceinline__ int bins_deal(int i, int nb)
+{
+    const int full = nb / (8 * kBinGroup) * (8 * kBinGroup);
+    if (i >= full) return i;
+    const int j = i >> 3;
+    return ((j / kBinGroup) * 8 + (i & 7)) * kBinGroup + j % kBinGroup;
+}
+
 // Dynamic LDS of the bins kernels: the run offsets and deltas (2 nsl + 1 ints).
 inline size_t bins_dyn_lds(int nsl) { return sizeof(int) * (2 * static_cast<size_t>(nsl) + 1); }
 
@@ -589,8 +606,8 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_kernel(int nsl, const i
     __shared__ unsigned short lsh[kBinCap];
     __shared__ unsigned short coarse[kBinCap / kBinStride];
     extern __shared__ int bins_dyn[];
-    bins_block<double>(blockIdx.x, nsl, bmeta, run, lpp, lst, part, x, xscale ? *xscale : 1.0, y, vout, nullptr, vld,
-                       vj, vtiled, ps, lsh, coarse, bins_dyn, bins_dyn + nsl + 1);
+    bins_block<double>(bins_deal(blockIdx.x, gridDim.x), nsl, bmeta, run, lpp, lst, part, x, xscale ? *xscale : 1.0, y,
+                       vout, nullptr, vld, vj, vtiled, ps, lsh, coarse, bins_dyn, bins_dyn + nsl + 1);
 }
 
 // The staged-slice SpMVs of every listed solve: blocks numbered by stg0
@@ -626,9 +643,10 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_seg_kernel(const SegMem
     __shared__ unsigned short lsh[kBinCap];
     __shared__ unsigned short coarse[kBinCap / kBinStride];
     extern __shared__ int bins_dyn[];
-    const SegStep st = S[seg_find_stg(S, ns, blockIdx.x, true)];
+    const int gb = bins_deal(blockIdx.x, gridDim.x); // a bijection on the whole grid, then the member
+    const SegStep st = S[seg_find_stg(S, ns, gb, true)];
     const SegMember mb = M[solo ? 0 : st.mem];
-    bins_block<double>(blockIdx.x - st.bin0, mb.stg_nsl, mb.s_bmeta, mb.s_run, mb.s_lpp, mb.s_lst, mb.part, mb.r,
+    bins_block<double>(gb - st.bin0, mb.stg_nsl, mb.s_bmeta, mb.s_run, mb.s_lpp, mb.s_lst, mb.part, mb.r,
                        mb.ctl->xscale, mb.w, static_cast<double *>(mb.V), mb.V32, mb.vld, st.j, true, ps, lsh, coarse,
                        bins_dyn, bins_dyn + mb.stg_nsl + 1);
 }
