@@ -559,22 +559,25 @@ __device__ __forceinline__ void bins_block(int b, int nsl, const int4 *__restric
     }
 }
 
-// The bin of bins-kernel block i of nb.  Blocks are dispatched to the 8 XCDs
-// in turn (block i to XCD i % 8), so consecutive bins would run on different
-// XCDs; here each XCD takes groups of kBinGroup consecutive bins instead, and
-// a run's end lines, shared with the neighbouring bins' runs of the same
-// slice, are read again from that XCD's L2.  Groups still go out in bin
-// order, so every XCD gets its share of the heavy (hub-row) bins at the
-// front.  A bijection on [0, nb): the last nb mod (8 kBinGroup) bins keep
-// their own block.
-constexpr int kBinGroup = 8;
-__device__ __forceinline__ int bins_deal(int i, int nb)
+// The work unit of block i of an nb-block launch, dealt to the XCDs in
+// groups of K consecutive units.  Blocks are dispatched to the 8 XCDs in turn
+// (block i to XCD i % 8), so consecutive units would run on different XCDs;
+// here each XCD takes K consecutive units at a time, and what neighbouring
+// units share is read again from that XCD's L2: a slice's x for the first
+// staged kernel's blocks (groups of 4), the lines that a bin's runs share
+// with the neighbouring bins' runs of the same slice for the bins kernel
+// (groups of 8).  Groups still go out in unit order, so every XCD gets its
+// share of the heavy units (slice 0, the hub-row bins) at the front.  A
+// bijection on [0, nb): the last nb mod (8 K) units keep their own block.
+template <int K>
+__device__ __forceinline__ int xcd_deal(int i, int nb)
 {
-    const int full = nb / (8 * kBinGroup) * (8 * kBinGroup);
+    const int full = nb / (8 * K) * (8 * K);
     if (i >= full) return i;
     const int j = i >> 3;
-    return ((j / kBinGroup) * 8 + (i & 7)) * kBinGroup + j % kBinGroup;
+    return ((j / K) * 8 + (i & 7)) * K + j % K;
 }
+constexpr int kStagedGroupXcd = 4, kBinGroupXcd = 8;
 
 // Dynamic LDS of the bins kernels: the run offsets and deltas (2 nsl + 1 ints).
 inline size_t bins_dyn_lds(int nsl) { return sizeof(int) * (2 * static_cast<size_t>(nsl) + 1); }
@@ -589,7 +592,8 @@ __global__ __launch_bounds__(kStagedThreads) void spmv_staged_kernel(const int4 
                                                                      double *__restrict__ part)
 {
     __shared__ double xs[1 << kStagedLw];
-    staged_block(sup[blockIdx.x], ncols, cptr, plen, ecol, eval, x, xscale ? *xscale : 1.0, part, xs);
+    staged_block(sup[xcd_deal<kStagedGroupXcd>(blockIdx.x, gridDim.x)], ncols, cptr, plen, ecol, eval, x,
+                 xscale ? *xscale : 1.0, part, xs);
 }
 
 __global__ __launch_bounds__(kBinThreads) void spmv_bins_kernel(int nsl, const int4 *__restrict__ bmeta,
@@ -606,8 +610,9 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_kernel(int nsl, const i
     __shared__ unsigned short lsh[kBinCap];
     __shared__ unsigned short coarse[kBinCap / kBinStride];
     extern __shared__ int bins_dyn[];
-    bins_block<double>(bins_deal(blockIdx.x, gridDim.x), nsl, bmeta, run, lpp, lst, part, x, xscale ? *xscale : 1.0, y,
-                       vout, nullptr, vld, vj, vtiled, ps, lsh, coarse, bins_dyn, bins_dyn + nsl + 1);
+    bins_block<double>(xcd_deal<kBinGroupXcd>(blockIdx.x, gridDim.x), nsl, bmeta, run, lpp, lst, part, x,
+                       xscale ? *xscale : 1.0, y, vout, nullptr, vld, vj, vtiled, ps, lsh, coarse, bins_dyn,
+                       bins_dyn + nsl + 1);
 }
 
 // The staged-slice SpMVs of every listed solve: blocks numbered by stg0
@@ -630,9 +635,10 @@ __global__ __launch_bounds__(kStagedThreads) void spmv_staged_seg_kernel(const S
                                                                          int solo)
 {
     __shared__ double xs[1 << kStagedLw];
-    const SegStep st = S[seg_find_stg(S, ns, blockIdx.x, false)];
+    const int gb = xcd_deal<kStagedGroupXcd>(blockIdx.x, gridDim.x); // a bijection on the whole grid, then the member
+    const SegStep st = S[seg_find_stg(S, ns, gb, false)];
     const SegMember mb = M[solo ? 0 : st.mem];
-    staged_block(mb.s_sup[blockIdx.x - st.stg0], mb.n, mb.s_cptr, mb.s_plen, mb.s_ecol, mb.s_eval, mb.r,
+    staged_block(mb.s_sup[gb - st.stg0], mb.n, mb.s_cptr, mb.s_plen, mb.s_ecol, mb.s_eval, mb.r,
                  mb.ctl->xscale, mb.part, xs);
 }
 
@@ -643,7 +649,7 @@ __global__ __launch_bounds__(kBinThreads) void spmv_bins_seg_kernel(const SegMem
     __shared__ unsigned short lsh[kBinCap];
     __shared__ unsigned short coarse[kBinCap / kBinStride];
     extern __shared__ int bins_dyn[];
-    const int gb = bins_deal(blockIdx.x, gridDim.x); // a bijection on the whole grid, then the member
+    const int gb = xcd_deal<kBinGroupXcd>(blockIdx.x, gridDim.x); // a bijection on the whole grid, then the member
     const SegStep st = S[seg_find_stg(S, ns, gb, true)];
     const SegMember mb = M[solo ? 0 : st.mem];
     bins_block<double>(gb - st.bin0, mb.stg_nsl, mb.s_bmeta, mb.s_run, mb.s_lpp, mb.s_lst, mb.part, mb.r,
